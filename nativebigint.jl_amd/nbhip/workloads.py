@@ -1,0 +1,115 @@
+"""Seeded synthetic operands and job lists for the multiplication workloads.
+
+Limbs are i.i.d. uniform UInt64 drawn from a SplitMix64 stream per
+(workload, job, operand), seed 0x4e42; a zero top limb is replaced by 1, so
+every operand has its full limb length (the reference benches draw
+`rand(UInt64, n)` the same way, bench/bench_kernels.jl:100-136).
+
+Workloads (sizes in 64-bit limbs):
+  single   4096 x 4096 product and a 4096-limb square
+  large    1,048,576 x 1,048,576, 1,048,576 x 262,144, square of 4,194,304
+  batch    4096 independent pairs, larger size log-uniform in [152, 65536],
+           25% squarings, otherwise size ratio uniform in [1, 4] (smaller
+           size at least 152, so the reference routes each to its NTT engine,
+           src/mul.jl:388)
+  hgcd     1024 instances of a 2x2 matrix of n/2-limb cofactors times a
+           2-vector of n-limb operands, n log-uniform in [600, 32768]
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+SEED = 0x4E42
+GAMMA = 0x9E3779B97F4A7C15
+_M1 = 0xBF58476D1CE4E5B9
+_M2 = 0x94D049BB133111EB
+MASK64 = (1 << 64) - 1
+
+WORKLOAD_IDS = {"single": 1, "large": 2, "batch": 3, "hgcd": 4, "serve": 5, "test": 9}
+
+
+def _mix(z: int) -> int:
+    z &= MASK64
+    z = ((z ^ (z >> 30)) * _M1) & MASK64
+    z = ((z ^ (z >> 27)) * _M2) & MASK64
+    return z ^ (z >> 31)
+
+
+def splitmix64(seed: int, n: int) -> np.ndarray:
+    """First n outputs of SplitMix64 started at `seed` (vectorised, exact)."""
+    i = np.arange(1, n + 1, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed & MASK64) + i * np.uint64(GAMMA)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(_M1)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(_M2)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def stream_seed(workload: str | int, job: int, operand: int) -> int:
+    wid = WORKLOAD_IDS[workload] if isinstance(workload, str) else int(workload)
+    return _mix(SEED ^ (wid << 56) ^ (job << 8) ^ operand)
+
+
+def operand(workload: str | int, job: int, which: int, n: int) -> np.ndarray:
+    """n uniform limbs with a nonzero top limb."""
+    a = splitmix64(stream_seed(workload, job, which), n)
+    if n and a[-1] == 0:
+        a[-1] = 1
+    return a
+
+
+@dataclass
+class Job:
+    m: int          # limbs of a (m >= n)
+    n: int          # limbs of b (n == m for a square)
+    sqr: bool
+
+
+def _uniform01(seed: int, n: int) -> np.ndarray:
+    return (splitmix64(seed, n) >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+
+
+def batch_jobs(count: int = 4096, lo: int = 152, hi: int = 65536, sqr_frac: float = 0.25,
+               max_ratio: float = 4.0, workload: str = "batch") -> list[Job]:
+    u = _uniform01(stream_seed(workload, 0xFFFFFF, 1), 3 * count).reshape(count, 3)
+    jobs = []
+    for k in range(count):
+        L = int(math.floor(math.exp(math.log(lo) + u[k, 0] * (math.log(hi) - math.log(lo)))))
+        L = max(lo, min(hi, L))
+        if u[k, 1] < sqr_frac:
+            jobs.append(Job(L, L, True))
+        else:
+            rho = 1.0 + u[k, 2] * (max_ratio - 1.0)
+            jobs.append(Job(L, max(lo, int(L // rho)), False))
+    return jobs
+
+
+def batch_operands(jobs: list[Job], workload: str = "batch"):
+    """(a, b) limb arrays per job; b is None for squarings."""
+    out = []
+    for k, j in enumerate(jobs):
+        a = operand(workload, k, 0, j.m)
+        b = None if j.sqr else operand(workload, k, 1, j.n)
+        out.append((a, b))
+    return out
+
+
+def hgcd_sizes(count: int = 1024, lo: int = 600, hi: int = 32768) -> list[int]:
+    u = _uniform01(stream_seed("hgcd", 0xFFFFFF, 1), count)
+    out = []
+    for k in range(count):
+        n = int(math.floor(math.exp(math.log(lo) + u[k] * (math.log(hi) - math.log(lo)))))
+        out.append(max(lo, min(hi, n)))
+    return out
+
+
+def limbs_to_int(a) -> int:
+    return int.from_bytes(np.ascontiguousarray(a, dtype="<u8").tobytes(), "little")
+
+
+def int_to_limbs(x: int, n: int) -> np.ndarray:
+    return np.frombuffer(x.to_bytes(8 * n, "little"), dtype="<u8").astype(np.uint64)
