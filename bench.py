@@ -1,0 +1,303 @@
+"""Benchmark: bit-exact limb-products/s of the MI355X NTT product engine.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batch|large]
+
+workload "batch" (default): 4096 independent products, larger size
+log-uniform in [152, 65536] limbs, 25% squarings, size ratio uniform in
+[1, 4].  A step is the whole job list in one planned batch (a fixed number of
+launches covering every job).  The same list is also run one product per
+launch for the batched/one-per-launch ratio.
+workload "large": 2^20 x 2^20, 2^20 x 2^18 and a 2^22-limb square, one
+product per launch; a step is the three products.
+
+limb-products/s = sum over completed bit-exact products of (m + n) result
+limbs / wall time, inputs already resident in HBM.  Every product of the
+timed run is checked bit-exact (libgmp when present, else the oracle).
+"""
+import argparse
+import ctypes
+import hashlib
+import json
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "nativebigint.jl_amd"))
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+FP64_VALU_PEAK_TFLOPS = 78.6   # MI355X FP64 vector
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=["batch", "large"], default="batch")
+    ap.add_argument("--jobs", type=int, default=4096)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-verify", action="store_true")
+    ap.add_argument("--no-one-per-launch", action="store_true")
+    return ap.parse_args()
+
+
+def kernel_bytes(plan_groups, N, m, n, sqr, nconv):
+    """Algorithmic HBM bytes per kernel kind for one product: each launch's
+    unavoidable reads + writes (spectra are 8-byte doubles, limbs 8 bytes)."""
+    nops = 1 if sqr else 2
+    K = len(plan_groups)
+    limbs_in = 8 * (m if sqr else m + n)
+    out = {"group_fwd": 0, "mid": 0, "group_inv": 0, "unpack": 0, "carry_fix": 0}
+    for g in range(K - 1):
+        rd = 2 * limbs_in if g == 0 else 2 * nops * 8 * N
+        out["group_fwd"] += rd + 2 * nops * 8 * N
+        out["group_inv"] += 2 * 2 * 8 * N
+    out["mid"] += (2 * limbs_in if K == 1 else 2 * nops * 8 * N) + 2 * 8 * N
+    out["unpack"] += 2 * 8 * nconv + 8 * (m + n)
+    return out
+
+
+def fp64_flops(N, plan_stage_radices, sqr, nconv):
+    """FP64 VALU operations per product (each add/mul/fma counted once):
+    per point per stage radix-4 10, radix-2 5, radix-3 10.67, radix-5 18.8,
+    radix-7 25; pointwise 7 per point per prime; unpack ~35 per coefficient."""
+    cost = {4: 10.0, 2: 5.0, 3: 32.0 / 3, 5: 18.8, 7: 25.0}
+    c = sum(cost[r] for r in plan_stage_radices)
+    T = 2 if sqr else 3
+    return 2 * (T * c * N + 7 * N) + 35 * nconv
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU: multi-GPU sharding is a design note, "
+                                   "not built; run with --gpus 1"}))
+        sys.exit(2)
+    import torch
+
+    from nbhip import device as D
+    from nbhip import engine
+    from nbhip import workloads as wl
+
+    engine.init()
+    info = D.device_info()
+    log(f"device {info}")
+    dev = torch.device("cuda")
+    stream = torch.cuda.Stream()
+
+    # ---- build the job list --------------------------------------------------
+    if args.workload == "batch":
+        jobs = wl.batch_jobs(count=args.jobs)
+        ops = wl.batch_operands(jobs)
+        wname = "batch"
+    else:
+        jobs = [wl.Job(1 << 20, 1 << 20, False), wl.Job(1 << 20, 1 << 18, False), wl.Job(1 << 22, 1 << 22, True)]
+        ops = [(wl.operand("large", 0, 0, 1 << 20), wl.operand("large", 0, 1, 1 << 20)),
+               (wl.operand("large", 1, 0, 1 << 20), wl.operand("large", 1, 1, 1 << 18)),
+               (wl.operand("large", 2, 0, 1 << 22), None)]
+        wname = "large"
+    t0 = time.time()
+    dtens = []
+    total_limbs = 0
+    meta = []
+    for j, (a, b) in zip(jobs, ops):
+        bits_a = engine.magnitude_bits(a, 0, a.size)
+        bits_b = bits_a if b is None else engine.magnitude_bits(b, 0, b.size)
+        rn = a.size + (a.size if b is None else b.size)
+        total_limbs += rn
+        bch, N = engine.fp_ntt_params(bits_a, bits_b)
+        nconv = -(-bits_a // bch) + -(-bits_b // bch) - 1
+        meta.append((N, a.size, rn - a.size, b is None, nconv))
+        dtens.append((torch.empty(rn, dtype=torch.int64, device=dev), D.to_dev(a),
+                      None if b is None else D.to_dev(b), bits_a, bits_b))
+    torch.cuda.synchronize()
+    log(f"{len(jobs)} jobs, {total_limbs} result limbs, staged in {time.time() - t0:.1f}s")
+
+    def run_one_per_launch():
+        for r, a, b, ba, bb in dtens:
+            if b is None:
+                D.sqr_dev(r, a, ba)
+            else:
+                D.mul_dev(r, a, b, ba, bb)
+
+    batch = None
+    if args.workload == "batch":
+        with torch.cuda.stream(stream):
+            batch = D.Batch(dtens)
+
+    def step():
+        if batch is not None:
+            batch.run(stream)
+        else:
+            run_one_per_launch()
+
+    # ---- warmup + timed region -------------------------------------------------
+    with torch.cuda.stream(stream):
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        D.profile_enable(True)
+        D.profile_read()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        tw0 = time.perf_counter()
+        ev0.record(stream)
+        for _ in range(args.steps):
+            step()
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        tw1 = time.perf_counter()
+        prof = D.profile_read()
+        D.profile_enable(False)
+    ms = ev0.elapsed_time(ev1) / args.steps
+    value = total_limbs / (ms * 1e-3)
+    log(f"timed: {ms:.3f} ms/step (wall {1e3 * (tw1 - tw0) / args.steps:.3f}), {value:.4g} limb-products/s")
+
+    # ---- verify every product of the timed run -----------------------------------
+    verified = None
+    if not args.no_verify:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from tests import gmp_ref
+        bad = 0
+        use_gmp = gmp_ref.available()
+        if not use_gmp:
+            from oracle import oracle as O
+        for (r, _, _, _, _), (a, b) in zip(dtens, ops):
+            got = D.to_host(r)
+            if use_gmp:
+                want = gmp_ref.sqr(a) if b is None else gmp_ref.mul(a, b)
+            else:
+                want = O.sqr(a) if b is None else O.mul(a, b)
+            bad += int(not np.array_equal(got, want))
+        verified = bad == 0
+        log(f"verified {len(jobs) - bad}/{len(jobs)} bit-exact ({'libgmp' if use_gmp else 'oracle'})")
+        if bad:
+            print(json.dumps({"error": f"{bad} products differ from the exact product"}))
+            sys.exit(1)
+
+    # ---- roofline of the dominant kernel ---------------------------------------------
+    bytes_by_kind = {k: 0 for k in D.PROF_KINDS}
+    flops = 0.0
+    groups_cache = {}
+    for (N, m, n, sq, nconv) in meta:
+        if N not in groups_cache:
+            groups_cache[N] = (D.plan_groups(N), [s[0] for s in D.plan_stages(N, 0)])
+        groups, radices = groups_cache[N]
+        for k, v in kernel_bytes(groups, N, m, n, sq, nconv).items():
+            bytes_by_kind[k] += v
+        flops += fp64_flops(N, radices, sq, nconv)
+    dom = max((k for k in prof if prof[k][1] > 0), key=lambda k: prof[k][0])
+    dom_ms, dom_launches = prof[dom]
+    launches_per_step = dom_launches / args.steps
+    bytes_per_launch = bytes_by_kind[dom] / launches_per_step
+    avg_launch_s = dom_ms * 1e-3 / dom_launches
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+    kernels = {k: {"ms_per_step": prof[k][0] / args.steps, "launches_per_step": prof[k][1] / args.steps,
+                   "algorithmic_GBps": (bytes_by_kind[k] / (prof[k][0] * 1e-3 / args.steps) / 1e9)
+                   if prof[k][0] > 0 else None}
+               for k in D.PROF_KINDS}
+    step_bytes = sum(bytes_by_kind.values())
+    roofline = {"bound": "hbm", "kernel": f"k_{'group' if dom.startswith('group') else dom}"
+                f"{' (' + dom + ')' if dom.startswith('group') else ''}",
+                "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_launch": int(bytes_per_launch), "avg_launch_us": round(avg_launch_s * 1e6, 2)}
+    valu = {"fp64_tflops_model": round(flops / (ms * 1e-3) / 1e12, 3), "peak_tflops": FP64_VALU_PEAK_TFLOPS,
+            "frac": round(flops / (ms * 1e-3) / 1e12 / FP64_VALU_PEAK_TFLOPS, 4),
+            "step_hbm_GBps_model": round(step_bytes / (ms * 1e-3) / 1e9, 1)}
+
+    # ---- one product per launch on the same list --------------------------------
+    opl = None
+    if batch is not None and not args.no_one_per_launch:
+        with torch.cuda.stream(stream):
+            run_one_per_launch()
+            torch.cuda.synchronize()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            reps = max(1, min(3, args.steps))
+            e0.record(stream)
+            for _ in range(reps):
+                run_one_per_launch()
+            e1.record(stream)
+            torch.cuda.synchronize()
+        opl_ms = e0.elapsed_time(e1) / reps
+        opl = {"value": total_limbs / (opl_ms * 1e-3), "ms_per_step": opl_ms,
+               "batched_speedup": opl_ms / ms}
+        log(f"one-per-launch: {opl_ms:.3f} ms/step, batched speedup {opl_ms / ms:.2f}x")
+
+    # ---- CPU baseline: the oracle on a bounded sample ---------------------------------
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        ncores = min(16, os.cpu_count() or 1)
+        if args.workload == "batch":
+            sample = list(range(0, len(jobs), max(1, len(jobs) // 512)))
+        else:
+            sample = [0, 1, 2]
+        lock = threading.Lock()
+        work = list(sample)
+        done_limbs = [0]
+
+        def worker():
+            while True:
+                with lock:
+                    if not work:
+                        return
+                    i = work.pop()
+                a, b = ops[i]
+                r = O.sqr(a) if b is None else O.mul(a, b)
+                with lock:
+                    done_limbs[0] += r.size
+        threads = [threading.Thread(target=worker) for _ in range(ncores)]
+        c0 = time.perf_counter()
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        cdt = time.perf_counter() - c0
+        cpu = {"value": done_limbs[0] / cdt, "unit": "limb-products/s", "cores": ncores, "kind": "port",
+               "sample": f"{len(sample)} of the {len(jobs)} {wname} products, C restatement of src/fpntt.jl "
+                         f"(oracle/, gcc -O3 -march=x86-64-v3), one product per thread, {cdt:.1f}s"}
+        log(f"cpu baseline {cpu['value']:.4g} limb-products/s on {ncores} threads")
+
+    out = {
+        "metric": "bit-exact limb-products/s",
+        "value": value,
+        "unit": "limb-products/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (SplitMix64 uniform limbs, nonzero top limb)",
+        "config": {"workload": "batch 4096 products, limbs log-uniform [152, 65536], 25% squares, ratio U[1,4]"
+                   if wname == "batch" else "large: 2^20x2^20, 2^20x2^18, 2^22 square, one per launch",
+                   "jobs": len(jobs), "result_limbs_per_step": total_limbs, "parallelism": "single GPU"},
+        "verified_bit_exact": verified,
+        "roofline": roofline,
+        "fp64_valu": valu,
+        "kernels": kernels,
+        "one_per_launch": opl,
+        "cpu_baseline": cpu,
+        "device": info,
+    }
+    print(json.dumps(out))
+    if batch is not None:
+        batch.close()
+
+
+if __name__ == "__main__":
+    main()

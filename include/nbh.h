@@ -1,0 +1,129 @@
+/*
+ * nbh.h -- C ABI of the MI355X FP64 two-prime NTT multiplication engine.
+ *
+ * Drop-in for NativeBigInt.jl's engine pair mul_fpntt2!/sqr_fpntt2!
+ * (src/fpntt.jl:982-1037), reached from mul!/sqr!/mul_bal! (src/mul.jl:134,
+ * 146, 161, 389) above MUL_FPNTT_THRESHOLD = 152 / SQR_FPNTT_THRESHOLD = 160
+ * limbs (src/mul.jl:24-26).  Limbs are little-endian uint64_t; r receives
+ * exactly m+n limbs (2n for a square) and must not alias a or b (a may equal
+ * b).  Inputs need not be top-normalised: bit lengths are taken from the top
+ * limb as magnitude_bits does (src/kernels/addsub.jl:150), an upper bound.
+ *
+ * Every function returns NBH_OK (0) or an error code; nbh_last_error_string()
+ * describes the last failure on the calling thread.  Calls are thread-safe
+ * (submissions are serialised inside the library).
+ */
+#ifndef NBH_H
+#define NBH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes.  NBH_ERR_LENGTH is the reference's ArgumentError for a
+ * transform length above 315*2^34 (src/fpntt.jl:948-949). */
+#define NBH_OK 0
+#define NBH_ERR_LENGTH 1
+#define NBH_ERR_ARG 2
+#define NBH_ERR_HIP 3
+#define NBH_ERR_OOM 4
+#define NBH_ERR_NOT_INIT 5
+#define NBH_ERR_BUDGET 6
+
+/* Library set-up on a HIP device; idempotent.  No reference counterpart (the
+ * reference's engine is a pure function with a lazily built plan cache,
+ * src/fpntt.jl:273-293); the plan/twiddle cache here is per device. */
+int32_t nbh_init(int32_t device);
+void nbh_shutdown(void);
+const char *nbh_last_error_string(void);
+
+/* ntt_len (src/fpntt.jl:936-951): smallest admissible m*2^k >= T. */
+int32_t nbh_ntt_len(int64_t T, int64_t *N);
+/* fp_ntt_params (src/fpntt.jl:964-977): chunk width b and length N. */
+int32_t nbh_ntt_params(int64_t bits_a, int64_t bits_b, int32_t *b, int64_t *N);
+
+/* mul_fpntt2! (src/fpntt.jl:982-1014) over HOST limb buffers: r[0..m+n) =
+ * a[0..m) * b[0..n).  Copies in, computes on the GPU, copies out, returns
+ * when r is written. */
+int32_t nbh_mul(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int64_t n);
+/* sqr_fpntt2! (src/fpntt.jl:1016-1037) over HOST buffers: r[0..2n) = a^2. */
+int32_t nbh_sqr(uint64_t *r, const uint64_t *a, int64_t n);
+
+/* Device-resident forms of the same: pointers are device (HBM) addresses, the
+ * work is enqueued on `stream` (a hipStream_t, NULL = the library's stream)
+ * and the call returns without waiting.  bits_a/bits_b are the operands'
+ * magnitude_bits; pass -1 to have the library read the top limbs (a small
+ * synchronous device read). */
+int32_t nbh_mul_dev(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int64_t n, int64_t bits_a,
+                    int64_t bits_b, void *stream);
+int32_t nbh_sqr_dev(uint64_t *r, const uint64_t *a, int64_t n, int64_t bits_a, void *stream);
+
+/* Batched products (device pointers).  One job is r = a*b, or r = a^2 when
+ * b == NULL.  There is no batched entry in the reference: each job is one
+ * mul_fpntt2!/sqr_fpntt2! call (src/fpntt.jl:982,1016), all of them run in a
+ * handful of launches that each cover every job.  bits_* <= 0 means "read
+ * the top limb". */
+typedef struct {
+    uint64_t *r;
+    const uint64_t *a;
+    const uint64_t *b;
+    int64_t m, n;
+    int64_t bits_a, bits_b;
+} nbh_job;
+
+typedef struct nbh_batch_s *nbh_batch;
+/* Plan a batch once (plans, workspace, device tile lists). */
+int32_t nbh_batch_create(const nbh_job *jobs, int64_t count, nbh_batch *out);
+/* Run it (enqueue on stream); can be called repeatedly. */
+int32_t nbh_batch_run(nbh_batch b, void *stream);
+void nbh_batch_destroy(nbh_batch b);
+/* One-shot: create, run, synchronise, destroy. */
+int32_t nbh_mul_batch_dev(const nbh_job *jobs, int64_t count, void *stream);
+
+/* Stage-level entry points over device buffers, one per reference function,
+ * used by the parity tests: fp_ntt_fwd! (src/fpntt.jl:715), fp_ntt_rev!
+ * (:727), fp_ntt_pack! (:744), fp_ntt_pointwise! (:770), fp_ntt_unpack2!
+ * (:869, with this N's 1/N constants). */
+int32_t nbh_ntt_fwd_dev(double *x, int64_t N, int32_t prime, void *stream);
+int32_t nbh_ntt_rev_dev(double *x, int64_t N, int32_t prime, void *stream);
+int32_t nbh_pack_dev(double *x, int64_t N, const uint64_t *limbs, int64_t n, int32_t b, int64_t nch, void *stream);
+int32_t nbh_pointwise_dev(double *xa, const double *xb, int64_t N, int32_t prime, void *stream);
+int32_t nbh_unpack_dev(uint64_t *r, int64_t rn, const double *x1, const double *x2, int64_t N, int64_t nconv,
+                       int32_t b, void *stream);
+
+/* Plan introspection (host copies): stage count, then per stage radix, span,
+ * Q and the 8 combine constants; a twiddle row (t = 1..m-1) of Q doubles. */
+int32_t nbh_plan_nstages(int64_t N, int32_t prime, int32_t *nst);
+int32_t nbh_plan_stage(int64_t N, int32_t prime, int32_t i, int32_t *m, int64_t *span, int64_t *Q, double *rot8);
+int32_t nbh_plan_twiddle_row(int64_t N, int32_t prime, int32_t i, int32_t t, double *out);
+int32_t nbh_plan_ninv(int64_t N, int32_t prime, double *ninv, double *ninvp);
+
+/* Schedule (how stages are grouped into HBM passes) and its tuning knobs:
+ * the LDS budget of a strided group tile (doubles), the largest contiguous
+ * last group, and the column range of strided tiles. */
+int32_t nbh_set_tuning(int32_t grp_lds_doubles, int32_t mid_max, int32_t cmin, int32_t cmax);
+int32_t nbh_get_tuning(int32_t *grp_lds_doubles, int32_t *mid_max, int32_t *cmin, int32_t *cmax);
+int32_t nbh_plan_groups(int64_t N, int32_t *ngroups, int64_t *la_lb_s_c /* 4 per group */);
+
+/* Per-kernel timing: when enabled, every launch of the product pipeline is
+ * bracketed by HIP events on the stream it is launched on.  nbh_profile_read
+ * synchronises, returns the summed milliseconds and launch counts per kernel
+ * kind (0 forward strided group + pack, 1 mid, 2 inverse strided group,
+ * 3 unpack, 4 carry fix) and clears them.  Instrumentation only: no reference
+ * counterpart. */
+#define NBH_PROF_KINDS 5
+int32_t nbh_profile_enable(int32_t on);
+int32_t nbh_profile_read(double *ms, int64_t *launches, int32_t nkinds);
+
+/* Synchronise the library stream (or `stream`). */
+int32_t nbh_synchronize(void *stream);
+/* Device properties the library sized itself by. */
+int32_t nbh_device_info(int32_t *cus, int64_t *lds_per_block, int64_t *hbm_bytes, char *name, int32_t name_len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NBH_H */

@@ -1,0 +1,1176 @@
+// nbh_api.hip -- host side of the engine and the C ABI declared in include/nbh.h.
+//
+// Plan building follows build_fp_plan (src/fpntt.jl:236-271): the same stage
+// list, roots, Winograd constants (src/fpntt.jl:127-168), 1/N and wi, computed
+// in host FP64 with the same operation sequence; the twiddle tables are built
+// on the device (k_twiddle_stage).  The schedule -- which consecutive stages
+// share one LDS tile -- is this engine's own and is set by the tuning knobs.
+// Sizing restates ntt_len/fp_ntt_params (src/fpntt.jl:933-977).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "nbh.h"
+#include "nbh_field.h"
+#include "nbh_internal.h"
+#include "nbh_launch.h"
+
+using namespace nbh;
+
+namespace {
+
+typedef unsigned __int128 u128;
+
+thread_local std::string t_err;
+
+int fail(int code, const std::string &msg) {
+    t_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(e_ == hipErrorOutOfMemory ? NBH_ERR_OOM : NBH_ERR_HIP,              \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                 \
+    } while (0)
+
+#define RCCHK(expr)                \
+    do {                           \
+        int rc_ = (expr);          \
+        if (rc_) return rc_;       \
+    } while (0)
+
+uint64_t powmod(uint64_t b, uint64_t e, uint64_t m) {
+    u128 r = 1, x = b % m;
+    while (e) {
+        if (e & 1) r = r * x % m;
+        x = x * x % m;
+        e >>= 1;
+    }
+    return (uint64_t)r;
+}
+
+uint64_t invmod(uint64_t a, uint64_t m) {
+    __int128 t = 0, nt = 1, r = m, nr = a % m;
+    while (nr) {
+        __int128 q = r / nr, tmp;
+        tmp = t - q * nt; t = nt; nt = tmp;
+        tmp = r - q * nr; r = nr; nr = tmp;
+    }
+    if (t < 0) t += m;
+    return (uint64_t)t;
+}
+
+int top_set_bit(uint64_t x) { return x ? 64 - __builtin_clzll(x) : 0; }
+int top_set_bit128(u128 x) {
+    uint64_t hi = (uint64_t)(x >> 64);
+    return hi ? 64 + top_set_bit(hi) : top_set_bit((uint64_t)x);
+}
+
+uint64_t prime_u(int k) { return k ? P2_U : P1_U; }
+double prime_d(int k) { return k ? P2_D : P1_D; }
+int two_adicity(int k) { return k ? 38 : 34; }
+
+// fp_generator (src/fpntt.jl:56-62): smallest g in 3..63 of full order.
+uint64_t generator(int k) {
+    static const uint64_t f1[] = {2, 3, 5, 7, 23}, f2[] = {2, 3, 5, 7, 13};
+    const uint64_t *f = k ? f2 : f1;
+    const uint64_t p = prime_u(k);
+    for (uint64_t g = 3; g <= 63; g++) {
+        bool ok = true;
+        for (int i = 0; i < 5; i++)
+            if (powmod(g, (p - 1) / f[i], p) == 1) { ok = false; break; }
+        if (ok) return g;
+    }
+    return 0;
+}
+
+// ntt_len (src/fpntt.jl:933-951); -1 where the reference throws ArgumentError.
+int64_t ntt_len(int64_t T) {
+    static const int64_t s[] = {1, 3, 5, 9};
+    static const int64_t md[] = {1, 3, 5, 7, 9, 15, 21, 105};
+    static const int64_t lg[] = {1, 3, 5, 7, 9, 15, 21, 35, 45, 63, 105, 315};
+    const int64_t *ms = T <= (1ll << 14) ? s : T <= (1ll << 20) ? md : lg;
+    const int nms = T <= (1ll << 14) ? 4 : T <= (1ll << 20) ? 8 : 12;
+    const int maxk = std::min(two_adicity(0), two_adicity(1));
+    int64_t best = INT64_MAX;
+    for (int i = 0; i < nms; i++) {
+        const int64_t m = ms[i];
+        const int64_t cl = (T + m - 1) / m;
+        int k = top_set_bit((uint64_t)(cl - 1));
+        if (k < 2) k = 2;
+        if (k > maxk) continue;
+        best = std::min(best, m << k);
+    }
+    return best == INT64_MAX ? -1 : best;
+}
+
+// fp_ntt_params (src/fpntt.jl:964-977).
+int ntt_params(int64_t bits_a, int64_t bits_b, int *b_out, int64_t *N_out) {
+    const u128 pp = (u128)P1_U * P2_U - 1;
+    const int maxb = top_set_bit(std::min(P1_U, P2_U)) - 1;
+    const int64_t mn = std::min(bits_a, bits_b);
+    const int nclow = top_set_bit((uint64_t)((mn + maxb - 1) / maxb));
+    const int bstart = std::min(maxb, (top_set_bit128(pp) - nclow + 3) >> 1);
+    for (int b = bstart; b >= 1; b--) {
+        const int64_t nca = (bits_a + b - 1) / b, ncb = (bits_b + b - 1) / b;
+        const u128 d = (u128)((1ull << b) - 1) * ((1ull << b) - 1);
+        if ((u128)std::min(nca, ncb) <= pp / d) {
+            const int64_t N = ntt_len(nca + ncb - 1);
+            if (N < 0) return fail(NBH_ERR_LENGTH, "fp NTT length exceeds the 315*2^34 the primes support");
+            *b_out = b;
+            *N_out = N;
+            return NBH_OK;
+        }
+    }
+    return fail(NBH_ERR_LENGTH, "no admissible chunk width");
+}
+
+// Winograd constants (src/fpntt.jl:127-168), host FP64, same op sequence.
+void dft5_consts(uint64_t c, int k, double *out) {
+    const uint64_t p = prime_u(k);
+    const double inv2 = (double)invmod(2, p);
+    const double c1 = fp_reduce_k((double)c, k);
+    const double c2 = fp_mulmod3_k(c1, c1, k), c3 = fp_mulmod3_k(c2, c1, k), c4 = fp_mulmod3_k(c3, c1, k);
+    const double inv4 = fp_mulmod3_k(inv2, inv2, k);
+    out[0] = fp_minbal_k(-inv4, k);
+    out[1] = fp_minbal_k(fp_mulmod3_k((c1 + c4) - (c2 + c3), inv4, k), k);
+    out[2] = fp_minbal_k(fp_mulmod3_k(c1 - c4, inv2, k), k);
+    out[3] = fp_minbal_k(fp_mulmod3_k(c2 - c3, inv2, k), k);
+}
+
+void dft7_consts(uint64_t c, int k, double *out) {
+    const uint64_t p = prime_u(k);
+    const double inv2 = (double)invmod(2, p), inv3 = (double)invmod(3, p);
+    double cj[7];
+    for (int j = 1; j <= 6; j++) cj[j] = (double)powmod(c, (uint64_t)j, p);
+    auto A = [&](int j) { return fp_mulmod3_k(cj[j] + cj[7 - j], inv2, k); };
+    auto B = [&](int j) { return fp_mulmod3_k(cj[j] - cj[7 - j], inv2, k); };
+    const double a0 = A(1), a1 = A(3), a2 = A(2);
+    const double b0 = B(1), b1 = B(3), b2 = B(2);
+    const double bp0 = fp_minbal_k(fp_mulmod3_k(a0 - a2, inv3, k), k);
+    const double bp1 = fp_minbal_k(fp_mulmod3_k(a1 - a2, inv3, k), k);
+    const double n0 = fp_minbal_k(fp_mulmod3_k(b0 - b2, inv3, k), k);
+    const double n1 = fp_minbal_k(fp_mulmod3_k(b1 + b2, inv3, k), k);
+    out[0] = fp_minbal_k(fp_mulmod3_k((a0 + a1) + a2, inv3, k), k);
+    out[1] = bp0;
+    out[2] = bp1;
+    out[3] = fp_minbal_k(bp0 + bp1, k);
+    out[4] = fp_minbal_k(fp_mulmod3_k((b0 + b2) - b1, inv3, k), k);
+    out[5] = n0;
+    out[6] = n1;
+    out[7] = fp_minbal_k(n0 + n1, k);
+}
+
+struct Tuning {
+    int grp_lds = 8192;   // doubles per strided group tile
+    int mid_max = 4096;   // largest contiguous last group (the mid kernel holds two)
+    int cmin = 8;
+    int cmax = 64;
+};
+
+// Stage list and device twiddle tables of one (N, prime); schedule-free.
+struct StageList {
+    int64_t N = 0, N2 = 0;
+    int prime = 0;
+    std::vector<StageDesc> st;
+    double ninv = 0, ninvp = 0;
+    double *tw = nullptr;
+    int64_t ntw = 0;
+};
+
+struct PlanEntry {
+    DevPlan host;
+    DevPlan *dev = nullptr;
+    size_t grp_lds_bytes[NBH_MAX_GROUPS] = {};
+    size_t mid_lds_bytes = 0;
+};
+
+struct Lib {
+    std::recursive_mutex mu;
+    bool init = false;
+    int device = -1;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    int cus = 0;
+    size_t lds_block = 65536;
+    size_t hbm = 0;
+    std::string name;
+    Tuning tun;
+    int tun_version = 0;
+    std::map<std::pair<int64_t, int>, std::unique_ptr<StageList>> stages;
+    std::map<std::tuple<int64_t, int, int>, std::unique_ptr<PlanEntry>> plans;
+    void *ws = nullptr;  // single-product workspace
+    size_t ws_size = 0;
+    void *io = nullptr;  // host-pointer API staging of a, b, r
+    size_t io_size = 0;
+    DevJob *slots = nullptr;
+    int slot_next = 0;
+    // per-kernel event timing
+    bool prof = false;
+    std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> prof_open;
+    std::vector<hipEvent_t> prof_pool;
+};
+Lib g;
+
+hipEvent_t prof_event() {
+    hipEvent_t e = nullptr;
+    if (!g.prof_pool.empty()) {
+        e = g.prof_pool.back();
+        g.prof_pool.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+        e = nullptr;
+    }
+    return e;
+}
+
+// Bracket one launch with events on its stream when profiling is on.
+struct ProfScope {
+    int kind;
+    hipStream_t s;
+    hipEvent_t a = nullptr;
+    ProfScope(int k, hipStream_t st) : kind(k), s(st) {
+        if (g.prof && (a = prof_event())) (void)hipEventRecord(a, s);
+    }
+    ~ProfScope() {
+        if (!a) return;
+        hipEvent_t b = prof_event();
+        if (!b) {
+            g.prof_pool.push_back(a);
+            return;
+        }
+        (void)hipEventRecord(b, s);
+        g.prof_open.emplace_back(kind, a, b);
+    }
+};
+constexpr int NSLOTS = 256;
+
+int ensure_init() {
+    if (g.init) return NBH_OK;
+    return fail(NBH_ERR_NOT_INIT, "nbh_init has not been called");
+}
+
+// Device twiddle tables for every stage: w_N^e via exact lo/hi power tables.
+int build_twiddles(StageList &L, uint64_t w) {
+    const int k = L.prime;
+    const uint64_t p = prime_u(k);
+    int64_t total = 0;
+    for (auto &s : L.st) {
+        if (s.m == 2) continue;
+        s.tw_off = total;
+        total += (int64_t)(s.m - 1) * s.Q;
+    }
+    L.ntw = total;
+    if (total == 0) return NBH_OK;
+    const int logN = top_set_bit((uint64_t)(L.N - 1));
+    const int h = (logN + 1) / 2;
+    const int64_t nlo = 1ll << h;
+    const int64_t nhi = (L.N + nlo - 1) / nlo;
+    std::vector<double> tab((size_t)(nlo + nhi));
+    const double half = prime_d(k) / 2;
+    auto minbal = [&](uint64_t v) { return (double)v > half ? (double)v - prime_d(k) : (double)v; };
+    uint64_t x = 1;
+    for (int64_t i = 0; i < nlo; i++) {
+        tab[(size_t)i] = minbal(x);
+        x = (uint64_t)((u128)x * w % p);
+    }
+    const uint64_t step = x;  // w^(2^h)
+    x = 1;
+    for (int64_t i = 0; i < nhi; i++) {
+        tab[(size_t)(nlo + i)] = minbal(x);
+        x = (uint64_t)((u128)x * step % p);
+    }
+    double *dtab = nullptr;
+    HIPCHK(hipMalloc((void **)&L.tw, sizeof(double) * (size_t)total));
+    HIPCHK(hipMalloc((void **)&dtab, sizeof(double) * tab.size()));
+    HIPCHK(hipMemcpyAsync(dtab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice, g.stream));
+    for (auto &s : L.st) {
+        if (s.m == 2) continue;
+        HIPCHK(launch_twiddle_stage(k, L.tw + s.tw_off, s.m, s.Q, L.N / s.L, dtab, dtab + nlo, h, g.stream));
+    }
+    HIPCHK(hipStreamSynchronize(g.stream));
+    HIPCHK(hipFree(dtab));
+    return NBH_OK;
+}
+
+// build_fp_plan (src/fpntt.jl:236-271), plus the trailing radix-2 pass that
+// fp_ntt_fwd!/rev! apply when k is odd (src/fpntt.jl:715-733) as a stage.
+int get_stages(int64_t N, int k, StageList **out) {
+    auto key = std::make_pair(N, k);
+    auto it = g.stages.find(key);
+    if (it != g.stages.end()) {
+        *out = it->second.get();
+        return NBH_OK;
+    }
+    const uint64_t p = prime_u(k);
+    if (N < 4) return fail(NBH_ERR_LENGTH, "transform length below 4");
+    const int kk = __builtin_ctzll((uint64_t)N);
+    int64_t m = N >> kk;
+    const bool m_ok = m == 1 || m == 3 || m == 5 || m == 7 || m == 9 || m == 15 || m == 21 || m == 35 || m == 45 ||
+                      m == 63 || m == 105 || m == 315;
+    if (!m_ok || !(m == 1 || kk >= 2) || kk > two_adicity(k) || (p - 1) % (uint64_t)N != 0)
+        return fail(NBH_ERR_LENGTH, "unsupported transform length " + std::to_string(N));
+    auto L = std::make_unique<StageList>();
+    L->N = N;
+    L->prime = k;
+    const uint64_t w = powmod(generator(k), (p - 1) / (uint64_t)N, p);
+    int64_t span = N;
+    uint64_t r = w;
+    const int mfs[4] = {7, 5, 3, 3};
+    for (int mf : mfs) {
+        if (m % mf) continue;
+        StageDesc s{};
+        s.m = mf;
+        s.L = span;
+        s.Q = span / mf;
+        const uint64_t c = powmod(r, (uint64_t)s.Q, p);
+        if (mf == 5) dft5_consts(c, k, s.rot);
+        else if (mf == 7) dft7_consts(c, k, s.rot);
+        else
+            for (int t = 1; t < mf; t++) s.rot[t - 1] = fp_reduce_k((double)powmod(c, (uint64_t)t, p), k);
+        L->st.push_back(s);
+        r = powmod(r, (uint64_t)mf, p);
+        span /= mf;
+        m /= mf;
+    }
+    L->N2 = span;
+    const double wi = span >= 4 ? fp_reduce_k((double)powmod(r, (uint64_t)(span >> 2), p), k) : 1.0;
+    for (int64_t Lspan = span; Lspan >= 4; Lspan >>= 2) {
+        StageDesc s{};
+        s.m = 4;
+        s.L = Lspan;
+        s.Q = Lspan / 4;
+        s.rot[0] = wi;
+        L->st.push_back(s);
+    }
+    if (__builtin_ctzll((uint64_t)span) & 1) {
+        StageDesc s{};
+        s.m = 2;
+        s.L = 2;
+        s.Q = 1;
+        L->st.push_back(s);
+    }
+    if ((int)L->st.size() > NBH_MAX_STAGES) return fail(NBH_ERR_LENGTH, "too many stages");
+    L->ninv = fp_reduce_k((double)invmod((uint64_t)N % p, p), k);
+    L->ninvp = L->ninv / prime_d(k);
+    RCCHK(build_twiddles(*L, w));
+    *out = L.get();
+    g.stages[key] = std::move(L);
+    return NBH_OK;
+}
+
+int64_t pow2_part(int64_t x) { return x & -x; }
+int64_t floor_pow2(int64_t x) { return x < 1 ? 0 : (int64_t)1 << (63 - __builtin_clzll((uint64_t)x)); }
+
+// Cut the stage list into HBM passes: the last group is every stage of span
+// <= mid_max (contiguous blocks); the earlier stages are packed greedily into
+// strided groups whose S x C tile fits grp_lds doubles with C >= cmin columns.
+int make_schedule(const StageList &L, const Tuning &T, DevPlan &P, PlanEntry &E) {
+    const int nst = (int)L.st.size();
+    int sl = 0;
+    while (sl < nst && L.st[sl].L > T.mid_max) sl++;
+    if (sl == nst) return fail(NBH_ERR_ARG, "schedule: no stage fits the mid budget");
+    int ng = 0;
+    auto colw = [&](int64_t Lb, int64_t S) {
+        return std::max<int64_t>(1, std::min({pow2_part(Lb), floor_pow2(T.grp_lds / S), (int64_t)T.cmax}));
+    };
+    int s = 0;
+    while (s < sl) {
+        int64_t S = L.st[s].m;
+        int e = s + 1;
+        while (e < sl) {
+            const int64_t S2 = S * L.st[e].m;
+            if (S2 > T.grp_lds || colw(L.st[e].Q, S2) < T.cmin) break;
+            S = S2;
+            e++;
+        }
+        if (ng >= NBH_MAX_GROUPS - 1) return fail(NBH_ERR_ARG, "schedule: too many groups");
+        GroupDesc &G = P.g[ng++];
+        G.s0 = s;
+        G.s1 = e;
+        G.La = L.st[s].L;
+        G.Lb = L.st[e - 1].Q;
+        G.S = (int32_t)S;
+        G.C = (int32_t)colw(G.Lb, S);
+        G.logC = __builtin_ctz((unsigned)G.C);
+        G.tiles = (L.N / G.La) * (G.Lb / G.C);
+        s = e;
+    }
+    GroupDesc &G = P.g[ng++];
+    G.s0 = sl;
+    G.s1 = nst;
+    G.La = L.st[sl].L;
+    G.Lb = 1;
+    G.S = (int32_t)G.La;
+    G.C = 1;
+    G.logC = 0;
+    G.tiles = L.N / G.La;
+    P.ngroups = ng;
+    for (int gi = 0; gi < ng; gi++) {
+        GroupDesc &Gd = P.g[gi];
+        for (int si = Gd.s0; si < Gd.s1; si++) {
+            StageDesc &sd = P.st[si];
+            sd.q = (int32_t)(sd.Q / Gd.Lb);
+            sd.q_magic = ((1ull << 32) + (uint64_t)sd.q - 1) / (uint64_t)sd.q;
+        }
+        E.grp_lds_bytes[gi] = sizeof(double) * (size_t)Gd.S * (size_t)Gd.C;
+    }
+    E.mid_lds_bytes = 2 * sizeof(double) * (size_t)P.g[ng - 1].S;
+    return NBH_OK;
+}
+
+int get_plan(int64_t N, int k, PlanEntry **out) {
+    auto key = std::make_tuple(N, k, g.tun_version);
+    auto it = g.plans.find(key);
+    if (it != g.plans.end()) {
+        *out = it->second.get();
+        return NBH_OK;
+    }
+    StageList *L = nullptr;
+    RCCHK(get_stages(N, k, &L));
+    auto E = std::make_unique<PlanEntry>();
+    DevPlan &P = E->host;
+    std::memset(&P, 0, sizeof P);
+    P.N = N;
+    P.N2 = L->N2;
+    P.prime = k;
+    P.nst = (int32_t)L->st.size();
+    P.ninv = L->ninv;
+    P.ninvp = L->ninvp;
+    for (int i = 0; i < P.nst; i++) P.st[i] = L->st[i];
+    P.tw = L->tw;
+    RCCHK(make_schedule(*L, g.tun, P, *E));
+    HIPCHK(hipMalloc((void **)&E->dev, sizeof(DevPlan)));
+    HIPCHK(hipMemcpy(E->dev, &P, sizeof(DevPlan), hipMemcpyHostToDevice));
+    *out = E.get();
+    g.plans[key] = std::move(E);
+    return NBH_OK;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int ensure_buf(void **buf, size_t *cur, size_t need) {
+    if (*cur >= need) return NBH_OK;
+    HIPCHK(hipStreamSynchronize(g.stream));
+    if (*buf) HIPCHK(hipFree(*buf));
+    *buf = nullptr;
+    *cur = 0;
+    const size_t sz = std::max(need, (size_t)1 << 20);
+    HIPCHK(hipMalloc(buf, sz));
+    *cur = sz;
+    return NBH_OK;
+}
+
+int64_t bits_from_top(int64_t n, uint64_t top) { return 64 * (n - 1) + top_set_bit(top); }
+
+// Fill a DevJob's sizing fields; returns NBH_OK with J.zero set for a zero operand.
+int size_job(DevJob &J, int64_t bits_a, int64_t bits_b, int64_t *N_out) {
+    J.rn = J.m + J.n;
+    J.nchunks = (J.rn + UNPACK_LPC - 1) / UNPACK_LPC;
+    if (bits_a == 0 || bits_b == 0) {
+        J.zero = 1;
+        *N_out = 0;
+        return NBH_OK;
+    }
+    int bch = 0;
+    int64_t N = 0;
+    RCCHK(ntt_params(bits_a, bits_b, &bch, &N));
+    J.bch = bch;
+    J.nca = (bits_a + bch - 1) / bch;
+    J.ncb = (bits_b + bch - 1) / bch;
+    J.nconv = J.nca + J.ncb - 1;
+    *N_out = N;
+    return NBH_OK;
+}
+
+int read_bits(const uint64_t *a, int64_t n, int64_t *bits) {
+    uint64_t top = 0;
+    HIPCHK(hipMemcpyAsync(&top, a + (n - 1), sizeof top, hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    *bits = bits_from_top(n, top);
+    return NBH_OK;
+}
+
+// One product on g.stream: pack+forward groups, mid, inverse groups, unpack.
+int run_single(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int64_t n, int64_t bits_a,
+               int64_t bits_b, bool sqr) {
+    if (!r || !a || m < 1 || (!sqr && (!b || n < 1))) return fail(NBH_ERR_ARG, "null buffer or empty operand");
+    if (sqr) { b = a; n = m; }
+    if (bits_a < 0) RCCHK(read_bits(a, m, &bits_a));
+    if (sqr) bits_b = bits_a;
+    else if (bits_b < 0) RCCHK(read_bits(b, n, &bits_b));
+    DevJob J;
+    std::memset(&J, 0, sizeof J);
+    J.a = a;
+    J.b = b;
+    J.r = r;
+    J.m = m;
+    J.n = n;
+    J.sqr = sqr ? 1 : 0;
+    int64_t N = 0;
+    RCCHK(size_job(J, bits_a, bits_b, &N));
+    if (J.zero) {
+        HIPCHK(launch_fill_u64(r, J.rn, 0, g.stream));
+        return NBH_OK;
+    }
+    PlanEntry *P[2];
+    RCCHK(get_plan(N, 0, &P[0]));
+    RCCHK(get_plan(N, 1, &P[1]));
+    const int nops = sqr ? 1 : 2;
+    const size_t xbytes = align256(sizeof(double) * (size_t)N);
+    const size_t need = align256(sizeof(uint32_t) * (size_t)J.nchunks) + xbytes * 2 * nops;
+    RCCHK(ensure_buf(&g.ws, &g.ws_size, need));
+    char *w = (char *)g.ws;
+    J.chunk_flags = (uint32_t *)w;
+    w += align256(sizeof(uint32_t) * (size_t)J.nchunks);
+    for (int k = 0; k < 2; k++) {
+        J.plan[k] = P[k]->dev;
+        J.xa[k] = (double *)w;
+        w += xbytes;
+        if (!sqr) {
+            J.xb[k] = (double *)w;
+            w += xbytes;
+        }
+    }
+    DevJob *slot = g.slots + (g.slot_next++ % NSLOTS);
+    HIPCHK(launch_store_job(J, slot, g.stream));
+    const DevPlan &H = P[0]->host;
+    const int K = H.ngroups;
+    for (int gi = 0; gi + 1 < K; gi++) {
+        TileGen tg{gi, nops, H.g[gi].tiles, gi == 0 ? TILE_PACK : 0, 0};
+        ProfScope ps(0, g.stream);
+        HIPCHK(launch_group(slot, nullptr, tg, 2 * nops * H.g[gi].tiles, 1, P[0]->grp_lds_bytes[gi], g.stream));
+    }
+    {
+        TileGen tg{K - 1, 1, H.g[K - 1].tiles, K == 1 ? TILE_PACK : 0, 0};
+        ProfScope ps(1, g.stream);
+        HIPCHK(launch_mid(slot, nullptr, tg, 2 * H.g[K - 1].tiles, P[0]->mid_lds_bytes, g.stream));
+    }
+    for (int gi = K - 2; gi >= 0; gi--) {
+        TileGen tg{gi, 1, H.g[gi].tiles, 0, 0};
+        ProfScope ps(2, g.stream);
+        HIPCHK(launch_group(slot, nullptr, tg, 2 * H.g[gi].tiles, 0, P[0]->grp_lds_bytes[gi], g.stream));
+    }
+    TileGen ug{0, 1, J.nchunks, 0, 0};
+    {
+        ProfScope ps(3, g.stream);
+        HIPCHK(launch_unpack(slot, nullptr, ug, J.nchunks, g.stream));
+    }
+    {
+        ProfScope ps(4, g.stream);
+        HIPCHK(launch_carry_fix(slot, nullptr, ug, J.nchunks, g.stream));
+    }
+    return NBH_OK;
+}
+
+// Make the library stream wait for `user`'s prior work; after the call, make
+// `user` wait for ours.  The single-product workspace is shared, so every
+// single-product call is serialised on g.stream.
+int stream_enter(void *user) {
+    if (!user) return NBH_OK;
+    HIPCHK(hipEventRecord(g.ev_in, (hipStream_t)user));
+    HIPCHK(hipStreamWaitEvent(g.stream, g.ev_in, 0));
+    return NBH_OK;
+}
+int stream_leave(void *user) {
+    if (!user) return NBH_OK;
+    HIPCHK(hipEventRecord(g.ev_out, g.stream));
+    HIPCHK(hipStreamWaitEvent((hipStream_t)user, g.ev_out, 0));
+    return NBH_OK;
+}
+
+int host_product(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int64_t n, bool sqr) {
+    if (!r || !a || m < 1 || (!sqr && (!b || n < 1))) return fail(NBH_ERR_ARG, "null buffer or empty operand");
+    if (sqr) n = m;
+    const int64_t bits_a = bits_from_top(m, a[m - 1]);
+    const int64_t bits_b = sqr ? bits_a : bits_from_top(n, b[n - 1]);
+    const size_t sa = align256(8 * (size_t)m), sb = sqr ? 0 : align256(8 * (size_t)n);
+    const size_t sr = align256(8 * (size_t)(m + n));
+    RCCHK(ensure_buf(&g.io, &g.io_size, sa + sb + sr));
+    uint64_t *da = (uint64_t *)g.io;
+    uint64_t *db = (uint64_t *)((char *)g.io + sa);
+    uint64_t *dr = (uint64_t *)((char *)g.io + sa + sb);
+    HIPCHK(hipMemcpyAsync(da, a, 8 * (size_t)m, hipMemcpyHostToDevice, g.stream));
+    if (!sqr) HIPCHK(hipMemcpyAsync(db, b, 8 * (size_t)n, hipMemcpyHostToDevice, g.stream));
+    RCCHK(run_single(dr, da, m, sqr ? nullptr : db, n, bits_a, bits_b, sqr));
+    HIPCHK(hipMemcpyAsync(r, dr, 8 * (size_t)(m + n), hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    return NBH_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Batches
+
+struct nbh_batch_s {
+    struct Phase {
+        int kind;        // 0 group, 1 mid, 2 unpack, 3 carry fix
+        int fwd;
+        int64_t off, count;
+        size_t lds;
+    };
+    std::vector<DevJob> jobs;
+    DevJob *djobs = nullptr;
+    Tile *dtiles = nullptr;
+    void *ws = nullptr;
+    std::vector<Phase> phases;
+};
+
+namespace {
+
+void batch_free(nbh_batch_s *B) {
+    if (!B) return;
+    if (B->djobs) (void)hipFree(B->djobs);
+    if (B->dtiles) (void)hipFree(B->dtiles);
+    if (B->ws) (void)hipFree(B->ws);
+    delete B;
+}
+
+int batch_create(const nbh_job *jobs, int64_t count, nbh_batch_s **out) {
+    if (count < 1 || !jobs) return fail(NBH_ERR_ARG, "empty batch");
+    std::unique_ptr<nbh_batch_s, void (*)(nbh_batch_s *)> B(new nbh_batch_s, batch_free);
+    std::vector<DevJob> &J = B->jobs;
+    J.resize((size_t)count);
+    std::vector<int64_t> bits((size_t)(2 * count), 0);
+    bool need_top = false;
+    for (int64_t i = 0; i < count; i++) {
+        const nbh_job &q = jobs[i];
+        DevJob &d = J[(size_t)i];
+        std::memset(&d, 0, sizeof d);
+        const bool sqr = q.b == nullptr;
+        if (!q.r || !q.a || q.m < 1 || (!sqr && q.n < 1)) return fail(NBH_ERR_ARG, "bad job " + std::to_string(i));
+        d.a = q.a;
+        d.b = sqr ? q.a : q.b;
+        d.r = q.r;
+        d.m = q.m;
+        d.n = sqr ? q.m : q.n;
+        d.sqr = sqr;
+        bits[(size_t)(2 * i)] = q.bits_a;
+        bits[(size_t)(2 * i + 1)] = sqr ? q.bits_a : q.bits_b;
+        if (q.bits_a <= 0 || (!sqr && q.bits_b <= 0)) need_top = true;
+    }
+    if (need_top) {
+        DevJob *tmp = nullptr;
+        uint64_t *tops = nullptr;
+        HIPCHK(hipMalloc((void **)&tmp, sizeof(DevJob) * (size_t)count));
+        HIPCHK(hipMalloc((void **)&tops, 16 * (size_t)count));
+        HIPCHK(hipMemcpyAsync(tmp, J.data(), sizeof(DevJob) * (size_t)count, hipMemcpyHostToDevice, g.stream));
+        HIPCHK(launch_top_limbs(tmp, count, tops, g.stream));
+        std::vector<uint64_t> h((size_t)(2 * count));
+        HIPCHK(hipMemcpyAsync(h.data(), tops, 16 * (size_t)count, hipMemcpyDeviceToHost, g.stream));
+        HIPCHK(hipStreamSynchronize(g.stream));
+        (void)hipFree(tmp);
+        (void)hipFree(tops);
+        for (int64_t i = 0; i < count; i++) {
+            if (bits[(size_t)(2 * i)] <= 0) bits[(size_t)(2 * i)] = bits_from_top(J[(size_t)i].m, h[(size_t)(2 * i)]);
+            if (bits[(size_t)(2 * i + 1)] <= 0)
+                bits[(size_t)(2 * i + 1)] = bits_from_top(J[(size_t)i].n, h[(size_t)(2 * i + 1)]);
+        }
+    }
+    // size every job, get plans, lay out the workspace
+    std::vector<PlanEntry *> plan0((size_t)count, nullptr), plan1((size_t)count, nullptr);
+    std::vector<int64_t> Ns((size_t)count, 0);
+    size_t ws = 0;
+    std::vector<size_t> offs((size_t)count);
+    for (int64_t i = 0; i < count; i++) {
+        DevJob &d = J[(size_t)i];
+        RCCHK(size_job(d, bits[(size_t)(2 * i)], bits[(size_t)(2 * i + 1)], &Ns[(size_t)i]));
+        offs[(size_t)i] = ws;
+        ws += align256(sizeof(uint32_t) * (size_t)d.nchunks);
+        if (d.zero) continue;
+        RCCHK(get_plan(Ns[(size_t)i], 0, &plan0[(size_t)i]));
+        RCCHK(get_plan(Ns[(size_t)i], 1, &plan1[(size_t)i]));
+        ws += align256(sizeof(double) * (size_t)Ns[(size_t)i]) * (d.sqr ? 2 : 4);
+    }
+    HIPCHK(hipMalloc(&B->ws, std::max<size_t>(ws, 256)));
+    for (int64_t i = 0; i < count; i++) {
+        DevJob &d = J[(size_t)i];
+        char *w = (char *)B->ws + offs[(size_t)i];
+        d.chunk_flags = (uint32_t *)w;
+        w += align256(sizeof(uint32_t) * (size_t)d.nchunks);
+        if (d.zero) continue;
+        const size_t xb = align256(sizeof(double) * (size_t)Ns[(size_t)i]);
+        d.plan[0] = plan0[(size_t)i]->dev;
+        d.plan[1] = plan1[(size_t)i]->dev;
+        for (int k = 0; k < 2; k++) {
+            d.xa[k] = (double *)w;
+            w += xb;
+            if (!d.sqr) {
+                d.xb[k] = (double *)w;
+                w += xb;
+            }
+        }
+    }
+    // phase tile lists; within a phase, bigger tiles first
+    std::vector<Tile> tiles;
+    int maxK = 0;
+    for (int64_t i = 0; i < count; i++)
+        if (!J[(size_t)i].zero) maxK = std::max(maxK, plan0[(size_t)i]->host.ngroups);
+    auto push_phase = [&](int kind, int fwd, std::vector<std::pair<size_t, Tile>> &list) {
+        if (list.empty()) return;
+        std::stable_sort(list.begin(), list.end(),
+                         [](const std::pair<size_t, Tile> &x, const std::pair<size_t, Tile> &y) {
+                             return x.first > y.first;
+                         });
+        nbh_batch_s::Phase ph{kind, fwd, (int64_t)tiles.size(), (int64_t)list.size(), 0};
+        for (auto &e : list) {
+            ph.lds = std::max(ph.lds, e.first);
+            tiles.push_back(e.second);
+        }
+        if (kind >= 2) ph.lds = 0;
+        B->phases.push_back(ph);
+        list.clear();
+    };
+    std::vector<std::pair<size_t, Tile>> list;
+    auto add_group_tiles = [&](int64_t i, int gi, int nops, uint8_t flags, size_t lds) {
+        const DevPlan &H = plan0[(size_t)i]->host;
+        for (int k = 0; k < 2; k++)
+            for (int op = 0; op < nops; op++)
+                for (int64_t t = 0; t < H.g[gi].tiles; t++)
+                    list.push_back({lds, Tile{(uint32_t)i, (uint8_t)k, (uint8_t)op, (uint8_t)gi, flags,
+                                              (uint32_t)t, 0}});
+    };
+    for (int d = maxK - 1; d >= 1; d--) {  // forward strided groups, farthest from the mid first
+        for (int64_t i = 0; i < count; i++) {
+            if (J[(size_t)i].zero) continue;
+            const int K = plan0[(size_t)i]->host.ngroups;
+            const int gi = K - 1 - d;
+            if (gi < 0) continue;
+            add_group_tiles(i, gi, J[(size_t)i].sqr ? 1 : 2, gi == 0 ? TILE_PACK : 0,
+                            plan0[(size_t)i]->grp_lds_bytes[gi]);
+        }
+        push_phase(0, 1, list);
+    }
+    for (int64_t i = 0; i < count; i++) {
+        if (J[(size_t)i].zero) continue;
+        const int K = plan0[(size_t)i]->host.ngroups;
+        add_group_tiles(i, K - 1, 1, K == 1 ? TILE_PACK : 0, plan0[(size_t)i]->mid_lds_bytes);
+    }
+    push_phase(1, 1, list);
+    for (int d = 1; d <= maxK - 1; d++) {
+        for (int64_t i = 0; i < count; i++) {
+            if (J[(size_t)i].zero) continue;
+            const int K = plan0[(size_t)i]->host.ngroups;
+            const int gi = K - 1 - d;
+            if (gi < 0) continue;
+            add_group_tiles(i, gi, 1, 0, plan0[(size_t)i]->grp_lds_bytes[gi]);
+        }
+        push_phase(0, 0, list);
+    }
+    for (int64_t i = 0; i < count; i++)
+        for (int64_t c = 0; c < J[(size_t)i].nchunks; c++)
+            list.push_back({(size_t)J[(size_t)i].nconv, Tile{(uint32_t)i, 0, 0, 0, 0, (uint32_t)c, 0}});
+    std::vector<std::pair<size_t, Tile>> list2 = list;
+    push_phase(2, 0, list);
+    push_phase(3, 0, list2);
+    HIPCHK(hipMalloc((void **)&B->djobs, sizeof(DevJob) * (size_t)count));
+    HIPCHK(hipMalloc((void **)&B->dtiles, sizeof(Tile) * std::max<size_t>(tiles.size(), 1)));
+    HIPCHK(hipMemcpy(B->djobs, J.data(), sizeof(DevJob) * (size_t)count, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(B->dtiles, tiles.data(), sizeof(Tile) * tiles.size(), hipMemcpyHostToDevice));
+    *out = B.release();
+    return NBH_OK;
+}
+
+int batch_run(nbh_batch_s *B, hipStream_t s) {
+    const TileGen none{0, 1, 1, 0, 0};
+    for (const auto &ph : B->phases) {
+        const Tile *t = B->dtiles + ph.off;
+        ProfScope ps(ph.kind == 0 ? (ph.fwd ? 0 : 2) : ph.kind == 1 ? 1 : ph.kind == 2 ? 3 : 4, s);
+        switch (ph.kind) {
+        case 0: HIPCHK(launch_group(B->djobs, t, none, ph.count, ph.fwd, ph.lds, s)); break;
+        case 1: HIPCHK(launch_mid(B->djobs, t, none, ph.count, ph.lds, s)); break;
+        case 2: HIPCHK(launch_unpack(B->djobs, t, none, ph.count, s)); break;
+        default: HIPCHK(launch_carry_fix(B->djobs, t, none, ph.count, s)); break;
+        }
+    }
+    return NBH_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+
+extern "C" {
+
+int32_t nbh_init(int32_t device) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    if (g.init) {
+        if (device != g.device) return fail(NBH_ERR_ARG, "already initialised on another device");
+        return NBH_OK;
+    }
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(NBH_ERR_ARG, "no such HIP device");
+    HIPCHK(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    g.device = device;
+    g.cus = prop.multiProcessorCount;
+    g.lds_block = prop.sharedMemPerBlock;
+    g.hbm = prop.totalGlobalMem;
+    g.name = prop.gcnArchName;
+    if (1.0 / P1_D != P1_INV || 1.0 / P2_D != P2_INV) return fail(NBH_ERR_HIP, "fl(1/p) constant mismatch");
+    // A blocking stream: it orders with the legacy default stream (handle 0),
+    // which is where callers such as torch enqueue by default.
+    HIPCHK(hipStreamCreate(&g.stream));
+    HIPCHK(hipEventCreateWithFlags(&g.ev_in, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&g.ev_out, hipEventDisableTiming));
+    HIPCHK(hipMalloc((void **)&g.slots, sizeof(DevJob) * NSLOTS));
+    HIPCHK(set_lds_limit(g.lds_block));
+    // keep the default tiles inside the device's per-block LDS
+    while ((size_t)g.tun.mid_max * 16 > g.lds_block && g.tun.mid_max > 256) g.tun.mid_max >>= 1;
+    while ((size_t)g.tun.grp_lds * 8 > g.lds_block && g.tun.grp_lds > 256) g.tun.grp_lds >>= 1;
+    g.init = true;
+    return NBH_OK;
+}
+
+void nbh_shutdown(void) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    if (!g.init) return;
+    (void)hipStreamSynchronize(g.stream);
+    for (auto &e : g.plans) (void)hipFree(e.second->dev);
+    for (auto &e : g.stages) (void)hipFree(e.second->tw);
+    g.plans.clear();
+    g.stages.clear();
+    if (g.ws) (void)hipFree(g.ws);
+    if (g.io) (void)hipFree(g.io);
+    if (g.slots) (void)hipFree(g.slots);
+    g.ws = g.io = nullptr;
+    g.ws_size = g.io_size = 0;
+    g.slots = nullptr;
+    (void)hipEventDestroy(g.ev_in);
+    (void)hipEventDestroy(g.ev_out);
+    (void)hipStreamDestroy(g.stream);
+    g.init = false;
+    g.device = -1;
+}
+
+const char *nbh_last_error_string(void) { return t_err.c_str(); }
+
+int32_t nbh_ntt_len(int64_t T, int64_t *N) {
+    if (!N || T < 1) return fail(NBH_ERR_ARG, "bad argument");
+    const int64_t n = ntt_len(T);
+    if (n < 0) return fail(NBH_ERR_LENGTH, "fp NTT length exceeds the 315*2^34 the primes support");
+    *N = n;
+    return NBH_OK;
+}
+
+int32_t nbh_ntt_params(int64_t bits_a, int64_t bits_b, int32_t *b, int64_t *N) {
+    if (!b || !N || bits_a < 1 || bits_b < 1) return fail(NBH_ERR_ARG, "bad argument");
+    int bb = 0;
+    RCCHK(ntt_params(bits_a, bits_b, &bb, N));
+    *b = bb;
+    return NBH_OK;
+}
+
+int32_t nbh_mul(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int64_t n) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    return host_product(r, a, m, b, n, false);
+}
+
+int32_t nbh_sqr(uint64_t *r, const uint64_t *a, int64_t n) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    return host_product(r, a, n, nullptr, n, true);
+}
+
+int32_t nbh_mul_dev(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int64_t n, int64_t bits_a,
+                    int64_t bits_b, void *stream) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    RCCHK(stream_enter(stream));
+    RCCHK(run_single(r, a, m, b, n, bits_a, bits_b, false));
+    return stream_leave(stream);
+}
+
+int32_t nbh_sqr_dev(uint64_t *r, const uint64_t *a, int64_t n, int64_t bits_a, void *stream) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    RCCHK(stream_enter(stream));
+    RCCHK(run_single(r, a, n, nullptr, n, bits_a, bits_a, true));
+    return stream_leave(stream);
+}
+
+int32_t nbh_batch_create(const nbh_job *jobs, int64_t count, nbh_batch *out) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    if (!out) return fail(NBH_ERR_ARG, "null handle");
+    nbh_batch_s *B = nullptr;
+    RCCHK(batch_create(jobs, count, &B));
+    *out = B;
+    return NBH_OK;
+}
+
+int32_t nbh_batch_run(nbh_batch b, void *stream) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    if (!b) return fail(NBH_ERR_ARG, "null handle");
+    return batch_run(b, stream ? (hipStream_t)stream : g.stream);
+}
+
+void nbh_batch_destroy(nbh_batch b) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    if (b) {
+        (void)hipDeviceSynchronize();
+        batch_free(b);
+    }
+}
+
+int32_t nbh_mul_batch_dev(const nbh_job *jobs, int64_t count, void *stream) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    nbh_batch_s *B = nullptr;
+    RCCHK(batch_create(jobs, count, &B));
+    hipStream_t s = stream ? (hipStream_t)stream : g.stream;
+    int rc = batch_run(B, s);
+    hipError_t e = hipStreamSynchronize(s);
+    batch_free(B);
+    if (rc) return rc;
+    HIPCHK(e);
+    return NBH_OK;
+}
+
+int32_t nbh_ntt_fwd_dev(double *x, int64_t N, int32_t prime, void *stream) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    if (!x || (prime != 0 && prime != 1)) return fail(NBH_ERR_ARG, "bad argument");
+    PlanEntry *P = nullptr;
+    RCCHK(get_plan(N, prime, &P));
+    RCCHK(stream_enter(stream));
+    DevJob J;
+    std::memset(&J, 0, sizeof J);
+    J.plan[prime] = P->dev;
+    J.xa[prime] = x;
+    DevJob *slot = g.slots + (g.slot_next++ % NSLOTS);
+    HIPCHK(launch_store_job(J, slot, g.stream));
+    const DevPlan &H = P->host;
+    for (int gi = 0; gi < H.ngroups; gi++) {
+        TileGen tg{gi, 1, H.g[gi].tiles, 0, prime};
+        HIPCHK(launch_group(slot, nullptr, tg, H.g[gi].tiles, 1, sizeof(double) * H.g[gi].S * H.g[gi].C, g.stream));
+    }
+    return stream_leave(stream);
+}
+
+int32_t nbh_ntt_rev_dev(double *x, int64_t N, int32_t prime, void *stream) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    if (!x || (prime != 0 && prime != 1)) return fail(NBH_ERR_ARG, "bad argument");
+    PlanEntry *P = nullptr;
+    RCCHK(get_plan(N, prime, &P));
+    RCCHK(stream_enter(stream));
+    DevJob J;
+    std::memset(&J, 0, sizeof J);
+    J.plan[prime] = P->dev;
+    J.xa[prime] = x;
+    DevJob *slot = g.slots + (g.slot_next++ % NSLOTS);
+    HIPCHK(launch_store_job(J, slot, g.stream));
+    const DevPlan &H = P->host;
+    for (int gi = H.ngroups - 1; gi >= 0; gi--) {
+        TileGen tg{gi, 1, H.g[gi].tiles, 0, prime};
+        HIPCHK(launch_group(slot, nullptr, tg, H.g[gi].tiles, 0, sizeof(double) * H.g[gi].S * H.g[gi].C, g.stream));
+    }
+    return stream_leave(stream);
+}
+
+int32_t nbh_pack_dev(double *x, int64_t N, const uint64_t *limbs, int64_t n, int32_t b, int64_t nch, void *stream) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    if (!x || !limbs || n < 1 || b < 1 || b > 52 || nch < 0 || nch > N || (nch - 1) * b >= 64 * n)
+        return fail(NBH_ERR_ARG, "bad argument");
+    RCCHK(stream_enter(stream));
+    HIPCHK(launch_pack(x, N, limbs, n, b, nch, g.stream));
+    return stream_leave(stream);
+}
+
+int32_t nbh_pointwise_dev(double *xa, const double *xb, int64_t N, int32_t prime, void *stream) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    if (!xa || !xb || N < 0 || (prime != 0 && prime != 1)) return fail(NBH_ERR_ARG, "bad argument");
+    RCCHK(stream_enter(stream));
+    HIPCHK(launch_pointwise(prime, xa, xb, N, g.stream));
+    return stream_leave(stream);
+}
+
+int32_t nbh_unpack_dev(uint64_t *r, int64_t rn, const double *x1, const double *x2, int64_t N, int64_t nconv,
+                       int32_t b, void *stream) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    if (!r || !x1 || !x2 || rn < 1 || nconv < 1 || nconv > N || b < 1 || b > 49)
+        return fail(NBH_ERR_ARG, "bad argument");
+    PlanEntry *P[2];
+    RCCHK(get_plan(N, 0, &P[0]));
+    RCCHK(get_plan(N, 1, &P[1]));
+    RCCHK(stream_enter(stream));
+    DevJob J;
+    std::memset(&J, 0, sizeof J);
+    J.r = r;
+    J.rn = rn;
+    J.nconv = nconv;
+    J.bch = b;
+    J.nchunks = (rn + UNPACK_LPC - 1) / UNPACK_LPC;
+    J.plan[0] = P[0]->dev;
+    J.plan[1] = P[1]->dev;
+    J.xa[0] = const_cast<double *>(x1);
+    J.xa[1] = const_cast<double *>(x2);
+    RCCHK(ensure_buf(&g.ws, &g.ws_size, align256(sizeof(uint32_t) * (size_t)J.nchunks)));
+    J.chunk_flags = (uint32_t *)g.ws;
+    DevJob *slot = g.slots + (g.slot_next++ % NSLOTS);
+    HIPCHK(launch_store_job(J, slot, g.stream));
+    TileGen ug{0, 1, J.nchunks, 0, 0};
+    HIPCHK(launch_unpack(slot, nullptr, ug, J.nchunks, g.stream));
+    HIPCHK(launch_carry_fix(slot, nullptr, ug, J.nchunks, g.stream));
+    return stream_leave(stream);
+}
+
+int32_t nbh_plan_nstages(int64_t N, int32_t prime, int32_t *nst) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    PlanEntry *P = nullptr;
+    RCCHK(get_plan(N, prime, &P));
+    *nst = P->host.nst;
+    return NBH_OK;
+}
+
+int32_t nbh_plan_stage(int64_t N, int32_t prime, int32_t i, int32_t *m, int64_t *span, int64_t *Q, double *rot8) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    PlanEntry *P = nullptr;
+    RCCHK(get_plan(N, prime, &P));
+    if (i < 0 || i >= P->host.nst) return fail(NBH_ERR_ARG, "no such stage");
+    const StageDesc &s = P->host.st[i];
+    *m = s.m;
+    *span = s.L;
+    *Q = s.Q;
+    std::memcpy(rot8, s.rot, sizeof s.rot);
+    return NBH_OK;
+}
+
+int32_t nbh_plan_twiddle_row(int64_t N, int32_t prime, int32_t i, int32_t t, double *out) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    PlanEntry *P = nullptr;
+    RCCHK(get_plan(N, prime, &P));
+    if (i < 0 || i >= P->host.nst) return fail(NBH_ERR_ARG, "no such stage");
+    const StageDesc &s = P->host.st[i];
+    if (t < 1 || t >= s.m || s.m == 2) return fail(NBH_ERR_ARG, "no such twiddle row");
+    HIPCHK(hipMemcpy(out, P->host.tw + s.tw_off + (int64_t)(t - 1) * s.Q, sizeof(double) * (size_t)s.Q,
+                     hipMemcpyDeviceToHost));
+    return NBH_OK;
+}
+
+int32_t nbh_plan_ninv(int64_t N, int32_t prime, double *ninv, double *ninvp) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    PlanEntry *P = nullptr;
+    RCCHK(get_plan(N, prime, &P));
+    *ninv = P->host.ninv;
+    *ninvp = P->host.ninvp;
+    return NBH_OK;
+}
+
+int32_t nbh_set_tuning(int32_t grp_lds_doubles, int32_t mid_max, int32_t cmin, int32_t cmax) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    auto pow2 = [](int32_t x) { return x > 0 && (x & (x - 1)) == 0; };
+    if (!pow2(grp_lds_doubles) || !pow2(mid_max) || !pow2(cmin) || !pow2(cmax) || cmin > cmax || mid_max < 4 ||
+        grp_lds_doubles < 16 || (size_t)grp_lds_doubles * 8 > g.lds_block || (size_t)mid_max * 16 > g.lds_block)
+        return fail(NBH_ERR_ARG, "tuning outside the device limits");
+    g.tun.grp_lds = grp_lds_doubles;
+    g.tun.mid_max = mid_max;
+    g.tun.cmin = cmin;
+    g.tun.cmax = cmax;
+    g.tun_version++;
+    return NBH_OK;
+}
+
+int32_t nbh_get_tuning(int32_t *grp_lds_doubles, int32_t *mid_max, int32_t *cmin, int32_t *cmax) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    *grp_lds_doubles = g.tun.grp_lds;
+    *mid_max = g.tun.mid_max;
+    *cmin = g.tun.cmin;
+    *cmax = g.tun.cmax;
+    return NBH_OK;
+}
+
+int32_t nbh_plan_groups(int64_t N, int32_t *ngroups, int64_t *out) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    PlanEntry *P = nullptr;
+    RCCHK(get_plan(N, 0, &P));
+    *ngroups = P->host.ngroups;
+    for (int i = 0; i < P->host.ngroups; i++) {
+        out[4 * i + 0] = P->host.g[i].La;
+        out[4 * i + 1] = P->host.g[i].Lb;
+        out[4 * i + 2] = P->host.g[i].S;
+        out[4 * i + 3] = P->host.g[i].C;
+    }
+    return NBH_OK;
+}
+
+int32_t nbh_profile_enable(int32_t on) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    g.prof = on != 0;
+    return NBH_OK;
+}
+
+int32_t nbh_profile_read(double *ms, int64_t *launches, int32_t nkinds) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    for (int i = 0; i < nkinds; i++) {
+        if (ms) ms[i] = 0;
+        if (launches) launches[i] = 0;
+    }
+    int rc = NBH_OK;
+    for (auto &e : g.prof_open) {
+        const int kind = std::get<0>(e);
+        hipEvent_t a = std::get<1>(e), b = std::get<2>(e);
+        if (hipEventSynchronize(b) != hipSuccess) rc = fail(NBH_ERR_HIP, "profile event");
+        float t = 0;
+        if (rc == NBH_OK && hipEventElapsedTime(&t, a, b) != hipSuccess) rc = fail(NBH_ERR_HIP, "elapsed time");
+        if (kind < nkinds) {
+            if (ms) ms[kind] += t;
+            if (launches) launches[kind] += 1;
+        }
+        g.prof_pool.push_back(a);
+        g.prof_pool.push_back(b);
+    }
+    g.prof_open.clear();
+    return rc;
+}
+
+int32_t nbh_synchronize(void *stream) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    HIPCHK(hipStreamSynchronize(stream ? (hipStream_t)stream : g.stream));
+    return NBH_OK;
+}
+
+int32_t nbh_device_info(int32_t *cus, int64_t *lds_per_block, int64_t *hbm_bytes, char *name, int32_t name_len) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    if (cus) *cus = g.cus;
+    if (lds_per_block) *lds_per_block = (int64_t)g.lds_block;
+    if (hbm_bytes) *hbm_bytes = (int64_t)g.hbm;
+    if (name && name_len > 0) {
+        std::strncpy(name, g.name.c_str(), (size_t)name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    return NBH_OK;
+}
+
+}  // extern "C"

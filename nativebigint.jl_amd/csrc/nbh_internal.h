@@ -1,0 +1,93 @@
+// nbh_internal.h -- device data layout shared by the plan builder, the
+// transform/unpack kernels and the C-ABI layer.
+//
+// A transform of length N = m * 2^k (m | 315) is the reference's flat list of
+// DIF stages (src/fpntt.jl:236-271): odd radices 7, 5, 3, 3 at the largest
+// spans, then the radix-4 tower, then a twiddle-free radix-2 when k is odd.
+// On the GPU the stage list is cut into GROUPS, each one HBM round trip:
+// a group's stages are applied to a tile held in LDS.  A group entered at block
+// length La and left at block length Lb works on La/Lb "rows" strided Lb apart;
+// a tile is C adjacent columns of one such sub-problem, so its global reads are
+// C*8-byte contiguous runs.  The last group always has Lb = 1 (contiguous
+// blocks) and is fused with the pointwise product and the first inverse group.
+#pragma once
+
+#include <stdint.h>
+
+#define NBH_MAX_STAGES 24
+#define NBH_MAX_GROUPS 6
+
+namespace nbh {
+
+struct StageDesc {
+    int32_t m;          // radix 2, 3, 4, 5 or 7
+    int32_t q;          // Q / Lb of the group this stage belongs to (row stride)
+    int64_t L;          // span (block length) of the stage
+    int64_t Q;          // L / m
+    int64_t tw_off;     // twiddle row t (1..m-1) is tw[tw_off + (t-1)*Q + j]
+    uint64_t q_magic;   // ceil(2^32 / q): floor(r / q) = (r * q_magic) >> 32
+    double rot[8];      // combine constants (radix-4: rot[0] = wi = r^(N2/4))
+};
+
+struct GroupDesc {
+    int32_t s0, s1;     // stages [s0, s1) in forward order
+    int32_t S;          // rows: La / Lb
+    int32_t C;          // columns per tile (power of two dividing Lb)
+    int32_t logC;
+    int32_t pad;
+    int64_t La, Lb;
+    int64_t tiles;      // tiles per transform: (N / La) * (Lb / C)
+};
+
+struct DevPlan {
+    int64_t N, N2;
+    int32_t prime, nst, ngroups, pad;
+    double ninv, ninvp;           // 1/N mod p (minimal balanced) and fl(ninv/p)
+    StageDesc st[NBH_MAX_STAGES];
+    GroupDesc g[NBH_MAX_GROUPS];
+    const double *tw;             // device twiddle tables
+};
+
+// One product r = a * b (or a^2 when sqr) resident on the device.
+struct DevJob {
+    const uint64_t *a, *b;
+    uint64_t *r;
+    int64_t m, n, rn;             // limbs of a, b, r (rn = m + n)
+    int64_t nca, ncb, nconv;      // chunk counts and convolution length
+    int32_t bch, sqr;             // chunk width in bits; squaring flag
+    const DevPlan *plan[2];
+    double *xa[2], *xb[2];        // spectra per prime (xb unused for squares)
+    uint32_t *chunk_flags;        // unpack chunk carry summaries
+    int64_t nchunks;
+    int32_t zero, pad;            // zero: an operand is zero -> r = 0
+};
+
+// One unit of work for a launch: a tile of a group of one transform.
+struct Tile {
+    uint32_t job;
+    uint8_t prime, op, group, flags;  // op: 0 operand a, 1 operand b
+    uint32_t tile;
+    uint32_t pad;
+};
+
+enum TileFlags : uint8_t { TILE_PACK = 1 };
+
+// Tiles generated from blockIdx when a launch covers a single job: block b is
+// (prime, op, tile) = (prime0 + b / (nops*per), (b / per) % nops, b % per).
+struct TileGen {
+    int32_t group;
+    int32_t nops;
+    int64_t per;
+    int32_t flags;
+    int32_t prime0;
+};
+
+// Unpack geometry: each thread owns UNPACK_LPT consecutive output limbs.
+constexpr int UNPACK_THREADS = 256;
+constexpr int UNPACK_LPT = 4;
+constexpr int UNPACK_LPC = UNPACK_THREADS * UNPACK_LPT;  // limbs per chunk
+
+constexpr uint32_t CHUNK_G = 1;  // carry out with carry-in 0
+constexpr uint32_t CHUNK_P = 2;  // every limb all-ones: carry passes through
+
+}  // namespace nbh

@@ -1,0 +1,672 @@
+// nbh_kernels.hip -- gfx950 kernels of the FP64 two-prime NTT product.
+//
+// Transform stages restate src/fpntt.jl:305-733 butterfly for butterfly (same
+// reduce/mulmod schedule, same minimal-balanced twiddles), so every spectrum
+// is bit-identical to the reference's.  What changes is the data movement:
+// a group of consecutive stages runs on an LDS-resident tile (one HBM round
+// trip per group instead of one sweep per stage), the chunk pack is fused into
+// the first forward group, and the last forward group, the pointwise product
+// and the first inverse group are one kernel.  The Garner unpack
+// (src/fpntt.jl:806-925) becomes a per-limb parallel sum with a
+// generate/propagate carry prefix (wavefront ballot + LDS across waves +
+// look-back across chunks) instead of a serial 128-bit window.
+//
+// Every kernel takes its work from a device tile list, so the same code runs
+// one product per launch or a whole heterogeneous batch per launch.
+
+#include <hip/hip_runtime.h>
+
+#include "nbh_field.h"
+#include "nbh_internal.h"
+
+namespace nbh {
+
+// ---------------------------------------------------------------------------
+// DFT combines (src/fpntt.jl:310-313, 371-382, 459-492, 543-549)
+
+template <int K>
+__device__ __forceinline__ void dft3(double &a, double &b, double &c, double w3) {
+    double u = fp_mulmod3<K>(b - c, w3);
+    double y0 = a + (b + c);
+    double y1 = (a - c) + u;
+    double y2 = (a - b) - u;
+    a = y0; b = y1; c = y2;
+}
+
+template <int K>
+__device__ __forceinline__ void dft4(double &a, double &b, double &c, double &d, double wi) {
+    double apc = a + c, amc = a - c;
+    double bpd = b + d;
+    double ibmd = fp_mulmod3<K>(b - d, wi);
+    a = apc + bpd;
+    b = amc + ibmd;
+    c = apc - bpd;
+    d = amc - ibmd;
+}
+
+template <int K>
+__device__ __forceinline__ void dft5(double *y, const double *k) {
+    double x0 = y[0];
+    double t1 = y[1] + y[4], t3 = y[1] - y[4];
+    double t2 = y[2] + y[3], t4 = y[2] - y[3];
+    double S = t1 + t2, D = t1 - t2;
+    double y0 = x0 + S;
+    double r = fp_reduce<K>(x0 + fp_mulmod3<K>(S, k[0]));
+    double q = fp_mulmod3<K>(D, k[1]);
+    double g1 = fp_mulmod3<K>(t3, k[2]) + fp_mulmod3<K>(t4, k[3]);
+    double g2 = fp_mulmod3<K>(t3, k[3]) - fp_mulmod3<K>(t4, k[2]);
+    double u = r + q, v = r - q;
+    y[0] = y0;
+    y[1] = u + g1;
+    y[2] = v + g2;
+    y[3] = v - g2;
+    y[4] = u - g1;
+}
+
+template <int K, bool RED>
+__device__ __forceinline__ void dft7(double *y, const double *rot) {
+    double x0 = y[0];
+    double s1 = y[1] + y[6], d1 = y[1] - y[6];
+    double s2 = y[2] + y[5], d2 = y[2] - y[5];
+    double s3 = y[3] + y[4], d3 = y[3] - y[4];
+    if (RED) {
+        s1 = fp_reduce<K>(s1); s2 = fp_reduce<K>(s2); s3 = fp_reduce<K>(s3);
+        d1 = fp_reduce<K>(d1); d2 = fp_reduce<K>(d2); d3 = fp_reduce<K>(d3);
+    }
+    double S = (s1 + s2) + s3;
+    double y0 = x0 + S;
+    double m1 = fp_mulmod3<K>(S, rot[0]);
+    double va = s1 - s3, vb = s2 - s3;
+    double w0 = fp_mulmod3<K>(va, rot[1]);
+    double w1 = fp_mulmod3<K>(vb, rot[2]);
+    double w2 = fp_mulmod3<K>(va + vb, rot[3]);
+    double mn = fp_mulmod3<K>((d1 + d2) - d3, rot[4]);
+    double wa = d1 + d3, wb = -(d2 + d3);
+    double q0 = fp_mulmod3<K>(wa, rot[5]);
+    double q1 = fp_mulmod3<K>(wb, rot[6]);
+    double q2 = fp_mulmod3<K>(wa + wb, rot[7]);
+    double u1 = fp_reduce<K>(m1 + (3.0 * w0 - w2));
+    double u3 = fp_reduce<K>((m1 + 2.0 * w2) - 3.0 * (w0 + w1));
+    double u2 = fp_reduce<K>(m1 + (3.0 * w1 - w2));
+    double r0 = q0 - q1, r1 = q2 - q0;
+    double v1 = fp_reduce<K>(mn + (2.0 * r0 + r1));
+    double v3 = fp_reduce<K>((r0 + 2.0 * r1) - mn);
+    double v2 = fp_reduce<K>((mn - r0) + r1);
+    double a1 = x0 + u1, a2 = x0 + u2, a3 = x0 + u3;
+    y[0] = y0;
+    y[1] = a1 + v1;
+    y[2] = a2 + v2;
+    y[3] = a3 + v3;
+    y[4] = a3 - v3;
+    y[5] = a2 - v2;
+    y[6] = a1 - v1;
+}
+
+// ---------------------------------------------------------------------------
+// One stage over an LDS tile.  The tile holds S rows x C columns (row-major,
+// lds[u*C + c]); the stage's butterfly legs are q rows apart inside blocks of
+// m*q rows, and the twiddle index of a butterfly is jj*Lb + jcol0 + c.
+// Forward: combine, reduce leg 0, twiddle legs 1..m-1 (src/fpntt.jl:315-352,
+// 384-440, 494-535, 645-701).  Reverse: reduce + twiddle, then combine; the
+// radix-5/7 reverse re-reduces leg 0.  Radix 2 (src/fpntt.jl:630-638) is the
+// same in both directions.
+
+template <int K, bool FWD>
+__device__ __forceinline__ void stage_lds(double *__restrict__ x, const StageDesc *__restrict__ sd,
+                                          int S, int logC, int64_t Lb, int64_t jcol0,
+                                          const double *__restrict__ twbase) {
+    const int m = sd->m;
+    const int q = sd->q;
+    const uint64_t qM = sd->q_magic;
+    const int64_t Q = sd->Q;
+    const int C = 1 << logC;
+    const int nb = (S / m) << logC;
+    const int strideC = q << logC;
+    const int blockRows = m * q;
+    const double *__restrict__ tw = twbase + sd->tw_off;
+    double rot[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) rot[i] = sd->rot[i];
+
+    for (int beta = threadIdx.x; beta < nb; beta += blockDim.x) {
+        const int c = beta & (C - 1);
+        const uint32_t r = (uint32_t)beta >> logC;
+        const uint32_t ob = (uint32_t)(((uint64_t)r * qM) >> 32);
+        const uint32_t jj = r - ob * (uint32_t)q;
+        const int base = ((int)(ob * blockRows + jj) << logC) + c;
+        const int64_t j = (int64_t)jj * Lb + jcol0 + c;
+        if (m == 4) {
+            double y0 = x[base], y1 = x[base + strideC], y2 = x[base + 2 * strideC], y3 = x[base + 3 * strideC];
+            const double w1 = tw[j], w2 = tw[Q + j], w3 = tw[2 * Q + j];
+            if (FWD) dft4<K>(y0, y1, y2, y3, rot[0]);
+            y0 = fp_reduce<K>(y0);
+            y1 = fp_mulmod3<K>(y1, w1);
+            y2 = fp_mulmod3<K>(y2, w2);
+            y3 = fp_mulmod3<K>(y3, w3);
+            if (!FWD) dft4<K>(y0, y1, y2, y3, rot[0]);
+            x[base] = y0; x[base + strideC] = y1; x[base + 2 * strideC] = y2; x[base + 3 * strideC] = y3;
+        } else if (m == 2) {
+            const double u = x[base], v = x[base + strideC];
+            x[base] = fp_reduce<K>(u + v);
+            x[base + strideC] = fp_reduce<K>(u - v);
+        } else if (m == 3) {
+            double y0 = x[base], y1 = x[base + strideC], y2 = x[base + 2 * strideC];
+            const double w1 = tw[j], w2 = tw[Q + j];
+            if (FWD) dft3<K>(y0, y1, y2, rot[0]);
+            y0 = fp_reduce<K>(y0);
+            y1 = fp_mulmod3<K>(y1, w1);
+            y2 = fp_mulmod3<K>(y2, w2);
+            if (!FWD) dft3<K>(y0, y1, y2, rot[0]);
+            x[base] = y0; x[base + strideC] = y1; x[base + 2 * strideC] = y2;
+        } else if (m == 5) {
+            double y[5];
+#pragma unroll
+            for (int t = 0; t < 5; t++) y[t] = x[base + t * strideC];
+            if (FWD) dft5<K>(y, rot);
+            y[0] = fp_reduce<K>(y[0]);
+#pragma unroll
+            for (int t = 1; t < 5; t++) y[t] = fp_mulmod3<K>(y[t], tw[(t - 1) * Q + j]);
+            if (!FWD) {
+                dft5<K>(y, rot);
+                y[0] = fp_reduce<K>(y[0]);
+            }
+#pragma unroll
+            for (int t = 0; t < 5; t++) x[base + t * strideC] = y[t];
+        } else {  // m == 7
+            double y[7];
+#pragma unroll
+            for (int t = 0; t < 7; t++) y[t] = x[base + t * strideC];
+            if (FWD) dft7<K, false>(y, rot);
+            y[0] = fp_reduce<K>(y[0]);
+#pragma unroll
+            for (int t = 1; t < 7; t++) y[t] = fp_mulmod3<K>(y[t], tw[(t - 1) * Q + j]);
+            if (!FWD) {
+                dft7<K, true>(y, rot);
+                y[0] = fp_reduce<K>(y[0]);
+            }
+#pragma unroll
+            for (int t = 0; t < 7; t++) x[base + t * strideC] = y[t];
+        }
+    }
+}
+
+// Chunk i of an operand: bits [i*b, i*b + b) (src/fpntt.jl:744-768); chunks
+// past nch are the zero fill.  Chunks are < 2^b < p: exact, canonical.
+__device__ __forceinline__ double pack_chunk(const uint64_t *__restrict__ limbs, int64_t n, int b,
+                                             int64_t nch, int64_t i) {
+    if (i >= nch) return 0.0;
+    const int64_t bit = i * b;
+    const int64_t w = bit >> 6;
+    const int sh = (int)(bit & 63);
+    uint64_t c = limbs[w] >> sh;
+    if (sh && w + 1 < n) c |= limbs[w + 1] << (64 - sh);
+    return (double)(c & ((1ull << b) - 1));
+}
+
+// The tile a block works on: from the launch's tile list, or generated.
+__device__ __forceinline__ Tile fetch_tile(const Tile *__restrict__ tiles, const TileGen &g) {
+    if (tiles) return tiles[blockIdx.x];
+    Tile t;
+    int64_t b = blockIdx.x;
+    t.job = 0;
+    t.tile = (uint32_t)(b % g.per);
+    b /= g.per;
+    t.op = (uint8_t)(b % g.nops);
+    t.prime = (uint8_t)(g.prime0 + b / g.nops);
+    t.group = (uint8_t)g.group;
+    t.flags = (uint8_t)g.flags;
+    t.pad = 0;
+    return t;
+}
+
+// ---------------------------------------------------------------------------
+// Group pass: one tile of one group of one transform, forward or inverse.
+
+template <int K>
+__device__ void group_body(const DevJob &J, const Tile &t, bool fwd, double *lds) {
+    const DevPlan *__restrict__ P = J.plan[K];
+    const GroupDesc &G = P->g[t.group];
+    double *__restrict__ x = t.op ? J.xb[K] : J.xa[K];
+    const int logC = G.logC, S = G.S;
+    const int64_t tpr = G.Lb >> logC;
+    const int64_t blk = (int64_t)t.tile / tpr;
+    const int64_t jc = (int64_t)t.tile - blk * tpr;
+    const int64_t jcol0 = jc << logC;
+    const int64_t gbase = blk * G.La + jcol0;
+    const int total = S << logC;
+    const int cmask = (1 << logC) - 1;
+
+    if (t.flags & TILE_PACK) {
+        const uint64_t *limbs = t.op ? J.b : J.a;
+        const int64_t n = t.op ? J.n : J.m;
+        const int64_t nch = t.op ? J.ncb : J.nca;
+        for (int e = threadIdx.x; e < total; e += blockDim.x)
+            lds[e] = pack_chunk(limbs, n, J.bch, nch, gbase + (int64_t)(e >> logC) * G.Lb + (e & cmask));
+    } else {
+        for (int e = threadIdx.x; e < total; e += blockDim.x)
+            lds[e] = x[gbase + (int64_t)(e >> logC) * G.Lb + (e & cmask)];
+    }
+    __syncthreads();
+    if (fwd) {
+        for (int s = G.s0; s < G.s1; s++) {
+            stage_lds<K, true>(lds, &P->st[s], S, logC, G.Lb, jcol0, P->tw);
+            __syncthreads();
+        }
+    } else {
+        for (int s = G.s1 - 1; s >= G.s0; s--) {
+            stage_lds<K, false>(lds, &P->st[s], S, logC, G.Lb, jcol0, P->tw);
+            __syncthreads();
+        }
+    }
+    for (int e = threadIdx.x; e < total; e += blockDim.x)
+        x[gbase + (int64_t)(e >> logC) * G.Lb + (e & cmask)] = lds[e];
+}
+
+__global__ void __launch_bounds__(512) k_group(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles,
+                                               TileGen gen, int fwd) {
+    extern __shared__ double lds[];
+    const Tile t = fetch_tile(tiles, gen);
+    const DevJob &J = jobs[t.job];
+    if (J.zero) return;
+    if (t.prime == 0) group_body<0>(J, t, fwd != 0, lds);
+    else group_body<1>(J, t, fwd != 0, lds);
+}
+
+// ---------------------------------------------------------------------------
+// Mid pass: last forward group on both operands, pointwise product
+// (src/fpntt.jl:770-785), first inverse group; for a single-group transform
+// the pack is fused in too, so the whole transform pair is one kernel.
+
+template <int K>
+__device__ void mid_body(const DevJob &J, const Tile &t, double *lds) {
+    const DevPlan *__restrict__ P = J.plan[K];
+    const GroupDesc &G = P->g[P->ngroups - 1];
+    const int S = G.S;
+    const int64_t gbase = (int64_t)t.tile * G.La;
+    const bool sq = J.sqr != 0;
+    double *la = lds;
+    double *lb = lds + S;
+    if (t.flags & TILE_PACK) {
+        for (int e = threadIdx.x; e < S; e += blockDim.x) {
+            la[e] = pack_chunk(J.a, J.m, J.bch, J.nca, gbase + e);
+            if (!sq) lb[e] = pack_chunk(J.b, J.n, J.bch, J.ncb, gbase + e);
+        }
+    } else {
+        const double *__restrict__ xa = J.xa[K];
+        const double *__restrict__ xb = J.xb[K];
+        for (int e = threadIdx.x; e < S; e += blockDim.x) {
+            la[e] = xa[gbase + e];
+            if (!sq) lb[e] = xb[gbase + e];
+        }
+    }
+    __syncthreads();
+    for (int s = G.s0; s < G.s1; s++) {
+        stage_lds<K, true>(la, &P->st[s], S, 0, 1, 0, P->tw);
+        if (!sq) stage_lds<K, true>(lb, &P->st[s], S, 0, 1, 0, P->tw);
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < S; e += blockDim.x) la[e] = fp_mulmod3<K>(la[e], sq ? la[e] : lb[e]);
+    __syncthreads();
+    for (int s = G.s1 - 1; s >= G.s0; s--) {
+        stage_lds<K, false>(la, &P->st[s], S, 0, 1, 0, P->tw);
+        __syncthreads();
+    }
+    double *__restrict__ xo = J.xa[K];
+    for (int e = threadIdx.x; e < S; e += blockDim.x) xo[gbase + e] = la[e];
+}
+
+__global__ void __launch_bounds__(512) k_mid(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles,
+                                             TileGen gen) {
+    extern __shared__ double lds[];
+    const Tile t = fetch_tile(tiles, gen);
+    const DevJob &J = jobs[t.job];
+    if (J.zero) return;
+    if (t.prime == 0) mid_body<0>(J, t, lds);
+    else mid_body<1>(J, t, lds);
+}
+
+// ---------------------------------------------------------------------------
+// Garner unpack (src/fpntt.jl:806-925), restated for a parallel machine.
+//
+// Coefficient i of the product convolution is c_i = c1 + p1*u (< p1*p2 < 2^100)
+// with c1, u the reference's canonical Garner pair read at (N-i) mod N and
+// scaled by 1/N.  The product is sum_i c_i 2^(b i).  Thread t owns the limbs
+// [k0, k0+4): it sums exactly the coefficients that START in those limbs
+// (a streamed 192-bit window, as the reference's) into four limb words and a
+// two-word spill into the next thread's limbs.  Adding the previous thread's
+// spill gives a 4-word value with a carry out; carries between threads are a
+// generate/propagate prefix (wave ballot + add trick, then waves in LDS),
+// and carries between chunks are resolved by k_carry_fix's look-back.
+
+__device__ __forceinline__ void garner_coeff(const DevJob &J, int64_t i, uint64_t &lo, uint64_t &hi) {
+    if (i >= J.nconv) { lo = 0; hi = 0; return; }
+    const DevPlan *P1 = J.plan[0];
+    const DevPlan *P2 = J.plan[1];
+    const int64_t N = P1->N;
+    const int64_t idx = i == 0 ? 0 : N - i;
+    double v1 = fp_mulmod4<0>(J.xa[0][idx], P1->ninv, P1->ninvp);
+    v1 = v1 < 0 ? v1 + P1_D : v1;
+    const double v2 = fp_mulmod4<1>(J.xa[1][idx], P2->ninv, P2->ninvp);
+    double u = fp_mulmod4<1>(v2 - fp_reduce<1>(v1), GARNER_G, GARNER_GP);
+    u = u < 0 ? u + P2_D : u;
+    const uint64_t c1 = (uint64_t)v1, uu = (uint64_t)u;
+    const uint64_t plo = uu * P1_U;
+    const uint64_t phi = __umul64hi(uu, P1_U);
+    lo = plo + c1;
+    hi = phi + (lo < c1 ? 1 : 0);
+}
+
+struct Seg {
+    uint64_t o0, o1, o2, o3;  // limbs k0..k0+3 (before neighbour spill and carries)
+    uint64_t s0, s1;          // spill into limbs k0+4, k0+5
+};
+
+__device__ __forceinline__ Seg segment_sum(const DevJob &J, int64_t k0) {
+    const int b = J.bch;
+    const int64_t ilo = (64 * k0 + b - 1) / b;
+    const int64_t iend = (64 * (k0 + UNPACK_LPT) + b - 1) / b;
+    int s = (int)(ilo * b - 64 * k0);
+    uint64_t w0 = 0, w1 = 0, w2 = 0;
+    Seg out{0, 0, 0, 0, 0, 0};
+    for (int64_t i = ilo; i < iend; i++) {
+        uint64_t lo, hi;
+        garner_coeff(J, i, lo, hi);
+        const unsigned __int128 a01 = (unsigned __int128)lo << s;
+        const unsigned __int128 a12 = (unsigned __int128)hi << s;
+        unsigned __int128 acc = (unsigned __int128)w0 + (uint64_t)a01;
+        w0 = (uint64_t)acc;
+        acc = (unsigned __int128)w1 + (uint64_t)(a01 >> 64) + (uint64_t)a12 + (uint64_t)(acc >> 64);
+        w1 = (uint64_t)acc;
+        w2 += (uint64_t)(a12 >> 64) + (uint64_t)(acc >> 64);
+        s += b;
+        if (s >= 64) {
+            out.o0 = out.o1; out.o1 = out.o2; out.o2 = out.o3; out.o3 = w0;
+            w0 = w1; w1 = w2; w2 = 0;
+            s -= 64;
+        }
+    }
+    out.s0 = w0;
+    out.s1 = w1;
+    return out;
+}
+
+__global__ void __launch_bounds__(UNPACK_THREADS) k_unpack(const DevJob *__restrict__ jobs,
+                                                           const Tile *__restrict__ tiles, TileGen gen) {
+    __shared__ uint64_t spill0[UNPACK_THREADS], spill1[UNPACK_THREADS];
+    __shared__ uint32_t wsum[UNPACK_THREADS / 64];
+    const Tile t = fetch_tile(tiles, gen);
+    const DevJob &J = jobs[t.job];
+    const int tid = threadIdx.x;
+    const int64_t k0 = (int64_t)t.tile * UNPACK_LPC + UNPACK_LPT * tid;
+    uint64_t *__restrict__ r = J.r;
+    if (J.zero) {
+        for (int w = 0; w < UNPACK_LPT; w++)
+            if (k0 + w < J.rn) r[k0 + w] = 0;
+        if (tid == 0) J.chunk_flags[t.tile] = 0;
+        return;
+    }
+    const Seg sg = segment_sum(J, k0);
+    spill0[tid] = sg.s0;
+    spill1[tid] = sg.s1;
+    uint64_t p0 = 0, p1 = 0;
+    if (tid == 0 && k0 >= UNPACK_LPT) {
+        const Seg pv = segment_sum(J, k0 - UNPACK_LPT);
+        p0 = pv.s0;
+        p1 = pv.s1;
+    }
+    __syncthreads();
+    if (tid > 0) {
+        p0 = spill0[tid - 1];
+        p1 = spill1[tid - 1];
+    }
+    // q = o + (p1:p0), four words with a carry out
+    uint64_t q0 = sg.o0 + p0;
+    uint64_t c = q0 < p0;
+    uint64_t q1 = sg.o1 + p1;
+    uint64_t c1 = q1 < p1;
+    q1 += c;
+    c1 += (q1 < c);
+    uint64_t q2 = sg.o2 + c1;
+    c = q2 < c1;
+    uint64_t q3 = sg.o3 + c;
+    c = q3 < c;
+    const bool G = c != 0;
+    const bool Pp = (q0 & q1 & q2 & q3) == ~0ull;
+
+    const int lane = tid & 63, wid = tid >> 6;
+    const uint64_t gm = __ballot(G);
+    const uint64_t pm = __ballot(Pp);
+    {
+        const uint64_t cv = (gm + (gm | pm)) ^ pm;  // carries into each lane with carry-in 0
+        const uint32_t wg = (uint32_t)(((gm >> 63) | ((pm >> 63) & (cv >> 63))) & 1);
+        const uint32_t wp = pm == ~0ull ? 1u : 0u;
+        if (lane == 0) wsum[wid] = wg | (wp << 1);
+    }
+    __syncthreads();
+    uint64_t cin = 0;
+    for (int w = 0; w < wid; w++) {
+        const uint32_t f = wsum[w];
+        cin = (f & 1) | ((f >> 1) & cin);
+    }
+    const uint64_t cv = (gm + (gm | pm) + cin) ^ pm;
+    const uint64_t my = (cv >> lane) & 1;
+    q0 += my;
+    c = q0 < my;
+    q1 += c;
+    c = q1 < c;
+    q2 += c;
+    c = q2 < c;
+    q3 += c;
+    if (k0 + 0 < J.rn) r[k0 + 0] = q0;
+    if (k0 + 1 < J.rn) r[k0 + 1] = q1;
+    if (k0 + 2 < J.rn) r[k0 + 2] = q2;
+    if (k0 + 3 < J.rn) r[k0 + 3] = q3;
+    if (tid == UNPACK_THREADS - 1) {
+        // carry out of the chunk with carry-in 0, and whether all limbs propagate
+        const uint32_t cg = (G || (Pp && my)) ? CHUNK_G : 0u;
+        uint32_t allp = 1;
+        for (int w = 0; w < UNPACK_THREADS / 64; w++) allp &= (wsum[w] >> 1);
+        J.chunk_flags[t.tile] = cg | (allp ? CHUNK_P : 0u);
+    }
+}
+
+// Carry into chunk ch = G(ch-1) | (P(ch-1) & carry into ch-1): walk back over
+// the chunk summaries (all written by the previous launch, so no waiting),
+// then add the carry into the chunk's leading run of all-ones limbs.
+__global__ void __launch_bounds__(UNPACK_THREADS) k_carry_fix(const DevJob *__restrict__ jobs,
+                                                              const Tile *__restrict__ tiles, TileGen gen) {
+    __shared__ int first;
+    const Tile t = fetch_tile(tiles, gen);
+    const DevJob &J = jobs[t.job];
+    if (J.zero || t.tile == 0) return;
+    int cin = 0;
+    for (int64_t ch = (int64_t)t.tile - 1; ch >= 0; ch--) {
+        const uint32_t f = J.chunk_flags[ch];
+        if (f & CHUNK_G) { cin = 1; break; }
+        if (!(f & CHUNK_P)) break;
+    }
+    if (!cin) return;
+    const int tid = threadIdx.x;
+    const int64_t k0 = (int64_t)t.tile * UNPACK_LPC + UNPACK_LPT * tid;
+    uint64_t *__restrict__ r = J.r;
+    uint64_t v[UNPACK_LPT];
+    bool ones = true;
+#pragma unroll
+    for (int w = 0; w < UNPACK_LPT; w++) {
+        v[w] = (k0 + w < J.rn) ? r[k0 + w] : 0;
+        ones = ones && (k0 + w < J.rn) && v[w] == ~0ull;
+    }
+    if (tid == 0) first = UNPACK_THREADS;
+    __syncthreads();
+    if (!ones) atomicMin(&first, tid);
+    __syncthreads();
+    if (tid < first) {
+#pragma unroll
+        for (int w = 0; w < UNPACK_LPT; w++)
+            if (k0 + w < J.rn) r[k0 + w] = 0;
+    } else if (tid == first) {
+        uint64_t cc = 1;
+#pragma unroll
+        for (int w = 0; w < UNPACK_LPT; w++) {
+            const uint64_t nv = v[w] + cc;
+            cc = nv < cc;
+            if (k0 + w < J.rn) r[k0 + w] = nv;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Twiddle tables.  Row t (1..m-1) of a stage of span L holds w_L^(t j) for
+// j < Q, w_L = w_N^(N/L), minimal balanced (src/fpntt.jl:173-225).  Each entry
+// is w_N^e = hi[e >> h] * lo[e & (2^h-1)] mod p with exact host-built tables;
+// the mulmod is exact and fp_minbal picks the unique |w| <= p/2 representative,
+// so the table equals the reference's chained fp_twiddles output bit for bit.
+
+template <int K>
+__global__ void k_twiddle_stage(double *__restrict__ out, int m, int64_t Q, int64_t NoverL,
+                                const double *__restrict__ tlo, const double *__restrict__ thi, int h) {
+    const int64_t total = (int64_t)(m - 1) * Q;
+    const int64_t mask = (1ll << h) - 1;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t t = idx / Q + 1;
+        const int64_t j = idx - (t - 1) * Q;
+        const int64_t e = t * j * NoverL;
+        out[idx] = fp_minbal<K>(fp_mulmod3<K>(thi[e >> h], tlo[e & mask]));
+    }
+}
+
+template __global__ void k_twiddle_stage<0>(double *, int, int64_t, int64_t, const double *, const double *, int);
+template __global__ void k_twiddle_stage<1>(double *, int, int64_t, int64_t, const double *, const double *, int);
+
+// ---------------------------------------------------------------------------
+// Stage-level kernels behind the debug/parity entry points.
+
+template <int K>
+__global__ void k_pointwise(double *__restrict__ xa, const double *__restrict__ xb, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        xa[i] = fp_mulmod3<K>(xa[i], xb[i]);
+}
+template __global__ void k_pointwise<0>(double *, const double *, int64_t);
+template __global__ void k_pointwise<1>(double *, const double *, int64_t);
+
+__global__ void k_pack(double *__restrict__ x, int64_t N, const uint64_t *__restrict__ limbs, int64_t n, int b,
+                       int64_t nch) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = pack_chunk(limbs, n, b, nch, i);
+}
+
+__global__ void k_fill_u64(uint64_t *__restrict__ p, int64_t n, uint64_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Stream-ordered upload of a single job descriptor (no host staging buffer).
+__global__ void k_store_job(DevJob j, DevJob *__restrict__ dst) {
+    if (threadIdx.x == 0) *dst = j;
+}
+
+// Top limbs of every operand of a batch (magnitude_bits needs them,
+// src/kernels/addsub.jl:150).
+__global__ void k_top_limbs(const DevJob *__restrict__ jobs, int64_t count, uint64_t *__restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
+        out[2 * i] = jobs[i].a[jobs[i].m - 1];
+        out[2 * i + 1] = jobs[i].sqr ? jobs[i].a[jobs[i].m - 1] : jobs[i].b[jobs[i].n - 1];
+    }
+}
+
+}  // namespace nbh
+
+// ---------------------------------------------------------------------------
+// Host launchers (nbh_launch.h)
+
+#include "nbh_launch.h"
+
+namespace nbh {
+
+static inline unsigned grid_for(int64_t n, int threads, int64_t cap = 1 << 16) {
+    int64_t g = (n + threads - 1) / threads;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+hipError_t launch_group(const DevJob *jobs, const Tile *tiles, const TileGen &gen, int64_t ntiles, int fwd,
+                        size_t lds_bytes, hipStream_t s) {
+    if (ntiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_group, dim3((unsigned)ntiles), dim3(GROUP_THREADS), lds_bytes, s, jobs, tiles, gen, fwd);
+    return hipGetLastError();
+}
+
+hipError_t launch_mid(const DevJob *jobs, const Tile *tiles, const TileGen &gen, int64_t ntiles, size_t lds_bytes,
+                      hipStream_t s) {
+    if (ntiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mid, dim3((unsigned)ntiles), dim3(GROUP_THREADS), lds_bytes, s, jobs, tiles, gen);
+    return hipGetLastError();
+}
+
+hipError_t launch_unpack(const DevJob *jobs, const Tile *tiles, const TileGen &gen, int64_t ntiles, hipStream_t s) {
+    if (ntiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_unpack, dim3((unsigned)ntiles), dim3(UNPACK_THREADS), 0, s, jobs, tiles, gen);
+    return hipGetLastError();
+}
+
+hipError_t launch_carry_fix(const DevJob *jobs, const Tile *tiles, const TileGen &gen, int64_t ntiles,
+                            hipStream_t s) {
+    if (ntiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_carry_fix, dim3((unsigned)ntiles), dim3(UNPACK_THREADS), 0, s, jobs, tiles, gen);
+    return hipGetLastError();
+}
+
+hipError_t launch_twiddle_stage(int prime, double *out, int m, int64_t Q, int64_t NoverL, const double *tlo,
+                                const double *thi, int h, hipStream_t s) {
+    const int64_t total = (int64_t)(m - 1) * Q;
+    if (total <= 0) return hipSuccess;
+    if (prime == 0)
+        hipLaunchKernelGGL(k_twiddle_stage<0>, dim3(grid_for(total, 256)), dim3(256), 0, s, out, m, Q, NoverL, tlo,
+                           thi, h);
+    else
+        hipLaunchKernelGGL(k_twiddle_stage<1>, dim3(grid_for(total, 256)), dim3(256), 0, s, out, m, Q, NoverL, tlo,
+                           thi, h);
+    return hipGetLastError();
+}
+
+hipError_t launch_pointwise(int prime, double *xa, const double *xb, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    if (prime == 0)
+        hipLaunchKernelGGL(k_pointwise<0>, dim3(grid_for(n, 256)), dim3(256), 0, s, xa, xb, n);
+    else
+        hipLaunchKernelGGL(k_pointwise<1>, dim3(grid_for(n, 256)), dim3(256), 0, s, xa, xb, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack(double *x, int64_t N, const uint64_t *limbs, int64_t n, int b, int64_t nch, hipStream_t s) {
+    if (N <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pack, dim3(grid_for(N, 256)), dim3(256), 0, s, x, N, limbs, n, b, nch);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(uint64_t *p, int64_t n, uint64_t v, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_store_job(const DevJob &j, DevJob *dst, hipStream_t s) {
+    hipLaunchKernelGGL(k_store_job, dim3(1), dim3(64), 0, s, j, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_top_limbs(const DevJob *jobs, int64_t count, uint64_t *out, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_top_limbs, dim3(grid_for(count, 256)), dim3(256), 0, s, jobs, count, out);
+    return hipGetLastError();
+}
+
+hipError_t set_lds_limit(size_t bytes) {
+    hipError_t e = hipFuncSetAttribute((const void *)k_group, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute((const void *)k_mid, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+}  // namespace nbh

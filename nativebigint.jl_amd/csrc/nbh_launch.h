@@ -1,0 +1,31 @@
+// nbh_launch.h -- host launchers for the kernels in nbh_kernels.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "nbh_internal.h"
+
+namespace nbh {
+
+constexpr int GROUP_THREADS = 512;
+
+// Tile-driven launches: `tiles` is a device tile list of ntiles entries, or
+// NULL to generate ntiles tiles of job 0 from `gen`.
+hipError_t launch_group(const DevJob *jobs, const Tile *tiles, const TileGen &gen, int64_t ntiles, int fwd,
+                        size_t lds_bytes, hipStream_t s);
+hipError_t launch_mid(const DevJob *jobs, const Tile *tiles, const TileGen &gen, int64_t ntiles, size_t lds_bytes,
+                      hipStream_t s);
+hipError_t launch_unpack(const DevJob *jobs, const Tile *tiles, const TileGen &gen, int64_t ntiles, hipStream_t s);
+hipError_t launch_carry_fix(const DevJob *jobs, const Tile *tiles, const TileGen &gen, int64_t ntiles,
+                            hipStream_t s);
+hipError_t launch_store_job(const DevJob &j, DevJob *dst, hipStream_t s);
+hipError_t launch_twiddle_stage(int prime, double *out, int m, int64_t Q, int64_t NoverL, const double *tlo,
+                                const double *thi, int h, hipStream_t s);
+hipError_t launch_pointwise(int prime, double *xa, const double *xb, int64_t n, hipStream_t s);
+hipError_t launch_pack(double *x, int64_t N, const uint64_t *limbs, int64_t n, int b, int64_t nch, hipStream_t s);
+hipError_t launch_fill_u64(uint64_t *p, int64_t n, uint64_t v, hipStream_t s);
+hipError_t launch_top_limbs(const DevJob *jobs, int64_t count, uint64_t *out, hipStream_t s);
+// Raise the dynamic-LDS ceiling of the LDS-tiled kernels to `bytes`.
+hipError_t set_lds_limit(size_t bytes);
+
+}  // namespace nbh

@@ -1,0 +1,198 @@
+"""Device-resident entry points (torch tensors as HBM buffers).
+
+torch supplies device memory and the stream; all arithmetic runs in
+libnbh.so's HIP kernels.  Limb buffers are int64 tensors holding the uint64
+limbs bit for bit; transform buffers are float64 tensors.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from ._lib import check, init, lib, nbh_job
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def to_dev(a: np.ndarray) -> torch.Tensor:
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.uint64).view(np.int64)).to("cuda")
+
+
+def to_host(t: torch.Tensor) -> np.ndarray:
+    return t.cpu().numpy().view(np.uint64).copy()
+
+
+def empty_limbs(n: int) -> torch.Tensor:
+    return torch.empty(n, dtype=torch.int64, device="cuda")
+
+
+def _p(t: torch.Tensor | None):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def mul_dev(r: torch.Tensor, a: torch.Tensor, b: torch.Tensor, bits_a: int = -1, bits_b: int = -1) -> None:
+    """r = a*b on the device, enqueued on torch's current stream."""
+    init()
+    assert r.numel() == a.numel() + b.numel()
+    check(lib().nbh_mul_dev(_p(r), _p(a), a.numel(), _p(b), b.numel(), bits_a, bits_b, _stream()))
+
+
+def sqr_dev(r: torch.Tensor, a: torch.Tensor, bits_a: int = -1) -> None:
+    init()
+    assert r.numel() == 2 * a.numel()
+    check(lib().nbh_sqr_dev(_p(r), _p(a), a.numel(), bits_a, _stream()))
+
+
+class Batch:
+    """A planned batch of independent products (device buffers).
+
+    jobs: list of (r, a, b) tensors, b None for a square, optionally with
+    (bits_a, bits_b) appended.  run() enqueues every product on the current
+    stream in a fixed number of launches."""
+
+    def __init__(self, jobs):
+        init()
+        arr = (nbh_job * len(jobs))()
+        self._keep = []
+        for i, j in enumerate(jobs):
+            r, a, b = j[0], j[1], j[2]
+            bits_a, bits_b = (j[3], j[4]) if len(j) > 3 else (-1, -1)
+            arr[i].r = r.data_ptr()
+            arr[i].a = a.data_ptr()
+            arr[i].b = b.data_ptr() if b is not None else None
+            arr[i].m = a.numel()
+            arr[i].n = b.numel() if b is not None else a.numel()
+            arr[i].bits_a = bits_a
+            arr[i].bits_b = bits_b
+            self._keep.append((r, a, b))
+        h = ctypes.c_void_p()
+        torch.cuda.synchronize()
+        check(lib().nbh_batch_create(arr, len(jobs), ctypes.byref(h)))
+        self._h = h
+
+    def run(self, stream=None) -> None:
+        s = ctypes.c_void_p(stream.cuda_stream) if stream is not None else _stream()
+        check(lib().nbh_batch_run(self._h, s))
+
+    def close(self) -> None:
+        if self._h:
+            lib().nbh_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# -- stage-level entry points (src/fpntt.jl:715, 727, 744, 770, 869) ---------
+
+def fp_ntt_fwd(x: torch.Tensor, prime: int) -> None:
+    init()
+    check(lib().nbh_ntt_fwd_dev(_p(x), x.numel(), prime, _stream()))
+
+
+def fp_ntt_rev(x: torch.Tensor, prime: int) -> None:
+    init()
+    check(lib().nbh_ntt_rev_dev(_p(x), x.numel(), prime, _stream()))
+
+
+def fp_ntt_pack(x: torch.Tensor, limbs: torch.Tensor, b: int, nch: int) -> None:
+    init()
+    check(lib().nbh_pack_dev(_p(x), x.numel(), _p(limbs), limbs.numel(), b, nch, _stream()))
+
+
+def fp_ntt_pointwise(xa: torch.Tensor, xb: torch.Tensor, prime: int) -> None:
+    init()
+    check(lib().nbh_pointwise_dev(_p(xa), _p(xb), xa.numel(), prime, _stream()))
+
+
+def fp_ntt_unpack2(r: torch.Tensor, x1: torch.Tensor, x2: torch.Tensor, nconv: int, b: int) -> None:
+    init()
+    check(lib().nbh_unpack_dev(_p(r), r.numel(), _p(x1), _p(x2), x1.numel(), nconv, b, _stream()))
+
+
+# -- plan introspection and tuning --------------------------------------------
+
+def plan_stages(N: int, prime: int):
+    init()
+    nst = ctypes.c_int32()
+    check(lib().nbh_plan_nstages(N, prime, ctypes.byref(nst)))
+    out = []
+    for i in range(nst.value):
+        m, span, Q = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int64()
+        rot = np.zeros(8, dtype=np.float64)
+        check(lib().nbh_plan_stage(N, prime, i, ctypes.byref(m), ctypes.byref(span), ctypes.byref(Q),
+                                   rot.ctypes.data_as(ctypes.c_void_p)))
+        out.append((m.value, span.value, Q.value, rot))
+    return out
+
+
+def plan_twiddle_row(N: int, prime: int, i: int, t: int, Q: int) -> np.ndarray:
+    out = np.empty(Q, dtype=np.float64)
+    check(lib().nbh_plan_twiddle_row(N, prime, i, t, out.ctypes.data_as(ctypes.c_void_p)))
+    return out
+
+
+def plan_ninv(N: int, prime: int):
+    init()
+    a, b = ctypes.c_double(), ctypes.c_double()
+    check(lib().nbh_plan_ninv(N, prime, ctypes.byref(a), ctypes.byref(b)))
+    return a.value, b.value
+
+
+def plan_groups(N: int):
+    """[(La, Lb, S, C), ...] of the current schedule for N."""
+    init()
+    ng = ctypes.c_int32()
+    buf = np.zeros(4 * 8, dtype=np.int64)
+    check(lib().nbh_plan_groups(N, ctypes.byref(ng), buf.ctypes.data_as(ctypes.c_void_p)))
+    return [tuple(int(v) for v in buf[4 * i:4 * i + 4]) for i in range(ng.value)]
+
+
+def set_tuning(grp_lds: int, mid_max: int, cmin: int, cmax: int) -> None:
+    init()
+    check(lib().nbh_set_tuning(grp_lds, mid_max, cmin, cmax))
+
+
+def get_tuning():
+    init()
+    v = [ctypes.c_int32() for _ in range(4)]
+    check(lib().nbh_get_tuning(*[ctypes.byref(x) for x in v]))
+    return tuple(x.value for x in v)
+
+
+def device_info():
+    init()
+    cus, lds, hbm = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int64()
+    name = ctypes.create_string_buffer(64)
+    check(lib().nbh_device_info(ctypes.byref(cus), ctypes.byref(lds), ctypes.byref(hbm), name, 64))
+    return {"cus": cus.value, "lds_per_block": lds.value, "hbm_bytes": hbm.value, "arch": name.value.decode()}
+
+
+PROF_KINDS = ("group_fwd", "mid", "group_inv", "unpack", "carry_fix")
+
+
+def profile_enable(on: bool = True) -> None:
+    init()
+    check(lib().nbh_profile_enable(1 if on else 0))
+
+
+def profile_read() -> dict:
+    """{kind: (total ms, launches)} since the last read (synchronises)."""
+    init()
+    ms = np.zeros(len(PROF_KINDS), dtype=np.float64)
+    cnt = np.zeros(len(PROF_KINDS), dtype=np.int64)
+    check(lib().nbh_profile_read(ms.ctypes.data_as(ctypes.c_void_p), cnt.ctypes.data_as(ctypes.c_void_p),
+                                 len(PROF_KINDS)))
+    return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(PROF_KINDS)}
+
+
+def synchronize() -> None:
+    init()
+    check(lib().nbh_synchronize(None))
