@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-one-per-launch", action="store_true")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="also run a plain device-to-device copy of this many MiB (HBM counter calibration)")
     return ap.parse_args()
 
 
@@ -93,6 +95,16 @@ def main():
     log(f"device {info}")
     dev = torch.device("cuda")
     stream = torch.cuda.Stream()
+
+    if args.calibrate_copy_mib:
+        n = args.calibrate_copy_mib * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.int64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: 3 x {args.calibrate_copy_mib} MiB device-to-device")
+        del src, dst
 
     # ---- build the job list --------------------------------------------------
     if args.workload == "batch":

@@ -173,8 +173,8 @@ void dft7_consts(uint64_t c, int k, double *out) {
 }
 
 struct Tuning {
-    int grp_lds = 8192;   // doubles per strided group tile
-    int mid_max = 4096;   // largest contiguous last group (the mid kernel holds two)
+    int grp_lds = 16384;  // points per strided group tile
+    int mid_max = 8192;   // largest contiguous last group (the mid kernel holds two)
     int cmin = 8;
     int cmax = 64;
 };
@@ -193,7 +193,9 @@ struct PlanEntry {
     DevPlan host;
     DevPlan *dev = nullptr;
     size_t grp_lds_bytes[NBH_MAX_GROUPS] = {};
-    size_t mid_lds_bytes = 0;
+    int grp_threads[NBH_MAX_GROUPS] = {};
+    size_t mid_lds_bytes = 0;  // two operands
+    int mid_threads[2] = {};   // [mul, sqr]
 };
 
 struct Lib {
@@ -423,9 +425,12 @@ int make_schedule(const StageList &L, const Tuning &T, DevPlan &P, PlanEntry &E)
             sd.q = (int32_t)(sd.Q / Gd.Lb);
             sd.q_magic = ((1ull << 32) + (uint64_t)sd.q - 1) / (uint64_t)sd.q;
         }
-        E.grp_lds_bytes[gi] = sizeof(double) * (size_t)Gd.S * (size_t)Gd.C;
+        E.grp_lds_bytes[gi] = sizeof(double) * (size_t)lds_doubles(Gd.S * Gd.C);
+        E.grp_threads[gi] = tile_threads((int64_t)Gd.S * Gd.C);
     }
-    E.mid_lds_bytes = 2 * sizeof(double) * (size_t)P.g[ng - 1].S;
+    E.mid_lds_bytes = 2 * sizeof(double) * (size_t)lds_doubles(P.g[ng - 1].S);
+    E.mid_threads[0] = tile_threads(2 * (int64_t)P.g[ng - 1].S);
+    E.mid_threads[1] = tile_threads((int64_t)P.g[ng - 1].S);
     return NBH_OK;
 }
 
@@ -486,6 +491,7 @@ int size_job(DevJob &J, int64_t bits_a, int64_t bits_b, int64_t *N_out) {
     int64_t N = 0;
     RCCHK(ntt_params(bits_a, bits_b, &bch, &N));
     J.bch = bch;
+    J.bmagic = ~0ull / (uint64_t)bch + 1;  // ceil(2^64 / bch)
     J.nca = (bits_a + bch - 1) / bch;
     J.ncb = (bits_b + bch - 1) / bch;
     J.nconv = J.nca + J.ncb - 1;
@@ -549,17 +555,20 @@ int run_single(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int
     for (int gi = 0; gi + 1 < K; gi++) {
         TileGen tg{gi, nops, H.g[gi].tiles, gi == 0 ? TILE_PACK : 0, 0};
         ProfScope ps(0, g.stream);
-        HIPCHK(launch_group(slot, nullptr, tg, 2 * nops * H.g[gi].tiles, 1, P[0]->grp_lds_bytes[gi], g.stream));
+        HIPCHK(launch_group(slot, nullptr, tg, 2 * nops * H.g[gi].tiles, 1, P[0]->grp_lds_bytes[gi],
+                            P[0]->grp_threads[gi], g.stream));
     }
     {
         TileGen tg{K - 1, 1, H.g[K - 1].tiles, K == 1 ? TILE_PACK : 0, 0};
         ProfScope ps(1, g.stream);
-        HIPCHK(launch_mid(slot, nullptr, tg, 2 * H.g[K - 1].tiles, P[0]->mid_lds_bytes, g.stream));
+        HIPCHK(launch_mid(slot, nullptr, tg, 2 * H.g[K - 1].tiles, P[0]->mid_lds_bytes, P[0]->mid_threads[sqr ? 1 : 0],
+                          g.stream));
     }
     for (int gi = K - 2; gi >= 0; gi--) {
         TileGen tg{gi, 1, H.g[gi].tiles, 0, 0};
         ProfScope ps(2, g.stream);
-        HIPCHK(launch_group(slot, nullptr, tg, 2 * H.g[gi].tiles, 0, P[0]->grp_lds_bytes[gi], g.stream));
+        HIPCHK(launch_group(slot, nullptr, tg, 2 * H.g[gi].tiles, 0, P[0]->grp_lds_bytes[gi], P[0]->grp_threads[gi],
+                            g.stream));
     }
     TileGen ug{0, 1, J.nchunks, 0, 0};
     {
@@ -619,6 +628,7 @@ struct nbh_batch_s {
         int fwd;
         int64_t off, count;
         size_t lds;
+        int threads;
     };
     std::vector<DevJob> jobs;
     DevJob *djobs = nullptr;
@@ -717,29 +727,40 @@ int batch_create(const nbh_job *jobs, int64_t count, nbh_batch_s **out) {
     int maxK = 0;
     for (int64_t i = 0; i < count; i++)
         if (!J[(size_t)i].zero) maxK = std::max(maxK, plan0[(size_t)i]->host.ngroups);
-    auto push_phase = [&](int kind, int fwd, std::vector<std::pair<size_t, Tile>> &list) {
+    // One launch per (phase, thread class): tiles of very different sizes get
+    // their own block shape.  Within a launch, bigger tiles first.
+    struct TileReq {
+        size_t lds;
+        int threads;
+        Tile t;
+    };
+    auto push_phase = [&](int kind, int fwd, std::vector<TileReq> &list) {
         if (list.empty()) return;
-        std::stable_sort(list.begin(), list.end(),
-                         [](const std::pair<size_t, Tile> &x, const std::pair<size_t, Tile> &y) {
-                             return x.first > y.first;
-                         });
-        nbh_batch_s::Phase ph{kind, fwd, (int64_t)tiles.size(), (int64_t)list.size(), 0};
-        for (auto &e : list) {
-            ph.lds = std::max(ph.lds, e.first);
-            tiles.push_back(e.second);
+        std::stable_sort(list.begin(), list.end(), [](const TileReq &x, const TileReq &y) {
+            return x.threads != y.threads ? x.threads > y.threads : x.lds > y.lds;
+        });
+        size_t i = 0;
+        while (i < list.size()) {
+            nbh_batch_s::Phase ph{kind, fwd, (int64_t)tiles.size(), 0, 0, list[i].threads};
+            while (i < list.size() && list[i].threads == ph.threads) {
+                ph.lds = std::max(ph.lds, list[i].lds);
+                tiles.push_back(list[i].t);
+                ph.count++;
+                i++;
+            }
+            if (kind >= 2) ph.lds = 0;
+            B->phases.push_back(ph);
         }
-        if (kind >= 2) ph.lds = 0;
-        B->phases.push_back(ph);
         list.clear();
     };
-    std::vector<std::pair<size_t, Tile>> list;
-    auto add_group_tiles = [&](int64_t i, int gi, int nops, uint8_t flags, size_t lds) {
+    std::vector<TileReq> list;
+    auto add_group_tiles = [&](int64_t i, int gi, int nops, uint8_t flags, size_t lds, int threads) {
         const DevPlan &H = plan0[(size_t)i]->host;
         for (int k = 0; k < 2; k++)
             for (int op = 0; op < nops; op++)
                 for (int64_t t = 0; t < H.g[gi].tiles; t++)
-                    list.push_back({lds, Tile{(uint32_t)i, (uint8_t)k, (uint8_t)op, (uint8_t)gi, flags,
-                                              (uint32_t)t, 0}});
+                    list.push_back({lds, threads,
+                                    Tile{(uint32_t)i, (uint8_t)k, (uint8_t)op, (uint8_t)gi, flags, (uint32_t)t, 0}});
     };
     for (int d = maxK - 1; d >= 1; d--) {  // forward strided groups, farthest from the mid first
         for (int64_t i = 0; i < count; i++) {
@@ -748,14 +769,15 @@ int batch_create(const nbh_job *jobs, int64_t count, nbh_batch_s **out) {
             const int gi = K - 1 - d;
             if (gi < 0) continue;
             add_group_tiles(i, gi, J[(size_t)i].sqr ? 1 : 2, gi == 0 ? TILE_PACK : 0,
-                            plan0[(size_t)i]->grp_lds_bytes[gi]);
+                            plan0[(size_t)i]->grp_lds_bytes[gi], plan0[(size_t)i]->grp_threads[gi]);
         }
         push_phase(0, 1, list);
     }
     for (int64_t i = 0; i < count; i++) {
         if (J[(size_t)i].zero) continue;
         const int K = plan0[(size_t)i]->host.ngroups;
-        add_group_tiles(i, K - 1, 1, K == 1 ? TILE_PACK : 0, plan0[(size_t)i]->mid_lds_bytes);
+        add_group_tiles(i, K - 1, 1, K == 1 ? TILE_PACK : 0, plan0[(size_t)i]->mid_lds_bytes,
+                        plan0[(size_t)i]->mid_threads[J[(size_t)i].sqr ? 1 : 0]);
     }
     push_phase(1, 1, list);
     for (int d = 1; d <= maxK - 1; d++) {
@@ -764,14 +786,14 @@ int batch_create(const nbh_job *jobs, int64_t count, nbh_batch_s **out) {
             const int K = plan0[(size_t)i]->host.ngroups;
             const int gi = K - 1 - d;
             if (gi < 0) continue;
-            add_group_tiles(i, gi, 1, 0, plan0[(size_t)i]->grp_lds_bytes[gi]);
+            add_group_tiles(i, gi, 1, 0, plan0[(size_t)i]->grp_lds_bytes[gi], plan0[(size_t)i]->grp_threads[gi]);
         }
         push_phase(0, 0, list);
     }
     for (int64_t i = 0; i < count; i++)
         for (int64_t c = 0; c < J[(size_t)i].nchunks; c++)
-            list.push_back({(size_t)J[(size_t)i].nconv, Tile{(uint32_t)i, 0, 0, 0, 0, (uint32_t)c, 0}});
-    std::vector<std::pair<size_t, Tile>> list2 = list;
+            list.push_back({0, UNPACK_THREADS, Tile{(uint32_t)i, 0, 0, 0, 0, (uint32_t)c, 0}});
+    std::vector<TileReq> list2 = list;
     push_phase(2, 0, list);
     push_phase(3, 0, list2);
     HIPCHK(hipMalloc((void **)&B->djobs, sizeof(DevJob) * (size_t)count));
@@ -788,8 +810,8 @@ int batch_run(nbh_batch_s *B, hipStream_t s) {
         const Tile *t = B->dtiles + ph.off;
         ProfScope ps(ph.kind == 0 ? (ph.fwd ? 0 : 2) : ph.kind == 1 ? 1 : ph.kind == 2 ? 3 : 4, s);
         switch (ph.kind) {
-        case 0: HIPCHK(launch_group(B->djobs, t, none, ph.count, ph.fwd, ph.lds, s)); break;
-        case 1: HIPCHK(launch_mid(B->djobs, t, none, ph.count, ph.lds, s)); break;
+        case 0: HIPCHK(launch_group(B->djobs, t, none, ph.count, ph.fwd, ph.lds, ph.threads, s)); break;
+        case 1: HIPCHK(launch_mid(B->djobs, t, none, ph.count, ph.lds, ph.threads, s)); break;
         case 2: HIPCHK(launch_unpack(B->djobs, t, none, ph.count, s)); break;
         default: HIPCHK(launch_carry_fix(B->djobs, t, none, ph.count, s)); break;
         }
@@ -830,8 +852,8 @@ int32_t nbh_init(int32_t device) {
     HIPCHK(hipMalloc((void **)&g.slots, sizeof(DevJob) * NSLOTS));
     HIPCHK(set_lds_limit(g.lds_block));
     // keep the default tiles inside the device's per-block LDS
-    while ((size_t)g.tun.mid_max * 16 > g.lds_block && g.tun.mid_max > 256) g.tun.mid_max >>= 1;
-    while ((size_t)g.tun.grp_lds * 8 > g.lds_block && g.tun.grp_lds > 256) g.tun.grp_lds >>= 1;
+    while (16 * (size_t)lds_doubles(g.tun.mid_max) > g.lds_block && g.tun.mid_max > 256) g.tun.mid_max >>= 1;
+    while (8 * (size_t)lds_doubles(g.tun.grp_lds) > g.lds_block && g.tun.grp_lds > 256) g.tun.grp_lds >>= 1;
     g.init = true;
     return NBH_OK;
 }
@@ -959,7 +981,7 @@ int32_t nbh_ntt_fwd_dev(double *x, int64_t N, int32_t prime, void *stream) {
     const DevPlan &H = P->host;
     for (int gi = 0; gi < H.ngroups; gi++) {
         TileGen tg{gi, 1, H.g[gi].tiles, 0, prime};
-        HIPCHK(launch_group(slot, nullptr, tg, H.g[gi].tiles, 1, sizeof(double) * H.g[gi].S * H.g[gi].C, g.stream));
+        HIPCHK(launch_group(slot, nullptr, tg, H.g[gi].tiles, 1, P->grp_lds_bytes[gi], P->grp_threads[gi], g.stream));
     }
     return stream_leave(stream);
 }
@@ -980,7 +1002,7 @@ int32_t nbh_ntt_rev_dev(double *x, int64_t N, int32_t prime, void *stream) {
     const DevPlan &H = P->host;
     for (int gi = H.ngroups - 1; gi >= 0; gi--) {
         TileGen tg{gi, 1, H.g[gi].tiles, 0, prime};
-        HIPCHK(launch_group(slot, nullptr, tg, H.g[gi].tiles, 0, sizeof(double) * H.g[gi].S * H.g[gi].C, g.stream));
+        HIPCHK(launch_group(slot, nullptr, tg, H.g[gi].tiles, 0, P->grp_lds_bytes[gi], P->grp_threads[gi], g.stream));
     }
     return stream_leave(stream);
 }
@@ -1020,6 +1042,7 @@ int32_t nbh_unpack_dev(uint64_t *r, int64_t rn, const double *x1, const double *
     J.rn = rn;
     J.nconv = nconv;
     J.bch = b;
+    J.bmagic = ~0ull / (uint64_t)b + 1;
     J.nchunks = (rn + UNPACK_LPC - 1) / UNPACK_LPC;
     J.plan[0] = P[0]->dev;
     J.plan[1] = P[1]->dev;
@@ -1086,7 +1109,8 @@ int32_t nbh_set_tuning(int32_t grp_lds_doubles, int32_t mid_max, int32_t cmin, i
     RCCHK(ensure_init());
     auto pow2 = [](int32_t x) { return x > 0 && (x & (x - 1)) == 0; };
     if (!pow2(grp_lds_doubles) || !pow2(mid_max) || !pow2(cmin) || !pow2(cmax) || cmin > cmax || mid_max < 4 ||
-        grp_lds_doubles < 16 || (size_t)grp_lds_doubles * 8 > g.lds_block || (size_t)mid_max * 16 > g.lds_block)
+        grp_lds_doubles < 16 || mid_max > (1 << 16) || grp_lds_doubles > (1 << 16) ||
+        8 * (size_t)lds_doubles(grp_lds_doubles) > g.lds_block || 16 * (size_t)lds_doubles(mid_max) > g.lds_block)
         return fail(NBH_ERR_ARG, "tuning outside the device limits");
     g.tun.grp_lds = grp_lds_doubles;
     g.tun.mid_max = mid_max;

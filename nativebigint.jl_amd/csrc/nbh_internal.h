@@ -14,6 +14,12 @@
 
 #include <stdint.h>
 
+#if defined(__HIPCC__)
+#define NBH_HDI __host__ __device__ __forceinline__
+#else
+#define NBH_HDI inline
+#endif
+
 #define NBH_MAX_STAGES 24
 #define NBH_MAX_GROUPS 6
 
@@ -60,6 +66,7 @@ struct DevJob {
     uint32_t *chunk_flags;        // unpack chunk carry summaries
     int64_t nchunks;
     int32_t zero, pad;            // zero: an operand is zero -> r = 0
+    uint64_t bmagic;              // ceil(2^64 / bch): floor(x / bch) = umulhi(x, bmagic) for x < 2^57
 };
 
 // One unit of work for a launch: a tile of a group of one transform.
@@ -81,6 +88,10 @@ struct TileGen {
     int32_t flags;
     int32_t prime0;
 };
+
+// Doubles of LDS that an n-point tile occupies with one pad double per 16
+// (the kernels index LDS as i + i/16).
+NBH_HDI constexpr int lds_doubles(int n) { return n + (n >> 4) + 1; }
 
 // Unpack geometry: each thread owns UNPACK_LPT consecutive output limbs.
 constexpr int UNPACK_THREADS = 256;

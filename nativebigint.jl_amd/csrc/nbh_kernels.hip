@@ -103,89 +103,223 @@ __device__ __forceinline__ void dft7(double *y, const double *rot) {
 }
 
 // ---------------------------------------------------------------------------
-// One stage over an LDS tile.  The tile holds S rows x C columns (row-major,
-// lds[u*C + c]); the stage's butterfly legs are q rows apart inside blocks of
-// m*q rows, and the twiddle index of a butterfly is jj*Lb + jcol0 + c.
-// Forward: combine, reduce leg 0, twiddle legs 1..m-1 (src/fpntt.jl:315-352,
-// 384-440, 494-535, 645-701).  Reverse: reduce + twiddle, then combine; the
-// radix-5/7 reverse re-reduces leg 0.  Radix 2 (src/fpntt.jl:630-638) is the
-// same in both directions.
+// Uniform loads.  Plan, job and tile records live in global memory that the
+// kernels also write through other pointers, so the compiler cannot prove
+// their loads uniform.  readfirstlane moves them to scalar registers: every
+// branch on a radix or a size becomes a scalar branch, and the index math
+// derived from them runs on the scalar unit instead of the FP64 VALU.
+
+__device__ __forceinline__ int32_t uni(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint32_t uniu(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)v); }
+__device__ __forceinline__ int64_t uni64(int64_t v) {
+    const uint32_t lo = uniu((uint32_t)v), hi = uniu((uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ double unid(double v) {
+    return __builtin_bit_cast(double, uni64(__builtin_bit_cast(int64_t, v)));
+}
+template <class T> __device__ __forceinline__ T *unip(T *p) { return (T *)uni64((int64_t)p); }
+
+// LDS index with one pad double per 16 (see lds_doubles in nbh_internal.h):
+// butterfly legs 16*k doubles apart and the lane strides of the last passes
+// land in distinct banks.
+__device__ __forceinline__ int pidx(int i) { return i + (i >> 4); }
+
+// One stage's descriptor in scalar registers.
+struct StageU {
+    int m, q;
+    uint64_t qmagic;
+    int64_t Q;
+    const double *tw;
+    double rot[8];
+};
+
+__device__ __forceinline__ StageU stage_u(const StageDesc *sd, const double *twbase) {
+    StageU s;
+    s.m = uni(sd->m);
+    s.q = uni(sd->q);
+    s.qmagic = (uint64_t)uni64((int64_t)sd->q_magic);
+    s.Q = uni64(sd->Q);
+    s.tw = twbase + uni64(sd->tw_off);
+#pragma unroll
+    for (int i = 0; i < 8; i++) s.rot[i] = unid(sd->rot[i]);
+    return s;
+}
+
+// Radix-4 butterfly with its twiddles (src/fpntt.jl:645-701): forward
+// combines then reduces leg 0 and twiddles legs 1..3; reverse twiddles first.
+template <int K, bool FWD>
+__device__ __forceinline__ void bfly4(double &y0, double &y1, double &y2, double &y3, double wi,
+                                      const double *__restrict__ tw, int64_t Q, int64_t j) {
+    const double w1 = tw[j], w2 = tw[Q + j], w3 = tw[2 * Q + j];
+    if (FWD) dft4<K>(y0, y1, y2, y3, wi);
+    y0 = fp_reduce<K>(y0);
+    y1 = fp_mulmod3<K>(y1, w1);
+    y2 = fp_mulmod3<K>(y2, w2);
+    y3 = fp_mulmod3<K>(y3, w3);
+    if (!FWD) dft4<K>(y0, y1, y2, y3, wi);
+}
+
+// Tile geometry shared by every pass: S rows x C columns, legs q rows apart,
+// twiddle index of a butterfly = jj*Lb + jcol0 + c.  A pass may run over two
+// tiles (the two operands of the mid kernel) in one loop.
+struct TileGeo {
+    int S, logC;
+    int64_t Lb, jcol0;
+};
+
+// Two consecutive radix-4 stages s (stride q0 = 4 q1) and s+1 (stride q1) on
+// 16 points per thread held in registers: the sub-problem of rows
+// ob*16 q1 + jj + a q1 + b q0, a, b in 0..3.  Stage s's butterflies run over
+// b (twiddle index jt + a Q1), stage s+1's over a (twiddle index jt); the
+// reverse runs them in the opposite order.  Exactly the butterflies of two
+// fp_radix4! sweeps, one LDS round trip instead of two.
+template <int K, bool FWD>
+__device__ void pass44(double *__restrict__ x0, double *__restrict__ x1, int narr, const StageU &s0,
+                       const StageU &s1, const TileGeo &g) {
+    const int q1 = s1.q;
+    const int logq1 = __builtin_ctz((unsigned)q1);
+    const int q0 = s0.q;
+    const int C = 1 << g.logC;
+    const int nsub = (g.S >> 4) << g.logC;
+    for (int sg = threadIdx.x; sg < nsub * narr; sg += blockDim.x) {
+        double *__restrict__ x = sg < nsub ? x0 : x1;
+        const int sl = sg < nsub ? sg : sg - nsub;
+        const int c = sl & (C - 1);
+        const int r = sl >> g.logC;
+        const int jj = r & (q1 - 1);
+        const int row0 = ((r >> logq1) << 4) * q1 + jj;
+        double y[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) y[a][b] = x[pidx(((row0 + a * q1 + b * q0) << g.logC) + c)];
+        const int64_t jt = (int64_t)jj * g.Lb + g.jcol0 + c;
+        if (FWD) {
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+                bfly4<K, true>(y[a][0], y[a][1], y[a][2], y[a][3], s0.rot[0], s0.tw, s0.Q, jt + a * s1.Q);
+#pragma unroll
+            for (int b = 0; b < 4; b++) bfly4<K, true>(y[0][b], y[1][b], y[2][b], y[3][b], s1.rot[0], s1.tw, s1.Q, jt);
+        } else {
+#pragma unroll
+            for (int b = 0; b < 4; b++)
+                bfly4<K, false>(y[0][b], y[1][b], y[2][b], y[3][b], s1.rot[0], s1.tw, s1.Q, jt);
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+                bfly4<K, false>(y[a][0], y[a][1], y[a][2], y[a][3], s0.rot[0], s0.tw, s0.Q, jt + a * s1.Q);
+        }
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) x[pidx(((row0 + a * q1 + b * q0) << g.logC) + c)] = y[a][b];
+    }
+}
+
+// One stage of radix M over the tile(s) (src/fpntt.jl:315-352, 384-440,
+// 494-535, 630-638, 645-701): forward = combine, reduce leg 0, twiddle legs
+// 1..M-1; reverse = reduce + twiddle, then combine (radix 5/7 re-reduce leg
+// 0); radix 2 is twiddle-free and the same both ways.
+template <int K, bool FWD, int M>
+__device__ void pass1(double *__restrict__ x0, double *__restrict__ x1, int narr, const StageU &s,
+                      const TileGeo &g) {
+    const int q = s.q;
+    const int C = 1 << g.logC;
+    const int nb = (g.S / M) << g.logC;
+    const int strideC = q << g.logC;
+    for (int bg = threadIdx.x; bg < nb * narr; bg += blockDim.x) {
+        double *__restrict__ x = bg < nb ? x0 : x1;
+        const int beta = bg < nb ? bg : bg - nb;
+        const int c = beta & (C - 1);
+        const uint32_t r = (uint32_t)beta >> g.logC;
+        const uint32_t ob = (uint32_t)(((uint64_t)r * s.qmagic) >> 32);
+        const uint32_t jj = r - ob * (uint32_t)q;
+        const int base = ((int)(ob * (uint32_t)(M * q) + jj) << g.logC) + c;
+        const int64_t j = (int64_t)jj * g.Lb + g.jcol0 + c;
+        double y[M];
+#pragma unroll
+        for (int t = 0; t < M; t++) y[t] = x[pidx(base + t * strideC)];
+        if (M == 2) {
+            const double u = y[0], v = y[1];
+            y[0] = fp_reduce<K>(u + v);
+            y[1] = fp_reduce<K>(u - v);
+        } else if (M == 4) {
+            bfly4<K, FWD>(y[0], y[1], y[2], y[3], s.rot[0], s.tw, s.Q, j);
+        } else if (M == 3) {
+            const double w1 = s.tw[j], w2 = s.tw[s.Q + j];
+            if (FWD) dft3<K>(y[0], y[1], y[2], s.rot[0]);
+            y[0] = fp_reduce<K>(y[0]);
+            y[1] = fp_mulmod3<K>(y[1], w1);
+            y[2] = fp_mulmod3<K>(y[2], w2);
+            if (!FWD) dft3<K>(y[0], y[1], y[2], s.rot[0]);
+        } else if (M == 5) {
+            if (FWD) dft5<K>(y, s.rot);
+            y[0] = fp_reduce<K>(y[0]);
+#pragma unroll
+            for (int t = 1; t < 5; t++) y[t] = fp_mulmod3<K>(y[t], s.tw[(t - 1) * s.Q + j]);
+            if (!FWD) {
+                dft5<K>(y, s.rot);
+                y[0] = fp_reduce<K>(y[0]);
+            }
+        } else {
+            if (FWD) dft7<K, false>(y, s.rot);
+            y[0] = fp_reduce<K>(y[0]);
+#pragma unroll
+            for (int t = 1; t < 7; t++) y[t] = fp_mulmod3<K>(y[t], s.tw[(t - 1) * s.Q + j]);
+            if (!FWD) {
+                dft7<K, true>(y, s.rot);
+                y[0] = fp_reduce<K>(y[0]);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < M; t++) x[pidx(base + t * strideC)] = y[t];
+    }
+}
 
 template <int K, bool FWD>
-__device__ __forceinline__ void stage_lds(double *__restrict__ x, const StageDesc *__restrict__ sd,
-                                          int S, int logC, int64_t Lb, int64_t jcol0,
-                                          const double *__restrict__ twbase) {
-    const int m = sd->m;
-    const int q = sd->q;
-    const uint64_t qM = sd->q_magic;
-    const int64_t Q = sd->Q;
-    const int C = 1 << logC;
-    const int nb = (S / m) << logC;
-    const int strideC = q << logC;
-    const int blockRows = m * q;
-    const double *__restrict__ tw = twbase + sd->tw_off;
-    double rot[8];
-#pragma unroll
-    for (int i = 0; i < 8; i++) rot[i] = sd->rot[i];
+__device__ __forceinline__ void pass_one(double *x0, double *x1, int narr, const StageU &s, const TileGeo &g) {
+    switch (s.m) {
+    case 4: pass1<K, FWD, 4>(x0, x1, narr, s, g); break;
+    case 2: pass1<K, FWD, 2>(x0, x1, narr, s, g); break;
+    case 3: pass1<K, FWD, 3>(x0, x1, narr, s, g); break;
+    case 5: pass1<K, FWD, 5>(x0, x1, narr, s, g); break;
+    default: pass1<K, FWD, 7>(x0, x1, narr, s, g); break;
+    }
+}
 
-    for (int beta = threadIdx.x; beta < nb; beta += blockDim.x) {
-        const int c = beta & (C - 1);
-        const uint32_t r = (uint32_t)beta >> logC;
-        const uint32_t ob = (uint32_t)(((uint64_t)r * qM) >> 32);
-        const uint32_t jj = r - ob * (uint32_t)q;
-        const int base = ((int)(ob * blockRows + jj) << logC) + c;
-        const int64_t j = (int64_t)jj * Lb + jcol0 + c;
-        if (m == 4) {
-            double y0 = x[base], y1 = x[base + strideC], y2 = x[base + 2 * strideC], y3 = x[base + 3 * strideC];
-            const double w1 = tw[j], w2 = tw[Q + j], w3 = tw[2 * Q + j];
-            if (FWD) dft4<K>(y0, y1, y2, y3, rot[0]);
-            y0 = fp_reduce<K>(y0);
-            y1 = fp_mulmod3<K>(y1, w1);
-            y2 = fp_mulmod3<K>(y2, w2);
-            y3 = fp_mulmod3<K>(y3, w3);
-            if (!FWD) dft4<K>(y0, y1, y2, y3, rot[0]);
-            x[base] = y0; x[base + strideC] = y1; x[base + 2 * strideC] = y2; x[base + 3 * strideC] = y3;
-        } else if (m == 2) {
-            const double u = x[base], v = x[base + strideC];
-            x[base] = fp_reduce<K>(u + v);
-            x[base + strideC] = fp_reduce<K>(u - v);
-        } else if (m == 3) {
-            double y0 = x[base], y1 = x[base + strideC], y2 = x[base + 2 * strideC];
-            const double w1 = tw[j], w2 = tw[Q + j];
-            if (FWD) dft3<K>(y0, y1, y2, rot[0]);
-            y0 = fp_reduce<K>(y0);
-            y1 = fp_mulmod3<K>(y1, w1);
-            y2 = fp_mulmod3<K>(y2, w2);
-            if (!FWD) dft3<K>(y0, y1, y2, rot[0]);
-            x[base] = y0; x[base + strideC] = y1; x[base + 2 * strideC] = y2;
-        } else if (m == 5) {
-            double y[5];
-#pragma unroll
-            for (int t = 0; t < 5; t++) y[t] = x[base + t * strideC];
-            if (FWD) dft5<K>(y, rot);
-            y[0] = fp_reduce<K>(y[0]);
-#pragma unroll
-            for (int t = 1; t < 5; t++) y[t] = fp_mulmod3<K>(y[t], tw[(t - 1) * Q + j]);
-            if (!FWD) {
-                dft5<K>(y, rot);
-                y[0] = fp_reduce<K>(y[0]);
+// Stages [s0, s1) of a plan over the LDS tile(s), radix-4 pairs fused.  Any
+// grouping of consecutive stages into passes applies the same butterflies.
+template <int K, bool FWD>
+__device__ void run_stages(double *x0, double *x1, int narr, const DevPlan *__restrict__ P, int s0, int s1,
+                           const TileGeo &g) {
+    const double *tw = unip(P->tw);
+    if (FWD) {
+        int s = s0;
+        while (s < s1) {
+            const StageU a = stage_u(&P->st[s], tw);
+            if (a.m == 4 && s + 1 < s1 && uni(P->st[s + 1].m) == 4) {
+                const StageU b = stage_u(&P->st[s + 1], tw);
+                pass44<K, true>(x0, x1, narr, a, b, g);
+                s += 2;
+            } else {
+                pass_one<K, true>(x0, x1, narr, a, g);
+                s += 1;
             }
-#pragma unroll
-            for (int t = 0; t < 5; t++) x[base + t * strideC] = y[t];
-        } else {  // m == 7
-            double y[7];
-#pragma unroll
-            for (int t = 0; t < 7; t++) y[t] = x[base + t * strideC];
-            if (FWD) dft7<K, false>(y, rot);
-            y[0] = fp_reduce<K>(y[0]);
-#pragma unroll
-            for (int t = 1; t < 7; t++) y[t] = fp_mulmod3<K>(y[t], tw[(t - 1) * Q + j]);
-            if (!FWD) {
-                dft7<K, true>(y, rot);
-                y[0] = fp_reduce<K>(y[0]);
+            __syncthreads();
+        }
+    } else {
+        int s = s1 - 1;
+        while (s >= s0) {
+            const StageU b = stage_u(&P->st[s], tw);
+            if (b.m == 4 && s - 1 >= s0 && uni(P->st[s - 1].m) == 4) {
+                const StageU a = stage_u(&P->st[s - 1], tw);
+                pass44<K, false>(x0, x1, narr, a, b, g);
+                s -= 2;
+            } else {
+                pass_one<K, false>(x0, x1, narr, b, g);
+                s -= 1;
             }
-#pragma unroll
-            for (int t = 0; t < 7; t++) x[base + t * strideC] = y[t];
+            __syncthreads();
         }
     }
 }
@@ -205,8 +339,19 @@ __device__ __forceinline__ double pack_chunk(const uint64_t *__restrict__ limbs,
 
 // The tile a block works on: from the launch's tile list, or generated.
 __device__ __forceinline__ Tile fetch_tile(const Tile *__restrict__ tiles, const TileGen &g) {
-    if (tiles) return tiles[blockIdx.x];
     Tile t;
+    if (tiles) {
+        const Tile v = tiles[blockIdx.x];
+        t.job = uniu(v.job);
+        const uint32_t w = uniu(*(const uint32_t *)&v.prime);
+        t.prime = (uint8_t)w;
+        t.op = (uint8_t)(w >> 8);
+        t.group = (uint8_t)(w >> 16);
+        t.flags = (uint8_t)(w >> 24);
+        t.tile = uniu(v.tile);
+        t.pad = 0;
+        return t;
+    }
     int64_t b = blockIdx.x;
     t.job = 0;
     t.tile = (uint32_t)(b % g.per);
@@ -223,51 +368,48 @@ __device__ __forceinline__ Tile fetch_tile(const Tile *__restrict__ tiles, const
 // Group pass: one tile of one group of one transform, forward or inverse.
 
 template <int K>
-__device__ void group_body(const DevJob &J, const Tile &t, bool fwd, double *lds) {
-    const DevPlan *__restrict__ P = J.plan[K];
-    const GroupDesc &G = P->g[t.group];
-    double *__restrict__ x = t.op ? J.xb[K] : J.xa[K];
-    const int logC = G.logC, S = G.S;
-    const int64_t tpr = G.Lb >> logC;
+__device__ void group_body(const DevJob *__restrict__ J, const Tile &t, bool fwd, double *lds) {
+    const DevPlan *__restrict__ P = unip(J->plan[K]);
+    const GroupDesc *G = &P->g[t.group];
+    TileGeo g;
+    g.S = uni(G->S);
+    g.logC = uni(G->logC);
+    g.Lb = uni64(G->Lb);
+    const int64_t La = uni64(G->La);
+    const int s0 = uni(G->s0), s1 = uni(G->s1);
+    double *__restrict__ x = unip(t.op ? J->xb[K] : J->xa[K]);
+    const int64_t tpr = g.Lb >> g.logC;
     const int64_t blk = (int64_t)t.tile / tpr;
     const int64_t jc = (int64_t)t.tile - blk * tpr;
-    const int64_t jcol0 = jc << logC;
-    const int64_t gbase = blk * G.La + jcol0;
-    const int total = S << logC;
-    const int cmask = (1 << logC) - 1;
+    g.jcol0 = jc << g.logC;
+    const int64_t gbase = blk * La + g.jcol0;
+    const int total = g.S << g.logC;
+    const int cmask = (1 << g.logC) - 1;
 
     if (t.flags & TILE_PACK) {
-        const uint64_t *limbs = t.op ? J.b : J.a;
-        const int64_t n = t.op ? J.n : J.m;
-        const int64_t nch = t.op ? J.ncb : J.nca;
+        const uint64_t *limbs = unip(t.op ? J->b : J->a);
+        const int64_t n = uni64(t.op ? J->n : J->m);
+        const int64_t nch = uni64(t.op ? J->ncb : J->nca);
+        const int bch = uni(J->bch);
         for (int e = threadIdx.x; e < total; e += blockDim.x)
-            lds[e] = pack_chunk(limbs, n, J.bch, nch, gbase + (int64_t)(e >> logC) * G.Lb + (e & cmask));
+            lds[pidx(e)] = pack_chunk(limbs, n, bch, nch, gbase + (int64_t)(e >> g.logC) * g.Lb + (e & cmask));
     } else {
         for (int e = threadIdx.x; e < total; e += blockDim.x)
-            lds[e] = x[gbase + (int64_t)(e >> logC) * G.Lb + (e & cmask)];
+            lds[pidx(e)] = x[gbase + (int64_t)(e >> g.logC) * g.Lb + (e & cmask)];
     }
     __syncthreads();
-    if (fwd) {
-        for (int s = G.s0; s < G.s1; s++) {
-            stage_lds<K, true>(lds, &P->st[s], S, logC, G.Lb, jcol0, P->tw);
-            __syncthreads();
-        }
-    } else {
-        for (int s = G.s1 - 1; s >= G.s0; s--) {
-            stage_lds<K, false>(lds, &P->st[s], S, logC, G.Lb, jcol0, P->tw);
-            __syncthreads();
-        }
-    }
+    if (fwd) run_stages<K, true>(lds, nullptr, 1, P, s0, s1, g);
+    else run_stages<K, false>(lds, nullptr, 1, P, s0, s1, g);
     for (int e = threadIdx.x; e < total; e += blockDim.x)
-        x[gbase + (int64_t)(e >> logC) * G.Lb + (e & cmask)] = lds[e];
+        x[gbase + (int64_t)(e >> g.logC) * g.Lb + (e & cmask)] = lds[pidx(e)];
 }
 
-__global__ void __launch_bounds__(512) k_group(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles,
-                                               TileGen gen, int fwd) {
+__global__ void __launch_bounds__(1024) k_group(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles,
+                                                TileGen gen, int fwd) {
     extern __shared__ double lds[];
     const Tile t = fetch_tile(tiles, gen);
-    const DevJob &J = jobs[t.job];
-    if (J.zero) return;
+    const DevJob *J = jobs + t.job;
+    if (uni(J->zero)) return;
     if (t.prime == 0) group_body<0>(J, t, fwd != 0, lds);
     else group_body<1>(J, t, fwd != 0, lds);
 }
@@ -278,49 +420,54 @@ __global__ void __launch_bounds__(512) k_group(const DevJob *__restrict__ jobs, 
 // the pack is fused in too, so the whole transform pair is one kernel.
 
 template <int K>
-__device__ void mid_body(const DevJob &J, const Tile &t, double *lds) {
-    const DevPlan *__restrict__ P = J.plan[K];
-    const GroupDesc &G = P->g[P->ngroups - 1];
-    const int S = G.S;
-    const int64_t gbase = (int64_t)t.tile * G.La;
-    const bool sq = J.sqr != 0;
+__device__ void mid_body(const DevJob *__restrict__ J, const Tile &t, double *lds) {
+    const DevPlan *__restrict__ P = unip(J->plan[K]);
+    const GroupDesc *G = &P->g[uni(P->ngroups) - 1];
+    TileGeo g;
+    g.S = uni(G->S);
+    g.logC = 0;
+    g.Lb = 1;
+    g.jcol0 = 0;
+    const int S = g.S;
+    const int s0 = uni(G->s0), s1 = uni(G->s1);
+    const int64_t gbase = (int64_t)t.tile * uni64(G->La);
+    const bool sq = uni(J->sqr) != 0;
     double *la = lds;
-    double *lb = lds + S;
+    double *lb = lds + lds_doubles(S);
     if (t.flags & TILE_PACK) {
+        const uint64_t *A = unip(J->a), *B = unip(J->b);
+        const int64_t m = uni64(J->m), n = uni64(J->n), nca = uni64(J->nca), ncb = uni64(J->ncb);
+        const int bch = uni(J->bch);
         for (int e = threadIdx.x; e < S; e += blockDim.x) {
-            la[e] = pack_chunk(J.a, J.m, J.bch, J.nca, gbase + e);
-            if (!sq) lb[e] = pack_chunk(J.b, J.n, J.bch, J.ncb, gbase + e);
+            la[pidx(e)] = pack_chunk(A, m, bch, nca, gbase + e);
+            if (!sq) lb[pidx(e)] = pack_chunk(B, n, bch, ncb, gbase + e);
         }
     } else {
-        const double *__restrict__ xa = J.xa[K];
-        const double *__restrict__ xb = J.xb[K];
+        const double *__restrict__ xa = unip(J->xa[K]);
+        const double *__restrict__ xb = unip(J->xb[K]);
         for (int e = threadIdx.x; e < S; e += blockDim.x) {
-            la[e] = xa[gbase + e];
-            if (!sq) lb[e] = xb[gbase + e];
+            la[pidx(e)] = xa[gbase + e];
+            if (!sq) lb[pidx(e)] = xb[gbase + e];
         }
     }
     __syncthreads();
-    for (int s = G.s0; s < G.s1; s++) {
-        stage_lds<K, true>(la, &P->st[s], S, 0, 1, 0, P->tw);
-        if (!sq) stage_lds<K, true>(lb, &P->st[s], S, 0, 1, 0, P->tw);
-        __syncthreads();
+    run_stages<K, true>(la, lb, sq ? 1 : 2, P, s0, s1, g);
+    for (int e = threadIdx.x; e < S; e += blockDim.x) {
+        const int i = pidx(e);
+        la[i] = fp_mulmod3<K>(la[i], sq ? la[i] : lb[i]);
     }
-    for (int e = threadIdx.x; e < S; e += blockDim.x) la[e] = fp_mulmod3<K>(la[e], sq ? la[e] : lb[e]);
     __syncthreads();
-    for (int s = G.s1 - 1; s >= G.s0; s--) {
-        stage_lds<K, false>(la, &P->st[s], S, 0, 1, 0, P->tw);
-        __syncthreads();
-    }
-    double *__restrict__ xo = J.xa[K];
-    for (int e = threadIdx.x; e < S; e += blockDim.x) xo[gbase + e] = la[e];
+    run_stages<K, false>(la, nullptr, 1, P, s0, s1, g);
+    double *__restrict__ xo = unip(J->xa[K]);
+    for (int e = threadIdx.x; e < S; e += blockDim.x) xo[gbase + e] = la[pidx(e)];
 }
 
-__global__ void __launch_bounds__(512) k_mid(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles,
-                                             TileGen gen) {
+__global__ void __launch_bounds__(1024) k_mid(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles,
+                                              TileGen gen) {
     extern __shared__ double lds[];
     const Tile t = fetch_tile(tiles, gen);
-    const DevJob &J = jobs[t.job];
-    if (J.zero) return;
+    const DevJob *J = jobs + t.job;
+    if (uni(J->zero)) return;
     if (t.prime == 0) mid_body<0>(J, t, lds);
     else mid_body<1>(J, t, lds);
 }
@@ -338,15 +485,39 @@ __global__ void __launch_bounds__(512) k_mid(const DevJob *__restrict__ jobs, co
 // generate/propagate prefix (wave ballot + add trick, then waves in LDS),
 // and carries between chunks are resolved by k_carry_fix's look-back.
 
-__device__ __forceinline__ void garner_coeff(const DevJob &J, int64_t i, uint64_t &lo, uint64_t &hi) {
-    if (i >= J.nconv) { lo = 0; hi = 0; return; }
-    const DevPlan *P1 = J.plan[0];
-    const DevPlan *P2 = J.plan[1];
-    const int64_t N = P1->N;
-    const int64_t idx = i == 0 ? 0 : N - i;
-    double v1 = fp_mulmod4<0>(J.xa[0][idx], P1->ninv, P1->ninvp);
+// The unpack's job constants in scalar registers.
+struct UnpackU {
+    const double *x1, *x2;
+    int64_t N, nconv, rn;
+    double n1, n1p, n2, n2p;
+    uint64_t bmagic;
+    int b;
+};
+
+__device__ __forceinline__ UnpackU unpack_u(const DevJob *__restrict__ J) {
+    UnpackU u;
+    const DevPlan *P1 = unip(J->plan[0]);
+    const DevPlan *P2 = unip(J->plan[1]);
+    u.x1 = unip(J->xa[0]);
+    u.x2 = unip(J->xa[1]);
+    u.N = uni64(P1->N);
+    u.nconv = uni64(J->nconv);
+    u.rn = uni64(J->rn);
+    u.n1 = unid(P1->ninv);
+    u.n1p = unid(P1->ninvp);
+    u.n2 = unid(P2->ninv);
+    u.n2p = unid(P2->ninvp);
+    u.bmagic = (uint64_t)uni64((int64_t)J->bmagic);
+    u.b = uni(J->bch);
+    return u;
+}
+
+__device__ __forceinline__ void garner_coeff(const UnpackU &U, int64_t i, uint64_t &lo, uint64_t &hi) {
+    if (i >= U.nconv) { lo = 0; hi = 0; return; }
+    const int64_t idx = i == 0 ? 0 : U.N - i;
+    double v1 = fp_mulmod4<0>(U.x1[idx], U.n1, U.n1p);
     v1 = v1 < 0 ? v1 + P1_D : v1;
-    const double v2 = fp_mulmod4<1>(J.xa[1][idx], P2->ninv, P2->ninvp);
+    const double v2 = fp_mulmod4<1>(U.x2[idx], U.n2, U.n2p);
     double u = fp_mulmod4<1>(v2 - fp_reduce<1>(v1), GARNER_G, GARNER_GP);
     u = u < 0 ? u + P2_D : u;
     const uint64_t c1 = (uint64_t)v1, uu = (uint64_t)u;
@@ -361,16 +532,16 @@ struct Seg {
     uint64_t s0, s1;          // spill into limbs k0+4, k0+5
 };
 
-__device__ __forceinline__ Seg segment_sum(const DevJob &J, int64_t k0) {
-    const int b = J.bch;
-    const int64_t ilo = (64 * k0 + b - 1) / b;
-    const int64_t iend = (64 * (k0 + UNPACK_LPT) + b - 1) / b;
+__device__ __forceinline__ Seg segment_sum(const UnpackU &U, int64_t k0) {
+    const int b = U.b;
+    const int64_t ilo = (int64_t)__umul64hi((uint64_t)(64 * k0 + b - 1), U.bmagic);
+    const int64_t iend = (int64_t)__umul64hi((uint64_t)(64 * (k0 + UNPACK_LPT) + b - 1), U.bmagic);
     int s = (int)(ilo * b - 64 * k0);
     uint64_t w0 = 0, w1 = 0, w2 = 0;
     Seg out{0, 0, 0, 0, 0, 0};
     for (int64_t i = ilo; i < iend; i++) {
         uint64_t lo, hi;
-        garner_coeff(J, i, lo, hi);
+        garner_coeff(U, i, lo, hi);
         const unsigned __int128 a01 = (unsigned __int128)lo << s;
         const unsigned __int128 a12 = (unsigned __int128)hi << s;
         unsigned __int128 acc = (unsigned __int128)w0 + (uint64_t)a01;
@@ -395,22 +566,25 @@ __global__ void __launch_bounds__(UNPACK_THREADS) k_unpack(const DevJob *__restr
     __shared__ uint64_t spill0[UNPACK_THREADS], spill1[UNPACK_THREADS];
     __shared__ uint32_t wsum[UNPACK_THREADS / 64];
     const Tile t = fetch_tile(tiles, gen);
-    const DevJob &J = jobs[t.job];
+    const DevJob *__restrict__ J = jobs + t.job;
     const int tid = threadIdx.x;
     const int64_t k0 = (int64_t)t.tile * UNPACK_LPC + UNPACK_LPT * tid;
-    uint64_t *__restrict__ r = J.r;
-    if (J.zero) {
+    uint64_t *__restrict__ r = unip(J->r);
+    const int64_t rn = uni64(J->rn);
+    uint32_t *__restrict__ flags = unip(J->chunk_flags);
+    if (uni(J->zero)) {
         for (int w = 0; w < UNPACK_LPT; w++)
-            if (k0 + w < J.rn) r[k0 + w] = 0;
-        if (tid == 0) J.chunk_flags[t.tile] = 0;
+            if (k0 + w < rn) r[k0 + w] = 0;
+        if (tid == 0) flags[t.tile] = 0;
         return;
     }
-    const Seg sg = segment_sum(J, k0);
+    const UnpackU U = unpack_u(J);
+    const Seg sg = segment_sum(U, k0);
     spill0[tid] = sg.s0;
     spill1[tid] = sg.s1;
     uint64_t p0 = 0, p1 = 0;
     if (tid == 0 && k0 >= UNPACK_LPT) {
-        const Seg pv = segment_sum(J, k0 - UNPACK_LPT);
+        const Seg pv = segment_sum(U, k0 - UNPACK_LPT);
         p0 = pv.s0;
         p1 = pv.s1;
     }
@@ -457,16 +631,16 @@ __global__ void __launch_bounds__(UNPACK_THREADS) k_unpack(const DevJob *__restr
     q2 += c;
     c = q2 < c;
     q3 += c;
-    if (k0 + 0 < J.rn) r[k0 + 0] = q0;
-    if (k0 + 1 < J.rn) r[k0 + 1] = q1;
-    if (k0 + 2 < J.rn) r[k0 + 2] = q2;
-    if (k0 + 3 < J.rn) r[k0 + 3] = q3;
+    if (k0 + 0 < rn) r[k0 + 0] = q0;
+    if (k0 + 1 < rn) r[k0 + 1] = q1;
+    if (k0 + 2 < rn) r[k0 + 2] = q2;
+    if (k0 + 3 < rn) r[k0 + 3] = q3;
     if (tid == UNPACK_THREADS - 1) {
         // carry out of the chunk with carry-in 0, and whether all limbs propagate
         const uint32_t cg = (G || (Pp && my)) ? CHUNK_G : 0u;
         uint32_t allp = 1;
         for (int w = 0; w < UNPACK_THREADS / 64; w++) allp &= (wsum[w] >> 1);
-        J.chunk_flags[t.tile] = cg | (allp ? CHUNK_P : 0u);
+        flags[t.tile] = cg | (allp ? CHUNK_P : 0u);
     }
 }
 
@@ -477,24 +651,26 @@ __global__ void __launch_bounds__(UNPACK_THREADS) k_carry_fix(const DevJob *__re
                                                               const Tile *__restrict__ tiles, TileGen gen) {
     __shared__ int first;
     const Tile t = fetch_tile(tiles, gen);
-    const DevJob &J = jobs[t.job];
-    if (J.zero || t.tile == 0) return;
+    const DevJob *__restrict__ J = jobs + t.job;
+    if (uni(J->zero) || t.tile == 0) return;
+    const uint32_t *__restrict__ flags = unip(J->chunk_flags);
     int cin = 0;
     for (int64_t ch = (int64_t)t.tile - 1; ch >= 0; ch--) {
-        const uint32_t f = J.chunk_flags[ch];
+        const uint32_t f = uniu(flags[ch]);
         if (f & CHUNK_G) { cin = 1; break; }
         if (!(f & CHUNK_P)) break;
     }
     if (!cin) return;
     const int tid = threadIdx.x;
     const int64_t k0 = (int64_t)t.tile * UNPACK_LPC + UNPACK_LPT * tid;
-    uint64_t *__restrict__ r = J.r;
+    uint64_t *__restrict__ r = unip(J->r);
+    const int64_t rn = uni64(J->rn);
     uint64_t v[UNPACK_LPT];
     bool ones = true;
 #pragma unroll
     for (int w = 0; w < UNPACK_LPT; w++) {
-        v[w] = (k0 + w < J.rn) ? r[k0 + w] : 0;
-        ones = ones && (k0 + w < J.rn) && v[w] == ~0ull;
+        v[w] = (k0 + w < rn) ? r[k0 + w] : 0;
+        ones = ones && (k0 + w < rn) && v[w] == ~0ull;
     }
     if (tid == 0) first = UNPACK_THREADS;
     __syncthreads();
@@ -503,14 +679,14 @@ __global__ void __launch_bounds__(UNPACK_THREADS) k_carry_fix(const DevJob *__re
     if (tid < first) {
 #pragma unroll
         for (int w = 0; w < UNPACK_LPT; w++)
-            if (k0 + w < J.rn) r[k0 + w] = 0;
+            if (k0 + w < rn) r[k0 + w] = 0;
     } else if (tid == first) {
         uint64_t cc = 1;
 #pragma unroll
         for (int w = 0; w < UNPACK_LPT; w++) {
             const uint64_t nv = v[w] + cc;
             cc = nv < cc;
-            if (k0 + w < J.rn) r[k0 + w] = nv;
+            if (k0 + w < rn) r[k0 + w] = nv;
         }
     }
 }
@@ -592,16 +768,16 @@ static inline unsigned grid_for(int64_t n, int threads, int64_t cap = 1 << 16) {
 }
 
 hipError_t launch_group(const DevJob *jobs, const Tile *tiles, const TileGen &gen, int64_t ntiles, int fwd,
-                        size_t lds_bytes, hipStream_t s) {
+                        size_t lds_bytes, int threads, hipStream_t s) {
     if (ntiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_group, dim3((unsigned)ntiles), dim3(GROUP_THREADS), lds_bytes, s, jobs, tiles, gen, fwd);
+    hipLaunchKernelGGL(k_group, dim3((unsigned)ntiles), dim3(threads), lds_bytes, s, jobs, tiles, gen, fwd);
     return hipGetLastError();
 }
 
 hipError_t launch_mid(const DevJob *jobs, const Tile *tiles, const TileGen &gen, int64_t ntiles, size_t lds_bytes,
-                      hipStream_t s) {
+                      int threads, hipStream_t s) {
     if (ntiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_mid, dim3((unsigned)ntiles), dim3(GROUP_THREADS), lds_bytes, s, jobs, tiles, gen);
+    hipLaunchKernelGGL(k_mid, dim3((unsigned)ntiles), dim3(threads), lds_bytes, s, jobs, tiles, gen);
     return hipGetLastError();
 }
 

@@ -7,14 +7,19 @@
 
 namespace nbh {
 
-constexpr int GROUP_THREADS = 512;
+// Threads of an LDS-tiled launch whose largest tile holds `points` points
+// (one register sub-problem of 16 points per thread), 64..1024.
+inline int tile_threads(int64_t points) {
+    int64_t t = (points / 16 + 63) / 64 * 64;
+    return (int)(t < 64 ? 64 : t > 1024 ? 1024 : t);
+}
 
 // Tile-driven launches: `tiles` is a device tile list of ntiles entries, or
 // NULL to generate ntiles tiles of job 0 from `gen`.
 hipError_t launch_group(const DevJob *jobs, const Tile *tiles, const TileGen &gen, int64_t ntiles, int fwd,
-                        size_t lds_bytes, hipStream_t s);
+                        size_t lds_bytes, int threads, hipStream_t s);
 hipError_t launch_mid(const DevJob *jobs, const Tile *tiles, const TileGen &gen, int64_t ntiles, size_t lds_bytes,
-                      hipStream_t s);
+                      int threads, hipStream_t s);
 hipError_t launch_unpack(const DevJob *jobs, const Tile *tiles, const TileGen &gen, int64_t ntiles, hipStream_t s);
 hipError_t launch_carry_fix(const DevJob *jobs, const Tile *tiles, const TileGen &gen, int64_t ntiles,
                             hipStream_t s);
