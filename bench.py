@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-one-per-launch", action="store_true")
+    ap.add_argument("--tuning", default="",
+                    help="grp_lds,mid_max,cmin,cmax schedule knobs (default: the library's)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run a plain device-to-device copy of this many MiB (HBM counter calibration)")
     return ap.parse_args()
@@ -91,7 +93,10 @@ def main():
     from nbhip import workloads as wl
 
     engine.init()
+    if args.tuning:
+        D.set_tuning(*[int(v) for v in args.tuning.split(",")])
     info = D.device_info()
+    info["tuning"] = D.get_tuning()
     log(f"device {info}")
     dev = torch.device("cuda")
     stream = torch.cuda.Stream()

@@ -102,3 +102,5 @@ constexpr uint32_t CHUNK_G = 1;  // carry out with carry-in 0
 constexpr uint32_t CHUNK_P = 2;  // every limb all-ones: carry passes through
 
 }  // namespace nbh
+
+static_assert(sizeof(nbh::Tile) == 16, "kernels read a Tile as four 32-bit words");

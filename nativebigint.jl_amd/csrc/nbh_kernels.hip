@@ -125,6 +125,22 @@ template <class T> __device__ __forceinline__ T *unip(T *p) { return (T *)uni64(
 // land in distinct banks.
 __device__ __forceinline__ int pidx(int i) { return i + (i >> 4); }
 
+// Loads and stores through the global address space: generic (flat) accesses
+// would make every wait cover both the LDS and the memory counters.
+typedef __attribute__((address_space(1))) const double gcdouble;
+typedef __attribute__((address_space(1))) double gdouble;
+typedef __attribute__((address_space(1))) const uint64_t gcu64;
+__device__ __forceinline__ double gld(const double *p, int64_t i) { return ((gcdouble *)p)[i]; }
+__device__ __forceinline__ void gst(double *p, int64_t i, double v) { ((gdouble *)p)[i] = v; }
+__device__ __forceinline__ uint64_t gldu(const uint64_t *p, int64_t i) { return ((gcu64 *)p)[i]; }
+
+// Descriptor records (plans, jobs, tiles) are read-only while any kernel runs;
+// reading them through the constant address space lets the compiler use
+// scalar loads (scalar cache) instead of vector loads after every barrier.
+template <class T> __device__ __forceinline__ const __attribute__((address_space(4))) T *cst(const T *p) {
+    return (const __attribute__((address_space(4))) T *)p;
+}
+
 // One stage's descriptor in scalar registers.
 struct StageU {
     int m, q;
@@ -134,7 +150,8 @@ struct StageU {
     double rot[8];
 };
 
-__device__ __forceinline__ StageU stage_u(const StageDesc *sd, const double *twbase) {
+__device__ __forceinline__ StageU stage_u(const StageDesc *sdp, const double *twbase) {
+    const auto *sd = cst(sdp);
     StageU s;
     s.m = uni(sd->m);
     s.q = uni(sd->q);
@@ -151,7 +168,7 @@ __device__ __forceinline__ StageU stage_u(const StageDesc *sd, const double *twb
 template <int K, bool FWD>
 __device__ __forceinline__ void bfly4(double &y0, double &y1, double &y2, double &y3, double wi,
                                       const double *__restrict__ tw, int64_t Q, int64_t j) {
-    const double w1 = tw[j], w2 = tw[Q + j], w3 = tw[2 * Q + j];
+    const double w1 = gld(tw, j), w2 = gld(tw, Q + j), w3 = gld(tw, 2 * Q + j);
     if (FWD) dft4<K>(y0, y1, y2, y3, wi);
     y0 = fp_reduce<K>(y0);
     y1 = fp_mulmod3<K>(y1, w1);
@@ -246,7 +263,7 @@ __device__ void pass1(double *__restrict__ x0, double *__restrict__ x1, int narr
         } else if (M == 4) {
             bfly4<K, FWD>(y[0], y[1], y[2], y[3], s.rot[0], s.tw, s.Q, j);
         } else if (M == 3) {
-            const double w1 = s.tw[j], w2 = s.tw[s.Q + j];
+            const double w1 = gld(s.tw, j), w2 = gld(s.tw, s.Q + j);
             if (FWD) dft3<K>(y[0], y[1], y[2], s.rot[0]);
             y[0] = fp_reduce<K>(y[0]);
             y[1] = fp_mulmod3<K>(y[1], w1);
@@ -256,7 +273,7 @@ __device__ void pass1(double *__restrict__ x0, double *__restrict__ x1, int narr
             if (FWD) dft5<K>(y, s.rot);
             y[0] = fp_reduce<K>(y[0]);
 #pragma unroll
-            for (int t = 1; t < 5; t++) y[t] = fp_mulmod3<K>(y[t], s.tw[(t - 1) * s.Q + j]);
+            for (int t = 1; t < 5; t++) y[t] = fp_mulmod3<K>(y[t], gld(s.tw, (t - 1) * s.Q + j));
             if (!FWD) {
                 dft5<K>(y, s.rot);
                 y[0] = fp_reduce<K>(y[0]);
@@ -265,7 +282,7 @@ __device__ void pass1(double *__restrict__ x0, double *__restrict__ x1, int narr
             if (FWD) dft7<K, false>(y, s.rot);
             y[0] = fp_reduce<K>(y[0]);
 #pragma unroll
-            for (int t = 1; t < 7; t++) y[t] = fp_mulmod3<K>(y[t], s.tw[(t - 1) * s.Q + j]);
+            for (int t = 1; t < 7; t++) y[t] = fp_mulmod3<K>(y[t], gld(s.tw, (t - 1) * s.Q + j));
             if (!FWD) {
                 dft7<K, true>(y, s.rot);
                 y[0] = fp_reduce<K>(y[0]);
@@ -292,12 +309,12 @@ __device__ __forceinline__ void pass_one(double *x0, double *x1, int narr, const
 template <int K, bool FWD>
 __device__ void run_stages(double *x0, double *x1, int narr, const DevPlan *__restrict__ P, int s0, int s1,
                            const TileGeo &g) {
-    const double *tw = unip(P->tw);
+    const double *tw = unip(cst(P)->tw);
     if (FWD) {
         int s = s0;
         while (s < s1) {
             const StageU a = stage_u(&P->st[s], tw);
-            if (a.m == 4 && s + 1 < s1 && uni(P->st[s + 1].m) == 4) {
+            if (a.m == 4 && s + 1 < s1 && uni(cst(P)->st[s + 1].m) == 4) {
                 const StageU b = stage_u(&P->st[s + 1], tw);
                 pass44<K, true>(x0, x1, narr, a, b, g);
                 s += 2;
@@ -311,7 +328,7 @@ __device__ void run_stages(double *x0, double *x1, int narr, const DevPlan *__re
         int s = s1 - 1;
         while (s >= s0) {
             const StageU b = stage_u(&P->st[s], tw);
-            if (b.m == 4 && s - 1 >= s0 && uni(P->st[s - 1].m) == 4) {
+            if (b.m == 4 && s - 1 >= s0 && uni(cst(P)->st[s - 1].m) == 4) {
                 const StageU a = stage_u(&P->st[s - 1], tw);
                 pass44<K, false>(x0, x1, narr, a, b, g);
                 s -= 2;
@@ -332,8 +349,8 @@ __device__ __forceinline__ double pack_chunk(const uint64_t *__restrict__ limbs,
     const int64_t bit = i * b;
     const int64_t w = bit >> 6;
     const int sh = (int)(bit & 63);
-    uint64_t c = limbs[w] >> sh;
-    if (sh && w + 1 < n) c |= limbs[w + 1] << (64 - sh);
+    uint64_t c = gldu(limbs, w) >> sh;
+    if (sh && w + 1 < n) c |= gldu(limbs, w + 1) << (64 - sh);
     return (double)(c & ((1ull << b) - 1));
 }
 
@@ -341,14 +358,15 @@ __device__ __forceinline__ double pack_chunk(const uint64_t *__restrict__ limbs,
 __device__ __forceinline__ Tile fetch_tile(const Tile *__restrict__ tiles, const TileGen &g) {
     Tile t;
     if (tiles) {
-        const Tile v = tiles[blockIdx.x];
-        t.job = uniu(v.job);
-        const uint32_t w = uniu(*(const uint32_t *)&v.prime);
+        // Tile is four words: job, (prime, op, group, flags), tile, pad
+        const auto *v = cst((const uint32_t *)(tiles + blockIdx.x));
+        t.job = uniu(v[0]);
+        const uint32_t w = uniu(v[1]);
         t.prime = (uint8_t)w;
         t.op = (uint8_t)(w >> 8);
         t.group = (uint8_t)(w >> 16);
         t.flags = (uint8_t)(w >> 24);
-        t.tile = uniu(v.tile);
+        t.tile = uniu(v[2]);
         t.pad = 0;
         return t;
     }
@@ -368,9 +386,10 @@ __device__ __forceinline__ Tile fetch_tile(const Tile *__restrict__ tiles, const
 // Group pass: one tile of one group of one transform, forward or inverse.
 
 template <int K>
-__device__ void group_body(const DevJob *__restrict__ J, const Tile &t, bool fwd, double *lds) {
+__device__ void group_body(const DevJob *__restrict__ Jp, const Tile &t, bool fwd, double *lds) {
+    const auto *J = cst(Jp);
     const DevPlan *__restrict__ P = unip(J->plan[K]);
-    const GroupDesc *G = &P->g[t.group];
+    const auto *G = &cst(P)->g[t.group];
     TileGeo g;
     g.S = uni(G->S);
     g.logC = uni(G->logC);
@@ -395,13 +414,13 @@ __device__ void group_body(const DevJob *__restrict__ J, const Tile &t, bool fwd
             lds[pidx(e)] = pack_chunk(limbs, n, bch, nch, gbase + (int64_t)(e >> g.logC) * g.Lb + (e & cmask));
     } else {
         for (int e = threadIdx.x; e < total; e += blockDim.x)
-            lds[pidx(e)] = x[gbase + (int64_t)(e >> g.logC) * g.Lb + (e & cmask)];
+            lds[pidx(e)] = gld(x, gbase + (int64_t)(e >> g.logC) * g.Lb + (e & cmask));
     }
     __syncthreads();
     if (fwd) run_stages<K, true>(lds, nullptr, 1, P, s0, s1, g);
     else run_stages<K, false>(lds, nullptr, 1, P, s0, s1, g);
     for (int e = threadIdx.x; e < total; e += blockDim.x)
-        x[gbase + (int64_t)(e >> g.logC) * g.Lb + (e & cmask)] = lds[pidx(e)];
+        gst(x, gbase + (int64_t)(e >> g.logC) * g.Lb + (e & cmask), lds[pidx(e)]);
 }
 
 __global__ void __launch_bounds__(1024) k_group(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles,
@@ -409,7 +428,7 @@ __global__ void __launch_bounds__(1024) k_group(const DevJob *__restrict__ jobs,
     extern __shared__ double lds[];
     const Tile t = fetch_tile(tiles, gen);
     const DevJob *J = jobs + t.job;
-    if (uni(J->zero)) return;
+    if (uni(cst(J)->zero)) return;
     if (t.prime == 0) group_body<0>(J, t, fwd != 0, lds);
     else group_body<1>(J, t, fwd != 0, lds);
 }
@@ -420,9 +439,10 @@ __global__ void __launch_bounds__(1024) k_group(const DevJob *__restrict__ jobs,
 // the pack is fused in too, so the whole transform pair is one kernel.
 
 template <int K>
-__device__ void mid_body(const DevJob *__restrict__ J, const Tile &t, double *lds) {
+__device__ void mid_body(const DevJob *__restrict__ Jp, const Tile &t, double *lds) {
+    const auto *J = cst(Jp);
     const DevPlan *__restrict__ P = unip(J->plan[K]);
-    const GroupDesc *G = &P->g[uni(P->ngroups) - 1];
+    const auto *G = &cst(P)->g[uni(cst(P)->ngroups) - 1];
     TileGeo g;
     g.S = uni(G->S);
     g.logC = 0;
@@ -446,8 +466,8 @@ __device__ void mid_body(const DevJob *__restrict__ J, const Tile &t, double *ld
         const double *__restrict__ xa = unip(J->xa[K]);
         const double *__restrict__ xb = unip(J->xb[K]);
         for (int e = threadIdx.x; e < S; e += blockDim.x) {
-            la[pidx(e)] = xa[gbase + e];
-            if (!sq) lb[pidx(e)] = xb[gbase + e];
+            la[pidx(e)] = gld(xa, gbase + e);
+            if (!sq) lb[pidx(e)] = gld(xb, gbase + e);
         }
     }
     __syncthreads();
@@ -459,7 +479,7 @@ __device__ void mid_body(const DevJob *__restrict__ J, const Tile &t, double *ld
     __syncthreads();
     run_stages<K, false>(la, nullptr, 1, P, s0, s1, g);
     double *__restrict__ xo = unip(J->xa[K]);
-    for (int e = threadIdx.x; e < S; e += blockDim.x) xo[gbase + e] = la[pidx(e)];
+    for (int e = threadIdx.x; e < S; e += blockDim.x) gst(xo, gbase + e, la[pidx(e)]);
 }
 
 __global__ void __launch_bounds__(1024) k_mid(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles,
@@ -467,7 +487,7 @@ __global__ void __launch_bounds__(1024) k_mid(const DevJob *__restrict__ jobs, c
     extern __shared__ double lds[];
     const Tile t = fetch_tile(tiles, gen);
     const DevJob *J = jobs + t.job;
-    if (uni(J->zero)) return;
+    if (uni(cst(J)->zero)) return;
     if (t.prime == 0) mid_body<0>(J, t, lds);
     else mid_body<1>(J, t, lds);
 }
@@ -494,10 +514,11 @@ struct UnpackU {
     int b;
 };
 
-__device__ __forceinline__ UnpackU unpack_u(const DevJob *__restrict__ J) {
+__device__ __forceinline__ UnpackU unpack_u(const DevJob *__restrict__ Jp) {
     UnpackU u;
-    const DevPlan *P1 = unip(J->plan[0]);
-    const DevPlan *P2 = unip(J->plan[1]);
+    const auto *J = cst(Jp);
+    const auto *P1 = cst(J->plan[0]);
+    const auto *P2 = cst(J->plan[1]);
     u.x1 = unip(J->xa[0]);
     u.x2 = unip(J->xa[1]);
     u.N = uni64(P1->N);
@@ -515,9 +536,9 @@ __device__ __forceinline__ UnpackU unpack_u(const DevJob *__restrict__ J) {
 __device__ __forceinline__ void garner_coeff(const UnpackU &U, int64_t i, uint64_t &lo, uint64_t &hi) {
     if (i >= U.nconv) { lo = 0; hi = 0; return; }
     const int64_t idx = i == 0 ? 0 : U.N - i;
-    double v1 = fp_mulmod4<0>(U.x1[idx], U.n1, U.n1p);
+    double v1 = fp_mulmod4<0>(gld(U.x1, idx), U.n1, U.n1p);
     v1 = v1 < 0 ? v1 + P1_D : v1;
-    const double v2 = fp_mulmod4<1>(U.x2[idx], U.n2, U.n2p);
+    const double v2 = fp_mulmod4<1>(gld(U.x2, idx), U.n2, U.n2p);
     double u = fp_mulmod4<1>(v2 - fp_reduce<1>(v1), GARNER_G, GARNER_GP);
     u = u < 0 ? u + P2_D : u;
     const uint64_t c1 = (uint64_t)v1, uu = (uint64_t)u;
@@ -566,7 +587,8 @@ __global__ void __launch_bounds__(UNPACK_THREADS) k_unpack(const DevJob *__restr
     __shared__ uint64_t spill0[UNPACK_THREADS], spill1[UNPACK_THREADS];
     __shared__ uint32_t wsum[UNPACK_THREADS / 64];
     const Tile t = fetch_tile(tiles, gen);
-    const DevJob *__restrict__ J = jobs + t.job;
+    const DevJob *__restrict__ Jp = jobs + t.job;
+    const auto *J = cst(Jp);
     const int tid = threadIdx.x;
     const int64_t k0 = (int64_t)t.tile * UNPACK_LPC + UNPACK_LPT * tid;
     uint64_t *__restrict__ r = unip(J->r);
@@ -578,7 +600,7 @@ __global__ void __launch_bounds__(UNPACK_THREADS) k_unpack(const DevJob *__restr
         if (tid == 0) flags[t.tile] = 0;
         return;
     }
-    const UnpackU U = unpack_u(J);
+    const UnpackU U = unpack_u(Jp);
     const Seg sg = segment_sum(U, k0);
     spill0[tid] = sg.s0;
     spill1[tid] = sg.s1;
@@ -651,7 +673,7 @@ __global__ void __launch_bounds__(UNPACK_THREADS) k_carry_fix(const DevJob *__re
                                                               const Tile *__restrict__ tiles, TileGen gen) {
     __shared__ int first;
     const Tile t = fetch_tile(tiles, gen);
-    const DevJob *__restrict__ J = jobs + t.job;
+    const auto *J = cst(jobs + t.job);
     if (uni(J->zero) || t.tile == 0) return;
     const uint32_t *__restrict__ flags = unip(J->chunk_flags);
     int cin = 0;
