@@ -490,6 +490,7 @@ int size_job(DevJob &J, int64_t bits_a, int64_t bits_b, int64_t *N_out) {
     int bch = 0;
     int64_t N = 0;
     RCCHK(ntt_params(bits_a, bits_b, &bch, &N));
+    if (bch < UNPACK_MIN_B) return fail(NBH_ERR_LENGTH, "product too large for device memory (chunk width below 32)");
     J.bch = bch;
     J.bmagic = ~0ull / (uint64_t)bch + 1;  // ceil(2^64 / bch)
     J.nca = (bits_a + bch - 1) / bch;
@@ -1030,7 +1031,7 @@ int32_t nbh_unpack_dev(uint64_t *r, int64_t rn, const double *x1, const double *
                        int32_t b, void *stream) {
     std::lock_guard<std::recursive_mutex> lk(g.mu);
     RCCHK(ensure_init());
-    if (!r || !x1 || !x2 || rn < 1 || nconv < 1 || nconv > N || b < 1 || b > 49)
+    if (!r || !x1 || !x2 || rn < 1 || nconv < 1 || nconv > N || b < UNPACK_MIN_B || b > 49)
         return fail(NBH_ERR_ARG, "bad argument");
     PlanEntry *P[2];
     RCCHK(get_plan(N, 0, &P[0]));

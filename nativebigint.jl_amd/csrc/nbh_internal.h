@@ -97,6 +97,11 @@ NBH_HDI constexpr int lds_doubles(int n) { return n + (n >> 4) + 1; }
 constexpr int UNPACK_THREADS = 256;
 constexpr int UNPACK_LPT = 4;
 constexpr int UNPACK_LPC = UNPACK_THREADS * UNPACK_LPT;  // limbs per chunk
+// Garner values staged per chunk: (LPC + LPT) * 64 / b + 2 with b >= 32, the
+// smallest chunk width of any product whose spectra fit in device memory.
+constexpr int UNPACK_MIN_B = 32;
+constexpr int UNPACK_STAGE = (UNPACK_LPC + UNPACK_LPT) * 64 / UNPACK_MIN_B + 2;
+constexpr int GU = 8;  // coefficients per thread per staging round
 
 constexpr uint32_t CHUNK_G = 1;  // carry out with carry-in 0
 constexpr uint32_t CHUNK_P = 2;  // every limb all-ones: carry passes through

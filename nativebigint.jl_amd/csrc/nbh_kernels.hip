@@ -341,6 +341,9 @@ __device__ void run_stages(double *x0, double *x1, int narr, const DevPlan *__re
     }
 }
 
+// Points each thread moves per round in the tile load loops.
+constexpr int LU = 16;
+
 // Chunk i of an operand: bits [i*b, i*b + b) (src/fpntt.jl:744-768); chunks
 // past nch are the zero fill.  Chunks are < 2^b < p: exact, canonical.
 __device__ __forceinline__ double pack_chunk(const uint64_t *__restrict__ limbs, int64_t n, int b,
@@ -405,16 +408,26 @@ __device__ void group_body(const DevJob *__restrict__ Jp, const Tile &t, bool fw
     const int total = g.S << g.logC;
     const int cmask = (1 << g.logC) - 1;
 
-    if (t.flags & TILE_PACK) {
-        const uint64_t *limbs = unip(t.op ? J->b : J->a);
-        const int64_t n = uni64(t.op ? J->n : J->m);
-        const int64_t nch = uni64(t.op ? J->ncb : J->nca);
-        const int bch = uni(J->bch);
-        for (int e = threadIdx.x; e < total; e += blockDim.x)
-            lds[pidx(e)] = pack_chunk(limbs, n, bch, nch, gbase + (int64_t)(e >> g.logC) * g.Lb + (e & cmask));
-    } else {
-        for (int e = threadIdx.x; e < total; e += blockDim.x)
-            lds[pidx(e)] = gld(x, gbase + (int64_t)(e >> g.logC) * g.Lb + (e & cmask));
+    // Every thread issues all of its loads before its first LDS store so the
+    // load latencies overlap (LU points per thread per round).
+    const bool pack = (t.flags & TILE_PACK) != 0;
+    const uint64_t *limbs = unip(t.op ? J->b : J->a);
+    const int64_t n = uni64(t.op ? J->n : J->m);
+    const int64_t nch = uni64(t.op ? J->ncb : J->nca);
+    const int bch = uni(J->bch);
+    for (int e0 = threadIdx.x; e0 < total; e0 += LU * blockDim.x) {
+        double v[LU];
+#pragma unroll
+        for (int k = 0; k < LU; k++) {
+            const int e = e0 + k * (int)blockDim.x;
+            const int64_t gi = gbase + (int64_t)(e >> g.logC) * g.Lb + (e & cmask);
+            v[k] = e < total ? (pack ? pack_chunk(limbs, n, bch, nch, gi) : gld(x, gi)) : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < LU; k++) {
+            const int e = e0 + k * (int)blockDim.x;
+            if (e < total) lds[pidx(e)] = v[k];
+        }
     }
     __syncthreads();
     if (fwd) run_stages<K, true>(lds, nullptr, 1, P, s0, s1, g);
@@ -454,20 +467,34 @@ __device__ void mid_body(const DevJob *__restrict__ Jp, const Tile &t, double *l
     const bool sq = uni(J->sqr) != 0;
     double *la = lds;
     double *lb = lds + lds_doubles(S);
-    if (t.flags & TILE_PACK) {
+    {
+        // both operands' points (or chunks), all loads of a round issued first
+        const bool pack = (t.flags & TILE_PACK) != 0;
         const uint64_t *A = unip(J->a), *B = unip(J->b);
         const int64_t m = uni64(J->m), n = uni64(J->n), nca = uni64(J->nca), ncb = uni64(J->ncb);
         const int bch = uni(J->bch);
-        for (int e = threadIdx.x; e < S; e += blockDim.x) {
-            la[pidx(e)] = pack_chunk(A, m, bch, nca, gbase + e);
-            if (!sq) lb[pidx(e)] = pack_chunk(B, n, bch, ncb, gbase + e);
-        }
-    } else {
         const double *__restrict__ xa = unip(J->xa[K]);
         const double *__restrict__ xb = unip(J->xb[K]);
-        for (int e = threadIdx.x; e < S; e += blockDim.x) {
-            la[pidx(e)] = gld(xa, gbase + e);
-            if (!sq) lb[pidx(e)] = gld(xb, gbase + e);
+        const int total = sq ? S : 2 * S;
+        for (int e0 = threadIdx.x; e0 < total; e0 += LU * blockDim.x) {
+            double v[LU];
+#pragma unroll
+            for (int k = 0; k < LU; k++) {
+                const int e = e0 + k * (int)blockDim.x;
+                const bool second = e >= S;
+                const int i = second ? e - S : e;
+                double val = 0.0;
+                if (e < total) {
+                    if (pack) val = second ? pack_chunk(B, n, bch, ncb, gbase + i) : pack_chunk(A, m, bch, nca, gbase + i);
+                    else val = gld(second ? xb : xa, gbase + i);
+                }
+                v[k] = val;
+            }
+#pragma unroll
+            for (int k = 0; k < LU; k++) {
+                const int e = e0 + k * (int)blockDim.x;
+                if (e < total) (e >= S ? lb : la)[pidx(e >= S ? e - S : e)] = v[k];
+            }
         }
     }
     __syncthreads();
@@ -533,12 +560,12 @@ __device__ __forceinline__ UnpackU unpack_u(const DevJob *__restrict__ Jp) {
     return u;
 }
 
-__device__ __forceinline__ void garner_coeff(const UnpackU &U, int64_t i, uint64_t &lo, uint64_t &hi) {
-    if (i >= U.nconv) { lo = 0; hi = 0; return; }
-    const int64_t idx = i == 0 ? 0 : U.N - i;
-    double v1 = fp_mulmod4<0>(gld(U.x1, idx), U.n1, U.n1p);
+// Garner pair of one coefficient from its two residues (src/fpntt.jl:806-816),
+// returned as c = c1 + p1*u < 2^100 in two words.
+__device__ __forceinline__ void garner(const UnpackU &U, double x1, double x2, uint64_t &lo, uint64_t &hi) {
+    double v1 = fp_mulmod4<0>(x1, U.n1, U.n1p);
     v1 = v1 < 0 ? v1 + P1_D : v1;
-    const double v2 = fp_mulmod4<1>(gld(U.x2, idx), U.n2, U.n2p);
+    const double v2 = fp_mulmod4<1>(x2, U.n2, U.n2p);
     double u = fp_mulmod4<1>(v2 - fp_reduce<1>(v1), GARNER_G, GARNER_GP);
     u = u < 0 ? u + P2_D : u;
     const uint64_t c1 = (uint64_t)v1, uu = (uint64_t)u;
@@ -553,7 +580,10 @@ struct Seg {
     uint64_t s0, s1;          // spill into limbs k0+4, k0+5
 };
 
-__device__ __forceinline__ Seg segment_sum(const UnpackU &U, int64_t k0) {
+// Sum of the staged coefficients that start in limbs [k0, k0+4), streamed
+// through a 192-bit window as the reference's unpack does.
+__device__ __forceinline__ Seg segment_sum(const UnpackU &U, int64_t k0, int64_t i_first,
+                                           const uint64_t *clo, const uint64_t *chi) {
     const int b = U.b;
     const int64_t ilo = (int64_t)__umul64hi((uint64_t)(64 * k0 + b - 1), U.bmagic);
     const int64_t iend = (int64_t)__umul64hi((uint64_t)(64 * (k0 + UNPACK_LPT) + b - 1), U.bmagic);
@@ -561,8 +591,8 @@ __device__ __forceinline__ Seg segment_sum(const UnpackU &U, int64_t k0) {
     uint64_t w0 = 0, w1 = 0, w2 = 0;
     Seg out{0, 0, 0, 0, 0, 0};
     for (int64_t i = ilo; i < iend; i++) {
-        uint64_t lo, hi;
-        garner_coeff(U, i, lo, hi);
+        const int li = (int)(i - i_first);
+        const uint64_t lo = clo[li], hi = chi[li];
         const unsigned __int128 a01 = (unsigned __int128)lo << s;
         const unsigned __int128 a12 = (unsigned __int128)hi << s;
         unsigned __int128 acc = (unsigned __int128)w0 + (uint64_t)a01;
@@ -582,15 +612,20 @@ __device__ __forceinline__ Seg segment_sum(const UnpackU &U, int64_t k0) {
     return out;
 }
 
+// Each chunk first stages the Garner values of every coefficient starting in
+// its limbs (and in the 4 limbs before, for the first thread's spill) in LDS:
+// coalesced reads of both spectra, each coefficient computed once.
 __global__ void __launch_bounds__(UNPACK_THREADS) k_unpack(const DevJob *__restrict__ jobs,
                                                            const Tile *__restrict__ tiles, TileGen gen) {
+    __shared__ uint64_t clo[UNPACK_STAGE], chi[UNPACK_STAGE];
     __shared__ uint64_t spill0[UNPACK_THREADS], spill1[UNPACK_THREADS];
     __shared__ uint32_t wsum[UNPACK_THREADS / 64];
     const Tile t = fetch_tile(tiles, gen);
     const DevJob *__restrict__ Jp = jobs + t.job;
     const auto *J = cst(Jp);
     const int tid = threadIdx.x;
-    const int64_t k0 = (int64_t)t.tile * UNPACK_LPC + UNPACK_LPT * tid;
+    const int64_t L0 = (int64_t)t.tile * UNPACK_LPC;
+    const int64_t k0 = L0 + UNPACK_LPT * tid;
     uint64_t *__restrict__ r = unip(J->r);
     const int64_t rn = uni64(J->rn);
     uint32_t *__restrict__ flags = unip(J->chunk_flags);
@@ -601,12 +636,39 @@ __global__ void __launch_bounds__(UNPACK_THREADS) k_unpack(const DevJob *__restr
         return;
     }
     const UnpackU U = unpack_u(Jp);
-    const Seg sg = segment_sum(U, k0);
+    const int64_t kstart = L0 >= UNPACK_LPT ? L0 - UNPACK_LPT : 0;
+    const int64_t i_first = (int64_t)__umul64hi((uint64_t)(64 * kstart + U.b - 1), U.bmagic);
+    const int64_t i_end = (int64_t)__umul64hi((uint64_t)(64 * (L0 + UNPACK_LPC) + U.b - 1), U.bmagic);
+    const int cnt = (int)(i_end - i_first);
+    for (int l0 = tid; l0 < cnt; l0 += GU * UNPACK_THREADS) {
+        double a1[GU], a2[GU];
+#pragma unroll
+        for (int k = 0; k < GU; k++) {
+            const int li = l0 + k * UNPACK_THREADS;
+            const int64_t i = i_first + li;
+            const bool valid = li < cnt && i < U.nconv;
+            const int64_t idx = i == 0 ? 0 : U.N - i;
+            a1[k] = valid ? gld(U.x1, idx) : 0.0;
+            a2[k] = valid ? gld(U.x2, idx) : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < GU; k++) {
+            const int li = l0 + k * UNPACK_THREADS;
+            if (li < cnt) {
+                uint64_t lo = 0, hi = 0;
+                if (i_first + li < U.nconv) garner(U, a1[k], a2[k], lo, hi);
+                clo[li] = lo;
+                chi[li] = hi;
+            }
+        }
+    }
+    __syncthreads();
+    const Seg sg = segment_sum(U, k0, i_first, clo, chi);
     spill0[tid] = sg.s0;
     spill1[tid] = sg.s1;
     uint64_t p0 = 0, p1 = 0;
     if (tid == 0 && k0 >= UNPACK_LPT) {
-        const Seg pv = segment_sum(U, k0 - UNPACK_LPT);
+        const Seg pv = segment_sum(U, k0 - UNPACK_LPT, i_first, clo, chi);
         p0 = pv.s0;
         p1 = pv.s1;
     }
