@@ -194,8 +194,8 @@ struct PlanEntry {
     DevPlan *dev = nullptr;
     size_t grp_lds_bytes[NBH_MAX_GROUPS] = {};
     int grp_threads[NBH_MAX_GROUPS] = {};
-    size_t mid_lds_bytes = 0;  // two operands
-    int mid_threads[2] = {};   // [mul, sqr]
+    size_t mid_lds_bytes[2] = {};  // [mul (two operands), sqr (one)]
+    int mid_threads[2] = {};       // [mul, sqr]
 };
 
 struct Lib {
@@ -218,6 +218,7 @@ struct Lib {
     size_t io_size = 0;
     DevJob *slots = nullptr;
     int slot_next = 0;
+    hipStream_t side[8] = {};  // batch phases fork onto these (default priority)
     // per-kernel event timing
     bool prof = false;
     std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> prof_open;
@@ -256,6 +257,7 @@ struct ProfScope {
     }
 };
 constexpr int NSLOTS = 256;
+constexpr int NSIDE = 4;
 
 int ensure_init() {
     if (g.init) return NBH_OK;
@@ -428,7 +430,8 @@ int make_schedule(const StageList &L, const Tuning &T, DevPlan &P, PlanEntry &E)
         E.grp_lds_bytes[gi] = sizeof(double) * (size_t)lds_doubles(Gd.S * Gd.C);
         E.grp_threads[gi] = tile_threads((int64_t)Gd.S * Gd.C);
     }
-    E.mid_lds_bytes = 2 * sizeof(double) * (size_t)lds_doubles(P.g[ng - 1].S);
+    E.mid_lds_bytes[0] = 2 * sizeof(double) * (size_t)lds_doubles(P.g[ng - 1].S);
+    E.mid_lds_bytes[1] = sizeof(double) * (size_t)lds_doubles(P.g[ng - 1].S);
     E.mid_threads[0] = tile_threads(2 * (int64_t)P.g[ng - 1].S);
     E.mid_threads[1] = tile_threads((int64_t)P.g[ng - 1].S);
     return NBH_OK;
@@ -562,8 +565,8 @@ int run_single(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int
     {
         TileGen tg{K - 1, 1, H.g[K - 1].tiles, K == 1 ? TILE_PACK : 0, 0};
         ProfScope ps(1, g.stream);
-        HIPCHK(launch_mid(slot, nullptr, tg, 2 * H.g[K - 1].tiles, P[0]->mid_lds_bytes, P[0]->mid_threads[sqr ? 1 : 0],
-                          g.stream));
+        HIPCHK(launch_mid(slot, nullptr, tg, 2 * H.g[K - 1].tiles, P[0]->mid_lds_bytes[sqr ? 1 : 0],
+                          P[0]->mid_threads[sqr ? 1 : 0], g.stream));
     }
     for (int gi = K - 2; gi >= 0; gi--) {
         TileGen tg{gi, 1, H.g[gi].tiles, 0, 0};
@@ -630,12 +633,15 @@ struct nbh_batch_s {
         int64_t off, count;
         size_t lds;
         int threads;
+        int step;        // launches of one step are independent of each other
     };
     std::vector<DevJob> jobs;
     DevJob *djobs = nullptr;
     Tile *dtiles = nullptr;
     void *ws = nullptr;
     std::vector<Phase> phases;
+    int nsteps = 0;
+    std::vector<hipEvent_t> ev;  // fork event per step + one join event per launch
 };
 
 namespace {
@@ -645,6 +651,7 @@ void batch_free(nbh_batch_s *B) {
     if (B->djobs) (void)hipFree(B->djobs);
     if (B->dtiles) (void)hipFree(B->dtiles);
     if (B->ws) (void)hipFree(B->ws);
+    for (auto e : B->ev) (void)hipEventDestroy(e);
     delete B;
 }
 
@@ -741,10 +748,10 @@ int batch_create(const nbh_job *jobs, int64_t count, nbh_batch_s **out) {
             return x.threads != y.threads ? x.threads > y.threads : x.lds > y.lds;
         });
         size_t i = 0;
+        const int step = B->nsteps++;
         while (i < list.size()) {
-            nbh_batch_s::Phase ph{kind, fwd, (int64_t)tiles.size(), 0, 0, list[i].threads};
-            while (i < list.size() && list[i].threads == ph.threads) {
-                ph.lds = std::max(ph.lds, list[i].lds);
+            nbh_batch_s::Phase ph{kind, fwd, (int64_t)tiles.size(), 0, list[i].lds, list[i].threads, step};
+            while (i < list.size() && list[i].threads == ph.threads && list[i].lds == ph.lds) {
                 tiles.push_back(list[i].t);
                 ph.count++;
                 i++;
@@ -777,7 +784,7 @@ int batch_create(const nbh_job *jobs, int64_t count, nbh_batch_s **out) {
     for (int64_t i = 0; i < count; i++) {
         if (J[(size_t)i].zero) continue;
         const int K = plan0[(size_t)i]->host.ngroups;
-        add_group_tiles(i, K - 1, 1, K == 1 ? TILE_PACK : 0, plan0[(size_t)i]->mid_lds_bytes,
+        add_group_tiles(i, K - 1, 1, K == 1 ? TILE_PACK : 0, plan0[(size_t)i]->mid_lds_bytes[J[(size_t)i].sqr ? 1 : 0],
                         plan0[(size_t)i]->mid_threads[J[(size_t)i].sqr ? 1 : 0]);
     }
     push_phase(1, 1, list);
@@ -801,21 +808,48 @@ int batch_create(const nbh_job *jobs, int64_t count, nbh_batch_s **out) {
     HIPCHK(hipMalloc((void **)&B->dtiles, sizeof(Tile) * std::max<size_t>(tiles.size(), 1)));
     HIPCHK(hipMemcpy(B->djobs, J.data(), sizeof(DevJob) * (size_t)count, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(B->dtiles, tiles.data(), sizeof(Tile) * tiles.size(), hipMemcpyHostToDevice));
+    B->ev.resize((size_t)B->nsteps + B->phases.size(), nullptr);
+    for (auto &e : B->ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     *out = B.release();
     return NBH_OK;
 }
 
-int batch_run(nbh_batch_s *B, hipStream_t s) {
+int launch_phase(nbh_batch_s *B, const nbh_batch_s::Phase &ph, hipStream_t s) {
     const TileGen none{0, 1, 1, 0, 0};
-    for (const auto &ph : B->phases) {
-        const Tile *t = B->dtiles + ph.off;
-        ProfScope ps(ph.kind == 0 ? (ph.fwd ? 0 : 2) : ph.kind == 1 ? 1 : ph.kind == 2 ? 3 : 4, s);
-        switch (ph.kind) {
-        case 0: HIPCHK(launch_group(B->djobs, t, none, ph.count, ph.fwd, ph.lds, ph.threads, s)); break;
-        case 1: HIPCHK(launch_mid(B->djobs, t, none, ph.count, ph.lds, ph.threads, s)); break;
-        case 2: HIPCHK(launch_unpack(B->djobs, t, none, ph.count, s)); break;
-        default: HIPCHK(launch_carry_fix(B->djobs, t, none, ph.count, s)); break;
+    const Tile *t = B->dtiles + ph.off;
+    ProfScope ps(ph.kind == 0 ? (ph.fwd ? 0 : 2) : ph.kind == 1 ? 1 : ph.kind == 2 ? 3 : 4, s);
+    switch (ph.kind) {
+    case 0: HIPCHK(launch_group(B->djobs, t, none, ph.count, ph.fwd, ph.lds, ph.threads, s)); break;
+    case 1: HIPCHK(launch_mid(B->djobs, t, none, ph.count, ph.lds, ph.threads, s)); break;
+    case 2: HIPCHK(launch_unpack(B->djobs, t, none, ph.count, s)); break;
+    default: HIPCHK(launch_carry_fix(B->djobs, t, none, ph.count, s)); break;
+    }
+    return NBH_OK;
+}
+
+// A step's launches (one per tile shape) are independent: they fork onto the
+// library's side streams after an event on `s` and join back before the next
+// step, so small shape classes run beside the big ones instead of after them.
+int batch_run(nbh_batch_s *B, hipStream_t s) {
+    size_t i = 0;
+    while (i < B->phases.size()) {
+        size_t j = i;
+        while (j < B->phases.size() && B->phases[j].step == B->phases[i].step) j++;
+        if (j - i == 1 || NSIDE == 0) {
+            for (size_t k = i; k < j; k++) RCCHK(launch_phase(B, B->phases[k], s));
+        } else {
+            hipEvent_t fork = B->ev[(size_t)B->phases[i].step];
+            HIPCHK(hipEventRecord(fork, s));
+            for (size_t k = i; k < j; k++) {
+                hipStream_t side = g.side[(k - i) % NSIDE];
+                HIPCHK(hipStreamWaitEvent(side, fork, 0));
+                RCCHK(launch_phase(B, B->phases[k], side));
+                hipEvent_t done = B->ev[(size_t)B->nsteps + k];
+                HIPCHK(hipEventRecord(done, side));
+                HIPCHK(hipStreamWaitEvent(s, done, 0));
+            }
         }
+        i = j;
     }
     return NBH_OK;
 }
@@ -851,6 +885,7 @@ int32_t nbh_init(int32_t device) {
     HIPCHK(hipEventCreateWithFlags(&g.ev_in, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&g.ev_out, hipEventDisableTiming));
     HIPCHK(hipMalloc((void **)&g.slots, sizeof(DevJob) * NSLOTS));
+    for (int i = 0; i < NSIDE; i++) HIPCHK(hipStreamCreateWithFlags(&g.side[i], hipStreamNonBlocking));
     HIPCHK(set_lds_limit(g.lds_block));
     // keep the default tiles inside the device's per-block LDS
     while (16 * (size_t)lds_doubles(g.tun.mid_max) > g.lds_block && g.tun.mid_max > 256) g.tun.mid_max >>= 1;
@@ -873,6 +908,7 @@ void nbh_shutdown(void) {
     g.ws = g.io = nullptr;
     g.ws_size = g.io_size = 0;
     g.slots = nullptr;
+    for (int i = 0; i < NSIDE; i++) (void)hipStreamDestroy(g.side[i]);
     (void)hipEventDestroy(g.ev_in);
     (void)hipEventDestroy(g.ev_out);
     (void)hipStreamDestroy(g.stream);

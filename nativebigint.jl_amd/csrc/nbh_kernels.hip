@@ -499,9 +499,17 @@ __device__ void mid_body(const DevJob *__restrict__ Jp, const Tile &t, double *l
     }
     __syncthreads();
     run_stages<K, true>(la, lb, sq ? 1 : 2, P, s0, s1, g);
-    for (int e = threadIdx.x; e < S; e += blockDim.x) {
-        const int i = pidx(e);
-        la[i] = fp_mulmod3<K>(la[i], sq ? la[i] : lb[i]);
+    // squares allocate one array: lb is never touched for them
+    if (sq) {
+        for (int e = threadIdx.x; e < S; e += blockDim.x) {
+            const int i = pidx(e);
+            la[i] = fp_mulmod3<K>(la[i], la[i]);
+        }
+    } else {
+        for (int e = threadIdx.x; e < S; e += blockDim.x) {
+            const int i = pidx(e);
+            la[i] = fp_mulmod3<K>(la[i], lb[i]);
+        }
     }
     __syncthreads();
     run_stages<K, false>(la, nullptr, 1, P, s0, s1, g);

@@ -63,6 +63,7 @@ def main():
     ap.add_argument("db")
     ap.add_argument("--title", default="rocprofv3 summary")
     ap.add_argument("--command", default="")
+    ap.add_argument("--dispatches", default="", help="also list every dispatch of kernels matching this")
     a = ap.parse_args()
     cur = sqlite3.connect(a.db).cursor()
     t = tables(cur)
@@ -73,6 +74,13 @@ def main():
         lines += counter_stats(cur)
     else:
         lines += kernel_stats(cur)
+    if a.dispatches:
+        lines += ["", f"dispatches of {a.dispatches}:", "",
+                  "| kernel | grid | workgroup | lds | duration us |", "|---|---|---|---|---|"]
+        for name, gx, wx, lds, d in cur.execute("select name, grid_x, workgroup_x, lds_size, duration from kernels "
+                                                "order by start"):
+            if a.dispatches in name:
+                lines.append(f"| {short(name)} | {gx} | {wx} | {lds} | {d / 1e3:.1f} |")
     print("\n".join(lines))
 
 
