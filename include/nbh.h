@@ -106,6 +106,12 @@ int32_t nbh_plan_ninv(int64_t N, int32_t prime, double *ninv, double *ninvp);
 int32_t nbh_set_tuning(int32_t grp_lds_doubles, int32_t mid_max, int32_t cmin, int32_t cmax);
 int32_t nbh_get_tuning(int32_t *grp_lds_doubles, int32_t *mid_max, int32_t *cmin, int32_t *cmax);
 int32_t nbh_plan_groups(int64_t N, int32_t *ngroups, int64_t *la_lb_s_c /* 4 per group */);
+/* Per-length knobs (the autotuner's output): entry i, 4 knobs each in the
+ * order of nbh_set_tuning, applies to lengths N <= nmax[i] (nmax increasing,
+ * first match wins); longer lengths use the nbh_set_tuning default.  count 0
+ * clears the table. */
+int32_t nbh_set_tuning_table(const int64_t *nmax, const int32_t *knobs, int32_t count);
+int32_t nbh_get_tuning_for(int64_t N, int32_t *knobs4);
 
 /* Per-kernel timing: when enabled, every launch of the product pipeline is
  * bracketed by HIP events on the stream it is launched on.  nbh_profile_read

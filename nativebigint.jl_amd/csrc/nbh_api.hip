@@ -9,7 +9,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <dlfcn.h>
+
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -208,10 +211,10 @@ struct Lib {
     size_t lds_block = 65536;
     size_t hbm = 0;
     std::string name;
-    Tuning tun;
-    int tun_version = 0;
+    Tuning tun;                                    // default schedule knobs
+    std::vector<std::pair<int64_t, Tuning>> table;  // per-length knobs: first entry with N <= Nmax
     std::map<std::pair<int64_t, int>, std::unique_ptr<StageList>> stages;
-    std::map<std::tuple<int64_t, int, int>, std::unique_ptr<PlanEntry>> plans;
+    std::map<std::tuple<int64_t, int, int, int, int, int>, std::unique_ptr<PlanEntry>> plans;
     void *ws = nullptr;  // single-product workspace
     size_t ws_size = 0;
     void *io = nullptr;  // host-pointer API staging of a, b, r
@@ -437,8 +440,51 @@ int make_schedule(const StageList &L, const Tuning &T, DevPlan &P, PlanEntry &E)
     return NBH_OK;
 }
 
+Tuning tuning_for(int64_t N) {
+    for (const auto &e : g.table)
+        if (N <= e.first) return e.second;
+    return g.tun;
+}
+
+bool tuning_ok(const Tuning &t) {
+    auto pow2 = [](int32_t x) { return x > 0 && (x & (x - 1)) == 0; };
+    return pow2(t.grp_lds) && pow2(t.mid_max) && pow2(t.cmin) && pow2(t.cmax) && t.cmin <= t.cmax &&
+           t.mid_max >= 4 && t.grp_lds >= 16 && t.mid_max <= (1 << 16) && t.grp_lds <= (1 << 16) &&
+           8 * (size_t)lds_doubles(t.grp_lds) <= g.lds_block && 16 * (size_t)lds_doubles(t.mid_max) <= g.lds_block;
+}
+
+// The autotuner's table, "Nmax grp_lds mid_max cmin cmax" per line, read at
+// init from tuning_<arch>.txt beside this library (if present).
+void load_tuning_file() {
+    Dl_info di;
+    if (!dladdr((void *)&load_tuning_file, &di) || !di.dli_fname) return;
+    std::string dir(di.dli_fname);
+    const size_t slash = dir.rfind('/');
+    dir = slash == std::string::npos ? "." : dir.substr(0, slash);
+    std::string arch = g.name.substr(0, g.name.find(':'));
+    FILE *f = std::fopen((dir + "/tuning_" + arch + ".txt").c_str(), "r");
+    if (!f) return;
+    std::vector<std::pair<int64_t, Tuning>> tab;
+    char line[256];
+    bool ok = true;
+    while (std::fgets(line, sizeof line, f)) {
+        if (line[0] == '#' || line[0] == '\n') continue;
+        long long nmax;
+        Tuning t;
+        if (std::sscanf(line, "%lld %d %d %d %d", &nmax, &t.grp_lds, &t.mid_max, &t.cmin, &t.cmax) != 5 ||
+            !tuning_ok(t) || (!tab.empty() && nmax <= tab.back().first)) {
+            ok = false;
+            break;
+        }
+        tab.emplace_back((int64_t)nmax, t);
+    }
+    std::fclose(f);
+    if (ok) g.table = tab;
+}
+
 int get_plan(int64_t N, int k, PlanEntry **out) {
-    auto key = std::make_tuple(N, k, g.tun_version);
+    const Tuning T = tuning_for(N);
+    auto key = std::make_tuple(N, k, T.grp_lds, T.mid_max, T.cmin, T.cmax);
     auto it = g.plans.find(key);
     if (it != g.plans.end()) {
         *out = it->second.get();
@@ -457,7 +503,7 @@ int get_plan(int64_t N, int k, PlanEntry **out) {
     P.ninvp = L->ninvp;
     for (int i = 0; i < P.nst; i++) P.st[i] = L->st[i];
     P.tw = L->tw;
-    RCCHK(make_schedule(*L, g.tun, P, *E));
+    RCCHK(make_schedule(*L, T, P, *E));
     HIPCHK(hipMalloc((void **)&E->dev, sizeof(DevPlan)));
     HIPCHK(hipMemcpy(E->dev, &P, sizeof(DevPlan), hipMemcpyHostToDevice));
     *out = E.get();
@@ -887,6 +933,7 @@ int32_t nbh_init(int32_t device) {
     HIPCHK(hipMalloc((void **)&g.slots, sizeof(DevJob) * NSLOTS));
     for (int i = 0; i < NSIDE; i++) HIPCHK(hipStreamCreateWithFlags(&g.side[i], hipStreamNonBlocking));
     HIPCHK(set_lds_limit(g.lds_block));
+    load_tuning_file();
     // keep the default tiles inside the device's per-block LDS
     while (16 * (size_t)lds_doubles(g.tun.mid_max) > g.lds_block && g.tun.mid_max > 256) g.tun.mid_max >>= 1;
     while (8 * (size_t)lds_doubles(g.tun.grp_lds) > g.lds_block && g.tun.grp_lds > 256) g.tun.grp_lds >>= 1;
@@ -1144,16 +1191,35 @@ int32_t nbh_plan_ninv(int64_t N, int32_t prime, double *ninv, double *ninvp) {
 int32_t nbh_set_tuning(int32_t grp_lds_doubles, int32_t mid_max, int32_t cmin, int32_t cmax) {
     std::lock_guard<std::recursive_mutex> lk(g.mu);
     RCCHK(ensure_init());
-    auto pow2 = [](int32_t x) { return x > 0 && (x & (x - 1)) == 0; };
-    if (!pow2(grp_lds_doubles) || !pow2(mid_max) || !pow2(cmin) || !pow2(cmax) || cmin > cmax || mid_max < 4 ||
-        grp_lds_doubles < 16 || mid_max > (1 << 16) || grp_lds_doubles > (1 << 16) ||
-        8 * (size_t)lds_doubles(grp_lds_doubles) > g.lds_block || 16 * (size_t)lds_doubles(mid_max) > g.lds_block)
-        return fail(NBH_ERR_ARG, "tuning outside the device limits");
-    g.tun.grp_lds = grp_lds_doubles;
-    g.tun.mid_max = mid_max;
-    g.tun.cmin = cmin;
-    g.tun.cmax = cmax;
-    g.tun_version++;
+    const Tuning t{grp_lds_doubles, mid_max, cmin, cmax};
+    if (!tuning_ok(t)) return fail(NBH_ERR_ARG, "tuning outside the device limits");
+    g.tun = t;
+    return NBH_OK;
+}
+
+int32_t nbh_set_tuning_table(const int64_t *nmax, const int32_t *knobs, int32_t count) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    if (count < 0 || (count > 0 && (!nmax || !knobs))) return fail(NBH_ERR_ARG, "bad table");
+    std::vector<std::pair<int64_t, Tuning>> tab;
+    for (int32_t i = 0; i < count; i++) {
+        const Tuning t{knobs[4 * i], knobs[4 * i + 1], knobs[4 * i + 2], knobs[4 * i + 3]};
+        if (!tuning_ok(t)) return fail(NBH_ERR_ARG, "table entry outside the device limits");
+        if (i > 0 && nmax[i] <= nmax[i - 1]) return fail(NBH_ERR_ARG, "table lengths must increase");
+        tab.emplace_back(nmax[i], t);
+    }
+    g.table = tab;
+    return NBH_OK;
+}
+
+int32_t nbh_get_tuning_for(int64_t N, int32_t *knobs4) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    const Tuning t = tuning_for(N);
+    knobs4[0] = t.grp_lds;
+    knobs4[1] = t.mid_max;
+    knobs4[2] = t.cmin;
+    knobs4[3] = t.cmax;
     return NBH_OK;
 }
 

@@ -167,6 +167,38 @@ def get_tuning():
     return tuple(x.value for x in v)
 
 
+def set_tuning_table(entries) -> None:
+    """entries: [(Nmax, (grp_lds, mid_max, cmin, cmax)), ...] with increasing Nmax."""
+    init()
+    nmax = np.array([e[0] for e in entries], dtype=np.int64)
+    knobs = np.array([list(e[1]) for e in entries], dtype=np.int32).reshape(-1)
+    check(lib().nbh_set_tuning_table(nmax.ctypes.data_as(ctypes.c_void_p) if len(entries) else None,
+                                     knobs.ctypes.data_as(ctypes.c_void_p) if len(entries) else None,
+                                     len(entries)))
+
+
+def tuning_file_entries():
+    """The table the library loaded at init (tuning_<arch>.txt beside libnbh.so)."""
+    import os
+    from ._lib import PKG_DIR
+    path = os.path.join(PKG_DIR, f"tuning_{device_info()['arch'].split(':')[0]}.txt")
+    if not os.path.exists(path):
+        return []
+    out = []
+    for line in open(path):
+        if line.strip() and not line.startswith("#"):
+            v = [int(x) for x in line.split()]
+            out.append((v[0], tuple(v[1:5])))
+    return out
+
+
+def get_tuning_for(N: int):
+    init()
+    out = np.zeros(4, dtype=np.int32)
+    check(lib().nbh_get_tuning_for(N, out.ctypes.data_as(ctypes.c_void_p)))
+    return tuple(int(v) for v in out)
+
+
 def device_info():
     init()
     cus, lds, hbm = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int64()

@@ -104,12 +104,15 @@ def test_transform_other_schedules(tuning, N):
         D.set_tuning(*tuning)
     except Exception as e:  # a tuning beyond this device's LDS
         pytest.skip(str(e))
+    D.set_tuning_table([])          # the autotuned per-length table would override it
     try:
+        assert D.get_tuning_for(N) == tuning
         say(f"  groups {D.plan_groups(N)}")
         for k in (0, 1):
             _transform_case(N, k, N + 17 * k)
     finally:
         D.set_tuning(*DEFAULT_TUNING)
+        D.set_tuning_table(D.tuning_file_entries())
 
 
 @pytest.mark.parametrize("n,b,nch,N", [(1, 49, 2, 4), (5, 48, 7, 12), (152, 46, 212, 512), (4096, 43, 6097, 12288),
