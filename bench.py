@@ -9,6 +9,10 @@ launches covering every job).  The same list is also run one product per
 launch for the batched/one-per-launch ratio.
 workload "large": 2^20 x 2^20, 2^20 x 2^18 and a 2^22-limb square, one
 product per launch; a step is the three products.
+workload "hgcd": 1024 batched 2x2 cofactor steps (u', v') = (m11 u - m01 v,
+m00 v - m10 u), n log-uniform in [600, 32768] limbs, cofactors n/2 limbs:
+four products per instance plus the signed combine, all instances in one
+planned batch.  limb-products counts the four products' result limbs.
 
 limb-products/s = sum over completed bit-exact products of (m + n) result
 limbs / wall time, inputs already resident in HBM.  Every product of the
@@ -42,7 +46,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["batch", "large"], default="batch")
+    ap.add_argument("--workload", choices=["batch", "large", "hgcd"], default="batch")
     ap.add_argument("--jobs", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
@@ -60,7 +64,7 @@ def kernel_bytes(plan_groups, N, m, n, sqr, nconv):
     nops = 1 if sqr else 2
     K = len(plan_groups)
     limbs_in = 8 * (m if sqr else m + n)
-    out = {"group_fwd": 0, "mid": 0, "group_inv": 0, "unpack": 0, "carry_fix": 0}
+    out = {"group_fwd": 0, "mid": 0, "group_inv": 0, "unpack": 0, "carry_fix": 0, "combine": 0}
     for g in range(K - 1):
         rd = 2 * limbs_in if g == 0 else 2 * nops * 8 * N
         out["group_fwd"] += rd + 2 * nops * 8 * N
@@ -110,6 +114,9 @@ def main():
         torch.cuda.synchronize()
         log(f"calibration copy: 3 x {args.calibrate_copy_mib} MiB device-to-device")
         del src, dst
+
+    if args.workload == "hgcd":
+        return bench_hgcd(args, D, engine, wl, info, stream)
 
     # ---- build the job list --------------------------------------------------
     if args.workload == "batch":
@@ -314,6 +321,76 @@ def main():
     print(json.dumps(out))
     if batch is not None:
         batch.close()
+
+
+def bench_hgcd(args, D, engine, wl, info, stream):
+    import torch
+    count = min(args.jobs, 1024)
+    sizes = wl.hgcd_sizes(count)
+    host = [wl.hgcd_instance(k, n) for k, n in enumerate(sizes)]
+    dev = [{k: D.to_dev(v) for k, v in q.items()} for q in host]
+    total_limbs = sum(4 * (q["u"].size + q["m00"].size) for q in host)
+    meta = []
+    for q in host:
+        n, mn = q["u"].size, q["m00"].size
+        bits_n = engine.magnitude_bits(q["u"], 0, n)
+        bits_m = engine.magnitude_bits(q["m00"], 0, mn)
+        bch, N = engine.fp_ntt_params(bits_n, bits_m)
+        nconv = -(-bits_n // bch) + -(-bits_m // bch) - 1
+        meta += [(N, n, mn, False, nconv)] * 4
+    torch.cuda.synchronize()
+    log(f"{count} cofactor steps, {total_limbs} product limbs")
+    with torch.cuda.stream(stream):
+        H = D.HgcdStep(dev, "adjust")
+        for _ in range(args.warmup):
+            H.run(stream)
+        torch.cuda.synchronize()
+        D.profile_enable(True)
+        D.profile_read()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        for _ in range(args.steps):
+            H.run(stream)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        prof = D.profile_read()
+        D.profile_enable(False)
+    ms = ev0.elapsed_time(ev1) / args.steps
+    value = total_limbs / (ms * 1e-3)
+    log(f"timed: {ms:.3f} ms/step, {value:.4g} limb-products/s")
+    verified = None
+    if not args.no_verify:
+        from tests import gmp_ref
+        bad = 0
+        for (r0, r1, sg, _), q in zip(H.out, host):
+            M = {k: wl.limbs_to_int(v) for k, v in q.items()}
+            s = sg.cpu().numpy()
+            w0 = M["m11"] * M["u"] - M["m01"] * M["v"]
+            w1 = M["m00"] * M["v"] - M["m10"] * M["u"]
+            ok = (int(s[0]) * wl.limbs_to_int(D.to_host(r0)) == w0 and
+                  int(s[1]) * wl.limbs_to_int(D.to_host(r1)) == w1)
+            bad += int(not ok)
+        verified = bad == 0
+        log(f"verified {count - bad}/{count} cofactor steps exact")
+        if bad:
+            print(json.dumps({"error": f"{bad} cofactor steps differ"}))
+            sys.exit(1)
+    out = {
+        "metric": "bit-exact limb-products/s", "value": value, "unit": "limb-products/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (SplitMix64 uniform limbs, nonzero top limb)",
+        "config": {"workload": "hgcd: 1024 cofactor steps (m11 u - m01 v, m00 v - m10 u), n log-uniform "
+                               "[600, 32768], cofactors n/2 limbs", "instances": count,
+                   "product_limbs_per_step": total_limbs, "parallelism": "single GPU"},
+        "verified_bit_exact": verified,
+        "kernels": {k: {"ms_per_step": prof[k][0] / args.steps, "launches_per_step": prof[k][1] / args.steps}
+                    for k in D.PROF_KINDS},
+        "device": info,
+    }
+    print(json.dumps(out))
+    H.close()
 
 
 if __name__ == "__main__":

@@ -82,6 +82,32 @@ void nbh_batch_destroy(nbh_batch b);
 /* One-shot: create, run, synchronise, destroy. */
 int32_t nbh_mul_batch_dev(const nbh_job *jobs, int64_t count, void *stream);
 
+/* Batched 2x2 cofactor step of HGCD (device pointers): per instance a 2x2
+ * matrix of mn-limb cofactors (zero-padded) and a 2-vector of n-limb operands.
+ *   NBH_HGCD_FORWARD: r0 = m00*u + m01*v, r1 = m10*u + m11*v, n+mn+1 limbs
+ *                     (hgcd_mul2!, src/gcd.jl:360-367, as hgcd_matrix_mul!
+ *                     and cof_mul2! use it)
+ *   NBH_HGCD_ADJUST:  r0 = |m11*u - m01*v|, r1 = |m00*v - m10*u|, n+mn limbs,
+ *                     with signs -1/0/+1 in s0/s1 (device int32): the M^-1
+ *                     application of hgcd_matrix_adjust! (src/gcd.jl:398-437)
+ *                     without the ra*2^(64p) term and for any matrix, so the
+ *                     results may be negative.
+ * The four products of every instance run as one planned product batch, the
+ * adds/subtractions as one more launch.  No reference counterpart as a batch. */
+#define NBH_HGCD_FORWARD 0
+#define NBH_HGCD_ADJUST 1
+typedef struct {
+    const uint64_t *m00, *m01, *m10, *m11;
+    const uint64_t *u, *v;
+    int64_t mn, n;
+    uint64_t *r0, *r1;
+    int32_t *s0, *s1;
+} nbh_hgcd_job;
+typedef struct nbh_hgcd_s *nbh_hgcd;
+int32_t nbh_hgcd_create(const nbh_hgcd_job *jobs, int64_t count, int32_t mode, nbh_hgcd *out);
+int32_t nbh_hgcd_run(nbh_hgcd h, void *stream);
+void nbh_hgcd_destroy(nbh_hgcd h);
+
 /* Stage-level entry points over device buffers, one per reference function,
  * used by the parity tests: fp_ntt_fwd! (src/fpntt.jl:715), fp_ntt_rev!
  * (:727), fp_ntt_pack! (:744), fp_ntt_pointwise! (:770), fp_ntt_unpack2!
@@ -117,9 +143,9 @@ int32_t nbh_get_tuning_for(int64_t N, int32_t *knobs4);
  * bracketed by HIP events on the stream it is launched on.  nbh_profile_read
  * synchronises, returns the summed milliseconds and launch counts per kernel
  * kind (0 forward strided group + pack, 1 mid, 2 inverse strided group,
- * 3 unpack, 4 carry fix) and clears them.  Instrumentation only: no reference
+ * 3 unpack, 4 carry fix, 5 cofactor-step combine) and clears them.  Instrumentation only: no reference
  * counterpart. */
-#define NBH_PROF_KINDS 5
+#define NBH_PROF_KINDS 6
 int32_t nbh_profile_enable(int32_t on);
 int32_t nbh_profile_read(double *ms, int64_t *launches, int32_t nkinds);
 

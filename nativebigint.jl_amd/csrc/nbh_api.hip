@@ -903,6 +903,73 @@ int batch_run(nbh_batch_s *B, hipStream_t s) {
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// Batched 2x2 cofactor steps: four products per instance in one product
+// batch, then one combine launch for the sums / signed differences.
+
+struct nbh_hgcd_s {
+    nbh_batch_s *products = nullptr;
+    void *tmp = nullptr;
+    CombineJob *dcomb = nullptr;
+    int64_t ncomb = 0;
+};
+
+namespace {
+
+void hgcd_free(nbh_hgcd_s *H) {
+    if (!H) return;
+    batch_free(H->products);
+    if (H->tmp) (void)hipFree(H->tmp);
+    if (H->dcomb) (void)hipFree(H->dcomb);
+    delete H;
+}
+
+int hgcd_create(const nbh_hgcd_job *jobs, int64_t count, int mode, nbh_hgcd_s **out) {
+    if (!jobs || count < 1 || (mode != NBH_HGCD_FORWARD && mode != NBH_HGCD_ADJUST))
+        return fail(NBH_ERR_ARG, "bad cofactor-step batch");
+    std::unique_ptr<nbh_hgcd_s, void (*)(nbh_hgcd_s *)> H(new nbh_hgcd_s, hgcd_free);
+    size_t words = 0;
+    for (int64_t i = 0; i < count; i++) {
+        const nbh_hgcd_job &q = jobs[i];
+        if (q.mn < 1 || q.n < 1 || !q.m00 || !q.m01 || !q.m10 || !q.m11 || !q.u || !q.v || !q.r0 || !q.r1 ||
+            (mode == NBH_HGCD_ADJUST && (!q.s0 || !q.s1)))
+            return fail(NBH_ERR_ARG, "bad cofactor-step instance " + std::to_string(i));
+        words += 4 * (size_t)(q.n + q.mn);
+    }
+    HIPCHK(hipMalloc(&H->tmp, 8 * words));
+    std::vector<nbh_job> prods;
+    std::vector<CombineJob> comb;
+    uint64_t *t = (uint64_t *)H->tmp;
+    for (int64_t i = 0; i < count; i++) {
+        const nbh_hgcd_job &q = jobs[i];
+        const int64_t lp = q.n + q.mn;
+        uint64_t *P[4] = {t, t + lp, t + 2 * lp, t + 3 * lp};
+        t += 4 * lp;
+        // forward: m00 u, m01 v, m10 u, m11 v; adjust: m11 u, m01 v, m00 v, m10 u
+        const uint64_t *mat[4] = {q.m00, q.m01, q.m10, q.m11};
+        const uint64_t *vec[4] = {q.u, q.v, q.u, q.v};
+        if (mode == NBH_HGCD_ADJUST) {
+            mat[0] = q.m11; vec[0] = q.u;
+            mat[1] = q.m01; vec[1] = q.v;
+            mat[2] = q.m00; vec[2] = q.v;
+            mat[3] = q.m10; vec[3] = q.u;
+        }
+        for (int k = 0; k < 4; k++) prods.push_back(nbh_job{P[k], vec[k], mat[k], q.n, q.mn, -1, -1});
+        const int sub = mode == NBH_HGCD_ADJUST;
+        const int64_t lr = sub ? lp : lp + 1;
+        comb.push_back(CombineJob{P[0], P[1], lp, lp, q.r0, lr, q.s0, sub, 0});
+        comb.push_back(CombineJob{P[2], P[3], lp, lp, q.r1, lr, q.s1, sub, 0});
+    }
+    RCCHK(batch_create(prods.data(), (int64_t)prods.size(), &H->products));
+    H->ncomb = (int64_t)comb.size();
+    HIPCHK(hipMalloc((void **)&H->dcomb, sizeof(CombineJob) * comb.size()));
+    HIPCHK(hipMemcpy(H->dcomb, comb.data(), sizeof(CombineJob) * comb.size(), hipMemcpyHostToDevice));
+    *out = H.release();
+    return NBH_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
 // C ABI
 
 extern "C" {
@@ -1047,6 +1114,35 @@ int32_t nbh_mul_batch_dev(const nbh_job *jobs, int64_t count, void *stream) {
     if (rc) return rc;
     HIPCHK(e);
     return NBH_OK;
+}
+
+int32_t nbh_hgcd_create(const nbh_hgcd_job *jobs, int64_t count, int32_t mode, nbh_hgcd *out) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    if (!out) return fail(NBH_ERR_ARG, "null handle");
+    nbh_hgcd_s *H = nullptr;
+    RCCHK(hgcd_create(jobs, count, mode, &H));
+    *out = H;
+    return NBH_OK;
+}
+
+int32_t nbh_hgcd_run(nbh_hgcd h, void *stream) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    if (!h) return fail(NBH_ERR_ARG, "null handle");
+    hipStream_t s = stream ? (hipStream_t)stream : g.stream;
+    RCCHK(batch_run(h->products, s));
+    ProfScope ps(5, s);
+    HIPCHK(launch_combine(h->dcomb, h->ncomb, s));
+    return NBH_OK;
+}
+
+void nbh_hgcd_destroy(nbh_hgcd h) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    if (h) {
+        (void)hipDeviceSynchronize();
+        hgcd_free(h);
+    }
 }
 
 int32_t nbh_ntt_fwd_dev(double *x, int64_t N, int32_t prime, void *stream) {

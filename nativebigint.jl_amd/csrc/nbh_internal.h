@@ -108,4 +108,17 @@ constexpr uint32_t CHUNK_P = 2;  // every limb all-ones: carry passes through
 
 }  // namespace nbh
 
+namespace nbh {
+// One output of a 2x2 cofactor step: r = A + B, or r = |A - B| with its sign
+// (A, B are products in the step's workspace, zero above their lengths).
+struct CombineJob {
+    const uint64_t *A, *B;
+    int64_t la, lb;
+    uint64_t *r;
+    int64_t lr;
+    int32_t *sign;  // sub only: -1, 0, +1 (device)
+    int32_t sub, pad;
+};
+}  // namespace nbh
+
 static_assert(sizeof(nbh::Tile) == 16, "kernels read a Tile as four 32-bit words");

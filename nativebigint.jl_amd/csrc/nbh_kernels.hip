@@ -784,6 +784,126 @@ __global__ void __launch_bounds__(UNPACK_THREADS) k_carry_fix(const DevJob *__re
 }
 
 // ---------------------------------------------------------------------------
+// 2x2 cofactor step outputs (the adds and subtractions of hgcd_mul2! and
+// hgcd_matrix_adjust!, src/gcd.jl:360-437): one workgroup per output walks
+// its limbs in chunks of UNPACK_LPC, each thread owning UNPACK_LPT limbs, with
+// the carry (borrow) between threads from the same generate/propagate ballot
+// prefix as the unpack and between chunks in a register.
+
+// Carry into the calling thread's segment given the block's carry-in; also
+// returns the carry out of the whole block.  gen: the segment's carry out with
+// carry-in 0; prop: the segment passes a carry-in through.
+__device__ __forceinline__ uint32_t block_carry(bool gen, bool prop, uint32_t cin_block, uint32_t *wsum,
+                                                uint32_t &cout_block) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
+    const uint64_t gm = __ballot(gen);
+    const uint64_t pm = __ballot(prop);
+    {
+        const uint64_t cv = (gm + (gm | pm)) ^ pm;
+        const uint32_t wg = (uint32_t)(((gm >> 63) | ((pm >> 63) & (cv >> 63))) & 1);
+        const uint32_t wp = pm == ~0ull ? 1u : 0u;
+        if (lane == 0) wsum[wid] = wg | (wp << 1);
+    }
+    __syncthreads();
+    uint64_t cin = cin_block;
+    for (int w = 0; w < wid; w++) {
+        const uint32_t f = wsum[w];
+        cin = (f & 1) | ((f >> 1) & cin);
+    }
+    uint32_t co = cin_block;
+    for (int w = 0; w < nw; w++) {
+        const uint32_t f = wsum[w];
+        co = (f & 1) | ((f >> 1) & co);
+    }
+    cout_block = co;
+    const uint64_t cv = (gm + (gm | pm) + cin) ^ pm;
+    __syncthreads();  // wsum is reused by the next call
+    return (uint32_t)((cv >> lane) & 1);
+}
+
+__global__ void __launch_bounds__(UNPACK_THREADS) k_combine(const CombineJob *__restrict__ jobs) {
+    __shared__ uint32_t wsum[UNPACK_THREADS / 64];
+    __shared__ unsigned long long top;  // 1 + highest differing limb, 0 if none
+    const auto *J = cst(jobs + blockIdx.x);
+    const uint64_t *A = unip(J->A), *B = unip(J->B);
+    const int64_t la = uni64(J->la), lb = uni64(J->lb), lr = uni64(J->lr);
+    uint64_t *__restrict__ r = unip(J->r);
+    const bool sub = uni(J->sub) != 0;
+    const int tid = threadIdx.x;
+    bool swap = false;
+    if (sub) {
+        // highest limb where A and B differ decides the sign
+        if (tid == 0) top = 0;
+        __syncthreads();
+        unsigned long long hi = 0;
+        const int64_t L = la > lb ? la : lb;
+        for (int64_t k = tid; k < L; k += blockDim.x) {
+            const uint64_t a = k < la ? gldu(A, k) : 0, b = k < lb ? gldu(B, k) : 0;
+            if (a != b) hi = (unsigned long long)k + 1;
+        }
+        if (hi) atomicMax(&top, hi);
+        __syncthreads();
+        const int64_t t = (int64_t)top - 1;
+        int32_t sg = 0;
+        if (t >= 0) {
+            const uint64_t a = t < la ? gldu(A, t) : 0, b = t < lb ? gldu(B, t) : 0;
+            sg = a > b ? 1 : -1;
+        }
+        if (tid == 0) *unip(J->sign) = sg;
+        swap = sg < 0;
+    }
+    const uint64_t *X = swap ? B : A, *Y = swap ? A : B;
+    const int64_t lx = swap ? lb : la, ly = swap ? la : lb;
+    uint32_t carry = 0;
+    for (int64_t c0 = 0; c0 < lr; c0 += UNPACK_LPC) {
+        const int64_t k0 = c0 + UNPACK_LPT * tid;
+        uint64_t d[UNPACK_LPT];
+        bool gen = false, prop = true;
+        // segment result with carry (borrow) in 0, and its generate/propagate
+        uint64_t cc = 0;
+#pragma unroll
+        for (int w = 0; w < UNPACK_LPT; w++) {
+            const int64_t k = k0 + w;
+            const uint64_t x = k < lx ? gldu(X, k) : 0, y = k < ly ? gldu(Y, k) : 0;
+            if (!sub) {
+                const uint64_t s1 = x + y;
+                const uint64_t c1 = s1 < x;
+                const uint64_t s2 = s1 + cc;
+                cc = c1 | (s2 < s1);
+                d[w] = s2;
+                prop = prop && s2 == ~0ull;
+            } else {
+                const uint64_t s1 = x - y;
+                const uint64_t b1 = x < y;
+                const uint64_t s2 = s1 - cc;
+                cc = b1 | (s1 < cc);
+                d[w] = s2;
+                prop = prop && s2 == 0;
+            }
+        }
+        gen = cc != 0;
+        uint32_t cout = 0;
+        const uint32_t cin = block_carry(gen, prop, carry, wsum, cout);
+        uint64_t cv = cin;
+#pragma unroll
+        for (int w = 0; w < UNPACK_LPT; w++) {
+            const int64_t k = k0 + w;
+            uint64_t v;
+            if (!sub) {
+                v = d[w] + cv;
+                cv = v < cv;
+            } else {
+                v = d[w] - cv;
+                cv = d[w] < cv;
+            }
+            if (k < lr) r[k] = v;
+        }
+        carry = cout;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Twiddle tables.  Row t (1..m-1) of a stage of span L holds w_L^(t j) for
 // j < Q, w_L = w_N^(N/L), minimal balanced (src/fpntt.jl:173-225).  Each entry
 // is w_N^e = hi[e >> h] * lo[e & (2^h-1)] mod p with exact host-built tables;
@@ -917,6 +1037,12 @@ hipError_t launch_pack(double *x, int64_t N, const uint64_t *limbs, int64_t n, i
 hipError_t launch_fill_u64(uint64_t *p, int64_t n, uint64_t v, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_combine(const CombineJob *jobs, int64_t count, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_combine, dim3((unsigned)count), dim3(UNPACK_THREADS), 0, s, jobs);
     return hipGetLastError();
 }
 

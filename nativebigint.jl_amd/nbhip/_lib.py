@@ -40,6 +40,17 @@ class nbh_job(ctypes.Structure):
                 ("bits_a", ctypes.c_int64), ("bits_b", ctypes.c_int64)]
 
 
+class nbh_hgcd_job(ctypes.Structure):
+    _fields_ = [("m00", ctypes.c_void_p), ("m01", ctypes.c_void_p), ("m10", ctypes.c_void_p),
+                ("m11", ctypes.c_void_p), ("u", ctypes.c_void_p), ("v", ctypes.c_void_p),
+                ("mn", ctypes.c_int64), ("n", ctypes.c_int64),
+                ("r0", ctypes.c_void_p), ("r1", ctypes.c_void_p),
+                ("s0", ctypes.c_void_p), ("s1", ctypes.c_void_p)]
+
+
+NBH_HGCD_FORWARD = 0
+NBH_HGCD_ADJUST = 1
+
 _i32, _i64, _vp, _d = ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p, ctypes.c_double
 _P = ctypes.POINTER
 
@@ -57,6 +68,9 @@ SIGNATURES = {
     "nbh_batch_run": (_i32, [_vp, _vp]),
     "nbh_batch_destroy": (None, [_vp]),
     "nbh_mul_batch_dev": (_i32, [_P(nbh_job), _i64, _vp]),
+    "nbh_hgcd_create": (_i32, [_P(nbh_hgcd_job), _i64, _i32, _P(_vp)]),
+    "nbh_hgcd_run": (_i32, [_vp, _vp]),
+    "nbh_hgcd_destroy": (None, [_vp]),
     "nbh_ntt_fwd_dev": (_i32, [_vp, _i64, _i32, _vp]),
     "nbh_ntt_rev_dev": (_i32, [_vp, _i64, _i32, _vp]),
     "nbh_pack_dev": (_i32, [_vp, _i64, _vp, _i64, _i32, _i64, _vp]),

@@ -11,7 +11,7 @@ import ctypes
 import numpy as np
 import torch
 
-from ._lib import check, init, lib, nbh_job
+from ._lib import NBH_HGCD_ADJUST, NBH_HGCD_FORWARD, check, init, lib, nbh_hgcd_job, nbh_job
 
 
 def _stream():
@@ -81,6 +81,52 @@ class Batch:
     def close(self) -> None:
         if self._h:
             lib().nbh_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class HgcdStep:
+    """Batched 2x2 cofactor step (device tensors).
+
+    instances: list of dicts with tensors m00, m01, m10, m11 (mn limbs each),
+    u, v (n limbs each).  mode "forward": r0 = m00 u + m01 v, r1 = m10 u + m11 v
+    (n+mn+1 limbs); mode "adjust": r0 = |m11 u - m01 v|, r1 = |m00 v - m10 u|
+    (n+mn limbs) with signs s0, s1.  Outputs are allocated here."""
+
+    def __init__(self, instances, mode: str = "forward"):
+        init()
+        self.mode = NBH_HGCD_FORWARD if mode == "forward" else NBH_HGCD_ADJUST
+        arr = (nbh_hgcd_job * len(instances))()
+        self.out = []
+        for i, q in enumerate(instances):
+            mn, n = q["m00"].numel(), q["u"].numel()
+            lr = n + mn + (1 if self.mode == NBH_HGCD_FORWARD else 0)
+            r0 = torch.empty(lr, dtype=torch.int64, device="cuda")
+            r1 = torch.empty(lr, dtype=torch.int64, device="cuda")
+            sg = torch.zeros(2, dtype=torch.int32, device="cuda")
+            self.out.append((r0, r1, sg, q))
+            for k in ("m00", "m01", "m10", "m11", "u", "v"):
+                setattr(arr[i], k, q[k].data_ptr())
+            arr[i].mn, arr[i].n = mn, n
+            arr[i].r0, arr[i].r1 = r0.data_ptr(), r1.data_ptr()
+            arr[i].s0, arr[i].s1 = sg.data_ptr(), sg.data_ptr() + 4
+        h = ctypes.c_void_p()
+        torch.cuda.synchronize()
+        check(lib().nbh_hgcd_create(arr, len(instances), self.mode, ctypes.byref(h)))
+        self._h = h
+
+    def run(self, stream=None) -> None:
+        s = ctypes.c_void_p(stream.cuda_stream) if stream is not None else _stream()
+        check(lib().nbh_hgcd_run(self._h, s))
+
+    def close(self) -> None:
+        if self._h:
+            lib().nbh_hgcd_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -207,7 +253,7 @@ def device_info():
     return {"cus": cus.value, "lds_per_block": lds.value, "hbm_bytes": hbm.value, "arch": name.value.decode()}
 
 
-PROF_KINDS = ("group_fwd", "mid", "group_inv", "unpack", "carry_fix")
+PROF_KINDS = ("group_fwd", "mid", "group_inv", "unpack", "carry_fix", "combine")
 
 
 def profile_enable(on: bool = True) -> None:

@@ -107,6 +107,14 @@ def hgcd_sizes(count: int = 1024, lo: int = 600, hi: int = 32768) -> list[int]:
     return out
 
 
+def hgcd_instance(k: int, n: int, workload: str = "hgcd") -> dict:
+    """Instance k of the cofactor-step workload: four n/2-limb cofactors and
+    two n-limb operands, uniform limbs with nonzero top limbs."""
+    mn = max(1, n // 2)
+    names = ("m00", "m01", "m10", "m11", "u", "v")
+    return {name: operand(workload, k, i, mn if i < 4 else n) for i, name in enumerate(names)}
+
+
 def limbs_to_int(a) -> int:
     return int.from_bytes(np.ascontiguousarray(a, dtype="<u8").tobytes(), "little")
 
