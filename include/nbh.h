@@ -79,6 +79,17 @@ int32_t nbh_batch_create(const nbh_job *jobs, int64_t count, nbh_batch *out);
 /* Run it (enqueue on stream); can be called repeatedly. */
 int32_t nbh_batch_run(nbh_batch b, void *stream);
 void nbh_batch_destroy(nbh_batch b);
+/* Arena form for servers: the bytes a batch occupies (workspace + its job and
+ * tile tables), then a batch planned inside a caller-owned device region
+ * with its tables uploaded on `stream` -- no device allocation and no host
+ * synchronisation, so batches can be planned while others run.  Every job
+ * must carry its operands' bit lengths.  nbh_batch_release frees the host
+ * side only and does not wait: the caller reuses the arena once the batch's
+ * last run has completed on its stream. */
+int32_t nbh_batch_arena_bytes(const nbh_job *jobs, int64_t count, int64_t *bytes);
+int32_t nbh_batch_create_in(const nbh_job *jobs, int64_t count, void *arena, int64_t arena_bytes, void *stream,
+                            nbh_batch *out);
+void nbh_batch_release(nbh_batch b);
 /* One-shot: create, run, synchronise, destroy. */
 int32_t nbh_mul_batch_dev(const nbh_job *jobs, int64_t count, void *stream);
 

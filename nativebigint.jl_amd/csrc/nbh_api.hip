@@ -226,6 +226,7 @@ struct Lib {
     bool prof = false;
     std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> prof_open;
     std::vector<hipEvent_t> prof_pool;
+    std::vector<hipEvent_t> ev_pool;  // fork/join events of released batches
 };
 Lib g;
 
@@ -682,9 +683,12 @@ struct nbh_batch_s {
         int step;        // launches of one step are independent of each other
     };
     std::vector<DevJob> jobs;
+    std::vector<TileRange> ranges;  // host copies (kept alive for async uploads)
+    int64_t ntiles = 0;
     DevJob *djobs = nullptr;
     Tile *dtiles = nullptr;
     void *ws = nullptr;
+    bool external = false;       // tables and workspace live in a caller's arena
     std::vector<Phase> phases;
     int nsteps = 0;
     std::vector<hipEvent_t> ev;  // fork event per step + one join event per launch
@@ -694,14 +698,22 @@ namespace {
 
 void batch_free(nbh_batch_s *B) {
     if (!B) return;
-    if (B->djobs) (void)hipFree(B->djobs);
-    if (B->dtiles) (void)hipFree(B->dtiles);
-    if (B->ws) (void)hipFree(B->ws);
-    for (auto e : B->ev) (void)hipEventDestroy(e);
+    if (!B->external) {
+        if (B->djobs) (void)hipFree(B->djobs);
+        if (B->dtiles) (void)hipFree(B->dtiles);
+        if (B->ws) (void)hipFree(B->ws);
+    }
+    for (auto e : B->ev) g.ev_pool.push_back(e);
     delete B;
 }
 
-int batch_create(const nbh_job *jobs, int64_t count, nbh_batch_s **out) {
+// Plan a batch.  With `need` set, only report the bytes the batch occupies in
+// an arena (workspace + job and tile tables) and create nothing.  With `arena`
+// set, place everything inside it and upload the tables on `astream` -- no
+// device allocation and no host synchronisation, so a server can plan batches
+// while others run.  Otherwise allocate with hipMalloc.
+int batch_create(const nbh_job *jobs, int64_t count, nbh_batch_s **out, size_t *need = nullptr,
+                 void *arena = nullptr, size_t arena_bytes = 0, hipStream_t astream = nullptr) {
     if (count < 1 || !jobs) return fail(NBH_ERR_ARG, "empty batch");
     std::unique_ptr<nbh_batch_s, void (*)(nbh_batch_s *)> B(new nbh_batch_s, batch_free);
     std::vector<DevJob> &J = B->jobs;
@@ -724,6 +736,7 @@ int batch_create(const nbh_job *jobs, int64_t count, nbh_batch_s **out) {
         bits[(size_t)(2 * i + 1)] = sqr ? q.bits_a : q.bits_b;
         if (q.bits_a <= 0 || (!sqr && q.bits_b <= 0)) need_top = true;
     }
+    if (need_top && (need || arena)) return fail(NBH_ERR_ARG, "arena batches need the operands' bit lengths");
     if (need_top) {
         DevJob *tmp = nullptr;
         uint64_t *tops = nullptr;
@@ -745,7 +758,7 @@ int batch_create(const nbh_job *jobs, int64_t count, nbh_batch_s **out) {
     // size every job, get plans, lay out the workspace
     std::vector<PlanEntry *> plan0((size_t)count, nullptr), plan1((size_t)count, nullptr);
     std::vector<int64_t> Ns((size_t)count, 0);
-    size_t ws = 0;
+    size_t ws = 256;  // offsets start past a reserved block: offset 0 would read as "no buffer"
     std::vector<size_t> offs((size_t)count);
     for (int64_t i = 0; i < count; i++) {
         DevJob &d = J[(size_t)i];
@@ -757,10 +770,12 @@ int batch_create(const nbh_job *jobs, int64_t count, nbh_batch_s **out) {
         RCCHK(get_plan(Ns[(size_t)i], 1, &plan1[(size_t)i]));
         ws += align256(sizeof(double) * (size_t)Ns[(size_t)i]) * (d.sqr ? 2 : 4);
     }
-    HIPCHK(hipMalloc(&B->ws, std::max<size_t>(ws, 256)));
+    B->ws = nullptr;
+    std::vector<size_t> spec_off((size_t)count);
+    for (int64_t i = 0; i < count; i++) spec_off[(size_t)i] = offs[(size_t)i];
     for (int64_t i = 0; i < count; i++) {
         DevJob &d = J[(size_t)i];
-        char *w = (char *)B->ws + offs[(size_t)i];
+        char *w = reinterpret_cast<char *>((uintptr_t)spec_off[(size_t)i]);  // offset, rebased below
         d.chunk_flags = (uint32_t *)w;
         w += align256(sizeof(uint32_t) * (size_t)d.nchunks);
         if (d.zero) continue;
@@ -777,7 +792,8 @@ int batch_create(const nbh_job *jobs, int64_t count, nbh_batch_s **out) {
         }
     }
     // phase tile lists; within a phase, bigger tiles first
-    std::vector<Tile> tiles;
+    std::vector<TileRange> &ranges = B->ranges;
+    int64_t &ntiles = B->ntiles;
     int maxK = 0;
     for (int64_t i = 0; i < count; i++)
         if (!J[(size_t)i].zero) maxK = std::max(maxK, plan0[(size_t)i]->host.ngroups);
@@ -786,7 +802,8 @@ int batch_create(const nbh_job *jobs, int64_t count, nbh_batch_s **out) {
     struct TileReq {
         size_t lds;
         int threads;
-        Tile t;
+        Tile t;          // first tile of the range
+        uint32_t count;
     };
     auto push_phase = [&](int kind, int fwd, std::vector<TileReq> &list) {
         if (list.empty()) return;
@@ -796,10 +813,11 @@ int batch_create(const nbh_job *jobs, int64_t count, nbh_batch_s **out) {
         size_t i = 0;
         const int step = B->nsteps++;
         while (i < list.size()) {
-            nbh_batch_s::Phase ph{kind, fwd, (int64_t)tiles.size(), 0, list[i].lds, list[i].threads, step};
+            nbh_batch_s::Phase ph{kind, fwd, ntiles, 0, list[i].lds, list[i].threads, step};
             while (i < list.size() && list[i].threads == ph.threads && list[i].lds == ph.lds) {
-                tiles.push_back(list[i].t);
-                ph.count++;
+                ranges.push_back(TileRange{list[i].t, list[i].count, (uint32_t)ntiles});
+                ntiles += list[i].count;
+                ph.count += list[i].count;
                 i++;
             }
             if (kind >= 2) ph.lds = 0;
@@ -812,9 +830,8 @@ int batch_create(const nbh_job *jobs, int64_t count, nbh_batch_s **out) {
         const DevPlan &H = plan0[(size_t)i]->host;
         for (int k = 0; k < 2; k++)
             for (int op = 0; op < nops; op++)
-                for (int64_t t = 0; t < H.g[gi].tiles; t++)
-                    list.push_back({lds, threads,
-                                    Tile{(uint32_t)i, (uint8_t)k, (uint8_t)op, (uint8_t)gi, flags, (uint32_t)t, 0}});
+                list.push_back({lds, threads, Tile{(uint32_t)i, (uint8_t)k, (uint8_t)op, (uint8_t)gi, flags, 0, 0},
+                                (uint32_t)H.g[gi].tiles});
     };
     for (int d = maxK - 1; d >= 1; d--) {  // forward strided groups, farthest from the mid first
         for (int64_t i = 0; i < count; i++) {
@@ -845,17 +862,62 @@ int batch_create(const nbh_job *jobs, int64_t count, nbh_batch_s **out) {
         push_phase(0, 0, list);
     }
     for (int64_t i = 0; i < count; i++)
-        for (int64_t c = 0; c < J[(size_t)i].nchunks; c++)
-            list.push_back({0, UNPACK_THREADS, Tile{(uint32_t)i, 0, 0, 0, 0, (uint32_t)c, 0}});
+        list.push_back({0, UNPACK_THREADS, Tile{(uint32_t)i, 0, 0, 0, 0, 0, 0}, (uint32_t)J[(size_t)i].nchunks});
     std::vector<TileReq> list2 = list;
     push_phase(2, 0, list);
     push_phase(3, 0, list2);
-    HIPCHK(hipMalloc((void **)&B->djobs, sizeof(DevJob) * (size_t)count));
-    HIPCHK(hipMalloc((void **)&B->dtiles, sizeof(Tile) * std::max<size_t>(tiles.size(), 1)));
-    HIPCHK(hipMemcpy(B->djobs, J.data(), sizeof(DevJob) * (size_t)count, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(B->dtiles, tiles.data(), sizeof(Tile) * tiles.size(), hipMemcpyHostToDevice));
+    // workspace offsets were laid out from 0: rebase them onto the real workspace
+    const size_t ws_bytes = align256(std::max<size_t>(ws, 256));
+    const size_t jobs_bytes = align256(sizeof(DevJob) * (size_t)count);
+    // tile list followed by the ranges it is expanded from
+    const size_t tiles_list_bytes = align256(sizeof(Tile) * (size_t)std::max<int64_t>(ntiles, 1));
+    const size_t tiles_bytes = tiles_list_bytes + align256(sizeof(TileRange) * std::max<size_t>(ranges.size(), 1));
+    if (need) {
+        *need = ws_bytes + jobs_bytes + tiles_bytes;
+        return NBH_OK;
+    }
+    if (arena) {
+        if (ws_bytes + jobs_bytes + tiles_bytes > arena_bytes) return fail(NBH_ERR_BUDGET, "arena too small");
+        B->external = true;
+        B->ws = arena;
+        B->djobs = (DevJob *)((char *)arena + ws_bytes);
+        B->dtiles = (Tile *)((char *)arena + ws_bytes + jobs_bytes);
+    } else {
+        HIPCHK(hipMalloc(&B->ws, ws_bytes));
+        HIPCHK(hipMalloc((void **)&B->djobs, jobs_bytes));
+        HIPCHK(hipMalloc((void **)&B->dtiles, tiles_bytes));
+    }
+    TileRange *dranges = (TileRange *)((char *)B->dtiles + tiles_list_bytes);
+    auto rebase = [&](auto *p) {
+        return p ? (decltype(p))((char *)B->ws + reinterpret_cast<uintptr_t>(p)) : p;
+    };
+    for (auto &d : J) {
+        d.chunk_flags = rebase(d.chunk_flags);
+        for (int k = 0; k < 2; k++) {
+            d.xa[k] = rebase(d.xa[k]);
+            d.xb[k] = rebase(d.xb[k]);
+        }
+    }
+    if (arena) {
+        HIPCHK(hipMemcpyAsync(B->djobs, J.data(), sizeof(DevJob) * (size_t)count, hipMemcpyHostToDevice, astream));
+        HIPCHK(hipMemcpyAsync(dranges, ranges.data(), sizeof(TileRange) * ranges.size(), hipMemcpyHostToDevice,
+                              astream));
+        HIPCHK(launch_expand_tiles(dranges, (int64_t)ranges.size(), B->dtiles, astream));
+    } else {
+        HIPCHK(hipMemcpy(B->djobs, J.data(), sizeof(DevJob) * (size_t)count, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dranges, ranges.data(), sizeof(TileRange) * ranges.size(), hipMemcpyHostToDevice));
+        HIPCHK(launch_expand_tiles(dranges, (int64_t)ranges.size(), B->dtiles, g.stream));
+        HIPCHK(hipStreamSynchronize(g.stream));
+    }
     B->ev.resize((size_t)B->nsteps + B->phases.size(), nullptr);
-    for (auto &e : B->ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    for (auto &e : B->ev) {
+        if (!g.ev_pool.empty()) {
+            e = g.ev_pool.back();
+            g.ev_pool.pop_back();
+        } else {
+            HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        }
+    }
     *out = B.release();
     return NBH_OK;
 }
@@ -1092,6 +1154,33 @@ int32_t nbh_batch_run(nbh_batch b, void *stream) {
     RCCHK(ensure_init());
     if (!b) return fail(NBH_ERR_ARG, "null handle");
     return batch_run(b, stream ? (hipStream_t)stream : g.stream);
+}
+
+int32_t nbh_batch_arena_bytes(const nbh_job *jobs, int64_t count, int64_t *bytes) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    if (!bytes) return fail(NBH_ERR_ARG, "null output");
+    size_t need = 0;
+    RCCHK(batch_create(jobs, count, nullptr, &need));
+    *bytes = (int64_t)need;
+    return NBH_OK;
+}
+
+int32_t nbh_batch_create_in(const nbh_job *jobs, int64_t count, void *arena, int64_t arena_bytes, void *stream,
+                            nbh_batch *out) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    if (!out || !arena || arena_bytes <= 0) return fail(NBH_ERR_ARG, "bad arena");
+    nbh_batch_s *B = nullptr;
+    RCCHK(batch_create(jobs, count, &B, nullptr, arena, (size_t)arena_bytes,
+                       stream ? (hipStream_t)stream : g.stream));
+    *out = B;
+    return NBH_OK;
+}
+
+void nbh_batch_release(nbh_batch b) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    batch_free(b);
 }
 
 void nbh_batch_destroy(nbh_batch b) {

@@ -109,6 +109,14 @@ constexpr uint32_t CHUNK_P = 2;  // every limb all-ones: carry passes through
 }  // namespace nbh
 
 namespace nbh {
+// A run of consecutive tiles of one (job, prime, operand, group): the host
+// plans batches as ranges; a device kernel expands them into the tile list.
+struct TileRange {
+    Tile head;       // head.tile is the first tile index
+    uint32_t count;
+    uint32_t out;    // position of the first tile in the expanded list
+};
+
 // One output of a 2x2 cofactor step: r = A + B, or r = |A - B| with its sign
 // (A, B are products in the step's workspace, zero above their lengths).
 struct CombineJob {

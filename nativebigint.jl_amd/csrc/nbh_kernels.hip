@@ -949,6 +949,16 @@ __global__ void k_fill_u64(uint64_t *__restrict__ p, int64_t n, uint64_t v) {
         p[i] = v;
 }
 
+// Expand a batch's tile ranges into its tile list (one block per range).
+__global__ void k_expand_tiles(const TileRange *__restrict__ ranges, Tile *__restrict__ tiles) {
+    const TileRange rg = ranges[blockIdx.x];
+    for (uint32_t i = threadIdx.x; i < rg.count; i += blockDim.x) {
+        Tile t = rg.head;
+        t.tile = rg.head.tile + i;
+        tiles[rg.out + i] = t;
+    }
+}
+
 // Stream-ordered upload of a single job descriptor (no host staging buffer).
 __global__ void k_store_job(DevJob j, DevJob *__restrict__ dst) {
     if (threadIdx.x == 0) *dst = j;
@@ -1043,6 +1053,12 @@ hipError_t launch_fill_u64(uint64_t *p, int64_t n, uint64_t v, hipStream_t s) {
 hipError_t launch_combine(const CombineJob *jobs, int64_t count, hipStream_t s) {
     if (count <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_combine, dim3((unsigned)count), dim3(UNPACK_THREADS), 0, s, jobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_expand_tiles(const TileRange *ranges, int64_t nranges, Tile *tiles, hipStream_t s) {
+    if (nranges <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_expand_tiles, dim3((unsigned)nranges), dim3(256), 0, s, ranges, tiles);
     return hipGetLastError();
 }
 

@@ -24,6 +24,7 @@ hipError_t launch_unpack(const DevJob *jobs, const Tile *tiles, const TileGen &g
 hipError_t launch_carry_fix(const DevJob *jobs, const Tile *tiles, const TileGen &gen, int64_t ntiles,
                             hipStream_t s);
 hipError_t launch_store_job(const DevJob &j, DevJob *dst, hipStream_t s);
+hipError_t launch_expand_tiles(const TileRange *ranges, int64_t nranges, Tile *tiles, hipStream_t s);
 // One workgroup per cofactor-step output (sum, or |difference| and sign).
 hipError_t launch_combine(const CombineJob *jobs, int64_t count, hipStream_t s);
 hipError_t launch_twiddle_stage(int prime, double *out, int m, int64_t Q, int64_t NoverL, const double *tlo,

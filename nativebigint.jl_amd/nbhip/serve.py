@@ -1,0 +1,236 @@
+"""Continuous-batching multiplication server on one GPU.
+
+Requests carry HBM-resident operands.  The serving loop (`Server.poll`)
+  - retires micro-batches whose completion event has fired (their arena space
+    is freed, their requests get a completion time),
+  - admits queued requests, oldest first, while their results and workspace
+    fit the free part of the HBM budget (one contiguous arena block per
+    micro-batch; a request that can never fit is rejected with
+    NBH_ERR_BUDGET),
+  - plans the admitted requests as one micro-batch *inside* that block
+    (nbh_batch_create_in: no device allocation, no host synchronisation) and
+    launches it on a free stream while earlier micro-batches are still running.
+So requests that arrive while others are on the GPU start as soon as a stream
+and memory are free, and their work overlaps on the device.  Plans and
+twiddle tables stay resident across batches (the library's caches).
+
+Latency of a request = time its completion was observed by the loop minus its
+arrival time (the device finished at or before the observation).
+"""
+from __future__ import annotations
+
+import ctypes
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from ._lib import NBH_ERR_BUDGET, NbhError, check, init, lib, nbh_job
+
+ALIGN = 256
+
+
+@dataclass
+class Request:
+    a: torch.Tensor                      # operand limbs (int64 view of uint64), device
+    b: torch.Tensor | None               # None for a square
+    bits_a: int
+    bits_b: int
+    t_arrival: float = 0.0
+    t_admit: float = 0.0
+    t_done: float = 0.0
+    result: torch.Tensor | None = None   # view into the arena while in flight
+    keep: bool = False                   # copy the result out on completion
+    out: torch.Tensor | None = None
+    rid: int = 0
+
+    def __post_init__(self):
+        self.dev_a = self.a.data_ptr()
+        self.dev_b = self.b.data_ptr() if self.b is not None else 0
+        self.m = self.a.numel()
+        self.n = self.m if self.b is None else self.b.numel()
+        self.limbs = self.m + self.n
+
+
+class Arena:
+    """First-fit allocator of byte ranges in one device buffer."""
+
+    def __init__(self, size: int):
+        self.size = size
+        self.free = [(0, size)]          # sorted (offset, length)
+
+    def alloc(self, n: int):
+        n = (n + ALIGN - 1) // ALIGN * ALIGN
+        for i, (off, ln) in enumerate(self.free):
+            if ln >= n:
+                if ln == n:
+                    self.free.pop(i)
+                else:
+                    self.free[i] = (off + n, ln - n)
+                return off, n
+        return None
+
+    def release(self, off: int, n: int):
+        self.free.append((off, n))
+        self.free.sort()
+        merged = []
+        for o, ln in self.free:
+            if merged and merged[-1][0] + merged[-1][1] == o:
+                merged[-1] = (merged[-1][0], merged[-1][1] + ln)
+            else:
+                merged.append((o, ln))
+        self.free = merged
+
+    @property
+    def largest_free(self) -> int:
+        return max((ln for _, ln in self.free), default=0)
+
+
+@dataclass
+class _Flight:
+    reqs: list
+    handle: ctypes.c_void_p
+    event: torch.cuda.Event
+    block: tuple
+    stream: int
+    rec: np.ndarray   # the job records (kept until the batch is released)
+
+
+@dataclass
+class Server:
+    budget_bytes: int
+    nstreams: int = 4
+    max_batch: int = 512
+    pending: deque = field(default_factory=deque)
+    done: list = field(default_factory=list)
+    rejected: list = field(default_factory=list)
+
+    def __post_init__(self):
+        init()
+        self.mem = torch.empty(self.budget_bytes, dtype=torch.uint8, device="cuda")
+        self.base = self.mem.data_ptr()
+        self.arena = Arena(self.budget_bytes)
+        self.streams = [torch.cuda.Stream() for _ in range(self.nstreams)]
+        self.busy = [None] * self.nstreams
+        self.batches = 0
+        self._rid = 0
+        self._need_cache = {}
+        self.host_admit_s = 0.0   # host time spent planning and launching micro-batches
+
+    # -- requests ---------------------------------------------------------------
+    def submit(self, req: Request, t: float | None = None) -> Request:
+        req.t_arrival = time.perf_counter() if t is None else t
+        req.rid = self._rid
+        self._rid += 1
+        self.pending.append(req)
+        return req
+
+    def idle(self) -> bool:
+        return not self.pending and all(b is None for b in self.busy)
+
+    # -- the loop -----------------------------------------------------------------
+    def _jobs(self, reqs, results_off):
+        """nbh_job records (r, a, b, m, n, bits_a, bits_b) as one int64 array."""
+        rec = np.empty((len(reqs), 7), dtype=np.int64)
+        off = results_off
+        for i, r in enumerate(reqs):
+            rec[i] = (self.base + off, r.dev_a, r.dev_b, r.m, r.n, r.bits_a, r.bits_b)
+            off += (8 * r.limbs + ALIGN - 1) // ALIGN * ALIGN
+        return rec, off - results_off
+
+    @staticmethod
+    def _ptr(rec):
+        return rec.ctypes.data_as(ctypes.POINTER(nbh_job))
+
+    def _need1(self, r):
+        """Arena bytes of one request alone (result + workspace + tables).  A
+        micro-batch needs at most the sum over its requests (its tables are
+        shared and aligned once), so admission sums these."""
+        key = (r.a.numel(), None if r.b is None else r.b.numel(), r.bits_a, r.bits_b)
+        v = self._need_cache.get(key)
+        if v is None:
+            rec, res_bytes = self._jobs([r], 0)
+            ws = ctypes.c_int64()
+            check(lib().nbh_batch_arena_bytes(self._ptr(rec), 1, ctypes.byref(ws)))
+            v = res_bytes + ws.value + ALIGN
+            self._need_cache[key] = v
+        return v
+
+    def _retire(self):
+        now = time.perf_counter()
+        for k, f in enumerate(self.busy):
+            if f is not None and f.event.query():
+                for r in f.reqs:
+                    r.t_done = now
+                    if r.keep:
+                        with torch.cuda.stream(self.streams[k]):
+                            r.out = r.result.clone()
+                    r.result = None
+                    self.done.append(r)
+                if any(r.keep for r in f.reqs):
+                    self.streams[k].synchronize()
+                lib().nbh_batch_release(f.handle)
+                self.arena.release(*f.block)
+                self.busy[k] = None
+
+    def _admit(self, k: int):
+        t_start = time.perf_counter()
+        reqs = []
+        need = 0
+        room = self.arena.largest_free
+        while self.pending and len(reqs) < self.max_batch:
+            nd = self._need1(self.pending[0])
+            if nd > self.budget_bytes:                    # can never fit
+                self.rejected.append(self.pending.popleft())
+                continue
+            if need + nd > room:
+                break
+            need += nd
+            reqs.append(self.pending.popleft())
+        if not reqs:
+            return
+        block = self.arena.alloc(need)
+        assert block is not None
+        off = block[0]
+        rec, res_bytes = self._jobs(reqs, off)
+        s = self.streams[k]
+        h = ctypes.c_void_p()
+        rc = lib().nbh_batch_create_in(self._ptr(rec), len(reqs), ctypes.c_void_p(self.base + off + res_bytes),
+                                       block[1] - res_bytes, ctypes.c_void_p(s.cuda_stream), ctypes.byref(h))
+        check(rc)
+        check(lib().nbh_batch_run(h, ctypes.c_void_p(s.cuda_stream)))
+        ev = torch.cuda.Event()
+        ev.record(s)
+        now = time.perf_counter()
+        o = off
+        for i, r in enumerate(reqs):
+            r.t_admit = now
+            r.result = self.mem[o:o + 8 * r.limbs].view(torch.int64)
+            o += (8 * r.limbs + ALIGN - 1) // ALIGN * ALIGN
+        self.busy[k] = _Flight(reqs, h, ev, block, k, rec)
+        self.batches += 1
+        self.host_admit_s += time.perf_counter() - t_start
+
+    def poll(self) -> None:
+        self._retire()
+        for k in range(self.nstreams):
+            if self.busy[k] is None and self.pending:
+                self._admit(k)
+
+    def drain(self) -> None:
+        while not self.idle():
+            self.poll()
+
+
+def budget_error(req: Request) -> NbhError:
+    return NbhError(NBH_ERR_BUDGET, f"request {req.rid} ({req.limbs} limbs) exceeds the HBM budget")
+
+
+def latency_stats(reqs) -> dict:
+    lat = np.array([r.t_done - r.t_arrival for r in reqs]) * 1e3
+    if lat.size == 0:
+        return {}
+    return {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_ms": float(lat.mean()), "max_ms": float(lat.max()), "requests": int(lat.size)}

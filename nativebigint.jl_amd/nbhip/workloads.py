@@ -88,6 +88,25 @@ def batch_jobs(count: int = 4096, lo: int = 152, hi: int = 65536, sqr_frac: floa
     return jobs
 
 
+def serve_jobs(count: int, lo: int = 152, hi: int = 262144, sqr_frac: float = 0.3,
+               unbal_frac: float = 0.2, max_ratio: float = 8.0, workload: str = "serve") -> list[Job]:
+    """Serving stream: sizes log-uniform in [lo, hi]; 30% squares, 20%
+    unbalanced with ratio uniform in [1, max_ratio], the rest balanced."""
+    u = _uniform01(stream_seed(workload, 0xFFFFFE, 2), 3 * count).reshape(count, 3)
+    jobs = []
+    for k in range(count):
+        L = int(math.floor(math.exp(math.log(lo) + u[k, 0] * (math.log(hi) - math.log(lo)))))
+        L = max(lo, min(hi, L))
+        if u[k, 1] < sqr_frac:
+            jobs.append(Job(L, L, True))
+        elif u[k, 1] < sqr_frac + unbal_frac:
+            rho = 1.0 + u[k, 2] * (max_ratio - 1.0)
+            jobs.append(Job(L, max(lo, int(L // rho)), False))
+        else:
+            jobs.append(Job(L, L, False))
+    return jobs
+
+
 def batch_operands(jobs: list[Job], workload: str = "batch"):
     """(a, b) limb arrays per job; b is None for squarings."""
     out = []
