@@ -58,13 +58,58 @@ def counter_stats(cur):
     return out
 
 
+def counter_per_dispatch(db: str, counter: str):
+    """{kernel short name: [per-dispatch sums of `counter`]}."""
+    cur = sqlite3.connect(db).cursor()
+    rows = cur.execute("select kernel_name, dispatch_id, sum(value) from counters_collection "
+                       "where counter_name = ? group by kernel_name, dispatch_id", (counter,)).fetchall()
+    out = collections.defaultdict(list)
+    for name, _, v in rows:
+        out[short(name)].append(v)
+    return out
+
+
+def traffic(fetch_db: str, write_db: str, kernels, copy_bytes: int, copy_kernel: str = "copyBuffer"):
+    """HBM bytes per launch of each kernel from FETCH_SIZE and WRITE_SIZE
+    (KiB per dispatch), each scaled by the factor that makes the calibration
+    copy of `copy_bytes` read and write exactly that many bytes."""
+    f = counter_per_dispatch(fetch_db, "FETCH_SIZE")
+    w = counter_per_dispatch(write_db, "WRITE_SIZE")
+    cf = [v for k, vs in f.items() if copy_kernel in k for v in vs]
+    cw = [v for k, vs in w.items() if copy_kernel in k for v in vs]
+    big_f, big_w = max(cf), max(cw)                 # the calibration copy is the largest copy
+    f_scale = copy_bytes / (big_f * 1024)
+    w_scale = copy_bytes / (big_w * 1024)
+    out = {"copy_bytes": copy_bytes, "fetch_scale": f_scale, "write_scale": w_scale, "kernels": {}}
+    for k in kernels:
+        fk = [v for name, vs in f.items() if k in name for v in vs]
+        wk = [v for name, vs in w.items() if k in name for v in vs]
+        if not fk or not wk:
+            continue
+        fetch = sum(fk) * 1024 * f_scale / len(fk)
+        write = sum(wk) * 1024 * w_scale / len(wk)
+        out["kernels"][k] = {"dispatches": len(fk), "fetch_bytes_per_launch": fetch,
+                             "write_bytes_per_launch": write, "bytes_per_launch": fetch + write}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("db")
     ap.add_argument("--title", default="rocprofv3 summary")
     ap.add_argument("--command", default="")
     ap.add_argument("--dispatches", default="", help="also list every dispatch of kernels matching this")
+    ap.add_argument("--traffic-write-db", default="",
+                    help="traffic mode: db is the FETCH_SIZE run, this the WRITE_SIZE run; prints JSON")
+    ap.add_argument("--copy-bytes", type=int, default=256 << 20)
+    ap.add_argument("--kernels", default="k_mid,k_group,k_unpack,k_carry_fix,k_combine")
     a = ap.parse_args()
+    if a.traffic_write_db:
+        import json
+        res = traffic(a.db, a.traffic_write_db, a.kernels.split(","), a.copy_bytes)
+        res["command"] = a.command
+        print(json.dumps(res, indent=1))
+        return
     cur = sqlite3.connect(a.db).cursor()
     t = tables(cur)
     lines = [f"# {a.title}", ""]
