@@ -191,7 +191,10 @@ struct TileGeo {
 // b (twiddle index jt + a Q1), stage s+1's over a (twiddle index jt); the
 // reverse runs them in the opposite order.  Exactly the butterflies of two
 // fp_radix4! sweeps, one LDS round trip instead of two.
-template <int K, bool FWD>
+// LIN: both leg strides are multiples of 16 points, so the padded index of
+// leg (a, b) is pidx(base) + a*pidx-stride(q1) + b*pidx-stride(q0), one add
+// per point instead of a pidx per point.
+template <int K, bool FWD, bool LIN>
 __device__ void pass44(double *__restrict__ x0, double *__restrict__ x1, int narr, const StageU &s0,
                        const StageU &s1, const TileGeo &g) {
     const int q1 = s1.q;
@@ -206,11 +209,16 @@ __device__ void pass44(double *__restrict__ x0, double *__restrict__ x1, int nar
         const int r = sl >> g.logC;
         const int jj = r & (q1 - 1);
         const int row0 = ((r >> logq1) << 4) * q1 + jj;
+        const int base = (row0 << g.logC) + c;
+        const int p0 = pidx(base);
+        const int sa = q1 << g.logC, sb = q0 << g.logC;
+        const int pa = sa + (sa >> 4), pb = sb + (sb >> 4);
+        auto ix = [&](int a, int b) { return LIN ? p0 + a * pa + b * pb : pidx(base + a * sa + b * sb); };
         double y[4][4];
 #pragma unroll
         for (int a = 0; a < 4; a++)
 #pragma unroll
-            for (int b = 0; b < 4; b++) y[a][b] = x[pidx(((row0 + a * q1 + b * q0) << g.logC) + c)];
+            for (int b = 0; b < 4; b++) y[a][b] = x[ix(a, b)];
         const int64_t jt = (int64_t)jj * g.Lb + g.jcol0 + c;
         if (FWD) {
 #pragma unroll
@@ -229,7 +237,7 @@ __device__ void pass44(double *__restrict__ x0, double *__restrict__ x1, int nar
 #pragma unroll
         for (int a = 0; a < 4; a++)
 #pragma unroll
-            for (int b = 0; b < 4; b++) x[pidx(((row0 + a * q1 + b * q0) << g.logC) + c)] = y[a][b];
+            for (int b = 0; b < 4; b++) x[ix(a, b)] = y[a][b];
     }
 }
 
@@ -237,7 +245,7 @@ __device__ void pass44(double *__restrict__ x0, double *__restrict__ x1, int nar
 // 494-535, 630-638, 645-701): forward = combine, reduce leg 0, twiddle legs
 // 1..M-1; reverse = reduce + twiddle, then combine (radix 5/7 re-reduce leg
 // 0); radix 2 is twiddle-free and the same both ways.
-template <int K, bool FWD, int M>
+template <int K, bool FWD, int M, bool LIN>
 __device__ void pass1(double *__restrict__ x0, double *__restrict__ x1, int narr, const StageU &s,
                       const TileGeo &g) {
     const int q = s.q;
@@ -253,9 +261,11 @@ __device__ void pass1(double *__restrict__ x0, double *__restrict__ x1, int narr
         const uint32_t jj = r - ob * (uint32_t)q;
         const int base = ((int)(ob * (uint32_t)(M * q) + jj) << g.logC) + c;
         const int64_t j = (int64_t)jj * g.Lb + g.jcol0 + c;
+        const int p0 = pidx(base), pstride = strideC + (strideC >> 4);
+        auto ix = [&](int t) { return LIN ? p0 + t * pstride : pidx(base + t * strideC); };
         double y[M];
 #pragma unroll
-        for (int t = 0; t < M; t++) y[t] = x[pidx(base + t * strideC)];
+        for (int t = 0; t < M; t++) y[t] = x[ix(t)];
         if (M == 2) {
             const double u = y[0], v = y[1];
             y[0] = fp_reduce<K>(u + v);
@@ -289,19 +299,32 @@ __device__ void pass1(double *__restrict__ x0, double *__restrict__ x1, int narr
             }
         }
 #pragma unroll
-        for (int t = 0; t < M; t++) x[pidx(base + t * strideC)] = y[t];
+        for (int t = 0; t < M; t++) x[ix(t)] = y[t];
+    }
+}
+
+template <int K, bool FWD, bool LIN>
+__device__ __forceinline__ void pass_one_t(double *x0, double *x1, int narr, const StageU &s, const TileGeo &g) {
+    switch (s.m) {
+    case 4: pass1<K, FWD, 4, LIN>(x0, x1, narr, s, g); break;
+    case 2: pass1<K, FWD, 2, LIN>(x0, x1, narr, s, g); break;
+    case 3: pass1<K, FWD, 3, LIN>(x0, x1, narr, s, g); break;
+    case 5: pass1<K, FWD, 5, LIN>(x0, x1, narr, s, g); break;
+    default: pass1<K, FWD, 7, LIN>(x0, x1, narr, s, g); break;
     }
 }
 
 template <int K, bool FWD>
 __device__ __forceinline__ void pass_one(double *x0, double *x1, int narr, const StageU &s, const TileGeo &g) {
-    switch (s.m) {
-    case 4: pass1<K, FWD, 4>(x0, x1, narr, s, g); break;
-    case 2: pass1<K, FWD, 2>(x0, x1, narr, s, g); break;
-    case 3: pass1<K, FWD, 3>(x0, x1, narr, s, g); break;
-    case 5: pass1<K, FWD, 5>(x0, x1, narr, s, g); break;
-    default: pass1<K, FWD, 7>(x0, x1, narr, s, g); break;
-    }
+    if (((s.q << g.logC) & 15) == 0) pass_one_t<K, FWD, true>(x0, x1, narr, s, g);
+    else pass_one_t<K, FWD, false>(x0, x1, narr, s, g);
+}
+
+template <int K, bool FWD>
+__device__ __forceinline__ void pass_44(double *x0, double *x1, int narr, const StageU &a, const StageU &b,
+                                        const TileGeo &g) {
+    if ((((a.q | b.q) << g.logC) & 15) == 0) pass44<K, FWD, true>(x0, x1, narr, a, b, g);
+    else pass44<K, FWD, false>(x0, x1, narr, a, b, g);
 }
 
 // Stages [s0, s1) of a plan over the LDS tile(s), radix-4 pairs fused.  Any
@@ -316,7 +339,7 @@ __device__ void run_stages(double *x0, double *x1, int narr, const DevPlan *__re
             const StageU a = stage_u(&P->st[s], tw);
             if (a.m == 4 && s + 1 < s1 && uni(cst(P)->st[s + 1].m) == 4) {
                 const StageU b = stage_u(&P->st[s + 1], tw);
-                pass44<K, true>(x0, x1, narr, a, b, g);
+                pass_44<K, true>(x0, x1, narr, a, b, g);
                 s += 2;
             } else {
                 pass_one<K, true>(x0, x1, narr, a, g);
@@ -330,7 +353,7 @@ __device__ void run_stages(double *x0, double *x1, int narr, const DevPlan *__re
             const StageU b = stage_u(&P->st[s], tw);
             if (b.m == 4 && s - 1 >= s0 && uni(cst(P)->st[s - 1].m) == 4) {
                 const StageU a = stage_u(&P->st[s - 1], tw);
-                pass44<K, false>(x0, x1, narr, a, b, g);
+                pass_44<K, false>(x0, x1, narr, a, b, g);
                 s -= 2;
             } else {
                 pass_one<K, false>(x0, x1, narr, b, g);
@@ -346,15 +369,150 @@ constexpr int LU = 16;
 
 // Chunk i of an operand: bits [i*b, i*b + b) (src/fpntt.jl:744-768); chunks
 // past nch are the zero fill.  Chunks are < 2^b < p: exact, canonical.
+// Branch-free: the caller passes bit = i*b and in = (i < nch); a lane that is
+// not `in` reads limb 0 and yields 0, and the second limb read is clamped to
+// the operand.  The chunk (< 2^49) converts exactly as hi*2^32 + lo.
+__device__ __forceinline__ double pack_bits(const uint64_t *__restrict__ limbs, int64_t n, uint64_t bmask, bool in,
+                                            int64_t bit) {
+    bit = in ? bit : 0;
+    const int64_t w = bit >> 6;
+    const uint32_t sh = (uint32_t)bit & 63u;
+    const bool two = w + 1 < n;
+    const uint64_t lo = gldu(limbs, w);
+    const uint64_t hi = gldu(limbs, two ? w + 1 : w);
+    // (hi << 1) << (63 - sh) is hi << (64 - sh) for sh > 0 and 0 for sh == 0
+    const uint64_t c = ((lo >> sh) | (two ? (hi << 1) << (63u - sh) : 0ull)) & bmask;
+    const double d = __builtin_fma((double)(uint32_t)(c >> 32), 4294967296.0, (double)(uint32_t)c);
+    return in ? d : 0.0;
+}
+
 __device__ __forceinline__ double pack_chunk(const uint64_t *__restrict__ limbs, int64_t n, int b,
                                              int64_t nch, int64_t i) {
-    if (i >= nch) return 0.0;
-    const int64_t bit = i * b;
-    const int64_t w = bit >> 6;
-    const int sh = (int)(bit & 63);
-    uint64_t c = gldu(limbs, w) >> sh;
-    if (sh && w + 1 < n) c |= gldu(limbs, w + 1) << (64 - sh);
-    return (double)(c & ((1ull << b) - 1));
+    return pack_bits(limbs, n, (1ull << b) - 1, i < nch, i * (int64_t)b);
+}
+
+// Where a tile's points come from: spectra x, or (pack) the b-bit chunks of
+// an operand's limbs.
+struct PointSrc {
+    const double *x;
+    const uint64_t *limbs;
+    int64_t n, nch;
+    int b;
+};
+
+// One round of NU loads per thread, all issued before any result is used:
+// tile point e = e0 + k*T is transform element gbase + (e >> logC)*Lb +
+// (e & (C-1)).  In a full round (every e < total) with T a multiple of C the
+// element and bit positions advance by a uniform step; otherwise a lane past
+// `total` re-reads point 0.
+template <bool PACK, int NU>
+__device__ __forceinline__ void load_round(double (&v)[NU], const PointSrc &src, int e0, int total, bool full,
+                                           const TileGeo &g, int64_t gbase) {
+    const int T = blockDim.x;
+    const int cmask = (1 << g.logC) - 1;
+    if (full && (T & cmask) == 0) {
+        const int64_t i0 = gbase + (int64_t)(e0 >> g.logC) * g.Lb + (e0 & cmask);
+        const int64_t step = (int64_t)(T >> g.logC) * g.Lb;
+        if (PACK) {
+            const uint64_t bmask = (1ull << src.b) - 1;
+            const int64_t bit0 = i0 * src.b, bstep = step * src.b;
+#pragma unroll
+            for (int k = 0; k < NU; k++)
+                v[k] = pack_bits(src.limbs, src.n, bmask, i0 + k * step < src.nch, bit0 + k * bstep);
+        } else {
+#pragma unroll
+            for (int k = 0; k < NU; k++) v[k] = gld(src.x, i0 + k * step);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < NU; k++) {
+            const int e = e0 + k * T;
+            const int ec = e < total ? e : 0;
+            const int64_t i = gbase + (int64_t)(ec >> g.logC) * g.Lb + (ec & cmask);
+            v[k] = PACK ? pack_chunk(src.limbs, src.n, src.b, src.nch, i) : gld(src.x, i);
+        }
+    }
+}
+
+// Stores of a round into the padded LDS tile: T is a multiple of 64, so
+// pidx(e0 + k*T) = pidx(e0) + k*(T + T/16).
+template <int NU>
+__device__ __forceinline__ void lds_round(double *__restrict__ lds, const double (&v)[NU], int e0, int total,
+                                          bool full) {
+    const int T = blockDim.x;
+    const int p0 = pidx(e0), pstep = T + (T >> 4);
+    if (full) {
+#pragma unroll
+        for (int k = 0; k < NU; k++) lds[p0 + k * pstep] = v[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < NU; k++)
+            if (e0 + k * T < total) lds[p0 + k * pstep] = v[k];
+    }
+}
+
+// A whole tile (total points) from src into LDS: full rounds of NU points per
+// thread, then full rounds of 4, then single points (the last one guarded),
+// so small tiles issue no redundant loads.
+template <int NU, bool PACK>
+__device__ __forceinline__ void load_tile_t(double *__restrict__ lds, const PointSrc &src, int total,
+                                            const TileGeo &g, int64_t gbase) {
+    const int T = blockDim.x;
+    int r0 = 0;
+    for (; r0 + NU * T <= total; r0 += NU * T) {
+        double v[NU];
+        load_round<PACK, NU>(v, src, r0 + (int)threadIdx.x, total, true, g, gbase);
+        lds_round<NU>(lds, v, r0 + (int)threadIdx.x, total, true);
+    }
+    for (; r0 + 4 * T <= total; r0 += 4 * T) {
+        double v[4];
+        load_round<PACK, 4>(v, src, r0 + (int)threadIdx.x, total, true, g, gbase);
+        lds_round<4>(lds, v, r0 + (int)threadIdx.x, total, true);
+    }
+    for (; r0 < total; r0 += T) {
+        double v[1];
+        const bool full = r0 + T <= total;
+        load_round<PACK, 1>(v, src, r0 + (int)threadIdx.x, total, full, g, gbase);
+        lds_round<1>(lds, v, r0 + (int)threadIdx.x, total, full);
+    }
+}
+
+template <int NU>
+__device__ __forceinline__ void load_tile(double *__restrict__ lds, const PointSrc &src, bool pack, int total,
+                                          const TileGeo &g, int64_t gbase) {
+    if (pack) load_tile_t<NU, true>(lds, src, total, g, gbase);
+    else load_tile_t<NU, false>(lds, src, total, g, gbase);
+}
+
+// Both operands of a product into la and lb, rounds interleaving NU loads of
+// each operand.
+template <int NU, bool PACK>
+__device__ __forceinline__ void load_pair(double *__restrict__ la, double *__restrict__ lb, const PointSrc &sa,
+                                          const PointSrc &sb, int S, const TileGeo &g, int64_t gbase) {
+    const int T = blockDim.x;
+    int r0 = 0;
+    for (; r0 + NU * T <= S; r0 += NU * T) {
+        double va[NU], vb[NU];
+        load_round<PACK, NU>(va, sa, r0 + (int)threadIdx.x, S, true, g, gbase);
+        load_round<PACK, NU>(vb, sb, r0 + (int)threadIdx.x, S, true, g, gbase);
+        lds_round<NU>(la, va, r0 + (int)threadIdx.x, S, true);
+        lds_round<NU>(lb, vb, r0 + (int)threadIdx.x, S, true);
+    }
+    for (; r0 + 2 * T <= S; r0 += 2 * T) {
+        double va[2], vb[2];
+        load_round<PACK, 2>(va, sa, r0 + (int)threadIdx.x, S, true, g, gbase);
+        load_round<PACK, 2>(vb, sb, r0 + (int)threadIdx.x, S, true, g, gbase);
+        lds_round<2>(la, va, r0 + (int)threadIdx.x, S, true);
+        lds_round<2>(lb, vb, r0 + (int)threadIdx.x, S, true);
+    }
+    for (; r0 < S; r0 += T) {
+        double va[1], vb[1];
+        const bool full = r0 + T <= S;
+        load_round<PACK, 1>(va, sa, r0 + (int)threadIdx.x, S, full, g, gbase);
+        load_round<PACK, 1>(vb, sb, r0 + (int)threadIdx.x, S, full, g, gbase);
+        lds_round<1>(la, va, r0 + (int)threadIdx.x, S, full);
+        lds_round<1>(lb, vb, r0 + (int)threadIdx.x, S, full);
+    }
 }
 
 // The tile a block works on: from the launch's tile list, or generated.
@@ -385,6 +543,35 @@ __device__ __forceinline__ Tile fetch_tile(const Tile *__restrict__ tiles, const
     return t;
 }
 
+// The LDS tile (rows of 2^logC columns, row r at gbase + r*Lb) back to HBM,
+// SU points per thread per round.
+constexpr int SU = 4;
+
+__device__ __forceinline__ void store_tile(double *__restrict__ x, const double *__restrict__ lds, int total,
+                                           const TileGeo &g, int64_t gbase) {
+    const int T = blockDim.x;
+    const int cmask = (1 << g.logC) - 1;
+    for (int r0 = 0; r0 < total; r0 += SU * T) {
+        const int e0 = r0 + (int)threadIdx.x;
+        if (r0 + SU * T <= total) {
+            double v[SU];
+#pragma unroll
+            for (int k = 0; k < SU; k++) v[k] = lds[pidx(e0 + k * T)];
+#pragma unroll
+            for (int k = 0; k < SU; k++) {
+                const int e = e0 + k * T;
+                gst(x, gbase + (int64_t)(e >> g.logC) * g.Lb + (e & cmask), v[k]);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < SU; k++) {
+                const int e = e0 + k * T;
+                if (e < total) gst(x, gbase + (int64_t)(e >> g.logC) * g.Lb + (e & cmask), lds[pidx(e)]);
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Group pass: one tile of one group of one transform, forward or inverse.
 
@@ -406,34 +593,20 @@ __device__ void group_body(const DevJob *__restrict__ Jp, const Tile &t, bool fw
     g.jcol0 = jc << g.logC;
     const int64_t gbase = blk * La + g.jcol0;
     const int total = g.S << g.logC;
-    const int cmask = (1 << g.logC) - 1;
 
     // Every thread issues all of its loads before its first LDS store so the
     // load latencies overlap (LU points per thread per round).
-    const bool pack = (t.flags & TILE_PACK) != 0;
-    const uint64_t *limbs = unip(t.op ? J->b : J->a);
-    const int64_t n = uni64(t.op ? J->n : J->m);
-    const int64_t nch = uni64(t.op ? J->ncb : J->nca);
-    const int bch = uni(J->bch);
-    for (int e0 = threadIdx.x; e0 < total; e0 += LU * blockDim.x) {
-        double v[LU];
-#pragma unroll
-        for (int k = 0; k < LU; k++) {
-            const int e = e0 + k * (int)blockDim.x;
-            const int64_t gi = gbase + (int64_t)(e >> g.logC) * g.Lb + (e & cmask);
-            v[k] = e < total ? (pack ? pack_chunk(limbs, n, bch, nch, gi) : gld(x, gi)) : 0.0;
-        }
-#pragma unroll
-        for (int k = 0; k < LU; k++) {
-            const int e = e0 + k * (int)blockDim.x;
-            if (e < total) lds[pidx(e)] = v[k];
-        }
-    }
+    PointSrc src;
+    src.x = x;
+    src.limbs = unip(t.op ? J->b : J->a);
+    src.n = uni64(t.op ? J->n : J->m);
+    src.nch = uni64(t.op ? J->ncb : J->nca);
+    src.b = uni(J->bch);
+    load_tile<LU>(lds, src, (t.flags & TILE_PACK) != 0, total, g, gbase);
     __syncthreads();
     if (fwd) run_stages<K, true>(lds, nullptr, 1, P, s0, s1, g);
     else run_stages<K, false>(lds, nullptr, 1, P, s0, s1, g);
-    for (int e = threadIdx.x; e < total; e += blockDim.x)
-        gst(x, gbase + (int64_t)(e >> g.logC) * g.Lb + (e & cmask), lds[pidx(e)]);
+    store_tile(x, lds, total, g, gbase);
 }
 
 __global__ void __launch_bounds__(1024) k_group(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles,
@@ -468,34 +641,22 @@ __device__ void mid_body(const DevJob *__restrict__ Jp, const Tile &t, double *l
     double *la = lds;
     double *lb = lds + lds_doubles(S);
     {
-        // both operands' points (or chunks), all loads of a round issued first
+        // Both operands' points (or chunks): a round issues LU/2 loads of
+        // each operand (LU of a square's) before its LDS stores.
         const bool pack = (t.flags & TILE_PACK) != 0;
-        const uint64_t *A = unip(J->a), *B = unip(J->b);
-        const int64_t m = uni64(J->m), n = uni64(J->n), nca = uni64(J->nca), ncb = uni64(J->ncb);
-        const int bch = uni(J->bch);
-        const double *__restrict__ xa = unip(J->xa[K]);
-        const double *__restrict__ xb = unip(J->xb[K]);
-        const int total = sq ? S : 2 * S;
-        for (int e0 = threadIdx.x; e0 < total; e0 += LU * blockDim.x) {
-            double v[LU];
-#pragma unroll
-            for (int k = 0; k < LU; k++) {
-                const int e = e0 + k * (int)blockDim.x;
-                const bool second = e >= S;
-                const int i = second ? e - S : e;
-                double val = 0.0;
-                if (e < total) {
-                    if (pack) val = second ? pack_chunk(B, n, bch, ncb, gbase + i) : pack_chunk(A, m, bch, nca, gbase + i);
-                    else val = gld(second ? xb : xa, gbase + i);
-                }
-                v[k] = val;
-            }
-#pragma unroll
-            for (int k = 0; k < LU; k++) {
-                const int e = e0 + k * (int)blockDim.x;
-                if (e < total) (e >= S ? lb : la)[pidx(e >= S ? e - S : e)] = v[k];
-            }
-        }
+        PointSrc sa, sb;
+        sa.x = unip(J->xa[K]);
+        sb.x = unip(J->xb[K]);
+        sa.limbs = unip(J->a);
+        sb.limbs = unip(J->b);
+        sa.n = uni64(J->m);
+        sb.n = uni64(J->n);
+        sa.nch = uni64(J->nca);
+        sb.nch = uni64(J->ncb);
+        sa.b = sb.b = uni(J->bch);
+        if (sq) load_tile<LU>(la, sa, pack, S, g, gbase);
+        else if (pack) load_pair<LU / 2, true>(la, lb, sa, sb, S, g, gbase);
+        else load_pair<LU / 2, false>(la, lb, sa, sb, S, g, gbase);
     }
     __syncthreads();
     run_stages<K, true>(la, lb, sq ? 1 : 2, P, s0, s1, g);
@@ -514,7 +675,7 @@ __device__ void mid_body(const DevJob *__restrict__ Jp, const Tile &t, double *l
     __syncthreads();
     run_stages<K, false>(la, nullptr, 1, P, s0, s1, g);
     double *__restrict__ xo = unip(J->xa[K]);
-    for (int e = threadIdx.x; e < S; e += blockDim.x) gst(xo, gbase + e, la[pidx(e)]);
+    store_tile(xo, la, S, g, gbase);
 }
 
 __global__ void __launch_bounds__(1024) k_mid(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles,
