@@ -119,6 +119,41 @@ int32_t nbh_hgcd_create(const nbh_hgcd_job *jobs, int64_t count, int32_t mode, n
 int32_t nbh_hgcd_run(nbh_hgcd h, void *stream);
 void nbh_hgcd_destroy(nbh_hgcd h);
 
+/* Cached forward spectra of a fixed operand.  barrett_reduce!
+ * (src/barrett.jl:44-81), which powermod calls once per step
+ * (src/algorithms.jl:17-107), multiplies by the same modulus m and reciprocal
+ * mu every time, and each mul! re-packs and re-transforms them
+ * (src/fpntt.jl:1002-1009).  A spectrum holds b's forward transforms under
+ * both primes at the chunk width and length that fit every a of up to
+ * bits_a_max bits (fp_ntt_params(bits_a_max, bits_b), src/fpntt.jl:964-977:
+ * a shorter a keeps the exactness bound and needs no longer transform), so a
+ * product against it transforms a and runs the inverse only: 4 of the 6
+ * transforms of mul_fpntt2!.  The product limbs are the same as
+ * nbh_mul_dev's.  No reference counterpart.
+ *   nbh_spectrum_create: b (n limbs, device) is read on `stream` only; the
+ *     spectrum is ready once `stream` reaches this point.  bits_b = -1 reads
+ *     the top limb.  The spectrum owns 2*N doubles of device memory.
+ *   nbh_mul_spec_dev: r[0..m+n) = a[0..m) * b, bits_a <= bits_a_max
+ *     (else NBH_ERR_ARG), bits_a = -1 reads the top limb.  The spectrum is
+ *     only read, so products against it may run concurrently.
+ *   nbh_spec_batch_create: a batch (run / destroy with nbh_batch_run /
+ *     nbh_batch_destroy) of products against spectra.
+ *   nbh_spectrum_destroy waits for the device to go idle, then frees it. */
+typedef struct nbh_spectrum_s *nbh_spectrum;
+int32_t nbh_spectrum_create(const uint64_t *b, int64_t n, int64_t bits_b, int64_t bits_a_max, void *stream,
+                            nbh_spectrum *out);
+void nbh_spectrum_destroy(nbh_spectrum s);
+int32_t nbh_spectrum_info(nbh_spectrum s, int64_t *N, int32_t *b, int64_t *n, int64_t *bits_a_max);
+int32_t nbh_mul_spec_dev(uint64_t *r, const uint64_t *a, int64_t m, int64_t bits_a, nbh_spectrum s, void *stream);
+typedef struct {
+    uint64_t *r;
+    const uint64_t *a;
+    int64_t m;
+    int64_t bits_a;
+    nbh_spectrum s;
+} nbh_spec_job;
+int32_t nbh_spec_batch_create(const nbh_spec_job *jobs, int64_t count, nbh_batch *out);
+
 /* Stage-level entry points over device buffers, one per reference function,
  * used by the parity tests: fp_ntt_fwd! (src/fpntt.jl:715), fp_ntt_rev!
  * (:727), fp_ntt_pack! (:744), fp_ntt_pointwise! (:770), fp_ntt_unpack2!

@@ -28,6 +28,15 @@
 
 using namespace nbh;
 
+// A fixed operand's finished forward spectra (both primes) at one length and
+// chunk width.
+struct nbh_spectrum_s {
+    const uint64_t *b = nullptr;
+    int64_t n = 0, bits_b = 0, bits_a_max = 0, N = 0, ncb = 0;
+    int32_t bch = 0, zero = 0;
+    double *x[2] = {nullptr, nullptr};
+};
+
 namespace {
 
 typedef unsigned __int128 u128;
@@ -550,6 +559,27 @@ int size_job(DevJob &J, int64_t bits_a, int64_t bits_b, int64_t *N_out) {
     return NBH_OK;
 }
 
+// Sizing of a product against a cached spectrum: the spectrum's chunk width
+// and length serve every a of up to bits_a_max bits.
+int size_job_spec(DevJob &J, int64_t bits_a, const nbh_spectrum_s &S, int64_t *N_out) {
+    J.rn = J.m + J.n;
+    J.nchunks = (J.rn + UNPACK_LPC - 1) / UNPACK_LPC;
+    if (bits_a > S.bits_a_max) return fail(NBH_ERR_ARG, "operand longer than the spectrum's bits_a_max");
+    if (bits_a == 0 || S.zero) {
+        J.zero = 1;
+        *N_out = 0;
+        return NBH_OK;
+    }
+    J.bch = S.bch;
+    J.bmagic = ~0ull / (uint64_t)S.bch + 1;
+    J.nca = (bits_a + S.bch - 1) / S.bch;
+    J.ncb = S.ncb;
+    J.nconv = J.nca + J.ncb - 1;
+    J.bspec = 1;
+    *N_out = S.N;
+    return NBH_OK;
+}
+
 int read_bits(const uint64_t *a, int64_t n, int64_t *bits) {
     uint64_t top = 0;
     HIPCHK(hipMemcpyAsync(&top, a + (n - 1), sizeof top, hipMemcpyDeviceToHost, g.stream));
@@ -560,7 +590,8 @@ int read_bits(const uint64_t *a, int64_t n, int64_t *bits) {
 
 // One product on g.stream: pack+forward groups, mid, inverse groups, unpack.
 int run_single(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int64_t n, int64_t bits_a,
-               int64_t bits_b, bool sqr) {
+               int64_t bits_b, bool sqr, const nbh_spectrum_s *spec = nullptr) {
+    if (spec) { b = spec->b; n = spec->n; bits_b = spec->bits_b; }
     if (!r || !a || m < 1 || (!sqr && (!b || n < 1))) return fail(NBH_ERR_ARG, "null buffer or empty operand");
     if (sqr) { b = a; n = m; }
     if (bits_a < 0) RCCHK(read_bits(a, m, &bits_a));
@@ -575,7 +606,8 @@ int run_single(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int
     J.n = n;
     J.sqr = sqr ? 1 : 0;
     int64_t N = 0;
-    RCCHK(size_job(J, bits_a, bits_b, &N));
+    if (spec) RCCHK(size_job_spec(J, bits_a, *spec, &N));
+    else RCCHK(size_job(J, bits_a, bits_b, &N));
     if (J.zero) {
         HIPCHK(launch_fill_u64(r, J.rn, 0, g.stream));
         return NBH_OK;
@@ -583,7 +615,8 @@ int run_single(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int
     PlanEntry *P[2];
     RCCHK(get_plan(N, 0, &P[0]));
     RCCHK(get_plan(N, 1, &P[1]));
-    const int nops = sqr ? 1 : 2;
+    // operands transformed here: a, and b unless it is a's square or cached
+    const int nops = sqr || spec ? 1 : 2;
     const size_t xbytes = align256(sizeof(double) * (size_t)N);
     const size_t need = align256(sizeof(uint32_t) * (size_t)J.nchunks) + xbytes * 2 * nops;
     RCCHK(ensure_buf(&g.ws, &g.ws_size, need));
@@ -594,7 +627,9 @@ int run_single(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int
         J.plan[k] = P[k]->dev;
         J.xa[k] = (double *)w;
         w += xbytes;
-        if (!sqr) {
+        if (spec) {
+            J.xb[k] = spec->x[k];
+        } else if (!sqr) {
             J.xb[k] = (double *)w;
             w += xbytes;
         }
@@ -612,8 +647,9 @@ int run_single(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int
     {
         TileGen tg{K - 1, 1, H.g[K - 1].tiles, K == 1 ? TILE_PACK : 0, 0};
         ProfScope ps(1, g.stream);
+        // a cached b still takes an LDS array, but only a is transformed
         HIPCHK(launch_mid(slot, nullptr, tg, 2 * H.g[K - 1].tiles, P[0]->mid_lds_bytes[sqr ? 1 : 0],
-                          P[0]->mid_threads[sqr ? 1 : 0], g.stream));
+                          P[0]->mid_threads[sqr || spec ? 1 : 0], g.stream));
     }
     for (int gi = K - 2; gi >= 0; gi--) {
         TileGen tg{gi, 1, H.g[gi].tiles, 0, 0};
@@ -712,8 +748,17 @@ void batch_free(nbh_batch_s *B) {
 // set, place everything inside it and upload the tables on `astream` -- no
 // device allocation and no host synchronisation, so a server can plan batches
 // while others run.  Otherwise allocate with hipMalloc.
-int batch_create(const nbh_job *jobs, int64_t count, nbh_batch_s **out, size_t *need = nullptr,
-                 void *arena = nullptr, size_t arena_bytes = 0, hipStream_t astream = nullptr) {
+// A job as the planner sees it: a product, a square (b == NULL) or a product
+// against a cached spectrum (spec set; b and n are the spectrum's).
+struct JobIn {
+    uint64_t *r;
+    const uint64_t *a, *b;
+    int64_t m, n, bits_a, bits_b;
+    const nbh_spectrum_s *spec;
+};
+
+int batch_plan(const JobIn *jobs, int64_t count, nbh_batch_s **out, size_t *need, void *arena,
+               size_t arena_bytes, hipStream_t astream) {
     if (count < 1 || !jobs) return fail(NBH_ERR_ARG, "empty batch");
     std::unique_ptr<nbh_batch_s, void (*)(nbh_batch_s *)> B(new nbh_batch_s, batch_free);
     std::vector<DevJob> &J = B->jobs;
@@ -721,10 +766,10 @@ int batch_create(const nbh_job *jobs, int64_t count, nbh_batch_s **out, size_t *
     std::vector<int64_t> bits((size_t)(2 * count), 0);
     bool need_top = false;
     for (int64_t i = 0; i < count; i++) {
-        const nbh_job &q = jobs[i];
+        const JobIn &q = jobs[i];
         DevJob &d = J[(size_t)i];
         std::memset(&d, 0, sizeof d);
-        const bool sqr = q.b == nullptr;
+        const bool sqr = q.b == nullptr && !q.spec;
         if (!q.r || !q.a || q.m < 1 || (!sqr && q.n < 1)) return fail(NBH_ERR_ARG, "bad job " + std::to_string(i));
         d.a = q.a;
         d.b = sqr ? q.a : q.b;
@@ -734,7 +779,7 @@ int batch_create(const nbh_job *jobs, int64_t count, nbh_batch_s **out, size_t *
         d.sqr = sqr;
         bits[(size_t)(2 * i)] = q.bits_a;
         bits[(size_t)(2 * i + 1)] = sqr ? q.bits_a : q.bits_b;
-        if (q.bits_a <= 0 || (!sqr && q.bits_b <= 0)) need_top = true;
+        if (q.bits_a <= 0 || (!sqr && !q.spec && q.bits_b <= 0)) need_top = true;
     }
     if (need_top && (need || arena)) return fail(NBH_ERR_ARG, "arena batches need the operands' bit lengths");
     if (need_top) {
@@ -762,13 +807,14 @@ int batch_create(const nbh_job *jobs, int64_t count, nbh_batch_s **out, size_t *
     std::vector<size_t> offs((size_t)count);
     for (int64_t i = 0; i < count; i++) {
         DevJob &d = J[(size_t)i];
-        RCCHK(size_job(d, bits[(size_t)(2 * i)], bits[(size_t)(2 * i + 1)], &Ns[(size_t)i]));
+        if (jobs[i].spec) RCCHK(size_job_spec(d, bits[(size_t)(2 * i)], *jobs[i].spec, &Ns[(size_t)i]));
+        else RCCHK(size_job(d, bits[(size_t)(2 * i)], bits[(size_t)(2 * i + 1)], &Ns[(size_t)i]));
         offs[(size_t)i] = ws;
         ws += align256(sizeof(uint32_t) * (size_t)d.nchunks);
         if (d.zero) continue;
         RCCHK(get_plan(Ns[(size_t)i], 0, &plan0[(size_t)i]));
         RCCHK(get_plan(Ns[(size_t)i], 1, &plan1[(size_t)i]));
-        ws += align256(sizeof(double) * (size_t)Ns[(size_t)i]) * (d.sqr ? 2 : 4);
+        ws += align256(sizeof(double) * (size_t)Ns[(size_t)i]) * (d.sqr || d.bspec ? 2 : 4);
     }
     B->ws = nullptr;
     std::vector<size_t> spec_off((size_t)count);
@@ -785,7 +831,9 @@ int batch_create(const nbh_job *jobs, int64_t count, nbh_batch_s **out, size_t *
         for (int k = 0; k < 2; k++) {
             d.xa[k] = (double *)w;
             w += xb;
-            if (!d.sqr) {
+            if (d.bspec) {
+                d.xb[k] = jobs[i].spec->x[k];  // a real address: not rebased
+            } else if (!d.sqr) {
                 d.xb[k] = (double *)w;
                 w += xb;
             }
@@ -839,7 +887,7 @@ int batch_create(const nbh_job *jobs, int64_t count, nbh_batch_s **out, size_t *
             const int K = plan0[(size_t)i]->host.ngroups;
             const int gi = K - 1 - d;
             if (gi < 0) continue;
-            add_group_tiles(i, gi, J[(size_t)i].sqr ? 1 : 2, gi == 0 ? TILE_PACK : 0,
+            add_group_tiles(i, gi, J[(size_t)i].sqr || J[(size_t)i].bspec ? 1 : 2, gi == 0 ? TILE_PACK : 0,
                             plan0[(size_t)i]->grp_lds_bytes[gi], plan0[(size_t)i]->grp_threads[gi]);
         }
         push_phase(0, 1, list);
@@ -848,7 +896,7 @@ int batch_create(const nbh_job *jobs, int64_t count, nbh_batch_s **out, size_t *
         if (J[(size_t)i].zero) continue;
         const int K = plan0[(size_t)i]->host.ngroups;
         add_group_tiles(i, K - 1, 1, K == 1 ? TILE_PACK : 0, plan0[(size_t)i]->mid_lds_bytes[J[(size_t)i].sqr ? 1 : 0],
-                        plan0[(size_t)i]->mid_threads[J[(size_t)i].sqr ? 1 : 0]);
+                        plan0[(size_t)i]->mid_threads[J[(size_t)i].sqr || J[(size_t)i].bspec ? 1 : 0]);
     }
     push_phase(1, 1, list);
     for (int d = 1; d <= maxK - 1; d++) {
@@ -895,7 +943,7 @@ int batch_create(const nbh_job *jobs, int64_t count, nbh_batch_s **out, size_t *
         d.chunk_flags = rebase(d.chunk_flags);
         for (int k = 0; k < 2; k++) {
             d.xa[k] = rebase(d.xa[k]);
-            d.xb[k] = rebase(d.xb[k]);
+            if (!d.bspec) d.xb[k] = rebase(d.xb[k]);
         }
     }
     if (arena) {
@@ -920,6 +968,17 @@ int batch_create(const nbh_job *jobs, int64_t count, nbh_batch_s **out, size_t *
     }
     *out = B.release();
     return NBH_OK;
+}
+
+int batch_create(const nbh_job *jobs, int64_t count, nbh_batch_s **out, size_t *need = nullptr,
+                 void *arena = nullptr, size_t arena_bytes = 0, hipStream_t astream = nullptr) {
+    if (count < 1 || !jobs) return fail(NBH_ERR_ARG, "empty batch");
+    std::vector<JobIn> in((size_t)count);
+    for (int64_t i = 0; i < count; i++) {
+        const nbh_job &q = jobs[i];
+        in[(size_t)i] = JobIn{q.r, q.a, q.b, q.m, q.n, q.bits_a, q.bits_b, nullptr};
+    }
+    return batch_plan(in.data(), count, out, need, arena, arena_bytes, astream);
 }
 
 int launch_phase(nbh_batch_s *B, const nbh_batch_s::Phase &ph, hipStream_t s) {
@@ -1202,6 +1261,108 @@ int32_t nbh_mul_batch_dev(const nbh_job *jobs, int64_t count, void *stream) {
     batch_free(B);
     if (rc) return rc;
     HIPCHK(e);
+    return NBH_OK;
+}
+
+int32_t nbh_spectrum_create(const uint64_t *b, int64_t n, int64_t bits_b, int64_t bits_a_max, void *stream,
+                            nbh_spectrum *out) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    if (!out || !b || n < 1 || bits_a_max < 1 || bits_b > 64 * n) return fail(NBH_ERR_ARG, "bad argument");
+    *out = nullptr;
+    RCCHK(stream_enter(stream));
+    if (bits_b < 0) RCCHK(read_bits(b, n, &bits_b));
+    std::unique_ptr<nbh_spectrum_s> S(new nbh_spectrum_s);
+    S->b = b;
+    S->n = n;
+    S->bits_b = bits_b;
+    S->bits_a_max = bits_a_max;
+    if (bits_b == 0) {
+        S->zero = 1;
+        *out = S.release();
+        return stream_leave(stream);
+    }
+    int bch = 0;
+    int64_t N = 0;
+    RCCHK(ntt_params(bits_a_max, bits_b, &bch, &N));
+    if (bch < UNPACK_MIN_B) return fail(NBH_ERR_LENGTH, "product too large for device memory (chunk width below 32)");
+    S->bch = bch;
+    S->N = N;
+    S->ncb = (bits_b + bch - 1) / bch;
+    PlanEntry *P[2];
+    RCCHK(get_plan(N, 0, &P[0]));
+    RCCHK(get_plan(N, 1, &P[1]));
+    for (int k = 0; k < 2; k++) {
+        if (hipMalloc((void **)&S->x[k], sizeof(double) * (size_t)N) != hipSuccess) {
+            if (S->x[0]) (void)hipFree(S->x[0]);
+            return fail(NBH_ERR_OOM, "spectrum allocation failed");
+        }
+    }
+    // b as operand a of a one-operand job: every forward group, the first
+    // one packing the chunks, both primes per launch
+    DevJob J;
+    std::memset(&J, 0, sizeof J);
+    J.a = b;
+    J.m = n;
+    J.nca = S->ncb;
+    J.bch = bch;
+    for (int k = 0; k < 2; k++) {
+        J.plan[k] = P[k]->dev;
+        J.xa[k] = S->x[k];
+    }
+    DevJob *slot = g.slots + (g.slot_next++ % NSLOTS);
+    HIPCHK(launch_store_job(J, slot, g.stream));
+    const DevPlan &H = P[0]->host;
+    for (int gi = 0; gi < H.ngroups; gi++) {
+        TileGen tg{gi, 1, H.g[gi].tiles, gi == 0 ? TILE_PACK : 0, 0};
+        HIPCHK(launch_group(slot, nullptr, tg, 2 * H.g[gi].tiles, 1, P[0]->grp_lds_bytes[gi], P[0]->grp_threads[gi],
+                            g.stream));
+    }
+    *out = S.release();
+    return stream_leave(stream);
+}
+
+void nbh_spectrum_destroy(nbh_spectrum s) {
+    if (!s) return;
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    // queued work on any stream may still read the spectrum
+    (void)hipDeviceSynchronize();
+    for (int k = 0; k < 2; k++)
+        if (s->x[k]) (void)hipFree(s->x[k]);
+    delete s;
+}
+
+int32_t nbh_spectrum_info(nbh_spectrum s, int64_t *N, int32_t *b, int64_t *n, int64_t *bits_a_max) {
+    if (!s) return fail(NBH_ERR_ARG, "null handle");
+    if (N) *N = s->N;
+    if (b) *b = s->bch;
+    if (n) *n = s->n;
+    if (bits_a_max) *bits_a_max = s->bits_a_max;
+    return NBH_OK;
+}
+
+int32_t nbh_mul_spec_dev(uint64_t *r, const uint64_t *a, int64_t m, int64_t bits_a, nbh_spectrum s, void *stream) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    if (!s) return fail(NBH_ERR_ARG, "null spectrum");
+    RCCHK(stream_enter(stream));
+    RCCHK(run_single(r, a, m, nullptr, 0, bits_a, 0, false, s));
+    return stream_leave(stream);
+}
+
+int32_t nbh_spec_batch_create(const nbh_spec_job *jobs, int64_t count, nbh_batch *out) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    if (!out || !jobs || count < 1) return fail(NBH_ERR_ARG, "bad argument");
+    std::vector<JobIn> in((size_t)count);
+    for (int64_t i = 0; i < count; i++) {
+        const nbh_spec_job &q = jobs[i];
+        if (!q.s) return fail(NBH_ERR_ARG, "job " + std::to_string(i) + " has no spectrum");
+        in[(size_t)i] = JobIn{q.r, q.a, q.s->b, q.m, q.s->n, q.bits_a, q.s->bits_b, q.s};
+    }
+    nbh_batch_s *B = nullptr;
+    RCCHK(batch_plan(in.data(), count, &B, nullptr, nullptr, 0, nullptr));
+    *out = B;
     return NBH_OK;
 }
 

@@ -65,7 +65,8 @@ struct DevJob {
     double *xa[2], *xb[2];        // spectra per prime (xb unused for squares)
     uint32_t *chunk_flags;        // unpack chunk carry summaries
     int64_t nchunks;
-    int32_t zero, pad;            // zero: an operand is zero -> r = 0
+    int32_t zero;                 // an operand is zero -> r = 0
+    int32_t bspec;                // xb holds b's finished forward spectra (shared, read-only)
     uint64_t bmagic;              // ceil(2^64 / bch): floor(x / bch) = umulhi(x, bmagic) for x < 2^57
 };
 

@@ -485,31 +485,31 @@ __device__ __forceinline__ void load_tile(double *__restrict__ lds, const PointS
 }
 
 // Both operands of a product into la and lb, rounds interleaving NU loads of
-// each operand.
-template <int NU, bool PACK>
+// each operand (PA / PB: chunks of limbs or spectra).
+template <int NU, bool PA, bool PB>
 __device__ __forceinline__ void load_pair(double *__restrict__ la, double *__restrict__ lb, const PointSrc &sa,
                                           const PointSrc &sb, int S, const TileGeo &g, int64_t gbase) {
     const int T = blockDim.x;
     int r0 = 0;
     for (; r0 + NU * T <= S; r0 += NU * T) {
         double va[NU], vb[NU];
-        load_round<PACK, NU>(va, sa, r0 + (int)threadIdx.x, S, true, g, gbase);
-        load_round<PACK, NU>(vb, sb, r0 + (int)threadIdx.x, S, true, g, gbase);
+        load_round<PA, NU>(va, sa, r0 + (int)threadIdx.x, S, true, g, gbase);
+        load_round<PB, NU>(vb, sb, r0 + (int)threadIdx.x, S, true, g, gbase);
         lds_round<NU>(la, va, r0 + (int)threadIdx.x, S, true);
         lds_round<NU>(lb, vb, r0 + (int)threadIdx.x, S, true);
     }
     for (; r0 + 2 * T <= S; r0 += 2 * T) {
         double va[2], vb[2];
-        load_round<PACK, 2>(va, sa, r0 + (int)threadIdx.x, S, true, g, gbase);
-        load_round<PACK, 2>(vb, sb, r0 + (int)threadIdx.x, S, true, g, gbase);
+        load_round<PA, 2>(va, sa, r0 + (int)threadIdx.x, S, true, g, gbase);
+        load_round<PB, 2>(vb, sb, r0 + (int)threadIdx.x, S, true, g, gbase);
         lds_round<2>(la, va, r0 + (int)threadIdx.x, S, true);
         lds_round<2>(lb, vb, r0 + (int)threadIdx.x, S, true);
     }
     for (; r0 < S; r0 += T) {
         double va[1], vb[1];
         const bool full = r0 + T <= S;
-        load_round<PACK, 1>(va, sa, r0 + (int)threadIdx.x, S, full, g, gbase);
-        load_round<PACK, 1>(vb, sb, r0 + (int)threadIdx.x, S, full, g, gbase);
+        load_round<PA, 1>(va, sa, r0 + (int)threadIdx.x, S, full, g, gbase);
+        load_round<PB, 1>(vb, sb, r0 + (int)threadIdx.x, S, full, g, gbase);
         lds_round<1>(la, va, r0 + (int)threadIdx.x, S, full);
         lds_round<1>(lb, vb, r0 + (int)threadIdx.x, S, full);
     }
@@ -622,7 +622,9 @@ __global__ void __launch_bounds__(1024) k_group(const DevJob *__restrict__ jobs,
 // ---------------------------------------------------------------------------
 // Mid pass: last forward group on both operands, pointwise product
 // (src/fpntt.jl:770-785), first inverse group; for a single-group transform
-// the pack is fused in too, so the whole transform pair is one kernel.
+// the pack is fused in too, so the whole transform pair is one kernel.  A
+// job whose b is a cached spectrum (bspec) reads b's finished forward
+// spectrum and transforms a only.
 
 template <int K>
 __device__ void mid_body(const DevJob *__restrict__ Jp, const Tile &t, double *lds) {
@@ -638,6 +640,7 @@ __device__ void mid_body(const DevJob *__restrict__ Jp, const Tile &t, double *l
     const int s0 = uni(G->s0), s1 = uni(G->s1);
     const int64_t gbase = (int64_t)t.tile * uni64(G->La);
     const bool sq = uni(J->sqr) != 0;
+    const bool bspec = uni(J->bspec) != 0;
     double *la = lds;
     double *lb = lds + lds_doubles(S);
     {
@@ -654,12 +657,16 @@ __device__ void mid_body(const DevJob *__restrict__ Jp, const Tile &t, double *l
         sa.nch = uni64(J->nca);
         sb.nch = uni64(J->ncb);
         sa.b = sb.b = uni(J->bch);
+        // with bspec, b's points are its finished spectrum: loaded, not packed
         if (sq) load_tile<LU>(la, sa, pack, S, g, gbase);
-        else if (pack) load_pair<LU / 2, true>(la, lb, sa, sb, S, g, gbase);
-        else load_pair<LU / 2, false>(la, lb, sa, sb, S, g, gbase);
+        else if (bspec) {
+            if (pack) load_pair<LU / 2, true, false>(la, lb, sa, sb, S, g, gbase);
+            else load_pair<LU / 2, false, false>(la, lb, sa, sb, S, g, gbase);
+        } else if (pack) load_pair<LU / 2, true, true>(la, lb, sa, sb, S, g, gbase);
+        else load_pair<LU / 2, false, false>(la, lb, sa, sb, S, g, gbase);
     }
     __syncthreads();
-    run_stages<K, true>(la, lb, sq ? 1 : 2, P, s0, s1, g);
+    run_stages<K, true>(la, lb, sq || bspec ? 1 : 2, P, s0, s1, g);
     // squares allocate one array: lb is never touched for them
     if (sq) {
         for (int e = threadIdx.x; e < S; e += blockDim.x) {

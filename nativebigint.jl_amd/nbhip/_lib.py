@@ -48,6 +48,11 @@ class nbh_hgcd_job(ctypes.Structure):
                 ("s0", ctypes.c_void_p), ("s1", ctypes.c_void_p)]
 
 
+class nbh_spec_job(ctypes.Structure):
+    _fields_ = [("r", ctypes.c_void_p), ("a", ctypes.c_void_p), ("m", ctypes.c_int64),
+                ("bits_a", ctypes.c_int64), ("s", ctypes.c_void_p)]
+
+
 NBH_HGCD_FORWARD = 0
 NBH_HGCD_ADJUST = 1
 
@@ -74,6 +79,11 @@ SIGNATURES = {
     "nbh_hgcd_create": (_i32, [_P(nbh_hgcd_job), _i64, _i32, _P(_vp)]),
     "nbh_hgcd_run": (_i32, [_vp, _vp]),
     "nbh_hgcd_destroy": (None, [_vp]),
+    "nbh_spectrum_create": (_i32, [_vp, _i64, _i64, _i64, _vp, _P(_vp)]),
+    "nbh_spectrum_destroy": (None, [_vp]),
+    "nbh_spectrum_info": (_i32, [_vp, _P(_i64), _P(_i32), _P(_i64), _P(_i64)]),
+    "nbh_mul_spec_dev": (_i32, [_vp, _vp, _i64, _i64, _vp, _vp]),
+    "nbh_spec_batch_create": (_i32, [_P(nbh_spec_job), _i64, _P(_vp)]),
     "nbh_ntt_fwd_dev": (_i32, [_vp, _i64, _i32, _vp]),
     "nbh_ntt_rev_dev": (_i32, [_vp, _i64, _i32, _vp]),
     "nbh_pack_dev": (_i32, [_vp, _i64, _vp, _i64, _i32, _i64, _vp]),

@@ -11,7 +11,7 @@ import ctypes
 import numpy as np
 import torch
 
-from ._lib import NBH_HGCD_ADJUST, NBH_HGCD_FORWARD, check, init, lib, nbh_hgcd_job, nbh_job
+from ._lib import NBH_HGCD_ADJUST, NBH_HGCD_FORWARD, check, init, lib, nbh_hgcd_job, nbh_job, nbh_spec_job
 
 
 def _stream():
@@ -134,6 +134,65 @@ class HgcdStep:
             self.close()
         except Exception:
             pass
+
+
+class Spectrum:
+    """Cached forward spectra of a fixed operand b (device tensor), for
+    products with any a of up to bits_a_max bits: the modulus and reciprocal
+    that barrett_reduce! (src/barrett.jl:44-81) multiplies by on every call.
+    mul(r, a) runs a's transforms and the inverse only."""
+
+    def __init__(self, b: torch.Tensor, bits_a_max: int, bits_b: int = -1):
+        init()
+        self.b = b
+        h = ctypes.c_void_p()
+        check(lib().nbh_spectrum_create(_p(b), b.numel(), bits_b, bits_a_max, _stream(), ctypes.byref(h)))
+        self._h = h
+        N, bch, n, amax = ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int64()
+        check(lib().nbh_spectrum_info(h, ctypes.byref(N), ctypes.byref(bch), ctypes.byref(n), ctypes.byref(amax)))
+        self.N, self.chunk_bits, self.n, self.bits_a_max = N.value, bch.value, n.value, amax.value
+
+    @property
+    def handle(self):
+        return self._h
+
+    def mul(self, r: torch.Tensor, a: torch.Tensor, bits_a: int = -1) -> None:
+        """r = a*b on the device, enqueued on torch's current stream."""
+        assert r.numel() == a.numel() + self.n
+        check(lib().nbh_mul_spec_dev(_p(r), _p(a), a.numel(), bits_a, self._h, _stream()))
+
+    def close(self) -> None:
+        if self._h:
+            lib().nbh_spectrum_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class SpecBatch(Batch):
+    """A planned batch of products against cached spectra: jobs are
+    (r, a, spectrum) or (r, a, spectrum, bits_a)."""
+
+    def __init__(self, jobs):
+        init()
+        arr = (nbh_spec_job * len(jobs))()
+        self._keep = []
+        for i, j in enumerate(jobs):
+            r, a, sp = j[0], j[1], j[2]
+            arr[i].r = r.data_ptr()
+            arr[i].a = a.data_ptr()
+            arr[i].m = a.numel()
+            arr[i].bits_a = j[3] if len(j) > 3 else -1
+            arr[i].s = sp.handle
+            self._keep.append((r, a, sp))
+        h = ctypes.c_void_p()
+        torch.cuda.synchronize()
+        check(lib().nbh_spec_batch_create(arr, len(jobs), ctypes.byref(h)))
+        self._h = h
 
 
 # -- stage-level entry points (src/fpntt.jl:715, 727, 744, 770, 869) ---------
