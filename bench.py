@@ -1,6 +1,6 @@
 """Benchmark: bit-exact limb-products/s of the MI355X NTT product engine.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batch|large]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batch|large|hgcd|spectra|serve]
 
 workload "batch" (default): 4096 independent products, larger size
 log-uniform in [152, 65536] limbs, 25% squarings, size ratio uniform in
@@ -13,6 +13,11 @@ workload "hgcd": 1024 batched 2x2 cofactor steps (u', v') = (m11 u - m01 v,
 m00 v - m10 u), n log-uniform in [600, 32768] limbs, cofactors n/2 limbs:
 four products per instance plus the signed combine, all instances in one
 planned batch.  limb-products counts the four products' result limbs.
+workload "spectra": the two products of 1024 Barrett reductions modulo a
+fixed 4096-limb modulus m (barrett_reduce!, src/barrett.jl:44-81): q1 * mu
+((k+1) x (k+1) limbs) and q3 * m ((k+1) x k), with the forward spectra of mu
+and m cached on the device, against the same 2048 products as a plain batch.
+workload "serve": Poisson arrivals into the continuous-batching server.
 
 limb-products/s = sum over completed bit-exact products of (m + n) result
 limbs / wall time, inputs already resident in HBM.  Every product of the
@@ -47,7 +52,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["batch", "large", "hgcd", "serve"], default="batch")
+    ap.add_argument("--workload", choices=["batch", "large", "hgcd", "spectra", "serve"], default="batch")
+    ap.add_argument("--modulus-limbs", type=int, default=4096, help="spectra: limbs of the fixed modulus")
     ap.add_argument("--requests", type=int, default=20000, help="serve: requests per offered load")
     ap.add_argument("--budget-gib", type=float, default=8.0, help="serve: HBM budget")
     ap.add_argument("--jobs", type=int, default=4096)
@@ -122,6 +128,8 @@ def main():
         return bench_hgcd(args, D, engine, wl, info, stream)
     if args.workload == "serve":
         return bench_serve(args, D, engine, wl, info, stream)
+    if args.workload == "spectra":
+        return bench_spectra(args, D, engine, wl, info, stream)
 
     # ---- build the job list --------------------------------------------------
     if args.workload == "batch":
@@ -401,6 +409,103 @@ def bench_hgcd(args, D, engine, wl, info, stream):
     }
     print(json.dumps(out))
     H.close()
+
+
+def _timed_runs(B, stream, steps, warmup, D, profile=True):
+    import torch
+    with torch.cuda.stream(stream):
+        for _ in range(warmup):
+            B.run(stream)
+        torch.cuda.synchronize()
+        if profile:
+            D.profile_enable(True)
+            D.profile_read()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        for _ in range(steps):
+            B.run(stream)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        prof = None
+        if profile:
+            prof = D.profile_read()
+            D.profile_enable(False)
+    return ev0.elapsed_time(ev1) / steps, prof
+
+
+def bench_spectra(args, D, engine, wl, info, stream):
+    """Barrett products against a fixed modulus with cached spectra, and the
+    same products as a plain batch."""
+    import torch
+    k = args.modulus_limbs
+    count = min(args.jobs, 1024)
+    m_l, mu_l, q1s, q3s = wl.barrett_operands(count, k)
+    dm, dmu = D.to_dev(m_l), D.to_dev(mu_l)
+    dq1 = [D.to_dev(q) for q in q1s]
+    dq3 = [D.to_dev(q) for q in q3s]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    with torch.cuda.stream(stream):
+        S_mu = D.Spectrum(dmu, 64 * (k + 1))
+        S_m = D.Spectrum(dm, 64 * (k + 1))
+    torch.cuda.synchronize()
+    t_spec = time.perf_counter() - t0
+    r_spec = [torch.empty(2 * k + 2, dtype=torch.int64, device="cuda") for _ in range(count)] + \
+             [torch.empty(2 * k + 1, dtype=torch.int64, device="cuda") for _ in range(count)]
+    r_plain = [torch.empty_like(r) for r in r_spec]
+    jobs_s = [(r_spec[i], dq1[i], S_mu) for i in range(count)] + \
+             [(r_spec[count + i], dq3[i], S_m) for i in range(count)]
+    jobs_p = [(r_plain[i], dq1[i], dmu) for i in range(count)] + \
+             [(r_plain[count + i], dq3[i], dm) for i in range(count)]
+    total_limbs = sum(r.numel() for r in r_spec)
+    log(f"{count} Barrett reductions mod a {k}-limb modulus: {2 * count} products, {total_limbs} limbs; "
+        f"spectra N={S_mu.N}, {S_m.N} built in {1e3 * t_spec:.2f} ms")
+    BS = D.SpecBatch(jobs_s)
+    BP = D.Batch(jobs_p)
+    ms_p, prof_p = _timed_runs(BP, stream, args.steps, args.warmup, D)
+    ms_s, prof_s = _timed_runs(BS, stream, args.steps, args.warmup, D)
+    value = total_limbs / (ms_s * 1e-3)
+    plain = total_limbs / (ms_p * 1e-3)
+    log(f"timed: cached spectra {ms_s:.3f} ms/step ({value:.4g} limb-products/s), "
+        f"plain {ms_p:.3f} ms/step ({plain:.4g}), x{ms_p / ms_s:.2f}")
+    verified = None
+    if not args.no_verify:
+        from tests import gmp_ref
+        bad = 0
+        for rs, rp in zip(r_spec, r_plain):
+            bad += int(not torch.equal(rs, rp))
+        ops = [(q1s[i], mu_l) for i in range(count)] + [(q3s[i], m_l) for i in range(count)]
+        for i in range(0, 2 * count, max(1, (2 * count) // 64)):     # 64 products against libgmp
+            a, b = ops[i]
+            bad += int(not np.array_equal(D.to_host(r_spec[i]), gmp_ref.mul(a, b)))
+        verified = bad == 0
+        log(f"verified: {2 * count} products equal to the plain batch, 64 equal to libgmp; {bad} differ")
+        if bad:
+            print(json.dumps({"error": f"{bad} products differ"}))
+            sys.exit(1)
+    out = {
+        "metric": "bit-exact limb-products/s", "value": value, "unit": "limb-products/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_s, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (SplitMix64 uniform limbs, nonzero top limb)",
+        "config": {"workload": f"spectra: {count} Barrett reductions mod a fixed {k}-limb modulus, "
+                               f"products q1*mu ((k+1)x(k+1)) and q3*m ((k+1)xk) against cached spectra",
+                   "products_per_step": 2 * count, "product_limbs_per_step": total_limbs,
+                   "parallelism": "single GPU"},
+        "plain_batch": {"value": plain, "ms_per_step": ms_p, "speedup_with_spectra": ms_p / ms_s},
+        "spectra_build_ms": 1e3 * t_spec,
+        "verified_bit_exact": verified,
+        "kernels": {kk: {"ms_per_step": prof_s[kk][0] / args.steps, "launches_per_step": prof_s[kk][1] / args.steps,
+                         "plain_ms_per_step": prof_p[kk][0] / args.steps}
+                    for kk in D.PROF_KINDS},
+        "device": info,
+    }
+    print(json.dumps(out))
+    BS.close()
+    BP.close()
+    S_mu.close()
+    S_m.close()
 
 
 def bench_serve(args, D, engine, wl, info, stream):

@@ -14,6 +14,9 @@ Workloads (sizes in 64-bit limbs):
            src/mul.jl:388)
   hgcd     1024 instances of a 2x2 matrix of n/2-limb cofactors times a
            2-vector of n-limb operands, n log-uniform in [600, 32768]
+  spectra  a fixed k-limb modulus m and (k+1)-limb reciprocal mu, and per
+           Barrett reduction a (k+1)-limb q1 and q3 (barrett_reduce!,
+           src/barrett.jl:44-81, multiplies q1*mu and q3*m)
 """
 from __future__ import annotations
 
@@ -28,7 +31,7 @@ _M1 = 0xBF58476D1CE4E5B9
 _M2 = 0x94D049BB133111EB
 MASK64 = (1 << 64) - 1
 
-WORKLOAD_IDS = {"single": 1, "large": 2, "batch": 3, "hgcd": 4, "serve": 5, "test": 9}
+WORKLOAD_IDS = {"single": 1, "large": 2, "batch": 3, "hgcd": 4, "serve": 5, "spectra": 6, "test": 9}
 
 
 def _mix(z: int) -> int:
@@ -132,6 +135,15 @@ def hgcd_instance(k: int, n: int, workload: str = "hgcd") -> dict:
     mn = max(1, n // 2)
     names = ("m00", "m01", "m10", "m11", "u", "v")
     return {name: operand(workload, k, i, mn if i < 4 else n) for i, name in enumerate(names)}
+
+
+def barrett_operands(count: int = 1024, k: int = 4096, workload: str = "spectra"):
+    """(m, mu, [q1], [q3]): m has k limbs, mu and every q1, q3 k+1 limbs."""
+    m = operand(workload, 0xFFFFF0, 0, k)
+    mu = operand(workload, 0xFFFFF0, 1, k + 1)
+    q1 = [operand(workload, i, 0, k + 1) for i in range(count)]
+    q3 = [operand(workload, i, 1, k + 1) for i in range(count)]
+    return m, mu, q1, q3
 
 
 def limbs_to_int(a) -> int:
