@@ -1,6 +1,6 @@
 """Benchmark: bit-exact limb-products/s of the MI355X NTT product engine.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batch|large|hgcd|spectra|serve]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batch|large|hgcd|spectra|powermod|serve]
 
 workload "batch" (default): 4096 independent products, larger size
 log-uniform in [152, 65536] limbs, 25% squarings, size ratio uniform in
@@ -17,6 +17,10 @@ workload "spectra": the two products of 1024 Barrett reductions modulo a
 fixed 4096-limb modulus m (barrett_reduce!, src/barrett.jl:44-81): q1 * mu
 ((k+1) x (k+1) limbs) and q3 * m ((k+1) x k), with the forward spectra of mu
 and m cached on the device, against the same 2048 products as a plain batch.
+workload "powermod": 64 bases raised to one 64-bit exponent modulo a fixed
+2048-limb modulus (powermod_limbs' Barrett branch, src/algorithms.jl:17-107):
+per modular product a product batch and a Barrett reduction batch against the
+cached spectra; limb-products counts the result limbs of all three products.
 workload "serve": Poisson arrivals into the continuous-batching server.
 
 limb-products/s = sum over completed bit-exact products of (m + n) result
@@ -52,7 +56,10 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["batch", "large", "hgcd", "spectra", "serve"], default="batch")
+    ap.add_argument("--workload", choices=["batch", "large", "hgcd", "spectra", "powermod", "serve"],
+                    default="batch")
+    ap.add_argument("--bases", type=int, default=64, help="powermod: bases sharing the exponent and modulus")
+    ap.add_argument("--exp-bits", type=int, default=64, help="powermod: exponent bits")
     ap.add_argument("--modulus-limbs", type=int, default=4096, help="spectra: limbs of the fixed modulus")
     ap.add_argument("--requests", type=int, default=20000, help="serve: requests per offered load")
     ap.add_argument("--budget-gib", type=float, default=8.0, help="serve: HBM budget")
@@ -130,6 +137,8 @@ def main():
         return bench_serve(args, D, engine, wl, info, stream)
     if args.workload == "spectra":
         return bench_spectra(args, D, engine, wl, info, stream)
+    if args.workload == "powermod":
+        return bench_powermod(args, D, engine, wl, info, stream)
 
     # ---- build the job list --------------------------------------------------
     if args.workload == "batch":
@@ -506,6 +515,75 @@ def bench_spectra(args, D, engine, wl, info, stream):
     BP.close()
     S_mu.close()
     S_m.close()
+
+
+def bench_powermod(args, D, engine, wl, info, stream):
+    """Batched powermod modulo a fixed modulus (Barrett with cached spectra)."""
+    import torch
+    from nbhip import barrett as BR
+    k = args.modulus_limbs if args.modulus_limbs != 4096 else 2048
+    count = args.bases
+    m = wl.operand("powermod", 0xFFFFF0, 0, k)
+    M = wl.limbs_to_int(m)
+    e_l = wl.operand("powermod", 0xFFFFF0, 1, (args.exp_bits + 63) // 64)
+    e = wl.limbs_to_int(e_l) % (1 << args.exp_bits) | (1 << (args.exp_bits - 1))
+    bases = []
+    for i in range(count):
+        b = wl.limbs_to_int(wl.operand("powermod", i, 0, k)) % M
+        bases.append(wl.int_to_limbs(max(b, 2), k))
+    P = BR.PowermodBatch(m, bases, e)
+    nmod = P.modmuls()
+    limbs_per_modmul = 2 * k + (2 * k + 2) + (2 * k + 1)
+    total_limbs = count * nmod * limbs_per_modmul
+    log(f"{count} bases ^ {args.exp_bits}-bit exponent mod a {k}-limb modulus: {nmod} modular products each "
+        f"(window {P.w}), {total_limbs} product limbs per run")
+    with torch.cuda.stream(stream):
+        for _ in range(max(1, args.warmup)):
+            P.run(stream)
+        torch.cuda.synchronize()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        steps = max(1, min(args.steps, 3))
+        ev0.record(stream)
+        for _ in range(steps):
+            out = P.run(stream)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        # per-kernel split from one more, profiled run (events around every launch)
+        D.profile_enable(True)
+        D.profile_read()
+        P.run(stream)
+        torch.cuda.synchronize()
+        prof = D.profile_read()
+        D.profile_enable(False)
+    ms = ev0.elapsed_time(ev1) / steps
+    value = total_limbs / (ms * 1e-3)
+    log(f"timed: {ms:.2f} ms per exponentiation batch, {count * nmod / (ms * 1e-3):.4g} modular products/s, "
+        f"{value:.4g} limb-products/s")
+    verified = None
+    if not args.no_verify:
+        sample = list(range(0, count, max(1, count // 4)))
+        bad = sum(int(wl.limbs_to_int(D.to_host(out[i])) != pow(wl.limbs_to_int(bases[i]), e, M)) for i in sample)
+        verified = bad == 0
+        log(f"verified {len(sample) - bad}/{len(sample)} results against pow()")
+        if bad:
+            print(json.dumps({"error": f"{bad} powermod results differ"}))
+            sys.exit(1)
+    out_json = {
+        "metric": "bit-exact limb-products/s", "value": value, "unit": "limb-products/s", "n_gpus": 1,
+        "steps": steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (SplitMix64 uniform limbs, nonzero top limb)",
+        "config": {"workload": f"powermod: {count} bases, {args.exp_bits}-bit exponent, {k}-limb modulus, "
+                               f"sliding window {P.w}, Barrett with cached spectra",
+                   "modular_products_per_step": count * nmod, "parallelism": "single GPU"},
+        "modular_products_per_s": count * nmod / (ms * 1e-3),
+        "verified_bit_exact": verified,
+        "kernels": {kk: {"ms_per_step": prof[kk][0], "launches_per_step": prof[kk][1]} for kk in D.PROF_KINDS},
+        "device": info,
+    }
+    print(json.dumps(out_json))
+    P.close()
 
 
 def bench_serve(args, D, engine, wl, info, stream):

@@ -154,6 +154,30 @@ typedef struct {
 } nbh_spec_job;
 int32_t nbh_spec_batch_create(const nbh_spec_job *jobs, int64_t count, nbh_batch *out);
 
+/* Batched Barrett reduction (barrett_reduce!, src/barrett.jl:44-81) modulo a
+ * fixed k-limb m (m[k-1] != 0) with mu = floor(beta^(2k) / m) of lmu limbs
+ * (barrett_mu!, src/barrett.jl:24-31; lmu is k+1, or k+2 when m = beta^(k-1)),
+ * both device buffers that must stay valid while the context lives.  The
+ * context caches the spectra of mu and m; a reduction batch is planned once
+ * for fixed buffers: per job r[0..k) = t[0..2k) mod m (t < beta^(2k), r must
+ * not alias t).  A run is the q1*mu product batch, the q3*m product batch
+ * (both against the cached spectra) and one correction launch
+ * (r = (T - q3 m) mod beta^(k+1), then at most two subtractions of m).
+ * Operands are taken at full length, so the buffers' contents may change
+ * between runs (powermod_limbs' mulred!, src/algorithms.jl:32-55). */
+typedef struct nbh_barrett_s *nbh_barrett;
+typedef struct nbh_barrett_batch_s *nbh_barrett_batch;
+typedef struct {
+    const uint64_t *t;
+    uint64_t *r;
+} nbh_barrett_job;
+int32_t nbh_barrett_create(const uint64_t *m, int64_t k, const uint64_t *mu, int64_t lmu, void *stream,
+                           nbh_barrett *out);
+void nbh_barrett_destroy(nbh_barrett b);
+int32_t nbh_barrett_batch_create(nbh_barrett b, const nbh_barrett_job *jobs, int64_t count, nbh_barrett_batch *out);
+int32_t nbh_barrett_batch_run(nbh_barrett_batch rb, void *stream);
+void nbh_barrett_batch_destroy(nbh_barrett_batch rb);
+
 /* Stage-level entry points over device buffers, one per reference function,
  * used by the parity tests: fp_ntt_fwd! (src/fpntt.jl:715), fp_ntt_rev!
  * (:727), fp_ntt_pack! (:744), fp_ntt_pointwise! (:770), fp_ntt_unpack2!

@@ -1366,6 +1366,105 @@ int32_t nbh_spec_batch_create(const nbh_spec_job *jobs, int64_t count, nbh_batch
     return NBH_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Barrett reduction (barrett_reduce!, src/barrett.jl:44-81) with the modulus
+// and reciprocal held as cached spectra.
+
+struct nbh_barrett_s {
+    const uint64_t *m = nullptr;
+    int64_t k = 0, lmu = 0;
+    nbh_spectrum_s *smu = nullptr, *sm = nullptr;
+};
+
+struct nbh_barrett_batch_s {
+    nbh_batch_s *p1 = nullptr, *p2 = nullptr;  // q1*mu, q3*m
+    void *ws = nullptr;                        // q2 and q3*m per job
+    BarrettFixJob *dfix = nullptr;
+    int64_t count = 0;
+};
+
+void nbh_barrett_batch_destroy(nbh_barrett_batch rb);
+
+int32_t nbh_barrett_create(const uint64_t *m, int64_t k, const uint64_t *mu, int64_t lmu, void *stream,
+                           nbh_barrett *out) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    if (!out || !m || !mu || k < 1 || lmu < 1 || lmu > k + 2) return fail(NBH_ERR_ARG, "bad argument");
+    *out = nullptr;
+    std::unique_ptr<nbh_barrett_s> B(new nbh_barrett_s);
+    B->m = m;
+    B->k = k;
+    B->lmu = lmu;
+    // q1 = floor(T / beta^(k-1)) has k+1 limbs; q3 = floor(q1 mu / beta^(k+1)) has lmu
+    int32_t rc = nbh_spectrum_create(mu, lmu, -1, 64 * (k + 1), stream, &B->smu);
+    if (rc == NBH_OK) rc = nbh_spectrum_create(m, k, -1, 64 * lmu, stream, &B->sm);
+    if (rc != NBH_OK) {
+        nbh_spectrum_destroy(B->smu);
+        return rc;
+    }
+    *out = B.release();
+    return NBH_OK;
+}
+
+void nbh_barrett_destroy(nbh_barrett b) {
+    if (!b) return;
+    nbh_spectrum_destroy(b->smu);
+    nbh_spectrum_destroy(b->sm);
+    delete b;
+}
+
+int32_t nbh_barrett_batch_create(nbh_barrett b, const nbh_barrett_job *jobs, int64_t count, nbh_barrett_batch *out) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    if (!b || !jobs || count < 1 || !out) return fail(NBH_ERR_ARG, "bad argument");
+    *out = nullptr;
+    const int64_t k = b->k, lmu = b->lmu;
+    const size_t q2_limbs = (size_t)(k + 1 + lmu), p_limbs = (size_t)(lmu + k);
+    const size_t per = align256(8 * q2_limbs) + align256(8 * p_limbs);
+    std::unique_ptr<nbh_barrett_batch_s, void (*)(nbh_barrett_batch)> R(new nbh_barrett_batch_s,
+                                                                       nbh_barrett_batch_destroy);
+    R->count = count;
+    HIPCHK(hipMalloc(&R->ws, per * (size_t)count));
+    std::vector<JobIn> j1((size_t)count), j2((size_t)count);
+    std::vector<BarrettFixJob> fx((size_t)count);
+    for (int64_t i = 0; i < count; i++) {
+        if (!jobs[i].t || !jobs[i].r) return fail(NBH_ERR_ARG, "bad reduction " + std::to_string(i));
+        uint64_t *q2 = (uint64_t *)((char *)R->ws + per * (size_t)i);
+        uint64_t *p = (uint64_t *)((char *)q2 + align256(8 * q2_limbs));
+        // operands are taken at their full lengths: the values change from run
+        // to run, and a length is an upper bound of the magnitude
+        j1[(size_t)i] = JobIn{q2, jobs[i].t + (k - 1), b->smu->b, k + 1, lmu, 64 * (k + 1), b->smu->bits_b, b->smu};
+        j2[(size_t)i] = JobIn{p, q2 + (k + 1), b->m, lmu, k, 64 * lmu, b->sm->bits_b, b->sm};
+        fx[(size_t)i] = BarrettFixJob{jobs[i].t, p, p, jobs[i].r, b->m, k};
+    }
+    RCCHK(batch_plan(j1.data(), count, &R->p1, nullptr, nullptr, 0, nullptr));
+    RCCHK(batch_plan(j2.data(), count, &R->p2, nullptr, nullptr, 0, nullptr));
+    HIPCHK(hipMalloc((void **)&R->dfix, sizeof(BarrettFixJob) * (size_t)count));
+    HIPCHK(hipMemcpy(R->dfix, fx.data(), sizeof(BarrettFixJob) * (size_t)count, hipMemcpyHostToDevice));
+    *out = R.release();
+    return NBH_OK;
+}
+
+int32_t nbh_barrett_batch_run(nbh_barrett_batch rb, void *stream) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    if (!rb) return fail(NBH_ERR_ARG, "null handle");
+    hipStream_t s = stream ? (hipStream_t)stream : g.stream;
+    RCCHK(batch_run(rb->p1, s));
+    RCCHK(batch_run(rb->p2, s));
+    HIPCHK(launch_barrett_fix(rb->dfix, rb->count, s));
+    return NBH_OK;
+}
+
+void nbh_barrett_batch_destroy(nbh_barrett_batch rb) {
+    if (!rb) return;
+    batch_free(rb->p1);
+    batch_free(rb->p2);
+    if (rb->ws) (void)hipFree(rb->ws);
+    if (rb->dfix) (void)hipFree(rb->dfix);
+    delete rb;
+}
+
 int32_t nbh_hgcd_create(const nbh_hgcd_job *jobs, int64_t count, int32_t mode, nbh_hgcd *out) {
     std::lock_guard<std::recursive_mutex> lk(g.mu);
     RCCHK(ensure_init());

@@ -128,6 +128,18 @@ struct CombineJob {
     int32_t *sign;  // sub only: -1, 0, +1 (device)
     int32_t sub, pad;
 };
+
+// Barrett correction of one reduction (barrett_reduce!, src/barrett.jl:
+// 67-80): w = (T - P) mod beta^(k+1) over the low k+1 limbs, then w -= m while
+// w >= m (at most twice), r = w's low k limbs.  w may be P's own buffer.
+struct BarrettFixJob {
+    const uint64_t *T;  // the reduced value's low k+1 limbs
+    const uint64_t *P;  // q3*m, at least k+1 limbs
+    uint64_t *w;        // k+1 limbs of working space
+    uint64_t *r;        // k limbs out
+    const uint64_t *m;  // k limbs
+    int64_t k;
+};
 }  // namespace nbh
 
 static_assert(sizeof(nbh::Tile) == 16, "kernels read a Tile as four 32-bit words");

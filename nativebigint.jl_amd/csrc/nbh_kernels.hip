@@ -1071,6 +1071,83 @@ __global__ void __launch_bounds__(UNPACK_THREADS) k_combine(const CombineJob *__
     }
 }
 
+// One workgroup per reduction.  Thread t owns limbs c0 + 4t .. c0 + 4t + 3 of
+// every 1024-limb chunk c0 in all three passes, so each pass reads back only
+// limbs the same thread wrote.
+__device__ __forceinline__ void barrett_sub_pass(const uint64_t *__restrict__ X, const uint64_t *__restrict__ Y,
+                                                 int64_t ly, uint64_t *__restrict__ w, int64_t L, uint32_t *wsum) {
+    uint32_t borrow = 0;
+    for (int64_t c0 = 0; c0 < L; c0 += UNPACK_LPC) {
+        const int64_t k0 = c0 + UNPACK_LPT * threadIdx.x;
+        uint64_t d[UNPACK_LPT];
+        bool prop = true;
+        uint64_t cc = 0;
+#pragma unroll
+        for (int q = 0; q < UNPACK_LPT; q++) {
+            const int64_t k = k0 + q;
+            const uint64_t x = k < L ? X[k] : 0, y = k < ly ? Y[k] : 0;
+            const uint64_t s1 = x - y;
+            const uint64_t b1 = x < y;
+            const uint64_t s2 = s1 - cc;
+            cc = b1 | (s1 < cc);
+            d[q] = s2;
+            prop = prop && s2 == 0;
+        }
+        uint32_t cout = 0;
+        const uint32_t cin = block_carry(cc != 0, prop, borrow, wsum, cout);
+        uint64_t cv = cin;
+#pragma unroll
+        for (int q = 0; q < UNPACK_LPT; q++) {
+            const int64_t k = k0 + q;
+            const uint64_t v = d[q] - cv;
+            cv = d[q] < cv;
+            if (k < L) w[k] = v;
+        }
+        borrow = cout;
+    }
+}
+
+__global__ void __launch_bounds__(UNPACK_THREADS) k_barrett_fix(const BarrettFixJob *__restrict__ jobs) {
+    __shared__ uint32_t wsum[UNPACK_THREADS / 64];
+    __shared__ unsigned long long top;  // 1 + highest limb where w and m differ, 0 if equal
+    __shared__ int ge;
+    const auto *J = cst(jobs + blockIdx.x);
+    const uint64_t *T = unip(J->T), *P = unip(J->P), *m = unip(J->m);
+    uint64_t *__restrict__ w = unip(J->w);
+    uint64_t *__restrict__ r = unip(J->r);
+    const int64_t k = uni64(J->k), L = k + 1;
+    barrett_sub_pass(T, P, L, w, L, wsum);
+    for (int rep = 0; rep < 2; rep++) {
+        if (threadIdx.x == 0) { top = 0; ge = 0; }
+        __syncthreads();
+        unsigned long long hi = 0;
+        for (int64_t c0 = 0; c0 < L; c0 += UNPACK_LPC)
+#pragma unroll
+            for (int q = 0; q < UNPACK_LPT; q++) {
+                const int64_t i = c0 + UNPACK_LPT * threadIdx.x + q;
+                if (i < L && w[i] != (i < k ? m[i] : 0)) hi = (unsigned long long)i + 1;
+            }
+        if (hi) atomicMax(&top, hi);
+        __syncthreads();
+        const int64_t t = (int64_t)top - 1;
+        // the owner of limb t decides w >= m; equal values count as >=
+        if (t < 0) {
+            if (threadIdx.x == 0) ge = 1;
+        } else if (((t % UNPACK_LPC) >> 2) == threadIdx.x) {
+            ge = w[t] > (t < k ? m[t] : 0) ? 1 : 0;
+        }
+        __syncthreads();
+        if (!ge) break;
+        barrett_sub_pass(w, m, k, w, L, wsum);
+    }
+    for (int64_t c0 = 0; c0 < L; c0 += UNPACK_LPC)
+#pragma unroll
+        for (int q = 0; q < UNPACK_LPT; q++) {
+            const int64_t i = c0 + UNPACK_LPT * threadIdx.x + q;
+            if (i < k) r[i] = w[i];
+        }
+}
+
 // ---------------------------------------------------------------------------
 // Twiddle tables.  Row t (1..m-1) of a stage of span L holds w_L^(t j) for
 // j < Q, w_L = w_N^(N/L), minimal balanced (src/fpntt.jl:173-225).  Each entry
@@ -1221,6 +1298,12 @@ hipError_t launch_fill_u64(uint64_t *p, int64_t n, uint64_t v, hipStream_t s) {
 hipError_t launch_combine(const CombineJob *jobs, int64_t count, hipStream_t s) {
     if (count <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_combine, dim3((unsigned)count), dim3(UNPACK_THREADS), 0, s, jobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_barrett_fix(const BarrettFixJob *jobs, int64_t count, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_barrett_fix, dim3((unsigned)count), dim3(UNPACK_THREADS), 0, s, jobs);
     return hipGetLastError();
 }
 

@@ -53,6 +53,10 @@ class nbh_spec_job(ctypes.Structure):
                 ("bits_a", ctypes.c_int64), ("s", ctypes.c_void_p)]
 
 
+class nbh_barrett_job(ctypes.Structure):
+    _fields_ = [("t", ctypes.c_void_p), ("r", ctypes.c_void_p)]
+
+
 NBH_HGCD_FORWARD = 0
 NBH_HGCD_ADJUST = 1
 
@@ -84,6 +88,11 @@ SIGNATURES = {
     "nbh_spectrum_info": (_i32, [_vp, _P(_i64), _P(_i32), _P(_i64), _P(_i64)]),
     "nbh_mul_spec_dev": (_i32, [_vp, _vp, _i64, _i64, _vp, _vp]),
     "nbh_spec_batch_create": (_i32, [_P(nbh_spec_job), _i64, _P(_vp)]),
+    "nbh_barrett_create": (_i32, [_vp, _i64, _vp, _i64, _vp, _P(_vp)]),
+    "nbh_barrett_destroy": (None, [_vp]),
+    "nbh_barrett_batch_create": (_i32, [_vp, _P(nbh_barrett_job), _i64, _P(_vp)]),
+    "nbh_barrett_batch_run": (_i32, [_vp, _vp]),
+    "nbh_barrett_batch_destroy": (None, [_vp]),
     "nbh_ntt_fwd_dev": (_i32, [_vp, _i64, _i32, _vp]),
     "nbh_ntt_rev_dev": (_i32, [_vp, _i64, _i32, _vp]),
     "nbh_pack_dev": (_i32, [_vp, _i64, _vp, _i64, _i32, _i64, _vp]),
