@@ -25,8 +25,8 @@ from . import engine
 from . import workloads as wl
 from ._lib import PKG_DIR
 
-CANDIDATES = [(16384, 8192, 8, 64), (8192, 4096, 8, 64), (4096, 2048, 8, 64), (4096, 1024, 8, 64),
-              (2048, 1024, 8, 64), (8192, 2048, 8, 64), (16384, 4096, 8, 256), (16384, 8192, 8, 1024)]
+CANDIDATES = [(16384, 8192, 8, 64), (8192, 4096, 8, 64), (4096, 2048, 8, 64), (4096, 2048, 8, 128),
+              (4096, 1024, 8, 64), (8192, 2048, 8, 64), (16384, 4096, 8, 64), (2048, 2048, 8, 64)]
 
 
 def limbs_for_length(N: int) -> int:
@@ -41,7 +41,7 @@ def limbs_for_length(N: int) -> int:
     return lo
 
 
-def time_batch(tens, reps=5):
+def time_batch(tens, reps=7):
     """Median over `reps` runs of one planned batch (ms)."""
     B = D.Batch(tens)
     s = torch.cuda.Stream()
@@ -68,10 +68,10 @@ def tune(max_log2=24, out=None, log=print):
     for j in range(9, max_log2 + 1):
         n = limbs_for_length(1 << j)
         N = engine.fp_ntt_params(64 * n, 64 * n)[1]
-        reps = max(1, min(1024, (1 << 24) // N))   # ~2^24 points per measurement
+        reps = max(1, min(2048, (1 << 25) // N))   # ~2^25 points per measurement
         a = [D.to_dev(wl.operand("test", 500 + i, 0, n)) for i in range(reps)]
         b = [D.to_dev(wl.operand("test", 500 + i, 1, n)) for i in range(reps)]
-        best = None
+        results = []
         ref = None
         for cand in CANDIDATES:
             try:
@@ -86,8 +86,18 @@ def tune(max_log2=24, out=None, log=print):
             elif not all(np.array_equal(x, y) for x, y in zip(got, ref)):
                 raise RuntimeError(f"schedule {cand} changed the product at N={N}")
             log(f"N={N:>9} ({n} limbs x{reps}) {cand}: {ms:.3f} ms")
-            if best is None or ms < best[0]:
-                best = (ms, cand)
+            results.append((ms, cand))
+        # re-time the two fastest against each other to damp run-to-run noise
+        results.sort()
+        best = results[0]
+        if len(results) > 1:
+            again = []
+            for _, cand in results[:2]:
+                D.set_tuning(*cand)
+                tens = [(torch.empty(2 * n, dtype=torch.int64, device="cuda"), a[i], b[i]) for i in range(reps)]
+                again.append((time_batch(tens, reps=9), cand))
+            best = min(again)
+            log(f"  recheck: {again}")
         entries.append((1 << j, best[1]))
         log(f"  -> N <= 2^{j}: {best[1]}")
     # drop entries equal to the next one (the table is first-match by Nmax)
