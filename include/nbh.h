@@ -217,6 +217,11 @@ int32_t nbh_get_tuning_for(int64_t N, int32_t *knobs4);
  * counterpart. */
 #define NBH_PROF_KINDS 6
 int32_t nbh_profile_enable(int32_t on);
+/* A batch step's launches (one per tile shape) are independent; by default
+ * they fork onto 4 library streams so small shapes run beside big ones.
+ * nside = 0 issues them in order on the caller's stream (every launch then
+ * has the GPU to itself, which is what a per-launch profile should see). */
+int32_t nbh_set_streams(int32_t nside);
 int32_t nbh_profile_read(double *ms, int64_t *launches, int32_t nkinds);
 
 /* Synchronise the library stream (or `stream`). */

@@ -231,6 +231,7 @@ struct Lib {
     DevJob *slots = nullptr;
     int slot_next = 0;
     hipStream_t side[8] = {};  // batch phases fork onto these (default priority)
+    int nside = 4;             // how many of them a step uses (0: all launches in order)
     // per-kernel event timing
     bool prof = false;
     std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> prof_open;
@@ -1002,13 +1003,13 @@ int batch_run(nbh_batch_s *B, hipStream_t s) {
     while (i < B->phases.size()) {
         size_t j = i;
         while (j < B->phases.size() && B->phases[j].step == B->phases[i].step) j++;
-        if (j - i == 1 || NSIDE == 0) {
+        if (j - i == 1 || g.nside == 0) {
             for (size_t k = i; k < j; k++) RCCHK(launch_phase(B, B->phases[k], s));
         } else {
             hipEvent_t fork = B->ev[(size_t)B->phases[i].step];
             HIPCHK(hipEventRecord(fork, s));
             for (size_t k = i; k < j; k++) {
-                hipStream_t side = g.side[(k - i) % NSIDE];
+                hipStream_t side = g.side[(k - i) % g.nside];
                 HIPCHK(hipStreamWaitEvent(side, fork, 0));
                 RCCHK(launch_phase(B, B->phases[k], side));
                 hipEvent_t done = B->ev[(size_t)B->nsteps + k];
@@ -1690,6 +1691,14 @@ int32_t nbh_plan_groups(int64_t N, int32_t *ngroups, int64_t *out) {
         out[4 * i + 2] = P->host.g[i].S;
         out[4 * i + 3] = P->host.g[i].C;
     }
+    return NBH_OK;
+}
+
+int32_t nbh_set_streams(int32_t nside) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    if (nside < 0 || nside > NSIDE) return fail(NBH_ERR_ARG, "side streams must be 0.." + std::to_string(NSIDE));
+    g.nside = nside;
     return NBH_OK;
 }
 

@@ -609,14 +609,28 @@ __device__ void group_body(const DevJob *__restrict__ Jp, const Tile &t, bool fw
     store_tile(x, lds, total, g, gbase);
 }
 
-__global__ void __launch_bounds__(1024) k_group(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles,
-                                                TileGen gen, int fwd) {
-    extern __shared__ double lds[];
+// Forward and inverse group passes are separate kernels (one code path each,
+// and separate names in a kernel trace).
+template <bool FWD>
+__device__ __forceinline__ void group_kernel(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles,
+                                             const TileGen &gen, double *lds) {
     const Tile t = fetch_tile(tiles, gen);
     const DevJob *J = jobs + t.job;
     if (uni(cst(J)->zero)) return;
-    if (t.prime == 0) group_body<0>(J, t, fwd != 0, lds);
-    else group_body<1>(J, t, fwd != 0, lds);
+    if (t.prime == 0) group_body<0>(J, t, FWD, lds);
+    else group_body<1>(J, t, FWD, lds);
+}
+
+__global__ void __launch_bounds__(1024) k_group_fwd(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles,
+                                                    TileGen gen) {
+    extern __shared__ double lds[];
+    group_kernel<true>(jobs, tiles, gen, lds);
+}
+
+__global__ void __launch_bounds__(1024) k_group_inv(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles,
+                                                    TileGen gen) {
+    extern __shared__ double lds[];
+    group_kernel<false>(jobs, tiles, gen, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -1237,7 +1251,8 @@ static inline unsigned grid_for(int64_t n, int threads, int64_t cap = 1 << 16) {
 hipError_t launch_group(const DevJob *jobs, const Tile *tiles, const TileGen &gen, int64_t ntiles, int fwd,
                         size_t lds_bytes, int threads, hipStream_t s) {
     if (ntiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_group, dim3((unsigned)ntiles), dim3(threads), lds_bytes, s, jobs, tiles, gen, fwd);
+    if (fwd) hipLaunchKernelGGL(k_group_fwd, dim3((unsigned)ntiles), dim3(threads), lds_bytes, s, jobs, tiles, gen);
+    else hipLaunchKernelGGL(k_group_inv, dim3((unsigned)ntiles), dim3(threads), lds_bytes, s, jobs, tiles, gen);
     return hipGetLastError();
 }
 
@@ -1325,7 +1340,10 @@ hipError_t launch_top_limbs(const DevJob *jobs, int64_t count, uint64_t *out, hi
 }
 
 hipError_t set_lds_limit(size_t bytes) {
-    hipError_t e = hipFuncSetAttribute((const void *)k_group, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    hipError_t e = hipFuncSetAttribute((const void *)k_group_fwd, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)bytes);
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void *)k_group_inv, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (e != hipSuccess) return e;
     return hipFuncSetAttribute((const void *)k_mid, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }

@@ -315,6 +315,12 @@ def device_info():
 PROF_KINDS = ("group_fwd", "mid", "group_inv", "unpack", "carry_fix", "combine")
 
 
+def set_streams(nside: int) -> None:
+    """Side streams a batch step forks onto (0: launches in order)."""
+    init()
+    check(lib().nbh_set_streams(nside))
+
+
 def profile_enable(on: bool = True) -> None:
     init()
     check(lib().nbh_profile_enable(1 if on else 0))
