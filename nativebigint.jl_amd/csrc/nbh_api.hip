@@ -446,8 +446,11 @@ int make_schedule(const StageList &L, const Tuning &T, DevPlan &P, PlanEntry &E)
     }
     E.mid_lds_bytes[0] = 2 * sizeof(double) * (size_t)lds_doubles(P.g[ng - 1].S);
     E.mid_lds_bytes[1] = sizeof(double) * (size_t)lds_doubles(P.g[ng - 1].S);
-    E.mid_threads[0] = tile_threads(2 * (int64_t)P.g[ng - 1].S);
+    // one thread per 16-point sub-problem of ONE array: the mid kernel runs a
+    // product's two operands in the same thread
+    E.mid_threads[0] = tile_threads((int64_t)P.g[ng - 1].S);
     E.mid_threads[1] = tile_threads((int64_t)P.g[ng - 1].S);
+    if (E.mid_threads[0] > MID_MAX_THREADS) return fail(NBH_ERR_ARG, "schedule: mid tile too large");
     return NBH_OK;
 }
 

@@ -90,6 +90,10 @@ struct TileGen {
     int32_t prime0;
 };
 
+// The mid kernel's block-size bound (its __launch_bounds__): one thread per
+// 16 points of a mid tile of at most 8192 points.
+constexpr int MID_MAX_THREADS = 512;
+
 // Doubles of LDS that an n-point tile occupies with one pad double per 16
 // (the kernels index LDS as i + i/16).
 NBH_HDI constexpr int lds_doubles(int n) { return n + (n >> 4) + 1; }

@@ -266,20 +266,20 @@ __device__ void pass1(double *__restrict__ x0, double *__restrict__ x1, int narr
         double y[M];
 #pragma unroll
         for (int t = 0; t < M; t++) y[t] = x[ix(t)];
-        if (M == 2) {
+        if constexpr (M == 2) {
             const double u = y[0], v = y[1];
             y[0] = fp_reduce<K>(u + v);
             y[1] = fp_reduce<K>(u - v);
-        } else if (M == 4) {
+        } else if constexpr (M == 4) {
             bfly4<K, FWD>(y[0], y[1], y[2], y[3], s.rot[0], s.tw, s.Q, j);
-        } else if (M == 3) {
+        } else if constexpr (M == 3) {
             const double w1 = gld(s.tw, j), w2 = gld(s.tw, s.Q + j);
             if (FWD) dft3<K>(y[0], y[1], y[2], s.rot[0]);
             y[0] = fp_reduce<K>(y[0]);
             y[1] = fp_mulmod3<K>(y[1], w1);
             y[2] = fp_mulmod3<K>(y[2], w2);
             if (!FWD) dft3<K>(y[0], y[1], y[2], s.rot[0]);
-        } else if (M == 5) {
+        } else if constexpr (M == 5) {
             if (FWD) dft5<K>(y, s.rot);
             y[0] = fp_reduce<K>(y[0]);
 #pragma unroll
@@ -362,6 +362,331 @@ __device__ void run_stages(double *x0, double *x1, int narr, const DevPlan *__re
             __syncthreads();
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Mid-kernel passes with both operands in one thread.  Thread t owns the same
+// sub-problem of a and of b, so a forward pass loads each twiddle once for
+// both operands, every thread has work in the single-array inverse, and the
+// last forward pass, the pointwise product (src/fpntt.jl:770-785) and the
+// first inverse pass run on registers: one LDS round trip instead of three.
+// The butterflies are exactly those of pass44 / pass1.
+
+// One radix-M stage on registers y[0..M) with its twiddles w[0..M-2].
+template <int K, bool FWD, int M>
+__device__ __forceinline__ void stage_regs(double *y, const double *w, const StageU &s) {
+    if constexpr (M == 2) {
+        const double u = y[0], v = y[1];
+        y[0] = fp_reduce<K>(u + v);
+        y[1] = fp_reduce<K>(u - v);
+    } else if constexpr (M == 4) {
+        if (FWD) dft4<K>(y[0], y[1], y[2], y[3], s.rot[0]);
+        y[0] = fp_reduce<K>(y[0]);
+        y[1] = fp_mulmod3<K>(y[1], w[0]);
+        y[2] = fp_mulmod3<K>(y[2], w[1]);
+        y[3] = fp_mulmod3<K>(y[3], w[2]);
+        if (!FWD) dft4<K>(y[0], y[1], y[2], y[3], s.rot[0]);
+    } else if constexpr (M == 3) {
+        if (FWD) dft3<K>(y[0], y[1], y[2], s.rot[0]);
+        y[0] = fp_reduce<K>(y[0]);
+        y[1] = fp_mulmod3<K>(y[1], w[0]);
+        y[2] = fp_mulmod3<K>(y[2], w[1]);
+        if (!FWD) dft3<K>(y[0], y[1], y[2], s.rot[0]);
+    } else if constexpr (M == 5) {
+        if (FWD) dft5<K>(y, s.rot);
+        y[0] = fp_reduce<K>(y[0]);
+#pragma unroll
+        for (int t = 1; t < 5; t++) y[t] = fp_mulmod3<K>(y[t], w[t - 1]);
+        if (!FWD) {
+            dft5<K>(y, s.rot);
+            y[0] = fp_reduce<K>(y[0]);
+        }
+    } else {
+        if (FWD) dft7<K, false>(y, s.rot);
+        y[0] = fp_reduce<K>(y[0]);
+#pragma unroll
+        for (int t = 1; t < 7; t++) y[t] = fp_mulmod3<K>(y[t], w[t - 1]);
+        if (!FWD) {
+            dft7<K, true>(y, s.rot);
+            y[0] = fp_reduce<K>(y[0]);
+        }
+    }
+}
+
+template <int M>
+__device__ __forceinline__ void load_tw(double *w, const StageU &s, int64_t j) {
+#pragma unroll
+    for (int t = 1; t < M; t++) w[t - 1] = gld(s.tw, (t - 1) * s.Q + j);
+}
+
+// Index geometry of a pass44 sub-problem (see pass44).
+struct Sub44 {
+    int p0, pa, pb, base, sa, sb;
+    int64_t jt;
+};
+
+__device__ __forceinline__ Sub44 sub44(int sl, const StageU &s0, const StageU &s1, const TileGeo &g) {
+    const int q1 = s1.q, q0 = s0.q;
+    const int logq1 = __builtin_ctz((unsigned)q1);
+    const int c = sl & ((1 << g.logC) - 1);
+    const int r = sl >> g.logC;
+    const int jj = r & (q1 - 1);
+    const int row0 = ((r >> logq1) << 4) * q1 + jj;
+    Sub44 u;
+    u.base = (row0 << g.logC) + c;
+    u.p0 = pidx(u.base);
+    u.sa = q1 << g.logC;
+    u.sb = q0 << g.logC;
+    u.pa = u.sa + (u.sa >> 4);
+    u.pb = u.sb + (u.sb >> 4);
+    u.jt = (int64_t)jj * g.Lb + g.jcol0 + c;
+    return u;
+}
+
+template <bool LIN>
+__device__ __forceinline__ int ix44(const Sub44 &u, int a, int b) {
+    return LIN ? u.p0 + a * u.pa + b * u.pb : pidx(u.base + a * u.sa + b * u.sb);
+}
+
+// Forward stages s0 (over b) and s1 (over a) of one sub-problem on registers.
+template <int K, bool FWD>
+__device__ __forceinline__ void regs44(double (&y)[4][4], const double (&w0)[4][3], const double (&w1)[3],
+                                       const StageU &s0, const StageU &s1) {
+    if (FWD) {
+#pragma unroll
+        for (int a = 0; a < 4; a++) stage_regs<K, true, 4>(y[a], w0[a], s0);
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            double t[4] = {y[0][b], y[1][b], y[2][b], y[3][b]};
+            stage_regs<K, true, 4>(t, w1, s1);
+#pragma unroll
+            for (int a = 0; a < 4; a++) y[a][b] = t[a];
+        }
+    } else {
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            double t[4] = {y[0][b], y[1][b], y[2][b], y[3][b]};
+            stage_regs<K, false, 4>(t, w1, s1);
+#pragma unroll
+            for (int a = 0; a < 4; a++) y[a][b] = t[a];
+        }
+#pragma unroll
+        for (int a = 0; a < 4; a++) stage_regs<K, false, 4>(y[a], w0[a], s0);
+    }
+}
+
+template <bool LIN>
+__device__ __forceinline__ void tw44(double (&w0)[4][3], double (&w1)[3], const Sub44 &u, const StageU &s0,
+                                     const StageU &s1) {
+#pragma unroll
+    for (int a = 0; a < 4; a++) load_tw<4>(w0[a], s0, u.jt + a * s1.Q);
+    load_tw<4>(w1, s1, u.jt);
+}
+
+// Forward radix-4 pair on NA arrays (a, and b when NA == 2).
+template <int K, bool LIN, int NA>
+__device__ void pass44f(double *__restrict__ xa, double *__restrict__ xb, const StageU &s0, const StageU &s1,
+                        const TileGeo &g) {
+    const int nsub = (g.S >> 4) << g.logC;
+    for (int sl = threadIdx.x; sl < nsub; sl += blockDim.x) {
+        const Sub44 u = sub44(sl, s0, s1, g);
+        double w0[4][3], w1[3];
+        tw44<LIN>(w0, w1, u, s0, s1);
+#pragma unroll
+        for (int n = 0; n < NA; n++) {
+            double *__restrict__ x = n ? xb : xa;
+            double y[4][4];
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 4; b++) y[a][b] = x[ix44<LIN>(u, a, b)];
+            regs44<K, true>(y, w0, w1, s0, s1);
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 4; b++) x[ix44<LIN>(u, a, b)] = y[a][b];
+        }
+    }
+}
+
+// Index geometry of a single-stage butterfly (see pass1).
+struct Sub1 {
+    int p0, pstride, base, strideC;
+    int64_t j;
+};
+
+template <int M>
+__device__ __forceinline__ Sub1 sub1(int beta, const StageU &s, const TileGeo &g) {
+    const int q = s.q;
+    const int c = beta & ((1 << g.logC) - 1);
+    const uint32_t r = (uint32_t)beta >> g.logC;
+    const uint32_t ob = (uint32_t)(((uint64_t)r * s.qmagic) >> 32);
+    const uint32_t jj = r - ob * (uint32_t)q;
+    Sub1 u;
+    u.base = ((int)(ob * (uint32_t)(M * q) + jj) << g.logC) + c;
+    u.strideC = q << g.logC;
+    u.p0 = pidx(u.base);
+    u.pstride = u.strideC + (u.strideC >> 4);
+    u.j = (int64_t)jj * g.Lb + g.jcol0 + c;
+    return u;
+}
+
+template <bool LIN>
+__device__ __forceinline__ int ix1(const Sub1 &u, int t) {
+    return LIN ? u.p0 + t * u.pstride : pidx(u.base + t * u.strideC);
+}
+
+// Forward single stage on NA arrays.
+template <int K, int M, bool LIN, int NA>
+__device__ void pass1f(double *__restrict__ xa, double *__restrict__ xb, const StageU &s, const TileGeo &g) {
+    const int nb = (g.S / M) << g.logC;
+    for (int beta = threadIdx.x; beta < nb; beta += blockDim.x) {
+        const Sub1 u = sub1<M>(beta, s, g);
+        double w[M > 1 ? M - 1 : 1];
+        load_tw<M>(w, s, u.j);
+#pragma unroll
+        for (int n = 0; n < NA; n++) {
+            double *__restrict__ x = n ? xb : xa;
+            double y[M];
+#pragma unroll
+            for (int t = 0; t < M; t++) y[t] = x[ix1<LIN>(u, t)];
+            stage_regs<K, true, M>(y, w, s);
+#pragma unroll
+            for (int t = 0; t < M; t++) x[ix1<LIN>(u, t)] = y[t];
+        }
+    }
+}
+
+// The last forward pass (a radix-4 pair), the pointwise product and the same
+// pair inverted, on registers.  NA == 2: a product; with bspec, b's points
+// are already its finished spectrum and are only loaded.  NA == 1: a square.
+template <int K, bool LIN, int NA, bool BSPEC>
+__device__ void fused44(double *__restrict__ xa, const double *__restrict__ xb, const StageU &s0, const StageU &s1,
+                        const TileGeo &g) {
+    const int nsub = (g.S >> 4) << g.logC;
+    for (int sl = threadIdx.x; sl < nsub; sl += blockDim.x) {
+        const Sub44 u = sub44(sl, s0, s1, g);
+        double ya[4][4], yb[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                ya[a][b] = xa[ix44<LIN>(u, a, b)];
+                if (NA == 2) yb[a][b] = xb[ix44<LIN>(u, a, b)];
+            }
+        double w0[4][3], w1[3];
+        tw44<LIN>(w0, w1, u, s0, s1);
+        regs44<K, true>(ya, w0, w1, s0, s1);
+        if (NA == 2 && !BSPEC) regs44<K, true>(yb, w0, w1, s0, s1);
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) ya[a][b] = fp_mulmod3<K>(ya[a][b], NA == 2 ? yb[a][b] : ya[a][b]);
+        regs44<K, false>(ya, w0, w1, s0, s1);
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) xa[ix44<LIN>(u, a, b)] = ya[a][b];
+    }
+}
+
+// The same for a last forward pass of one radix-M stage.
+template <int K, int M, bool LIN, int NA, bool BSPEC>
+__device__ void fused1(double *__restrict__ xa, const double *__restrict__ xb, const StageU &s, const TileGeo &g) {
+    const int nb = (g.S / M) << g.logC;
+    for (int beta = threadIdx.x; beta < nb; beta += blockDim.x) {
+        const Sub1 u = sub1<M>(beta, s, g);
+        double w[M > 1 ? M - 1 : 1];
+        load_tw<M>(w, s, u.j);
+        double ya[M], yb[M];
+#pragma unroll
+        for (int t = 0; t < M; t++) {
+            ya[t] = xa[ix1<LIN>(u, t)];
+            if (NA == 2) yb[t] = xb[ix1<LIN>(u, t)];
+        }
+        stage_regs<K, true, M>(ya, w, s);
+        if (NA == 2 && !BSPEC) stage_regs<K, true, M>(yb, w, s);
+#pragma unroll
+        for (int t = 0; t < M; t++) ya[t] = fp_mulmod3<K>(ya[t], NA == 2 ? yb[t] : ya[t]);
+        stage_regs<K, false, M>(ya, w, s);
+#pragma unroll
+        for (int t = 0; t < M; t++) xa[ix1<LIN>(u, t)] = ya[t];
+    }
+}
+
+template <int K, int NA>
+__device__ __forceinline__ void pass1f_any(double *xa, double *xb, const StageU &s, const TileGeo &g) {
+    const bool lin = ((s.q << g.logC) & 15) == 0;
+    switch (s.m) {
+    case 4: lin ? pass1f<K, 4, true, NA>(xa, xb, s, g) : pass1f<K, 4, false, NA>(xa, xb, s, g); break;
+    case 2: lin ? pass1f<K, 2, true, NA>(xa, xb, s, g) : pass1f<K, 2, false, NA>(xa, xb, s, g); break;
+    case 3: lin ? pass1f<K, 3, true, NA>(xa, xb, s, g) : pass1f<K, 3, false, NA>(xa, xb, s, g); break;
+    case 5: lin ? pass1f<K, 5, true, NA>(xa, xb, s, g) : pass1f<K, 5, false, NA>(xa, xb, s, g); break;
+    default: lin ? pass1f<K, 7, true, NA>(xa, xb, s, g) : pass1f<K, 7, false, NA>(xa, xb, s, g); break;
+    }
+}
+
+template <int K, int NA, bool BSPEC>
+__device__ __forceinline__ void fused1_any(double *xa, const double *xb, const StageU &s, const TileGeo &g) {
+    const bool lin = ((s.q << g.logC) & 15) == 0;
+    switch (s.m) {
+    case 4:
+        lin ? fused1<K, 4, true, NA, BSPEC>(xa, xb, s, g) : fused1<K, 4, false, NA, BSPEC>(xa, xb, s, g);
+        break;
+    case 2:
+        lin ? fused1<K, 2, true, NA, BSPEC>(xa, xb, s, g) : fused1<K, 2, false, NA, BSPEC>(xa, xb, s, g);
+        break;
+    case 3:
+        lin ? fused1<K, 3, true, NA, BSPEC>(xa, xb, s, g) : fused1<K, 3, false, NA, BSPEC>(xa, xb, s, g);
+        break;
+    case 5:
+        lin ? fused1<K, 5, true, NA, BSPEC>(xa, xb, s, g) : fused1<K, 5, false, NA, BSPEC>(xa, xb, s, g);
+        break;
+    default:
+        lin ? fused1<K, 7, true, NA, BSPEC>(xa, xb, s, g) : fused1<K, 7, false, NA, BSPEC>(xa, xb, s, g);
+        break;
+    }
+}
+
+// What the mid kernel multiplies: two transformed operands, a square, or an
+// operand by a cached spectrum.
+enum MidMode { MID_MUL = 0, MID_SQR = 1, MID_SPEC = 2 };
+
+// Stages [s0, s1) forward on a (and on b for MID_MUL), the pointwise product,
+// then [s0, s1) inverse on a: every forward pass but the last as in
+// run_stages (radix-4 pairs from the start), the last forward pass fused with
+// the product and its own inverse, the rest of the inverse by run_stages.
+template <int K, int MODE>
+__device__ void mid_stages(double *la, double *lb, const DevPlan *__restrict__ P, int s0, int s1, const TileGeo &g) {
+    constexpr int NF = MODE == MID_MUL ? 2 : 1;   // arrays transformed forward
+    constexpr int NL = MODE == MID_SQR ? 1 : 2;   // arrays read by the product
+    constexpr bool BS = MODE == MID_SPEC;
+    const double *tw = unip(cst(P)->tw);
+    int s = s0;
+    for (;;) {
+        const StageU a = stage_u(&P->st[s], tw);
+        const bool pair = a.m == 4 && s + 1 < s1 && uni(cst(P)->st[s + 1].m) == 4;
+        const int e = s + (pair ? 2 : 1);
+        if (pair) {
+            const StageU b = stage_u(&P->st[s + 1], tw);
+            const bool lin = (((a.q | b.q) << g.logC) & 15) == 0;
+            if (e >= s1) {
+                if (lin) fused44<K, true, NL, BS>(la, lb, a, b, g);
+                else fused44<K, false, NL, BS>(la, lb, a, b, g);
+            } else {
+                if (lin) pass44f<K, true, NF>(la, lb, a, b, g);
+                else pass44f<K, false, NF>(la, lb, a, b, g);
+            }
+        } else if (e >= s1) {
+            fused1_any<K, NL, BS>(la, lb, a, g);
+        } else {
+            pass1f_any<K, NF>(la, lb, a, g);
+        }
+        __syncthreads();
+        if (e >= s1) break;
+        s = e;
+    }
+    run_stages<K, false>(la, nullptr, 1, P, s0, s, g);
 }
 
 // Points each thread moves per round in the tile load loops.
@@ -680,26 +1005,15 @@ __device__ void mid_body(const DevJob *__restrict__ Jp, const Tile &t, double *l
         else load_pair<LU / 2, false, false>(la, lb, sa, sb, S, g, gbase);
     }
     __syncthreads();
-    run_stages<K, true>(la, lb, sq || bspec ? 1 : 2, P, s0, s1, g);
     // squares allocate one array: lb is never touched for them
-    if (sq) {
-        for (int e = threadIdx.x; e < S; e += blockDim.x) {
-            const int i = pidx(e);
-            la[i] = fp_mulmod3<K>(la[i], la[i]);
-        }
-    } else {
-        for (int e = threadIdx.x; e < S; e += blockDim.x) {
-            const int i = pidx(e);
-            la[i] = fp_mulmod3<K>(la[i], lb[i]);
-        }
-    }
-    __syncthreads();
-    run_stages<K, false>(la, nullptr, 1, P, s0, s1, g);
+    if (sq) mid_stages<K, MID_SQR>(la, lb, P, s0, s1, g);
+    else if (bspec) mid_stages<K, MID_SPEC>(la, lb, P, s0, s1, g);
+    else mid_stages<K, MID_MUL>(la, lb, P, s0, s1, g);
     double *__restrict__ xo = unip(J->xa[K]);
     store_tile(xo, la, S, g, gbase);
 }
 
-__global__ void __launch_bounds__(1024) k_mid(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles,
+__global__ void __launch_bounds__(MID_MAX_THREADS) k_mid(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles,
                                               TileGen gen) {
     extern __shared__ double lds[];
     const Tile t = fetch_tile(tiles, gen);
