@@ -79,6 +79,12 @@ int32_t nbh_batch_create(const nbh_job *jobs, int64_t count, nbh_batch *out);
 /* Run it (enqueue on stream); can be called repeatedly. */
 int32_t nbh_batch_run(nbh_batch b, void *stream);
 void nbh_batch_destroy(nbh_batch b);
+/* The batch's launch sequence (every phase, with the fork/join across the
+ * library's side streams) captured once into a HIP graph; replay launches the
+ * graph on `stream` (capturing first if needed): the same work as
+ * nbh_batch_run in one submission.  The graph holds kernels and events only. */
+int32_t nbh_batch_capture(nbh_batch b);
+int32_t nbh_batch_replay(nbh_batch b, void *stream);
 /* Arena form for servers: the bytes a batch occupies (workspace + its job and
  * tile tables), then a batch planned inside a caller-owned device region
  * with its tables uploaded on `stream` -- no device allocation and no host

@@ -732,12 +732,16 @@ struct nbh_batch_s {
     std::vector<Phase> phases;
     int nsteps = 0;
     std::vector<hipEvent_t> ev;  // fork event per step + one join event per launch
+    hipGraph_t graph = nullptr;  // the launch sequence captured once (nbh_batch_capture)
+    hipGraphExec_t gexec = nullptr;
 };
 
 namespace {
 
 void batch_free(nbh_batch_s *B) {
     if (!B) return;
+    if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
+    if (B->graph) (void)hipGraphDestroy(B->graph);
     if (!B->external) {
         if (B->djobs) (void)hipFree(B->djobs);
         if (B->dtiles) (void)hipFree(B->dtiles);
@@ -1217,6 +1221,47 @@ int32_t nbh_batch_run(nbh_batch b, void *stream) {
     RCCHK(ensure_init());
     if (!b) return fail(NBH_ERR_ARG, "null handle");
     return batch_run(b, stream ? (hipStream_t)stream : g.stream);
+}
+
+// Capture the batch's whole launch sequence -- every phase, and the fork /
+// join events across the side streams -- into a HIP graph on a private
+// stream, so a run is one graph launch.  The sequence records kernels and
+// events only (no memset nodes).  Per-launch profiling is off while capturing.
+int32_t nbh_batch_capture(nbh_batch b) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    if (!b) return fail(NBH_ERR_ARG, "null handle");
+    if (b->gexec) return NBH_OK;
+    hipStream_t cs = nullptr;
+    HIPCHK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    const bool prof = g.prof;
+    g.prof = false;
+    hipGraph_t graph = nullptr;
+    int rc = NBH_OK;
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        rc = batch_run(b, cs);
+        e = hipStreamEndCapture(cs, &graph);
+    }
+    g.prof = prof;
+    if (e == hipSuccess && rc == NBH_OK) e = hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0);
+    (void)hipStreamDestroy(cs);
+    if (rc != NBH_OK || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        b->gexec = nullptr;
+        return rc != NBH_OK ? rc : fail(NBH_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    }
+    b->graph = graph;
+    return NBH_OK;
+}
+
+int32_t nbh_batch_replay(nbh_batch b, void *stream) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    if (!b) return fail(NBH_ERR_ARG, "null handle");
+    if (!b->gexec) RCCHK(nbh_batch_capture(b));
+    HIPCHK(hipGraphLaunch(b->gexec, stream ? (hipStream_t)stream : g.stream));
+    return NBH_OK;
 }
 
 int32_t nbh_batch_arena_bytes(const nbh_job *jobs, int64_t count, int64_t *bytes) {

@@ -78,6 +78,14 @@ class Batch:
         s = ctypes.c_void_p(stream.cuda_stream) if stream is not None else _stream()
         check(lib().nbh_batch_run(self._h, s))
 
+    def capture(self) -> None:
+        """Capture the launch sequence into a HIP graph (replay() runs it)."""
+        check(lib().nbh_batch_capture(self._h))
+
+    def replay(self, stream=None) -> None:
+        s = ctypes.c_void_p(stream.cuda_stream) if stream is not None else _stream()
+        check(lib().nbh_batch_replay(self._h, s))
+
     def close(self) -> None:
         if self._h:
             lib().nbh_batch_destroy(self._h)
