@@ -677,6 +677,8 @@ def bench_serve(args, D, engine, wl, info, stream):
         srv = serve.Server(budget_bytes=int(args.budget_gib * (1 << 30)), nstreams=4, max_batch=256)
         keep = set(range(0, count, max(1, count // 64)))
         reqs = [serve.Request(dev[p][0], dev[p][1], *bits[p], keep=i in keep) for i, p in enumerate(pick)]
+        srv.prepare(reqs)                 # start-up: plans of every request shape
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
         i = 0
         while i < count or not srv.idle():

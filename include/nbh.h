@@ -228,6 +228,10 @@ int32_t nbh_profile_enable(int32_t on);
  * nside = 0 issues them in order on the caller's stream (every launch then
  * has the GPU to itself, which is what a per-launch profile should see). */
 int32_t nbh_set_streams(int32_t nside);
+/* The same for one batch (-1: follow the library setting).  A serving loop
+ * that already overlaps micro-batches on its own streams sets 0: each
+ * fork/join costs host API calls per launch. */
+int32_t nbh_batch_set_streams(nbh_batch b, int32_t nside);
 int32_t nbh_profile_read(double *ms, int64_t *launches, int32_t nkinds);
 
 /* Synchronise the library stream (or `stream`). */

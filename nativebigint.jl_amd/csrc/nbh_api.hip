@@ -732,6 +732,7 @@ struct nbh_batch_s {
     std::vector<Phase> phases;
     int nsteps = 0;
     std::vector<hipEvent_t> ev;  // fork event per step + one join event per launch
+    int nside = -1;              // side streams for this batch's steps (-1: the library setting)
     hipGraph_t graph = nullptr;  // the launch sequence captured once (nbh_batch_capture)
     hipGraphExec_t gexec = nullptr;
 };
@@ -1006,17 +1007,18 @@ int launch_phase(nbh_batch_s *B, const nbh_batch_s::Phase &ph, hipStream_t s) {
 // library's side streams after an event on `s` and join back before the next
 // step, so small shape classes run beside the big ones instead of after them.
 int batch_run(nbh_batch_s *B, hipStream_t s) {
+    const int nside = B->nside >= 0 ? B->nside : g.nside;
     size_t i = 0;
     while (i < B->phases.size()) {
         size_t j = i;
         while (j < B->phases.size() && B->phases[j].step == B->phases[i].step) j++;
-        if (j - i == 1 || g.nside == 0) {
+        if (j - i == 1 || nside == 0) {
             for (size_t k = i; k < j; k++) RCCHK(launch_phase(B, B->phases[k], s));
         } else {
             hipEvent_t fork = B->ev[(size_t)B->phases[i].step];
             HIPCHK(hipEventRecord(fork, s));
             for (size_t k = i; k < j; k++) {
-                hipStream_t side = g.side[(k - i) % g.nside];
+                hipStream_t side = g.side[(k - i) % nside];
                 HIPCHK(hipStreamWaitEvent(side, fork, 0));
                 RCCHK(launch_phase(B, B->phases[k], side));
                 hipEvent_t done = B->ev[(size_t)B->nsteps + k];
@@ -1747,6 +1749,15 @@ int32_t nbh_set_streams(int32_t nside) {
     RCCHK(ensure_init());
     if (nside < 0 || nside > NSIDE) return fail(NBH_ERR_ARG, "side streams must be 0.." + std::to_string(NSIDE));
     g.nside = nside;
+    return NBH_OK;
+}
+
+int32_t nbh_batch_set_streams(nbh_batch b, int32_t nside) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    if (!b) return fail(NBH_ERR_ARG, "null handle");
+    if (nside < -1 || nside > NSIDE) return fail(NBH_ERR_ARG, "side streams must be -1.." + std::to_string(NSIDE));
+    b->nside = nside;
     return NBH_OK;
 }
 

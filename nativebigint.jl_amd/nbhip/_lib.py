@@ -89,6 +89,7 @@ SIGNATURES = {
     "nbh_mul_spec_dev": (_i32, [_vp, _vp, _i64, _i64, _vp, _vp]),
     "nbh_spec_batch_create": (_i32, [_P(nbh_spec_job), _i64, _P(_vp)]),
     "nbh_set_streams": (_i32, [_i32]),
+    "nbh_batch_set_streams": (_i32, [_vp, _i32]),
     "nbh_batch_capture": (_i32, [_vp]),
     "nbh_batch_replay": (_i32, [_vp, _vp]),
     "nbh_barrett_create": (_i32, [_vp, _i64, _vp, _i64, _vp, _P(_vp)]),

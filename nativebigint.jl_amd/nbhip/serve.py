@@ -41,7 +41,7 @@ class Request:
     t_arrival: float = 0.0
     t_admit: float = 0.0
     t_done: float = 0.0
-    result: torch.Tensor | None = None   # view into the arena while in flight
+    off: int = -1                        # byte offset of the result in the arena while in flight
     keep: bool = False                   # copy the result out on completion
     out: torch.Tensor | None = None
     rid: int = 0
@@ -52,6 +52,9 @@ class Request:
         self.m = self.a.numel()
         self.n = self.m if self.b is None else self.b.numel()
         self.limbs = self.m + self.n
+        # the nbh_job record (r filled in at admission) and the sizing key
+        self.row = np.array([0, self.dev_a, self.dev_b, self.m, self.n, self.bits_a, self.bits_b], dtype=np.int64)
+        self.key = (self.m, None if self.b is None else self.n, self.bits_a, self.bits_b)
 
 
 class Arena:
@@ -132,13 +135,14 @@ class Server:
 
     # -- the loop -----------------------------------------------------------------
     def _jobs(self, reqs, results_off):
-        """nbh_job records (r, a, b, m, n, bits_a, bits_b) as one int64 array."""
-        rec = np.empty((len(reqs), 7), dtype=np.int64)
-        off = results_off
-        for i, r in enumerate(reqs):
-            rec[i] = (self.base + off, r.dev_a, r.dev_b, r.m, r.n, r.bits_a, r.bits_b)
-            off += (8 * r.limbs + ALIGN - 1) // ALIGN * ALIGN
-        return rec, off - results_off
+        """nbh_job records (r, a, b, m, n, bits_a, bits_b) as one int64 array,
+        the results laid out back to back (ALIGN-aligned) from results_off;
+        returns the records, the results' bytes and their offsets."""
+        rec = np.stack([r.row for r in reqs])
+        size = (8 * (rec[:, 3] + rec[:, 4]) + ALIGN - 1) // ALIGN * ALIGN
+        offs = results_off + np.cumsum(size) - size
+        rec[:, 0] = self.base + offs
+        return rec, int(size.sum()), offs
 
     @staticmethod
     def _ptr(rec):
@@ -148,15 +152,22 @@ class Server:
         """Arena bytes of one request alone (result + workspace + tables).  A
         micro-batch needs at most the sum over its requests (its tables are
         shared and aligned once), so admission sums these."""
-        key = (r.a.numel(), None if r.b is None else r.b.numel(), r.bits_a, r.bits_b)
+        key = r.key
         v = self._need_cache.get(key)
         if v is None:
-            rec, res_bytes = self._jobs([r], 0)
+            rec, res_bytes, _ = self._jobs([r], 0)
             ws = ctypes.c_int64()
             check(lib().nbh_batch_arena_bytes(self._ptr(rec), 1, ctypes.byref(ws)))
             v = res_bytes + ws.value + ALIGN
             self._need_cache[key] = v
         return v
+
+    def prepare(self, requests) -> None:
+        """Size every distinct request shape once (building the plans and
+        twiddle tables of their transform lengths), as a server does at
+        start-up, so admission never builds a plan."""
+        for r in requests:
+            self._need1(r)
 
     def _retire(self):
         now = time.perf_counter()
@@ -166,9 +177,9 @@ class Server:
                     r.t_done = now
                     if r.keep:
                         with torch.cuda.stream(self.streams[k]):
-                            r.out = r.result.clone()
-                    r.result = None
-                    self.done.append(r)
+                            r.out = self.result_view(r).clone()
+                    r.off = -1
+                self.done.extend(f.reqs)
                 if any(r.keep for r in f.reqs):
                     self.streams[k].synchronize()
                 lib().nbh_batch_release(f.handle)
@@ -194,24 +205,30 @@ class Server:
         block = self.arena.alloc(need)
         assert block is not None
         off = block[0]
-        rec, res_bytes = self._jobs(reqs, off)
+        rec, res_bytes, offs = self._jobs(reqs, off)
         s = self.streams[k]
         h = ctypes.c_void_p()
         rc = lib().nbh_batch_create_in(self._ptr(rec), len(reqs), ctypes.c_void_p(self.base + off + res_bytes),
                                        block[1] - res_bytes, ctypes.c_void_p(s.cuda_stream), ctypes.byref(h))
         check(rc)
+        # micro-batches already overlap on the server's streams: issue each
+        # one's launches in order (no per-launch fork/join host calls)
+        check(lib().nbh_batch_set_streams(h, 0))
         check(lib().nbh_batch_run(h, ctypes.c_void_p(s.cuda_stream)))
         ev = torch.cuda.Event()
         ev.record(s)
         now = time.perf_counter()
-        o = off
-        for i, r in enumerate(reqs):
+        for r, o in zip(reqs, offs.tolist()):
             r.t_admit = now
-            r.result = self.mem[o:o + 8 * r.limbs].view(torch.int64)
-            o += (8 * r.limbs + ALIGN - 1) // ALIGN * ALIGN
+            r.off = o
         self.busy[k] = _Flight(reqs, h, ev, block, k, rec)
         self.batches += 1
         self.host_admit_s += time.perf_counter() - t_start
+
+    def result_view(self, r: Request) -> torch.Tensor:
+        """The arena limbs of an in-flight request's result (valid until the
+        request's micro-batch is retired)."""
+        return self.mem[r.off:r.off + 8 * r.limbs].view(torch.int64)
 
     def poll(self) -> None:
         self._retire()
