@@ -62,9 +62,13 @@ int32_t nbh_sqr_dev(uint64_t *r, const uint64_t *a, int64_t n, int64_t bits_a, v
 
 /* Batched products (device pointers).  One job is r = a*b, or r = a^2 when
  * b == NULL.  There is no batched entry in the reference: each job is one
- * mul_fpntt2!/sqr_fpntt2! call (src/fpntt.jl:982,1016), all of them run in a
- * handful of launches that each cover every job.  bits_* <= 0 means "read
- * the top limb". */
+ * mul!/sqr! call, all of them run in a handful of launches that each cover
+ * every job.  Jobs the reference sends to mul_fpntt2!/sqr_fpntt2!
+ * (src/fpntt.jl:982,1016; mul!: smaller operand >= 64 limbs and m + n >= 304,
+ * sqr!: n >= 160, src/mul.jl:24-26,127-164,383-389) run in the NTT
+ * pipeline; the rest (its Karatsuba / mul_basecase! range) run in a
+ * schoolbook kernel, one workgroup per product, beside the first NTT launch.
+ * bits_* <= 0 means "read the top limb". */
 typedef struct {
     uint64_t *r;
     const uint64_t *a;
@@ -219,9 +223,10 @@ int32_t nbh_get_tuning_for(int64_t N, int32_t *knobs4);
  * bracketed by HIP events on the stream it is launched on.  nbh_profile_read
  * synchronises, returns the summed milliseconds and launch counts per kernel
  * kind (0 forward strided group + pack, 1 mid, 2 inverse strided group,
- * 3 unpack, 4 carry fix, 5 cofactor-step combine) and clears them.  Instrumentation only: no reference
+ * 3 unpack, 4 carry fix, 5 cofactor-step combine, 6 schoolbook products below
+ * the NTT thresholds) and clears them.  Instrumentation only: no reference
  * counterpart. */
-#define NBH_PROF_KINDS 6
+#define NBH_PROF_KINDS 7
 int32_t nbh_profile_enable(int32_t on);
 /* A batch step's launches (one per tile shape) are independent; by default
  * they fork onto 4 library streams so small shapes run beside big ones.

@@ -715,7 +715,7 @@ int host_product(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, i
 
 struct nbh_batch_s {
     struct Phase {
-        int kind;        // 0 group, 1 mid, 2 unpack, 3 carry fix
+        int kind;        // 0 group, 1 mid, 2 unpack, 3 carry fix, 5 schoolbook products
         int fwd;
         int64_t off, count;
         size_t lds;
@@ -724,6 +724,8 @@ struct nbh_batch_s {
     };
     std::vector<DevJob> jobs;
     std::vector<TileRange> ranges;  // host copies (kept alive for async uploads)
+    std::vector<BaseJob> base;      // products below the NTT thresholds (schoolbook)
+    BaseJob *dbase = nullptr;
     int64_t ntiles = 0;
     DevJob *djobs = nullptr;
     Tile *dtiles = nullptr;
@@ -747,6 +749,7 @@ void batch_free(nbh_batch_s *B) {
         if (B->djobs) (void)hipFree(B->djobs);
         if (B->dtiles) (void)hipFree(B->dtiles);
         if (B->ws) (void)hipFree(B->ws);
+        if (B->dbase) (void)hipFree(B->dbase);
     }
     for (auto e : B->ev) g.ev_pool.push_back(e);
     delete B;
@@ -773,6 +776,7 @@ int batch_plan(const JobIn *jobs, int64_t count, nbh_batch_s **out, size_t *need
     std::vector<DevJob> &J = B->jobs;
     J.resize((size_t)count);
     std::vector<int64_t> bits((size_t)(2 * count), 0);
+    std::vector<char> isbase((size_t)count, 0);
     bool need_top = false;
     for (int64_t i = 0; i < count; i++) {
         const JobIn &q = jobs[i];
@@ -789,6 +793,20 @@ int batch_plan(const JobIn *jobs, int64_t count, nbh_batch_s **out, size_t *need
         bits[(size_t)(2 * i)] = q.bits_a;
         bits[(size_t)(2 * i + 1)] = sqr ? q.bits_a : q.bits_b;
         if (q.bits_a <= 0 || (!sqr && !q.spec && q.bits_b <= 0)) need_top = true;
+        // below the reference's NTT thresholds: schoolbook (src/mul.jl:131,145,388)
+        const int64_t lo = std::min(d.m, d.n), hi = std::max(d.m, d.n);
+        if (!q.spec && (sqr ? d.m < SQR_FPNTT_THRESHOLD : lo < MUL_FPNTT_MIN || d.m + d.n < 2 * MUL_FPNTT_THRESHOLD)) {
+            isbase[(size_t)i] = 1;
+            const bool swap = d.n > d.m;
+            B->base.push_back(BaseJob{swap ? d.b : d.a, swap ? d.a : d.b, d.r, hi, lo});
+        }
+    }
+    // schoolbook products need no sizing, plans or workspace
+    if (need_top) {
+        need_top = false;
+        for (int64_t i = 0; i < count; i++)
+            if (!isbase[(size_t)i] && (jobs[i].bits_a <= 0 || (jobs[i].b && !jobs[i].spec && jobs[i].bits_b <= 0)))
+                need_top = true;
     }
     if (need_top && (need || arena)) return fail(NBH_ERR_ARG, "arena batches need the operands' bit lengths");
     if (need_top) {
@@ -816,6 +834,11 @@ int batch_plan(const JobIn *jobs, int64_t count, nbh_batch_s **out, size_t *need
     std::vector<size_t> offs((size_t)count);
     for (int64_t i = 0; i < count; i++) {
         DevJob &d = J[(size_t)i];
+        offs[(size_t)i] = ws;
+        if (isbase[(size_t)i]) {
+            d.zero = 1;  // no NTT work; the schoolbook phase writes r
+            continue;
+        }
         if (jobs[i].spec) RCCHK(size_job_spec(d, bits[(size_t)(2 * i)], *jobs[i].spec, &Ns[(size_t)i]));
         else RCCHK(size_job(d, bits[(size_t)(2 * i)], bits[(size_t)(2 * i + 1)], &Ns[(size_t)i]));
         offs[(size_t)i] = ws;
@@ -923,26 +946,36 @@ int batch_plan(const JobIn *jobs, int64_t count, nbh_batch_s **out, size_t *need
     std::vector<TileReq> list2 = list;
     push_phase(2, 0, list);
     push_phase(3, 0, list2);
+    // the schoolbook products: one launch, beside the first NTT step
+    if (!B->base.empty()) {
+        const int step = B->phases.empty() ? B->nsteps++ : B->phases.front().step;
+        B->phases.insert(B->phases.begin(), nbh_batch_s::Phase{5, 0, 0, (int64_t)B->base.size(), 0,
+                                                              UNPACK_THREADS, step});
+    }
     // workspace offsets were laid out from 0: rebase them onto the real workspace
     const size_t ws_bytes = align256(std::max<size_t>(ws, 256));
     const size_t jobs_bytes = align256(sizeof(DevJob) * (size_t)count);
     // tile list followed by the ranges it is expanded from
     const size_t tiles_list_bytes = align256(sizeof(Tile) * (size_t)std::max<int64_t>(ntiles, 1));
     const size_t tiles_bytes = tiles_list_bytes + align256(sizeof(TileRange) * std::max<size_t>(ranges.size(), 1));
+    const size_t base_bytes = B->base.empty() ? 0 : align256(sizeof(BaseJob) * B->base.size());
     if (need) {
-        *need = ws_bytes + jobs_bytes + tiles_bytes;
+        *need = ws_bytes + jobs_bytes + tiles_bytes + base_bytes;
         return NBH_OK;
     }
     if (arena) {
-        if (ws_bytes + jobs_bytes + tiles_bytes > arena_bytes) return fail(NBH_ERR_BUDGET, "arena too small");
+        if (ws_bytes + jobs_bytes + tiles_bytes + base_bytes > arena_bytes)
+            return fail(NBH_ERR_BUDGET, "arena too small");
         B->external = true;
         B->ws = arena;
         B->djobs = (DevJob *)((char *)arena + ws_bytes);
         B->dtiles = (Tile *)((char *)arena + ws_bytes + jobs_bytes);
+        if (base_bytes) B->dbase = (BaseJob *)((char *)arena + ws_bytes + jobs_bytes + tiles_bytes);
     } else {
         HIPCHK(hipMalloc(&B->ws, ws_bytes));
         HIPCHK(hipMalloc((void **)&B->djobs, jobs_bytes));
         HIPCHK(hipMalloc((void **)&B->dtiles, tiles_bytes));
+        if (base_bytes) HIPCHK(hipMalloc((void **)&B->dbase, base_bytes));
     }
     TileRange *dranges = (TileRange *)((char *)B->dtiles + tiles_list_bytes);
     auto rebase = [&](auto *p) {
@@ -960,10 +993,15 @@ int batch_plan(const JobIn *jobs, int64_t count, nbh_batch_s **out, size_t *need
         HIPCHK(hipMemcpyAsync(dranges, ranges.data(), sizeof(TileRange) * ranges.size(), hipMemcpyHostToDevice,
                               astream));
         HIPCHK(launch_expand_tiles(dranges, (int64_t)ranges.size(), B->dtiles, astream));
+        if (base_bytes)
+            HIPCHK(hipMemcpyAsync(B->dbase, B->base.data(), sizeof(BaseJob) * B->base.size(), hipMemcpyHostToDevice,
+                                  astream));
     } else {
         HIPCHK(hipMemcpy(B->djobs, J.data(), sizeof(DevJob) * (size_t)count, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(dranges, ranges.data(), sizeof(TileRange) * ranges.size(), hipMemcpyHostToDevice));
         HIPCHK(launch_expand_tiles(dranges, (int64_t)ranges.size(), B->dtiles, g.stream));
+        if (base_bytes)
+            HIPCHK(hipMemcpy(B->dbase, B->base.data(), sizeof(BaseJob) * B->base.size(), hipMemcpyHostToDevice));
         HIPCHK(hipStreamSynchronize(g.stream));
     }
     B->ev.resize((size_t)B->nsteps + B->phases.size(), nullptr);
@@ -993,8 +1031,9 @@ int batch_create(const nbh_job *jobs, int64_t count, nbh_batch_s **out, size_t *
 int launch_phase(nbh_batch_s *B, const nbh_batch_s::Phase &ph, hipStream_t s) {
     const TileGen none{0, 1, 1, 0, 0};
     const Tile *t = B->dtiles + ph.off;
-    ProfScope ps(ph.kind == 0 ? (ph.fwd ? 0 : 2) : ph.kind == 1 ? 1 : ph.kind == 2 ? 3 : 4, s);
+    ProfScope ps(ph.kind == 0 ? (ph.fwd ? 0 : 2) : ph.kind == 1 ? 1 : ph.kind == 2 ? 3 : ph.kind == 5 ? 6 : 4, s);
     switch (ph.kind) {
+    case 5: HIPCHK(launch_basecase(B->dbase, ph.count, s)); break;
     case 0: HIPCHK(launch_group(B->djobs, t, none, ph.count, ph.fwd, ph.lds, ph.threads, s)); break;
     case 1: HIPCHK(launch_mid(B->djobs, t, none, ph.count, ph.lds, ph.threads, s)); break;
     case 2: HIPCHK(launch_unpack(B->djobs, t, none, ph.count, s)); break;

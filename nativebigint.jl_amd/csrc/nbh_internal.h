@@ -144,6 +144,19 @@ struct BarrettFixJob {
     const uint64_t *m;  // k limbs
     int64_t k;
 };
+
+// A product below the NTT thresholds (mul!: smaller operand < 64 limbs or
+// m + n < 304; sqr!: n < 160, src/mul.jl:24-26,127-164,383-389): schoolbook
+// on the device.  b is the shorter operand (n <= BASE_NMAX limbs).
+constexpr int BASE_NMAX = 160;
+constexpr int64_t MUL_FPNTT_MIN = 64;        // src/mul.jl:24
+constexpr int64_t MUL_FPNTT_THRESHOLD = 152; // src/mul.jl:25
+constexpr int64_t SQR_FPNTT_THRESHOLD = 160; // src/mul.jl:26
+struct BaseJob {
+    const uint64_t *a, *b;
+    uint64_t *r;
+    int64_t m, n;  // limbs of a (the longer) and b; r has m + n
+};
 }  // namespace nbh
 
 static_assert(sizeof(nbh::Tile) == 16, "kernels read a Tile as four 32-bit words");

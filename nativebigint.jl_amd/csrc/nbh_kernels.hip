@@ -1477,6 +1477,110 @@ __global__ void __launch_bounds__(UNPACK_THREADS) k_barrett_fix(const BarrettFix
 }
 
 // ---------------------------------------------------------------------------
+// Schoolbook products below the NTT thresholds (the results of mul_basecase! /
+// sqr_basecase! / Karatsuba, src/kernels/mul.jl:305-320,406, src/mul.jl:
+// 66-118: the product is unique, so any exact method gives the reference's
+// limbs).  One workgroup per product.  Thread u owns result limbs
+// [c0 + 4u, c0 + 4u + 4) of each 1024-limb chunk: it sums the four columns
+// S_t = sum_j a[t-j] b[j] (< 2^136, three words each) that start in those
+// limbs into six words -- four of its own and a two-word spill into the
+// next thread's limbs.  Adding the previous thread's spill leaves at most one
+// carry out of the four words; carries between threads are the block
+// generate/propagate prefix, between chunks a register (as in the unpack).
+
+__device__ __forceinline__ void add3(uint64_t &c0, uint64_t &c1, uint64_t &c2, uint64_t lo, uint64_t hi) {
+    c0 += lo;
+    const uint64_t k0 = c0 < lo;
+    const uint64_t t = hi + k0;  // hi < 2^64 - 1, so no overflow here
+    c1 += t;
+    c2 += c1 < t;
+}
+
+__device__ void basecase_body(const BaseJob &J, uint64_t *sb, uint64_t (*spill)[2], uint32_t *wsum) {
+    const uint64_t *__restrict__ a = J.a;
+    const int64_t m = J.m, n = J.n, rn = m + n;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) sb[j] = J.b[j];
+    if (threadIdx.x == 0) spill[0][0] = spill[0][1] = 0;
+    __syncthreads();
+    uint32_t carry = 0;
+    for (int64_t c0 = 0; c0 < rn; c0 += UNPACK_LPC) {
+        const int64_t k0 = c0 + UNPACK_LPT * threadIdx.x;
+        uint64_t W[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < UNPACK_LPT; i++) {
+            const int64_t t = k0 + i;
+            uint64_t x0 = 0, x1 = 0, x2 = 0;
+            if (t < rn - 1) {
+                const int64_t jlo = t - m + 1 > 0 ? t - m + 1 : 0;
+                const int64_t jhi = t < n - 1 ? t : n - 1;
+                for (int64_t j = jlo; j <= jhi; j++) {
+                    const uint64_t x = gldu(a, t - j), y = sb[j];
+                    add3(x0, x1, x2, x * y, __umul64hi(x, y));
+                }
+            }
+            // W += (x0, x1, x2) << 64 i
+            uint64_t cc = 0;
+#pragma unroll
+            for (int w = 0; w < 6; w++) {
+                const uint64_t add = w == i ? x0 : w == i + 1 ? x1 : w == i + 2 ? x2 : 0;
+                const uint64_t s1 = W[w] + add;
+                const uint64_t k1 = s1 < add;
+                const uint64_t s2 = s1 + cc;
+                cc = k1 | (s2 < s1);
+                W[w] = s2;
+            }
+        }
+        spill[threadIdx.x + 1][0] = W[4];
+        spill[threadIdx.x + 1][1] = W[5];
+        __syncthreads();
+        const uint64_t p0 = spill[threadIdx.x][0], p1 = spill[threadIdx.x][1];
+        // own four words + the previous thread's spill: one carry out at most
+        uint64_t o[4];
+        uint64_t cc = 0;
+#pragma unroll
+        for (int w = 0; w < 4; w++) {
+            const uint64_t add = w == 0 ? p0 : w == 1 ? p1 : 0;
+            const uint64_t s1 = W[w] + add;
+            const uint64_t k1 = s1 < add;
+            const uint64_t s2 = s1 + cc;
+            cc = k1 | (s2 < s1);
+            o[w] = s2;
+        }
+        const bool prop = (o[0] & o[1] & o[2] & o[3]) == ~0ull;
+        __syncthreads();  // every thread has read its spill slot
+        if (threadIdx.x == blockDim.x - 1) {
+            spill[0][0] = W[4];
+            spill[0][1] = W[5];
+        }
+        uint32_t cout = 0;
+        const uint32_t cin = block_carry(cc != 0, prop, carry, wsum, cout);
+        uint64_t cv = cin;
+#pragma unroll
+        for (int w = 0; w < 4; w++) {
+            const int64_t k = k0 + w;
+            const uint64_t v = o[w] + cv;
+            cv = v < cv;
+            if (k < rn) J.r[k] = v;
+        }
+        carry = cout;
+    }
+}
+
+__global__ void __launch_bounds__(UNPACK_THREADS) k_basecase(const BaseJob *__restrict__ jobs) {
+    __shared__ uint64_t sb[BASE_NMAX];
+    __shared__ uint64_t spill[UNPACK_THREADS + 1][2];
+    __shared__ uint32_t wsum[UNPACK_THREADS / 64];
+    const auto *Jc = cst(jobs + blockIdx.x);
+    BaseJob J;
+    J.a = unip(Jc->a);
+    J.b = unip(Jc->b);
+    J.r = unip(Jc->r);
+    J.m = uni64(Jc->m);
+    J.n = uni64(Jc->n);
+    basecase_body(J, sb, spill, wsum);
+}
+
+// ---------------------------------------------------------------------------
 // Twiddle tables.  Row t (1..m-1) of a stage of span L holds w_L^(t j) for
 // j < Q, w_L = w_N^(N/L), minimal balanced (src/fpntt.jl:173-225).  Each entry
 // is w_N^e = hi[e >> h] * lo[e & (2^h-1)] mod p with exact host-built tables;
@@ -1627,6 +1731,12 @@ hipError_t launch_fill_u64(uint64_t *p, int64_t n, uint64_t v, hipStream_t s) {
 hipError_t launch_combine(const CombineJob *jobs, int64_t count, hipStream_t s) {
     if (count <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_combine, dim3((unsigned)count), dim3(UNPACK_THREADS), 0, s, jobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_basecase(const BaseJob *jobs, int64_t count, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_basecase, dim3((unsigned)count), dim3(UNPACK_THREADS), 0, s, jobs);
     return hipGetLastError();
 }
 

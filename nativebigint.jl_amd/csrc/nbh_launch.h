@@ -27,6 +27,8 @@ hipError_t launch_store_job(const DevJob &j, DevJob *dst, hipStream_t s);
 hipError_t launch_expand_tiles(const TileRange *ranges, int64_t nranges, Tile *tiles, hipStream_t s);
 // One workgroup per cofactor-step output (sum, or |difference| and sign).
 hipError_t launch_combine(const CombineJob *jobs, int64_t count, hipStream_t s);
+// One workgroup per product below the NTT thresholds (schoolbook).
+hipError_t launch_basecase(const BaseJob *jobs, int64_t count, hipStream_t s);
 // One workgroup per Barrett reduction: subtraction and the two corrections.
 hipError_t launch_barrett_fix(const BarrettFixJob *jobs, int64_t count, hipStream_t s);
 hipError_t launch_twiddle_stage(int prime, double *out, int m, int64_t Q, int64_t NoverL, const double *tlo,
