@@ -144,6 +144,33 @@ class HgcdStep:
             pass
 
 
+class HgcdMatrixMul:
+    """hgcd_matrix_mul! (src/gcd.jl:372-391), M <- M * M1, batched over many
+    matrix pairs: row (x, y) of M becomes (x m1_00 + y m1_10, x m1_01 + y m1_11),
+    i.e. one forward cofactor step with matrix M1 transposed and vector (x, y)
+    (hgcd_mul2!, src/gcd.jl:360-367, twice).  pairs: list of (M, M1), dicts
+    of device tensors m00..m11 (n and n1 limbs, zero-padded).  After run(),
+    result(i) is the new M of pair i, entries of n + n1 + 1 limbs."""
+
+    def __init__(self, pairs):
+        inst = []
+        for M, M1 in pairs:
+            t = {"m00": M1["m00"], "m01": M1["m10"], "m10": M1["m01"], "m11": M1["m11"]}
+            inst.append({**t, "u": M["m00"], "v": M["m01"]})
+            inst.append({**t, "u": M["m10"], "v": M["m11"]})
+        self.step = HgcdStep(inst, "forward")
+
+    def run(self, stream=None) -> None:
+        self.step.run(stream)
+
+    def result(self, i: int) -> dict:
+        (a0, a1, _, _), (b0, b1, _, _) = self.step.out[2 * i], self.step.out[2 * i + 1]
+        return {"m00": a0, "m01": a1, "m10": b0, "m11": b1}
+
+    def close(self) -> None:
+        self.step.close()
+
+
 class Spectrum:
     """Cached forward spectra of a fixed operand b (device tensor), for
     products with any a of up to bits_a_max bits: the modulus and reciprocal

@@ -67,3 +67,35 @@ def test_adjust_signed(instances):
             assert int(s[1]) * ints(D.to_host(r1)) == w1
             assert int(s[0]) == (w0 > 0) - (w0 < 0) and int(s[1]) == (w1 > 0) - (w1 < 0)
     H.close()
+
+
+def test_matrix_mul():
+    """hgcd_matrix_mul! (src/gcd.jl:372-391): M <- M * M1 for a batch of
+    matrix pairs, as two forward cofactor steps per pair."""
+    rng = np.random.default_rng(44)
+
+    def mat(k, pad=0):
+        out = {}
+        for name in ("m00", "m01", "m10", "m11"):
+            a = rng.integers(0, 2**64, k, dtype=np.uint64)
+            if pad:
+                a[-pad:] = 0
+            out[name] = a
+        return out
+    shapes = [(300, 200), (1000, 1000), (160, 2500), (4000, 90, 3)]
+    host = [(mat(n), mat(n1, *rest)) for n, n1, *rest in shapes]
+    pairs = [({k: D.to_dev(v) for k, v in M.items()}, {k: D.to_dev(v) for k, v in M1.items()}) for M, M1 in host]
+    H = D.HgcdMatrixMul(pairs)
+    H.run()
+    torch.cuda.synchronize()
+    for i, (M, M1) in enumerate(host):
+        A = {k: ints(v) for k, v in M.items()}
+        B = {k: ints(v) for k, v in M1.items()}
+        R = H.result(i)
+        assert ints(D.to_host(R["m00"])) == A["m00"] * B["m00"] + A["m01"] * B["m10"]
+        assert ints(D.to_host(R["m01"])) == A["m00"] * B["m01"] + A["m01"] * B["m11"]
+        assert ints(D.to_host(R["m10"])) == A["m10"] * B["m00"] + A["m11"] * B["m10"]
+        assert ints(D.to_host(R["m11"])) == A["m10"] * B["m01"] + A["m11"] * B["m11"]
+        n, n1 = M["m00"].size, M1["m00"].size
+        assert R["m00"].numel() == n + n1 + 1
+    H.close()
