@@ -56,3 +56,18 @@ def test_host_mirror_argument_checks():
         engine.mul_fpntt2(a, 0, a, 0, 100, a, 100, 100)       # r aliases a
     with pytest.raises(TypeError):
         engine.mul_fpntt2(np.empty(10), 0, a, 0, 5, a, 0, 5)
+
+
+def test_newer_entry_points_refuse_before_init():
+    """Every entry point that needs the device answers NBH_ERR_NOT_INIT (with
+    a message) instead of touching it when nbh_init has not succeeded."""
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    assert L.nbh_spectrum_create(None, 1, -1, 64, None, ctypes.byref(h)) == _lib.NBH_ERR_NOT_INIT
+    assert L.nbh_barrett_create(None, 1, None, 2, None, ctypes.byref(h)) == _lib.NBH_ERR_NOT_INIT
+    assert L.nbh_batch_capture(None) == _lib.NBH_ERR_NOT_INIT
+    assert L.nbh_batch_replay(None, None) == _lib.NBH_ERR_NOT_INIT
+    assert L.nbh_set_streams(0) == _lib.NBH_ERR_NOT_INIT
+    assert L.nbh_batch_set_streams(None, 0) == _lib.NBH_ERR_NOT_INIT
+    assert L.nbh_mul_spec_dev(None, None, 1, -1, None, None) == _lib.NBH_ERR_NOT_INIT
+    assert b"nbh_init" in L.nbh_last_error_string()

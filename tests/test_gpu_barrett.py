@@ -90,3 +90,16 @@ def test_powermod_batch(k, nbits, count):
     torch.cuda.synchronize()
     assert all(np.array_equal(x, D.to_host(r)) for x, r in zip(first, out))
     P.close()
+
+
+def test_barrett_arguments_are_checked():
+    import ctypes
+    from nbhip._lib import NBH_ERR_ARG, lib
+    m = D.to_dev(np.array([5, 7], dtype=np.uint64))
+    h = ctypes.c_void_p()
+    # lmu above k + 2 and a zero k are refused
+    assert lib().nbh_barrett_create(D._p(m), 2, D._p(m), 5, None, ctypes.byref(h)) == NBH_ERR_ARG
+    assert lib().nbh_barrett_create(D._p(m), 0, D._p(m), 1, None, ctypes.byref(h)) == NBH_ERR_ARG
+    # a spectrum needs a positive bound on the other operand
+    assert lib().nbh_spectrum_create(D._p(m), 2, -1, 0, None, ctypes.byref(h)) == NBH_ERR_ARG
+    assert lib().nbh_spectrum_info(None, None, None, None, None) == NBH_ERR_ARG

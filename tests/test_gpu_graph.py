@@ -66,3 +66,20 @@ def test_graph_replays_equal_eager(nside):
         assert np.array_equal(D.to_host(r), w)
     B.close()
     B2.close()
+
+
+def test_stream_settings_are_checked():
+    from nbhip._lib import NbhError, check, lib
+    with pytest.raises(NbhError):
+        D.set_streams(5)
+    with pytest.raises(NbhError):
+        D.set_streams(-1)
+    tens = _batch(4, 0)
+    B = D.Batch(tens)
+    with pytest.raises(NbhError):
+        check(lib().nbh_batch_set_streams(B._h, 9))
+    check(lib().nbh_batch_set_streams(B._h, -1))
+    B.run()
+    torch.cuda.synchronize()
+    B.close()
+    D.set_streams(4)
