@@ -444,7 +444,8 @@ int make_schedule(const StageList &L, const Tuning &T, DevPlan &P, PlanEntry &E)
         E.grp_lds_bytes[gi] = sizeof(double) * (size_t)lds_doubles(Gd.S * Gd.C);
         E.grp_threads[gi] = tile_threads((int64_t)Gd.S * Gd.C);
     }
-    E.mid_lds_bytes[0] = 2 * sizeof(double) * (size_t)lds_doubles(P.g[ng - 1].S);
+    // the mid kernel holds one array per tile (products keep b's last pass in registers)
+    E.mid_lds_bytes[0] = sizeof(double) * (size_t)lds_doubles(P.g[ng - 1].S);
     E.mid_lds_bytes[1] = sizeof(double) * (size_t)lds_doubles(P.g[ng - 1].S);
     // one thread per 16-point sub-problem of ONE array: the mid kernel runs a
     // product's two operands in the same thread

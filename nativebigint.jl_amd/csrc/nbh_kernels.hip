@@ -557,63 +557,6 @@ __device__ void pass1f(double *__restrict__ xa, double *__restrict__ xb, const S
     }
 }
 
-// The last forward pass (a radix-4 pair), the pointwise product and the same
-// pair inverted, on registers.  NA == 2: a product; with bspec, b's points
-// are already its finished spectrum and are only loaded.  NA == 1: a square.
-template <int K, bool LIN, int NA, bool BSPEC>
-__device__ void fused44(double *__restrict__ xa, const double *__restrict__ xb, const StageU &s0, const StageU &s1,
-                        const TileGeo &g) {
-    const int nsub = (g.S >> 4) << g.logC;
-    for (int sl = threadIdx.x; sl < nsub; sl += blockDim.x) {
-        const Sub44 u = sub44(sl, s0, s1, g);
-        double ya[4][4], yb[4][4];
-#pragma unroll
-        for (int a = 0; a < 4; a++)
-#pragma unroll
-            for (int b = 0; b < 4; b++) {
-                ya[a][b] = xa[ix44<LIN>(u, a, b)];
-                if (NA == 2) yb[a][b] = xb[ix44<LIN>(u, a, b)];
-            }
-        double w0[4][3], w1[3];
-        tw44<LIN>(w0, w1, u, s0, s1);
-        regs44<K, true>(ya, w0, w1, s0, s1);
-        if (NA == 2 && !BSPEC) regs44<K, true>(yb, w0, w1, s0, s1);
-#pragma unroll
-        for (int a = 0; a < 4; a++)
-#pragma unroll
-            for (int b = 0; b < 4; b++) ya[a][b] = fp_mulmod3<K>(ya[a][b], NA == 2 ? yb[a][b] : ya[a][b]);
-        regs44<K, false>(ya, w0, w1, s0, s1);
-#pragma unroll
-        for (int a = 0; a < 4; a++)
-#pragma unroll
-            for (int b = 0; b < 4; b++) xa[ix44<LIN>(u, a, b)] = ya[a][b];
-    }
-}
-
-// The same for a last forward pass of one radix-M stage.
-template <int K, int M, bool LIN, int NA, bool BSPEC>
-__device__ void fused1(double *__restrict__ xa, const double *__restrict__ xb, const StageU &s, const TileGeo &g) {
-    const int nb = (g.S / M) << g.logC;
-    for (int beta = threadIdx.x; beta < nb; beta += blockDim.x) {
-        const Sub1 u = sub1<M>(beta, s, g);
-        double w[M > 1 ? M - 1 : 1];
-        load_tw<M>(w, s, u.j);
-        double ya[M], yb[M];
-#pragma unroll
-        for (int t = 0; t < M; t++) {
-            ya[t] = xa[ix1<LIN>(u, t)];
-            if (NA == 2) yb[t] = xb[ix1<LIN>(u, t)];
-        }
-        stage_regs<K, true, M>(ya, w, s);
-        if (NA == 2 && !BSPEC) stage_regs<K, true, M>(yb, w, s);
-#pragma unroll
-        for (int t = 0; t < M; t++) ya[t] = fp_mulmod3<K>(ya[t], NA == 2 ? yb[t] : ya[t]);
-        stage_regs<K, false, M>(ya, w, s);
-#pragma unroll
-        for (int t = 0; t < M; t++) xa[ix1<LIN>(u, t)] = ya[t];
-    }
-}
-
 template <int K, int NA>
 __device__ __forceinline__ void pass1f_any(double *xa, double *xb, const StageU &s, const TileGeo &g) {
     const bool lin = ((s.q << g.logC) & 15) == 0;
@@ -626,67 +569,137 @@ __device__ __forceinline__ void pass1f_any(double *xa, double *xb, const StageU 
     }
 }
 
-template <int K, int NA, bool BSPEC>
-__device__ __forceinline__ void fused1_any(double *xa, const double *xb, const StageU &s, const TileGeo &g) {
-    const bool lin = ((s.q << g.logC) & 15) == 0;
-    switch (s.m) {
-    case 4:
-        lin ? fused1<K, 4, true, NA, BSPEC>(xa, xb, s, g) : fused1<K, 4, false, NA, BSPEC>(xa, xb, s, g);
-        break;
-    case 2:
-        lin ? fused1<K, 2, true, NA, BSPEC>(xa, xb, s, g) : fused1<K, 2, false, NA, BSPEC>(xa, xb, s, g);
-        break;
-    case 3:
-        lin ? fused1<K, 3, true, NA, BSPEC>(xa, xb, s, g) : fused1<K, 3, false, NA, BSPEC>(xa, xb, s, g);
-        break;
-    case 5:
-        lin ? fused1<K, 5, true, NA, BSPEC>(xa, xb, s, g) : fused1<K, 5, false, NA, BSPEC>(xa, xb, s, g);
-        break;
-    default:
-        lin ? fused1<K, 7, true, NA, BSPEC>(xa, xb, s, g) : fused1<K, 7, false, NA, BSPEC>(xa, xb, s, g);
-        break;
-    }
-}
+// The mid kernel keeps ONE array in LDS, so twice as many product tiles share
+// a CU as with both operands resident.  A product transforms b first and
+// keeps the result of b's last forward pass in registers (the thread's own
+// index set of that pass: one 16-point sub-problem, or 16/M butterflies);
+// then a is loaded into the same array, transformed, and its last forward
+// pass, the pointwise product against the held values and the same pass
+// inverted run on registers.  A square multiplies a by itself there; a
+// product by a cached spectrum reads b's points from HBM there.
+enum MidSrc { SRC_SELF = 0, SRC_HELD = 1, SRC_GLOB = 2 };
+// Held values per thread: 16 for a radix-4 pair, M * (16/M) for one radix-M
+// stage.  The last stage of every admissible length m*2^k is radix 4 or 2
+// (k >= 2 unless m = 1, src/fpntt.jl:236-271), so the last pass is one of
+// those.
+constexpr int HOLD = 16;
 
-// What the mid kernel multiplies: two transformed operands, a square, or an
-// operand by a cached spectrum.
-enum MidMode { MID_MUL = 0, MID_SQR = 1, MID_SPEC = 2 };
-
-// Stages [s0, s1) forward on a (and on b for MID_MUL), the pointwise product,
-// then [s0, s1) inverse on a: every forward pass but the last as in
-// run_stages (radix-4 pairs from the start), the last forward pass fused with
-// the product and its own inverse, the rest of the inverse by run_stages.
-template <int K, int MODE>
-__device__ void mid_stages(double *la, double *lb, const DevPlan *__restrict__ P, int s0, int s1, const TileGeo &g) {
-    constexpr int NF = MODE == MID_MUL ? 2 : 1;   // arrays transformed forward
-    constexpr int NL = MODE == MID_SQR ? 1 : 2;   // arrays read by the product
-    constexpr bool BS = MODE == MID_SPEC;
+// Forward passes from s0 up to (not including) the last pass of [s0, s1), on
+// one array; returns the first stage of the last pass.
+template <int K>
+__device__ int fwd_until_last(double *x, const DevPlan *__restrict__ P, int s0, int s1, const TileGeo &g) {
     const double *tw = unip(cst(P)->tw);
     int s = s0;
     for (;;) {
         const StageU a = stage_u(&P->st[s], tw);
         const bool pair = a.m == 4 && s + 1 < s1 && uni(cst(P)->st[s + 1].m) == 4;
         const int e = s + (pair ? 2 : 1);
+        if (e >= s1) return s;
         if (pair) {
             const StageU b = stage_u(&P->st[s + 1], tw);
-            const bool lin = (((a.q | b.q) << g.logC) & 15) == 0;
-            if (e >= s1) {
-                if (lin) fused44<K, true, NL, BS>(la, lb, a, b, g);
-                else fused44<K, false, NL, BS>(la, lb, a, b, g);
-            } else {
-                if (lin) pass44f<K, true, NF>(la, lb, a, b, g);
-                else pass44f<K, false, NF>(la, lb, a, b, g);
-            }
-        } else if (e >= s1) {
-            fused1_any<K, NL, BS>(la, lb, a, g);
+            if ((((a.q | b.q) << g.logC) & 15) == 0) pass44f<K, true, 1>(x, nullptr, a, b, g);
+            else pass44f<K, false, 1>(x, nullptr, a, b, g);
         } else {
-            pass1f_any<K, NF>(la, lb, a, g);
+            pass1f_any<K, 1>(x, nullptr, a, g);
         }
         __syncthreads();
-        if (e >= s1) break;
         s = e;
     }
-    run_stages<K, false>(la, nullptr, 1, P, s0, s, g);
+}
+
+// Last pass = a radix-4 pair.  HOLDB: compute x's last forward pass into hb
+// (no store).  Otherwise: the fused pass on x with b from SRC.
+template <int K, bool LIN, bool HOLDB, int SRC>
+__device__ void last44(double *__restrict__ x, double (&hb)[HOLD], const double *__restrict__ gb, int64_t gbase,
+                       const StageU &s0, const StageU &s1, const TileGeo &g) {
+    const int nsub = (g.S >> 4) << g.logC;
+    const int sl = threadIdx.x;  // one sub-problem per thread: blockDim >= S/16
+    if (sl >= nsub) return;
+    const Sub44 u = sub44(sl, s0, s1, g);
+    double y[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) y[a][b] = x[ix44<LIN>(u, a, b)];
+    double w0[4][3], w1[3];
+    tw44<LIN>(w0, w1, u, s0, s1);
+    regs44<K, true>(y, w0, w1, s0, s1);
+    if (HOLDB) {
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) hb[4 * a + b] = y[a][b];
+        return;
+    }
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const double v = SRC == SRC_SELF ? y[a][b]
+                             : SRC == SRC_HELD ? hb[4 * a + b]
+                                               : gld(gb, gbase + u.base + a * u.sa + b * u.sb);
+            y[a][b] = fp_mulmod3<K>(y[a][b], v);
+        }
+    regs44<K, false>(y, w0, w1, s0, s1);
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) x[ix44<LIN>(u, a, b)] = y[a][b];
+}
+
+// Last pass = one radix-M stage: the thread's butterflies beta = t + it*T.
+template <int K, int M, bool LIN, bool HOLDB, int SRC>
+__device__ void last1(double *__restrict__ x, double (&hb)[HOLD], const double *__restrict__ gb, int64_t gbase,
+                      const StageU &s, const TileGeo &g) {
+    static_assert(16 % M == 0, "the last pass is radix 4 or 2");
+    constexpr int MAXIT = 16 / M;  // blockDim >= S/16, so nb/blockDim <= 16/M
+    const int nb = (g.S / M) << g.logC;
+#pragma unroll
+    for (int it = 0; it < MAXIT; it++) {
+        const int beta = threadIdx.x + it * (int)blockDim.x;
+        if (beta >= nb) break;
+        const Sub1 u = sub1<M>(beta, s, g);
+        double w[M > 1 ? M - 1 : 1];
+        load_tw<M>(w, s, u.j);
+        double y[M];
+#pragma unroll
+        for (int t = 0; t < M; t++) y[t] = x[ix1<LIN>(u, t)];
+        stage_regs<K, true, M>(y, w, s);
+        if (HOLDB) {
+#pragma unroll
+            for (int t = 0; t < M; t++) hb[it * M + t] = y[t];
+            continue;
+        }
+#pragma unroll
+        for (int t = 0; t < M; t++) {
+            const double v = SRC == SRC_SELF ? y[t]
+                             : SRC == SRC_HELD ? hb[it * M + t]
+                                               : gld(gb, gbase + u.base + t * u.strideC);
+            y[t] = fp_mulmod3<K>(y[t], v);
+        }
+        stage_regs<K, false, M>(y, w, s);
+#pragma unroll
+        for (int t = 0; t < M; t++) x[ix1<LIN>(u, t)] = y[t];
+    }
+}
+
+template <int K, bool HOLDB, int SRC>
+__device__ __forceinline__ void last_pass(double *x, double (&hb)[HOLD], const double *gb, int64_t gbase,
+                                          const DevPlan *__restrict__ P, int s, int s1, const TileGeo &g) {
+    const double *tw = unip(cst(P)->tw);
+    const StageU a = stage_u(&P->st[s], tw);
+    if (s1 - s == 2) {
+        const StageU b = stage_u(&P->st[s + 1], tw);
+        if ((((a.q | b.q) << g.logC) & 15) == 0) last44<K, true, HOLDB, SRC>(x, hb, gb, gbase, a, b, g);
+        else last44<K, false, HOLDB, SRC>(x, hb, gb, gbase, a, b, g);
+        return;
+    }
+    const bool lin = ((a.q << g.logC) & 15) == 0;
+    if (a.m == 2) {
+        lin ? last1<K, 2, true, HOLDB, SRC>(x, hb, gb, gbase, a, g) : last1<K, 2, false, HOLDB, SRC>(x, hb, gb, gbase, a, g);
+    } else {
+        lin ? last1<K, 4, true, HOLDB, SRC>(x, hb, gb, gbase, a, g) : last1<K, 4, false, HOLDB, SRC>(x, hb, gb, gbase, a, g);
+    }
 }
 
 // Points each thread moves per round in the tile load loops.
@@ -807,37 +820,6 @@ __device__ __forceinline__ void load_tile(double *__restrict__ lds, const PointS
                                           const TileGeo &g, int64_t gbase) {
     if (pack) load_tile_t<NU, true>(lds, src, total, g, gbase);
     else load_tile_t<NU, false>(lds, src, total, g, gbase);
-}
-
-// Both operands of a product into la and lb, rounds interleaving NU loads of
-// each operand (PA / PB: chunks of limbs or spectra).
-template <int NU, bool PA, bool PB>
-__device__ __forceinline__ void load_pair(double *__restrict__ la, double *__restrict__ lb, const PointSrc &sa,
-                                          const PointSrc &sb, int S, const TileGeo &g, int64_t gbase) {
-    const int T = blockDim.x;
-    int r0 = 0;
-    for (; r0 + NU * T <= S; r0 += NU * T) {
-        double va[NU], vb[NU];
-        load_round<PA, NU>(va, sa, r0 + (int)threadIdx.x, S, true, g, gbase);
-        load_round<PB, NU>(vb, sb, r0 + (int)threadIdx.x, S, true, g, gbase);
-        lds_round<NU>(la, va, r0 + (int)threadIdx.x, S, true);
-        lds_round<NU>(lb, vb, r0 + (int)threadIdx.x, S, true);
-    }
-    for (; r0 + 2 * T <= S; r0 += 2 * T) {
-        double va[2], vb[2];
-        load_round<PA, 2>(va, sa, r0 + (int)threadIdx.x, S, true, g, gbase);
-        load_round<PB, 2>(vb, sb, r0 + (int)threadIdx.x, S, true, g, gbase);
-        lds_round<2>(la, va, r0 + (int)threadIdx.x, S, true);
-        lds_round<2>(lb, vb, r0 + (int)threadIdx.x, S, true);
-    }
-    for (; r0 < S; r0 += T) {
-        double va[1], vb[1];
-        const bool full = r0 + T <= S;
-        load_round<PA, 1>(va, sa, r0 + (int)threadIdx.x, S, full, g, gbase);
-        load_round<PB, 1>(vb, sb, r0 + (int)threadIdx.x, S, full, g, gbase);
-        lds_round<1>(la, va, r0 + (int)threadIdx.x, S, full);
-        lds_round<1>(lb, vb, r0 + (int)threadIdx.x, S, full);
-    }
 }
 
 // The tile a block works on: from the launch's tile list, or generated.
@@ -980,35 +962,35 @@ __device__ void mid_body(const DevJob *__restrict__ Jp, const Tile &t, double *l
     const int64_t gbase = (int64_t)t.tile * uni64(G->La);
     const bool sq = uni(J->sqr) != 0;
     const bool bspec = uni(J->bspec) != 0;
-    double *la = lds;
-    double *lb = lds + lds_doubles(S);
-    {
-        // Both operands' points (or chunks): a round issues LU/2 loads of
-        // each operand (LU of a square's) before its LDS stores.
-        const bool pack = (t.flags & TILE_PACK) != 0;
-        PointSrc sa, sb;
-        sa.x = unip(J->xa[K]);
-        sb.x = unip(J->xb[K]);
-        sa.limbs = unip(J->a);
-        sb.limbs = unip(J->b);
-        sa.n = uni64(J->m);
-        sb.n = uni64(J->n);
-        sa.nch = uni64(J->nca);
-        sb.nch = uni64(J->ncb);
-        sa.b = sb.b = uni(J->bch);
-        // with bspec, b's points are its finished spectrum: loaded, not packed
-        if (sq) load_tile<LU>(la, sa, pack, S, g, gbase);
-        else if (bspec) {
-            if (pack) load_pair<LU / 2, true, false>(la, lb, sa, sb, S, g, gbase);
-            else load_pair<LU / 2, false, false>(la, lb, sa, sb, S, g, gbase);
-        } else if (pack) load_pair<LU / 2, true, true>(la, lb, sa, sb, S, g, gbase);
-        else load_pair<LU / 2, false, false>(la, lb, sa, sb, S, g, gbase);
+    double *la = lds;  // the one LDS array
+    const bool pack = (t.flags & TILE_PACK) != 0;
+    PointSrc sa, sb;
+    sa.x = unip(J->xa[K]);
+    sb.x = unip(J->xb[K]);
+    sa.limbs = unip(J->a);
+    sb.limbs = unip(J->b);
+    sa.n = uni64(J->m);
+    sb.n = uni64(J->n);
+    sa.nch = uni64(J->nca);
+    sb.nch = uni64(J->ncb);
+    sa.b = sb.b = uni(J->bch);
+    double hb[HOLD];
+    if (!sq && !bspec) {
+        // b first: its forward passes, then its last pass held in registers
+        load_tile<LU>(la, sb, pack, S, g, gbase);
+        __syncthreads();
+        const int sl = fwd_until_last<K>(la, P, s0, s1, g);
+        last_pass<K, true, SRC_HELD>(la, hb, nullptr, 0, P, sl, s1, g);
+        __syncthreads();
     }
+    load_tile<LU>(la, sa, pack, S, g, gbase);
     __syncthreads();
-    // squares allocate one array: lb is never touched for them
-    if (sq) mid_stages<K, MID_SQR>(la, lb, P, s0, s1, g);
-    else if (bspec) mid_stages<K, MID_SPEC>(la, lb, P, s0, s1, g);
-    else mid_stages<K, MID_MUL>(la, lb, P, s0, s1, g);
+    const int sl = fwd_until_last<K>(la, P, s0, s1, g);
+    if (sq) last_pass<K, false, SRC_SELF>(la, hb, nullptr, 0, P, sl, s1, g);
+    else if (bspec) last_pass<K, false, SRC_GLOB>(la, hb, sb.x, gbase, P, sl, s1, g);
+    else last_pass<K, false, SRC_HELD>(la, hb, nullptr, 0, P, sl, s1, g);
+    __syncthreads();
+    run_stages<K, false>(la, nullptr, 1, P, s0, sl, g);
     double *__restrict__ xo = unip(J->xa[K]);
     store_tile(xo, la, S, g, gbase);
 }
