@@ -24,6 +24,10 @@ cached spectra; limb-products counts the result limbs of all three products.
 workload "serve": Poisson arrivals into the continuous-batching server.
 workload "small": 4096 products below the NTT thresholds (limbs log-uniform
 in [1, 151], 25% squares): the batch's schoolbook kernel.
+workload "shortprod": 1024 low short products k x k -> k limbs (mullo!/sqrlo!,
+src/mul.jl:305-378), k log-uniform in [224, 65536] where the reference falls
+back to a full product, 25% sqrlo, in one planned batch, against the same
+operands as full products.  limb-products counts the k kept limbs.
 
 limb-products/s = sum over completed bit-exact products of (m + n) result
 limbs / wall time, inputs already resident in HBM.  Every product of the
@@ -58,7 +62,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["batch", "large", "hgcd", "spectra", "powermod", "serve", "small"],
+    ap.add_argument("--workload", choices=["batch", "large", "hgcd", "spectra", "powermod", "serve", "small",
+                                           "shortprod"],
                     default="batch")
     ap.add_argument("--bases", type=int, default=64, help="powermod: bases sharing the exponent and modulus")
     ap.add_argument("--exp-bits", type=int, default=64, help="powermod: exponent bits")
@@ -143,6 +148,8 @@ def main():
         return bench_spectra(args, D, engine, wl, info, stream)
     if args.workload == "powermod":
         return bench_powermod(args, D, engine, wl, info, stream)
+    if args.workload == "shortprod":
+        return bench_shortprod(args, D, engine, wl, info, stream)
 
     # ---- build the job list --------------------------------------------------
     if args.workload == "batch":
@@ -574,6 +581,71 @@ def bench_spectra(args, D, engine, wl, info, stream):
     BP.close()
     S_mu.close()
     S_m.close()
+
+
+def bench_shortprod(args, D, engine, wl, info, stream):
+    """Low short products in one batch, and the same operands as full
+    products (what the reference's mullo!/sqrlo! run at these sizes)."""
+    import torch
+    count = min(args.jobs, 1024)
+    jobs = wl.batch_jobs(count=count, lo=224, hi=65536, max_ratio=1.0, workload="shortprod")
+    ops = wl.batch_operands(jobs, workload="shortprod")
+    js, jf, keep = [], [], []
+    for j, (a, b) in zip(jobs, ops):
+        k = a.size
+        da = D.to_dev(a)
+        db = None if b is None else D.to_dev(b)
+        ba = engine.magnitude_bits(a, 0, a.size)
+        bb = ba if b is None else engine.magnitude_bits(b, 0, b.size)
+        js.append((D.empty_limbs(k), da, db, ba, bb, k))
+        jf.append((D.empty_limbs(2 * k), da, db, ba, bb))
+        keep.append(k)
+    total_limbs = sum(keep)
+    torch.cuda.synchronize()
+    log(f"{count} short products, {total_limbs} kept limbs")
+    BS = D.Batch(js)
+    BF = D.Batch(jf)
+    D.set_streams(args.streams)
+    ms_f, _ = _timed_runs(BF, stream, args.steps, args.warmup, D, profile=False)
+    ms_s, _ = _timed_runs(BS, stream, args.steps, args.warmup, D, profile=False)
+    D.set_streams(0)
+    _, prof = _timed_runs(BS, stream, min(args.steps, 3), 0, D)
+    value = total_limbs / (ms_s * 1e-3)
+    log(f"timed: short {ms_s:.3f} ms/step ({value:.4g} kept limbs/s), full products of the same operands "
+        f"{ms_f:.3f} ms/step, x{ms_f / ms_s:.2f}")
+    verified = None
+    if not args.no_verify:
+        from tests import gmp_ref
+        bad = 0
+        for (rs, *_), (rf, *_), k in zip(js, jf, keep):
+            bad += int(not torch.equal(rs, rf[:k]))
+        for i in range(0, count, max(1, count // 32)):            # 32 against libgmp
+            a, b = ops[i]
+            full = gmp_ref.sqr(a) if b is None else gmp_ref.mul(a, b)
+            bad += int(not np.array_equal(D.to_host(js[i][0]), full[:keep[i]]))
+        verified = bad == 0
+        log(f"verified: {count} short products equal to the full products' low limbs, 32 to libgmp; {bad} differ")
+        if bad:
+            print(json.dumps({"error": f"{bad} products differ"}))
+            sys.exit(1)
+    out = {
+        "metric": "bit-exact limb-products/s", "value": value, "unit": "limb-products/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_s, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (SplitMix64 uniform limbs, nonzero top limb)",
+        "config": {"workload": f"shortprod: {count} low short products k x k -> k limbs (mullo!/sqrlo!), "
+                               "k log-uniform [224, 65536], 25% sqrlo", "products": count,
+                   "kept_limbs_per_step": total_limbs, "parallelism": "single GPU"},
+        "full_products_ms_per_step": ms_f,
+        "speedup_vs_full_products": ms_f / ms_s,
+        "verified_bit_exact": verified,
+        "kernels": {k: {"ms_per_step": prof[k][0] / min(args.steps, 3),
+                        "launches_per_step": prof[k][1] / min(args.steps, 3)} for k in D.PROF_KINDS},
+        "device": info,
+    }
+    print(json.dumps(out))
+    BS.close()
+    BF.close()
 
 
 def bench_powermod(args, D, engine, wl, info, stream):

@@ -60,6 +60,24 @@ int32_t nbh_mul_dev(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b
                     int64_t bits_b, void *stream);
 int32_t nbh_sqr_dev(uint64_t *r, const uint64_t *a, int64_t n, int64_t bits_a, void *stream);
 
+/* Low short products: mullo! (src/mul.jl:305-354) r[0..k) = a[0..la) *
+ * b[0..lb) mod beta^k, and sqrlo! (:360-378) r[0..k) = a[0..la)^2 mod beta^k,
+ * la, lb, k >= 1.  Exactly k limbs of r are written (the reference's two
+ * clobber limbs r[k..k+2) are left alone), zero-filled above a product
+ * shorter than k; r must not alias a or b.  Operand limbs at or above k are
+ * ignored, as the reference clamps la and lb to k.  The product of the
+ * clamped operands runs in the NTT pipeline when mul!/sqr! would send it
+ * there (see the batch routing below) and in the schoolbook kernel
+ * otherwise; either way only the coefficients and limbs below k are
+ * unpacked.  Host forms copy in and out and return when r is written; the
+ * device forms are enqueued on `stream` like nbh_mul_dev, with bits_* the
+ * magnitude_bits of the operands as passed (-1: read the top limb). */
+int32_t nbh_mullo(uint64_t *r, const uint64_t *a, int64_t la, const uint64_t *b, int64_t lb, int64_t k);
+int32_t nbh_sqrlo(uint64_t *r, const uint64_t *a, int64_t la, int64_t k);
+int32_t nbh_mullo_dev(uint64_t *r, const uint64_t *a, int64_t la, const uint64_t *b, int64_t lb, int64_t k,
+                      int64_t bits_a, int64_t bits_b, void *stream);
+int32_t nbh_sqrlo_dev(uint64_t *r, const uint64_t *a, int64_t la, int64_t k, int64_t bits_a, void *stream);
+
 /* Batched products (device pointers).  One job is r = a*b, or r = a^2 when
  * b == NULL.  There is no batched entry in the reference: each job is one
  * mul!/sqr! call, all of them run in a handful of launches that each cover
@@ -68,13 +86,17 @@ int32_t nbh_sqr_dev(uint64_t *r, const uint64_t *a, int64_t n, int64_t bits_a, v
  * sqr!: n >= 160, src/mul.jl:24-26,127-164,383-389) run in the NTT
  * pipeline; the rest (its Karatsuba / mul_basecase! range) run in a
  * schoolbook kernel, one workgroup per product, beside the first NTT launch.
- * bits_* <= 0 means "read the top limb". */
+ * bits_* <= 0 means "read the top limb".  k > 0 makes the job a low short
+ * product (mullo!/sqrlo!, as nbh_mullo_dev): r[0..k) = the product of the
+ * operands clamped to k limbs, mod beta^k; k = 0 is the full product,
+ * r[0..m+n). */
 typedef struct {
     uint64_t *r;
     const uint64_t *a;
     const uint64_t *b;
     int64_t m, n;
     int64_t bits_a, bits_b;
+    int64_t k;
 } nbh_job;
 
 typedef struct nbh_batch_s *nbh_batch;

@@ -585,6 +585,21 @@ int size_job_spec(DevJob &J, int64_t bits_a, const nbh_spectrum_s &S, int64_t *N
     return NBH_OK;
 }
 
+// A low short product keeps r[0..keep) only: the unpack stops at limb keep
+// and its carry out of the top is dropped (keep = 0: the full m + n limbs).
+void keep_limbs(DevJob &J, int64_t keep) {
+    if (keep <= 0) return;
+    J.rn = keep;
+    J.nchunks = (keep + UNPACK_LPC - 1) / UNPACK_LPC;
+}
+
+// The products mul!/sqr! keep out of the NTT engine (Karatsuba and
+// mul_basecase!, src/mul.jl:24-26,131,145,388): smaller operand below 64
+// limbs or m + n below 304; squares below 160 limbs.
+bool schoolbook_route(int64_t m, int64_t n, bool sqr) {
+    return sqr ? m < SQR_FPNTT_THRESHOLD : std::min(m, n) < MUL_FPNTT_MIN || m + n < 2 * MUL_FPNTT_THRESHOLD;
+}
+
 int read_bits(const uint64_t *a, int64_t n, int64_t *bits) {
     uint64_t top = 0;
     HIPCHK(hipMemcpyAsync(&top, a + (n - 1), sizeof top, hipMemcpyDeviceToHost, g.stream));
@@ -594,8 +609,9 @@ int read_bits(const uint64_t *a, int64_t n, int64_t *bits) {
 }
 
 // One product on g.stream: pack+forward groups, mid, inverse groups, unpack.
+// keep > 0: only r[0..keep) is written, the product mod beta^keep.
 int run_single(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int64_t n, int64_t bits_a,
-               int64_t bits_b, bool sqr, const nbh_spectrum_s *spec = nullptr) {
+               int64_t bits_b, bool sqr, const nbh_spectrum_s *spec = nullptr, int64_t keep = 0) {
     if (spec) { b = spec->b; n = spec->n; bits_b = spec->bits_b; }
     if (!r || !a || m < 1 || (!sqr && (!b || n < 1))) return fail(NBH_ERR_ARG, "null buffer or empty operand");
     if (sqr) { b = a; n = m; }
@@ -613,6 +629,7 @@ int run_single(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int
     int64_t N = 0;
     if (spec) RCCHK(size_job_spec(J, bits_a, *spec, &N));
     else RCCHK(size_job(J, bits_a, bits_b, &N));
+    keep_limbs(J, keep);
     if (J.zero) {
         HIPCHK(launch_fill_u64(r, J.rn, 0, g.stream));
         return NBH_OK;
@@ -674,6 +691,30 @@ int run_single(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int
     return NBH_OK;
 }
 
+// Low short product on g.stream (mullo!/sqrlo!, src/mul.jl:305-378):
+// r[0..k) = (a mod beta^k) * (b mod beta^k) mod beta^k.  The clamped product
+// takes the engine mul!/sqr! would give it (every engine is exact, so the low
+// limbs are the reference's whichever it takes), with only its low k limbs
+// unpacked or summed.
+int run_lo(uint64_t *r, const uint64_t *a, int64_t la, const uint64_t *b, int64_t lb, int64_t k, int64_t bits_a,
+           int64_t bits_b, bool sqr) {
+    if (!r || !a || la < 1 || k < 1 || (!sqr && (!b || lb < 1)))
+        return fail(NBH_ERR_ARG, "null buffer, empty operand or k < 1");
+    if (sqr) { b = a; lb = la; bits_b = bits_a; }
+    la = std::min(la, k);
+    lb = std::min(lb, k);
+    // a magnitude bound of the whole operand bounds its low limbs too
+    if (bits_a >= 0) bits_a = std::min(bits_a, 64 * la);
+    if (bits_b >= 0) bits_b = std::min(bits_b, 64 * lb);
+    if (schoolbook_route(la, lb, sqr)) {
+        const bool swap = lb > la;
+        const BaseJob j{swap ? b : a, swap ? a : b, r, std::max(la, lb), std::min(la, lb), k};
+        HIPCHK(launch_basecase_one(j, g.stream));
+        return NBH_OK;
+    }
+    return run_single(r, a, la, sqr ? nullptr : b, lb, bits_a, bits_b, sqr, nullptr, k);
+}
+
 // Make the library stream wait for `user`'s prior work; after the call, make
 // `user` wait for ours.  The single-product workspace is shared, so every
 // single-product call is serialised on g.stream.
@@ -690,21 +731,30 @@ int stream_leave(void *user) {
     return NBH_OK;
 }
 
-int host_product(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int64_t n, bool sqr) {
-    if (!r || !a || m < 1 || (!sqr && (!b || n < 1))) return fail(NBH_ERR_ARG, "null buffer or empty operand");
+// Host buffers in and out; keep > 0 is the low short product r[0..keep).
+int host_product(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int64_t n, bool sqr,
+                 int64_t keep = 0) {
+    if (!r || !a || m < 1 || (!sqr && (!b || n < 1)) || keep < 0)
+        return fail(NBH_ERR_ARG, "null buffer, empty operand or negative k");
     if (sqr) n = m;
+    if (keep) {
+        m = std::min(m, keep);
+        n = std::min(n, keep);
+    }
     const int64_t bits_a = bits_from_top(m, a[m - 1]);
     const int64_t bits_b = sqr ? bits_a : bits_from_top(n, b[n - 1]);
+    const int64_t rl = keep ? keep : m + n;
     const size_t sa = align256(8 * (size_t)m), sb = sqr ? 0 : align256(8 * (size_t)n);
-    const size_t sr = align256(8 * (size_t)(m + n));
+    const size_t sr = align256(8 * (size_t)rl);
     RCCHK(ensure_buf(&g.io, &g.io_size, sa + sb + sr));
     uint64_t *da = (uint64_t *)g.io;
     uint64_t *db = (uint64_t *)((char *)g.io + sa);
     uint64_t *dr = (uint64_t *)((char *)g.io + sa + sb);
     HIPCHK(hipMemcpyAsync(da, a, 8 * (size_t)m, hipMemcpyHostToDevice, g.stream));
     if (!sqr) HIPCHK(hipMemcpyAsync(db, b, 8 * (size_t)n, hipMemcpyHostToDevice, g.stream));
-    RCCHK(run_single(dr, da, m, sqr ? nullptr : db, n, bits_a, bits_b, sqr));
-    HIPCHK(hipMemcpyAsync(r, dr, 8 * (size_t)(m + n), hipMemcpyDeviceToHost, g.stream));
+    if (keep) RCCHK(run_lo(dr, da, m, sqr ? nullptr : db, n, keep, bits_a, bits_b, sqr));
+    else RCCHK(run_single(dr, da, m, sqr ? nullptr : db, n, bits_a, bits_b, sqr));
+    HIPCHK(hipMemcpyAsync(r, dr, 8 * (size_t)rl, hipMemcpyDeviceToHost, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
     return NBH_OK;
 }
@@ -768,6 +818,7 @@ struct JobIn {
     const uint64_t *a, *b;
     int64_t m, n, bits_a, bits_b;
     const nbh_spectrum_s *spec;
+    int64_t keep;  // > 0: low short product r[0..keep) (a, and b unless cached, clamped to keep limbs)
 };
 
 int batch_plan(const JobIn *jobs, int64_t count, nbh_batch_s **out, size_t *need, void *arena,
@@ -784,29 +835,40 @@ int batch_plan(const JobIn *jobs, int64_t count, nbh_batch_s **out, size_t *need
         DevJob &d = J[(size_t)i];
         std::memset(&d, 0, sizeof d);
         const bool sqr = q.b == nullptr && !q.spec;
-        if (!q.r || !q.a || q.m < 1 || (!sqr && q.n < 1)) return fail(NBH_ERR_ARG, "bad job " + std::to_string(i));
+        if (!q.r || !q.a || q.m < 1 || (!sqr && q.n < 1) || q.keep < 0)
+            return fail(NBH_ERR_ARG, "bad job " + std::to_string(i));
         d.a = q.a;
         d.b = sqr ? q.a : q.b;
         d.r = q.r;
         d.m = q.m;
         d.n = sqr ? q.m : q.n;
         d.sqr = sqr;
-        bits[(size_t)(2 * i)] = q.bits_a;
-        bits[(size_t)(2 * i + 1)] = sqr ? q.bits_a : q.bits_b;
-        if (q.bits_a <= 0 || (!sqr && !q.spec && q.bits_b <= 0)) need_top = true;
-        // below the reference's NTT thresholds: schoolbook (src/mul.jl:131,145,388)
-        const int64_t lo = std::min(d.m, d.n), hi = std::max(d.m, d.n);
-        if (!q.spec && (sqr ? d.m < SQR_FPNTT_THRESHOLD : lo < MUL_FPNTT_MIN || d.m + d.n < 2 * MUL_FPNTT_THRESHOLD)) {
+        int64_t ba = q.bits_a, bb = sqr ? q.bits_a : q.bits_b;
+        if (q.keep > 0) {
+            // limbs at or above keep cannot reach r (mullo!, src/mul.jl:308-309);
+            // a cached spectrum's operand stays whole
+            d.m = std::min(d.m, q.keep);
+            if (!q.spec) d.n = std::min(d.n, q.keep);
+            if (ba > 0) ba = std::min(ba, 64 * d.m);
+            if (bb > 0 && !q.spec) bb = std::min(bb, 64 * d.n);
+        }
+        bits[(size_t)(2 * i)] = ba;
+        bits[(size_t)(2 * i + 1)] = bb;
+        if (ba <= 0 || (!sqr && !q.spec && bb <= 0)) need_top = true;
+        // below the reference's NTT thresholds: schoolbook
+        if (!q.spec && schoolbook_route(d.m, d.n, sqr)) {
             isbase[(size_t)i] = 1;
             const bool swap = d.n > d.m;
-            B->base.push_back(BaseJob{swap ? d.b : d.a, swap ? d.a : d.b, d.r, hi, lo});
+            B->base.push_back(BaseJob{swap ? d.b : d.a, swap ? d.a : d.b, d.r, std::max(d.m, d.n),
+                                      std::min(d.m, d.n), q.keep > 0 ? q.keep : d.m + d.n});
         }
     }
     // schoolbook products need no sizing, plans or workspace
     if (need_top) {
         need_top = false;
         for (int64_t i = 0; i < count; i++)
-            if (!isbase[(size_t)i] && (jobs[i].bits_a <= 0 || (jobs[i].b && !jobs[i].spec && jobs[i].bits_b <= 0)))
+            if (!isbase[(size_t)i] && (bits[(size_t)(2 * i)] <= 0 ||
+                                       (jobs[i].b && !jobs[i].spec && bits[(size_t)(2 * i + 1)] <= 0)))
                 need_top = true;
     }
     if (need_top && (need || arena)) return fail(NBH_ERR_ARG, "arena batches need the operands' bit lengths");
@@ -842,6 +904,7 @@ int batch_plan(const JobIn *jobs, int64_t count, nbh_batch_s **out, size_t *need
         }
         if (jobs[i].spec) RCCHK(size_job_spec(d, bits[(size_t)(2 * i)], *jobs[i].spec, &Ns[(size_t)i]));
         else RCCHK(size_job(d, bits[(size_t)(2 * i)], bits[(size_t)(2 * i + 1)], &Ns[(size_t)i]));
+        keep_limbs(d, jobs[i].keep);
         offs[(size_t)i] = ws;
         ws += align256(sizeof(uint32_t) * (size_t)d.nchunks);
         if (d.zero) continue;
@@ -1024,7 +1087,7 @@ int batch_create(const nbh_job *jobs, int64_t count, nbh_batch_s **out, size_t *
     std::vector<JobIn> in((size_t)count);
     for (int64_t i = 0; i < count; i++) {
         const nbh_job &q = jobs[i];
-        in[(size_t)i] = JobIn{q.r, q.a, q.b, q.m, q.n, q.bits_a, q.bits_b, nullptr};
+        in[(size_t)i] = JobIn{q.r, q.a, q.b, q.m, q.n, q.bits_a, q.bits_b, nullptr, q.k};
     }
     return batch_plan(in.data(), count, out, need, arena, arena_bytes, astream);
 }
@@ -1245,6 +1308,37 @@ int32_t nbh_sqr_dev(uint64_t *r, const uint64_t *a, int64_t n, int64_t bits_a, v
     RCCHK(ensure_init());
     RCCHK(stream_enter(stream));
     RCCHK(run_single(r, a, n, nullptr, n, bits_a, bits_a, true));
+    return stream_leave(stream);
+}
+
+int32_t nbh_mullo(uint64_t *r, const uint64_t *a, int64_t la, const uint64_t *b, int64_t lb, int64_t k) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    if (k < 1) return fail(NBH_ERR_ARG, "k < 1");
+    return host_product(r, a, la, b, lb, false, k);
+}
+
+int32_t nbh_sqrlo(uint64_t *r, const uint64_t *a, int64_t la, int64_t k) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    if (k < 1) return fail(NBH_ERR_ARG, "k < 1");
+    return host_product(r, a, la, nullptr, la, true, k);
+}
+
+int32_t nbh_mullo_dev(uint64_t *r, const uint64_t *a, int64_t la, const uint64_t *b, int64_t lb, int64_t k,
+                      int64_t bits_a, int64_t bits_b, void *stream) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    RCCHK(stream_enter(stream));
+    RCCHK(run_lo(r, a, la, b, lb, k, bits_a, bits_b, false));
+    return stream_leave(stream);
+}
+
+int32_t nbh_sqrlo_dev(uint64_t *r, const uint64_t *a, int64_t la, int64_t k, int64_t bits_a, void *stream) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    RCCHK(stream_enter(stream));
+    RCCHK(run_lo(r, a, la, nullptr, la, k, bits_a, bits_a, true));
     return stream_leave(stream);
 }
 
@@ -1525,7 +1619,8 @@ int32_t nbh_barrett_batch_create(nbh_barrett b, const nbh_barrett_job *jobs, int
         // operands are taken at their full lengths: the values change from run
         // to run, and a length is an upper bound of the magnitude
         j1[(size_t)i] = JobIn{q2, jobs[i].t + (k - 1), b->smu->b, k + 1, lmu, 64 * (k + 1), b->smu->bits_b, b->smu};
-        j2[(size_t)i] = JobIn{p, q2 + (k + 1), b->m, lmu, k, 64 * lmu, b->sm->bits_b, b->sm};
+        // r = T - q3*m is taken mod beta^(k+1): q3*m's low k+1 limbs suffice
+        j2[(size_t)i] = JobIn{p, q2 + (k + 1), b->m, lmu, k, 64 * lmu, b->sm->bits_b, b->sm, k + 1};
         fx[(size_t)i] = BarrettFixJob{jobs[i].t, p, p, jobs[i].r, b->m, k};
     }
     RCCHK(batch_plan(j1.data(), count, &R->p1, nullptr, nullptr, 0, nullptr));

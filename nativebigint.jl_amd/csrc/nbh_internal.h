@@ -155,7 +155,8 @@ constexpr int64_t SQR_FPNTT_THRESHOLD = 160; // src/mul.jl:26
 struct BaseJob {
     const uint64_t *a, *b;
     uint64_t *r;
-    int64_t m, n;  // limbs of a (the longer) and b; r has m + n
+    int64_t m, n;  // limbs of a (the longer) and b
+    int64_t rn;    // limbs of r written: the product mod beta^rn (m + n for the full product)
 };
 }  // namespace nbh
 

@@ -1480,7 +1480,7 @@ __device__ __forceinline__ void add3(uint64_t &c0, uint64_t &c1, uint64_t &c2, u
 
 __device__ void basecase_body(const BaseJob &J, uint64_t *sb, uint64_t (*spill)[2], uint32_t *wsum) {
     const uint64_t *__restrict__ a = J.a;
-    const int64_t m = J.m, n = J.n, rn = m + n;
+    const int64_t m = J.m, n = J.n, rn = J.rn, top = m + n - 1;  // columns >= top hold no products
     for (int j = threadIdx.x; j < n; j += blockDim.x) sb[j] = J.b[j];
     if (threadIdx.x == 0) spill[0][0] = spill[0][1] = 0;
     __syncthreads();
@@ -1492,7 +1492,7 @@ __device__ void basecase_body(const BaseJob &J, uint64_t *sb, uint64_t (*spill)[
         for (int i = 0; i < UNPACK_LPT; i++) {
             const int64_t t = k0 + i;
             uint64_t x0 = 0, x1 = 0, x2 = 0;
-            if (t < rn - 1) {
+            if (t < top) {
                 const int64_t jlo = t - m + 1 > 0 ? t - m + 1 : 0;
                 const int64_t jhi = t < n - 1 ? t : n - 1;
                 for (int64_t j = jlo; j <= jhi; j++) {
@@ -1559,6 +1559,15 @@ __global__ void __launch_bounds__(UNPACK_THREADS) k_basecase(const BaseJob *__re
     J.r = unip(Jc->r);
     J.m = uni64(Jc->m);
     J.n = uni64(Jc->n);
+    J.rn = uni64(Jc->rn);
+    basecase_body(J, sb, spill, wsum);
+}
+
+// One schoolbook product passed by value (single calls need no job table).
+__global__ void __launch_bounds__(UNPACK_THREADS) k_basecase_one(BaseJob J) {
+    __shared__ uint64_t sb[BASE_NMAX];
+    __shared__ uint64_t spill[UNPACK_THREADS + 1][2];
+    __shared__ uint32_t wsum[UNPACK_THREADS / 64];
     basecase_body(J, sb, spill, wsum);
 }
 
@@ -1719,6 +1728,11 @@ hipError_t launch_combine(const CombineJob *jobs, int64_t count, hipStream_t s) 
 hipError_t launch_basecase(const BaseJob *jobs, int64_t count, hipStream_t s) {
     if (count <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_basecase, dim3((unsigned)count), dim3(UNPACK_THREADS), 0, s, jobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_basecase_one(const BaseJob &j, hipStream_t s) {
+    hipLaunchKernelGGL(k_basecase_one, dim3(1), dim3(UNPACK_THREADS), 0, s, j);
     return hipGetLastError();
 }
 

@@ -29,6 +29,7 @@ hipError_t launch_expand_tiles(const TileRange *ranges, int64_t nranges, Tile *t
 hipError_t launch_combine(const CombineJob *jobs, int64_t count, hipStream_t s);
 // One workgroup per product below the NTT thresholds (schoolbook).
 hipError_t launch_basecase(const BaseJob *jobs, int64_t count, hipStream_t s);
+hipError_t launch_basecase_one(const BaseJob &j, hipStream_t s);
 // One workgroup per Barrett reduction: subtraction and the two corrections.
 hipError_t launch_barrett_fix(const BarrettFixJob *jobs, int64_t count, hipStream_t s);
 hipError_t launch_twiddle_stage(int prime, double *out, int m, int64_t Q, int64_t NoverL, const double *tlo,

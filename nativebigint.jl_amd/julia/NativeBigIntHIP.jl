@@ -42,6 +42,32 @@ function sqr_fpntt2!(r::Memory{UInt64}, ro::Int, a::Memory{UInt64}, ao::Int, n::
     return nothing
 end
 
+# Low short products, mullo!/sqrlo! (src/mul.jl:305-378) with the same
+# signatures (the scratch arguments are accepted and unused): r[1..k] of
+# a*b mod beta^k.  Writes exactly k limbs; the reference's two clobber limbs
+# are left alone.  For k at or above MULLO_FULL_THRESHOLD the reference runs
+# a full mul! and copies k limbs; here only the low k limbs are unpacked and
+# copied back.
+function mullo!(r::Memory{UInt64}, ro::Int, a::Memory{UInt64}, ao::Int, la::Int,
+                b::Memory{UInt64}, bo::Int, lb::Int, k::Int,
+                scratch::Union{Memory{UInt64},Nothing}=nothing, so::Int=0)
+    GC.@preserve r a b begin
+        check(ccall((:nbh_mullo, libnbh), Int32,
+                    (Ptr{UInt64}, Ptr{UInt64}, Int64, Ptr{UInt64}, Int64, Int64),
+                    pointer(r, ro + 1), pointer(a, ao + 1), la, pointer(b, bo + 1), lb, k))
+    end
+    return nothing
+end
+
+function sqrlo!(r::Memory{UInt64}, ro::Int, a::Memory{UInt64}, ao::Int, la::Int, k::Int,
+                scratch::Union{Memory{UInt64},Nothing}=nothing, so::Int=0)
+    GC.@preserve r a begin
+        check(ccall((:nbh_sqrlo, libnbh), Int32, (Ptr{UInt64}, Ptr{UInt64}, Int64, Int64),
+                    pointer(r, ro + 1), pointer(a, ao + 1), la, k))
+    end
+    return nothing
+end
+
 function ntt_len(T::Int)
     N = Ref{Int64}(0)
     check(ccall((:nbh_ntt_len, libnbh), Int32, (Int64, Ptr{Int64}), T, N))

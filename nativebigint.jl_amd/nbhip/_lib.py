@@ -37,7 +37,7 @@ class ArgumentError(NbhError, ValueError):
 class nbh_job(ctypes.Structure):
     _fields_ = [("r", ctypes.c_void_p), ("a", ctypes.c_void_p), ("b", ctypes.c_void_p),
                 ("m", ctypes.c_int64), ("n", ctypes.c_int64),
-                ("bits_a", ctypes.c_int64), ("bits_b", ctypes.c_int64)]
+                ("bits_a", ctypes.c_int64), ("bits_b", ctypes.c_int64), ("k", ctypes.c_int64)]
 
 
 class nbh_hgcd_job(ctypes.Structure):
@@ -73,6 +73,10 @@ SIGNATURES = {
     "nbh_sqr": (_i32, [_vp, _vp, _i64]),
     "nbh_mul_dev": (_i32, [_vp, _vp, _i64, _vp, _i64, _i64, _i64, _vp]),
     "nbh_sqr_dev": (_i32, [_vp, _vp, _i64, _i64, _vp]),
+    "nbh_mullo": (_i32, [_vp, _vp, _i64, _vp, _i64, _i64]),
+    "nbh_sqrlo": (_i32, [_vp, _vp, _i64, _i64]),
+    "nbh_mullo_dev": (_i32, [_vp, _vp, _i64, _vp, _i64, _i64, _i64, _i64, _vp]),
+    "nbh_sqrlo_dev": (_i32, [_vp, _vp, _i64, _i64, _i64, _vp]),
     "nbh_batch_create": (_i32, [_P(nbh_job), _i64, _P(_vp)]),
     "nbh_batch_run": (_i32, [_vp, _vp]),
     "nbh_batch_destroy": (None, [_vp]),

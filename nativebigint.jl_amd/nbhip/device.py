@@ -47,12 +47,29 @@ def sqr_dev(r: torch.Tensor, a: torch.Tensor, bits_a: int = -1) -> None:
     check(lib().nbh_sqr_dev(_p(r), _p(a), a.numel(), bits_a, _stream()))
 
 
+def mullo_dev(r: torch.Tensor, a: torch.Tensor, b: torch.Tensor, k: int, bits_a: int = -1,
+              bits_b: int = -1) -> None:
+    """r[0:k] = a*b mod beta^k (mullo!, src/mul.jl:305-354) on the device,
+    enqueued on torch's current stream; r has at least k limbs."""
+    init()
+    assert r.numel() >= k
+    check(lib().nbh_mullo_dev(_p(r), _p(a), a.numel(), _p(b), b.numel(), k, bits_a, bits_b, _stream()))
+
+
+def sqrlo_dev(r: torch.Tensor, a: torch.Tensor, k: int, bits_a: int = -1) -> None:
+    """r[0:k] = a^2 mod beta^k (sqrlo!, src/mul.jl:360-378)."""
+    init()
+    assert r.numel() >= k
+    check(lib().nbh_sqrlo_dev(_p(r), _p(a), a.numel(), k, bits_a, _stream()))
+
+
 class Batch:
     """A planned batch of independent products (device buffers).
 
     jobs: list of (r, a, b) tensors, b None for a square, optionally with
-    (bits_a, bits_b) appended.  run() enqueues every product on the current
-    stream in a fixed number of launches."""
+    (bits_a, bits_b) appended, and then k for a low short product r[0:k]
+    (mullo!/sqrlo!).  run() enqueues every product on the current stream in a
+    fixed number of launches."""
 
     def __init__(self, jobs):
         init()
@@ -61,6 +78,7 @@ class Batch:
         for i, j in enumerate(jobs):
             r, a, b = j[0], j[1], j[2]
             bits_a, bits_b = (j[3], j[4]) if len(j) > 3 else (-1, -1)
+            arr[i].k = j[5] if len(j) > 5 else 0
             arr[i].r = r.data_ptr()
             arr[i].a = a.data_ptr()
             arr[i].b = b.data_ptr() if b is not None else None

@@ -23,7 +23,7 @@ SQR_FPNTT_THRESHOLD = 160
 
 __all__ = ["MUL_FPNTT_MIN", "MUL_FPNTT_THRESHOLD", "SQR_FPNTT_THRESHOLD", "ArgumentError", "NbhError",
            "magnitude_bits", "ntt_len", "fp_ntt_params", "mul_fpntt2", "sqr_fpntt2", "mul", "sqr", "mul_bal",
-           "init"]
+           "mullo", "sqrlo", "init"]
 
 
 def _u64(buf) -> np.ndarray:
@@ -103,3 +103,29 @@ def sqr(r, ro: int, a, ao: int, n: int) -> None:
     if n < SQR_FPNTT_THRESHOLD:
         raise NotImplementedError("below the NTT threshold the reference uses Karatsuba/basecase")
     sqr_fpntt2(r, ro, a, ao, n)
+
+
+def mullo(r, ro: int, a, ao: int, la: int, b, bo: int, lb: int, k: int) -> None:
+    """mullo! (src/mul.jl:305-354): r[ro:ro+k] = a*b mod beta^k, la, lb,
+    k >= 1, r not aliasing a or b.  Operand limbs at or above k are ignored,
+    as the reference clamps la and lb to k.  Exactly k limbs are written: the
+    reference's two clobber limbs above k are not needed.  Every size runs on
+    the GPU (schoolbook below the mul! NTT thresholds)."""
+    r, a, b = _u64(r), _u64(a), _u64(b)
+    if k < 1:
+        raise ValueError("k >= 1")
+    la, lb = min(la, k), min(lb, k)
+    _check_ranges(r, ro, k, a, ao, la, b, bo, lb)
+    init()
+    check(lib().nbh_mullo(_ptr(r, ro), _ptr(a, ao), la, _ptr(b, bo), lb, k))
+
+
+def sqrlo(r, ro: int, a, ao: int, la: int, k: int) -> None:
+    """sqrlo! (src/mul.jl:360-378): r[ro:ro+k] = a^2 mod beta^k."""
+    r, a = _u64(r), _u64(a)
+    if k < 1:
+        raise ValueError("k >= 1")
+    la = min(la, k)
+    _check_ranges(r, ro, k, a, ao, la)
+    init()
+    check(lib().nbh_sqrlo(_ptr(r, ro), _ptr(a, ao), la, k))

@@ -53,7 +53,7 @@ class Request:
         self.n = self.m if self.b is None else self.b.numel()
         self.limbs = self.m + self.n
         # the nbh_job record (r filled in at admission) and the sizing key
-        self.row = np.array([0, self.dev_a, self.dev_b, self.m, self.n, self.bits_a, self.bits_b], dtype=np.int64)
+        self.row = np.array([0, self.dev_a, self.dev_b, self.m, self.n, self.bits_a, self.bits_b, 0], dtype=np.int64)
         self.key = (self.m, None if self.b is None else self.n, self.bits_a, self.bits_b)
 
 
@@ -135,7 +135,7 @@ class Server:
 
     # -- the loop -----------------------------------------------------------------
     def _jobs(self, reqs, results_off):
-        """nbh_job records (r, a, b, m, n, bits_a, bits_b) as one int64 array,
+        """nbh_job records (r, a, b, m, n, bits_a, bits_b, k=0) as one int64 array,
         the results laid out back to back (ALIGN-aligned) from results_off;
         returns the records, the results' bytes and their offsets."""
         rec = np.stack([r.row for r in reqs])

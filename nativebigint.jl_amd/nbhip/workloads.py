@@ -31,7 +31,8 @@ _M1 = 0xBF58476D1CE4E5B9
 _M2 = 0x94D049BB133111EB
 MASK64 = (1 << 64) - 1
 
-WORKLOAD_IDS = {"single": 1, "large": 2, "batch": 3, "hgcd": 4, "serve": 5, "spectra": 6, "powermod": 7, "small": 8, "test": 9}
+WORKLOAD_IDS = {"single": 1, "large": 2, "batch": 3, "hgcd": 4, "serve": 5, "spectra": 6, "powermod": 7, "small": 8,
+                "test": 9, "shortprod": 10}
 
 
 def _mix(z: int) -> int:

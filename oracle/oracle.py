@@ -265,3 +265,89 @@ def sqr(a) -> np.ndarray:
     if lib().or_sqr(_ptr(r), _ptr(a), a.size):
         raise ValueError("unsupported operand size")
     return r
+
+
+# -- low short products (src/mul.jl:166-378) ----------------------------------
+#
+# mullo!/sqrlo! restated over Python integers: the same clamping, the same
+# full-product cases, the same Mulders split ((11k) >> 4) with the two cross
+# terms taken mod beta^(k-m) and the carry out of the top discarded.  The
+# sub-products (Karatsuba / basecase / NTT in the reference) are exact integer
+# products here; the truncated basecases are the product mod beta^k, which is
+# what they compute.
+
+MULLO_BASECASE_THRESHOLD = 140   # src/mul.jl:181-184
+SQRLO_BASECASE_THRESHOLD = 224
+MULLO_FULL_THRESHOLD = 224
+SQRLO_FULL_THRESHOLD = 448
+MUL_KARATSUBA_THRESHOLD = 29     # src/mul.jl:12
+
+
+def _limbs_int(a, n: int) -> int:
+    v = 0
+    for i in range(n - 1, -1, -1):
+        v = (v << 64) | int(a[i])
+    return v
+
+
+def _int_limbs(v: int, n: int) -> np.ndarray:
+    out = np.zeros(n, dtype=np.uint64)
+    for i in range(n):
+        out[i] = v & 0xFFFFFFFFFFFFFFFF
+        v >>= 64
+    return out
+
+
+def mullo_split(k: int) -> int:
+    """src/mul.jl:185."""
+    return (11 * k) >> 4
+
+
+def _mullo_int(A, la: int, B, lb: int, k: int) -> int:
+    """mullo! (src/mul.jl:305-354) on limb arrays A, B; returns the low k
+    limbs as an integer."""
+    la, lb = min(la, k), min(lb, k)
+    if la < lb:
+        A, la, B, lb = B, lb, A, la
+    beta_k = 1 << (64 * k)
+    if la + lb <= k + 2:                               # full product in place
+        return (_limbs_int(A, la) * _limbs_int(B, lb)) % beta_k
+    if k < MULLO_BASECASE_THRESHOLD or lb < MUL_KARATSUBA_THRESHOLD or k >= MULLO_FULL_THRESHOLD:
+        # truncated basecase, or full product with the low k limbs copied
+        return (_limbs_int(A, la) * _limbs_int(B, lb)) % beta_k
+    m = mullo_split(k)
+    kk = k - m
+    lla, llb = min(la, m), min(lb, m)
+    r = (_limbs_int(A, lla) * _limbs_int(B, llb)) % beta_k
+    if la > m:   # a_hi x b_lo, columns m..k
+        c = _mullo_int(A[m:], la - m, B, min(lb, kk), kk)
+        r = (r + (c << (64 * m))) % beta_k
+    if lb > m:
+        c = _mullo_int(B[m:], lb - m, A, min(la, kk), kk)
+        r = (r + (c << (64 * m))) % beta_k
+    return r
+
+
+def mullo(a, b, k: int) -> np.ndarray:
+    """mullo! (src/mul.jl:305-354): the k limbs of a*b mod beta^k."""
+    a, b = _u64a(a), _u64a(b)
+    return _int_limbs(_mullo_int(a, a.size, b, b.size, k), k)
+
+
+def sqrlo(a, k: int) -> np.ndarray:
+    """sqrlo! (src/mul.jl:360-378): the k limbs of a^2 mod beta^k."""
+    a = _u64a(a)
+    la = min(a.size, k)
+    beta_k = 1 << (64 * k)
+    A = _limbs_int(a, la)
+    if 2 * la <= k + 2 or k < SQRLO_BASECASE_THRESHOLD or k >= SQRLO_FULL_THRESHOLD:
+        return _int_limbs((A * A) % beta_k, k)
+    m = mullo_split(k)
+    kk = k - m
+    lla = min(la, m)
+    lo = _limbs_int(a, lla)
+    r = (lo * lo) % beta_k
+    if la > m:   # the cross term, added in twice; a_hi^2 starts at column 2m >= k
+        c = _mullo_int(a[m:], la - m, a, min(la, kk), kk)
+        r = (r + (c << (64 * m)) + (c << (64 * m))) % beta_k
+    return _int_limbs(r, k)
