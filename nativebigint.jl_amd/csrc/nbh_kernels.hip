@@ -1046,6 +1046,12 @@ __device__ __forceinline__ UnpackU unpack_u(const DevJob *__restrict__ Jp) {
     return u;
 }
 
+// An integer-valued double in [0, 2^52) as an integer: v + 2^52 is exact and
+// its mantissa field is v.
+__device__ __forceinline__ uint64_t exact_u64(double v) {
+    return (uint64_t)__double_as_longlong(v + 4503599627370496.0) - 0x4330000000000000ull;
+}
+
 // Garner pair of one coefficient from its two residues (src/fpntt.jl:806-816),
 // returned as c = c1 + p1*u < 2^100 in two words.
 __device__ __forceinline__ void garner(const UnpackU &U, double x1, double x2, uint64_t &lo, uint64_t &hi) {
@@ -1054,11 +1060,9 @@ __device__ __forceinline__ void garner(const UnpackU &U, double x1, double x2, u
     const double v2 = fp_mulmod4<1>(x2, U.n2, U.n2p);
     double u = fp_mulmod4<1>(v2 - fp_reduce<1>(v1), GARNER_G, GARNER_GP);
     u = u < 0 ? u + P2_D : u;
-    const uint64_t c1 = (uint64_t)v1, uu = (uint64_t)u;
-    const uint64_t plo = uu * P1_U;
-    const uint64_t phi = __umul64hi(uu, P1_U);
-    lo = plo + c1;
-    hi = phi + (lo < c1 ? 1 : 0);
+    const unsigned __int128 c = (unsigned __int128)exact_u64(u) * P1_U + exact_u64(v1);
+    lo = (uint64_t)c;
+    hi = (uint64_t)(c >> 64);
 }
 
 struct Seg {
@@ -1073,19 +1077,22 @@ __device__ __forceinline__ Seg segment_sum(const UnpackU &U, int64_t k0, int64_t
     const int b = U.b;
     const int64_t ilo = (int64_t)__umul64hi((uint64_t)(64 * k0 + b - 1), U.bmagic);
     const int64_t iend = (int64_t)__umul64hi((uint64_t)(64 * (k0 + UNPACK_LPT) + b - 1), U.bmagic);
-    int s = (int)(ilo * b - 64 * k0);
+    int s = (int)(ilo * b - 64 * k0);  // in [0, 64) at every add: b < 64, one shift per step
+    const int n = (int)(iend - ilo);
+    const uint64_t *__restrict__ plo = clo + (ilo - i_first);
+    const uint64_t *__restrict__ phi = chi + (ilo - i_first);
     uint64_t w0 = 0, w1 = 0, w2 = 0;
     Seg out{0, 0, 0, 0, 0, 0};
-    for (int64_t i = ilo; i < iend; i++) {
-        const int li = (int)(i - i_first);
-        const uint64_t lo = clo[li], hi = chi[li];
-        const unsigned __int128 a01 = (unsigned __int128)lo << s;
-        const unsigned __int128 a12 = (unsigned __int128)hi << s;
-        unsigned __int128 acc = (unsigned __int128)w0 + (uint64_t)a01;
-        w0 = (uint64_t)acc;
-        acc = (unsigned __int128)w1 + (uint64_t)(a01 >> 64) + (uint64_t)a12 + (uint64_t)(acc >> 64);
-        w1 = (uint64_t)acc;
-        w2 += (uint64_t)(a12 >> 64) + (uint64_t)(acc >> 64);
+    for (int t = 0; t < n; t++) {
+        const uint64_t lo = plo[t], hi = phi[t];
+        // (hi:lo) << s in three words; (x >> 1) >> (63 - s) is x >> (64 - s), and 0 for s = 0
+        const uint64_t a0 = lo << s;
+        const uint64_t a1 = ((lo >> 1) >> (63 - s)) | (hi << s);
+        const uint64_t a2 = (hi >> 1) >> (63 - s);
+        unsigned long long c0, c1;
+        w0 = __builtin_addcll(w0, a0, 0ull, &c0);
+        w1 = __builtin_addcll(w1, a1, c0, &c1);
+        w2 += a2 + c1;
         s += b;
         if (s >= 64) {
             out.o0 = out.o1; out.o1 = out.o2; out.o2 = out.o3; out.o3 = w0;
