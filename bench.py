@@ -109,6 +109,33 @@ def fp64_flops(N, plan_stage_radices, sqr, nconv):
     return 2 * (T * c * N + 7 * N) + 35 * nconv
 
 
+FP64_STAGE_COST = {4: 10.0, 2: 5.0, 3: 32.0 / 3, 5: 18.8, 7: 25.0}
+
+
+def fp64_by_kind(plan_groups, plan_stage_radices, N, sqr, nconv):
+    """fp64_flops split by kernel kind: each group's stages (consecutive,
+    radix product S) run in that group's kernel; forward groups transform
+    every operand, inverse groups the product, and the mid kernel does its
+    stages forward for every operand, the pointwise product and its stages
+    inverse (both primes throughout)."""
+    nops = 1 if sqr else 2
+    costs, i = [], 0
+    for (_, _, S, _) in plan_groups:
+        c, prod = 0.0, 1
+        while prod < S:
+            r = plan_stage_radices[i]
+            c += FP64_STAGE_COST[r]
+            prod *= r
+            i += 1
+        costs.append(c)
+    out = {"group_fwd": 0.0, "mid": 0.0, "group_inv": 0.0, "unpack": 35.0 * nconv}
+    for c in costs[:-1]:
+        out["group_fwd"] += 2 * nops * c * N
+        out["group_inv"] += 2 * c * N
+    out["mid"] += 2 * ((nops + 1) * costs[-1] * N + 7 * N)
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -291,6 +318,7 @@ def main():
 
     # ---- roofline of the dominant kernel ---------------------------------------------
     bytes_by_kind = {k: 0 for k in D.PROF_KINDS}
+    fp64_kind = {k: 0.0 for k in D.PROF_KINDS}
     flops = 0.0
     groups_cache = {}
     for (N, m, n, sq, nconv) in meta:
@@ -299,6 +327,8 @@ def main():
         groups, radices = groups_cache[N]
         for k, v in kernel_bytes(groups, N, m, n, sq, nconv).items():
             bytes_by_kind[k] += v
+        for k, v in fp64_by_kind(groups, radices, N, sq, nconv).items():
+            fp64_kind[k] += v
         flops += fp64_flops(N, radices, sq, nconv)
     dom = max((k for k in prof if prof[k][1] > 0), key=lambda k: prof[k][0])
     dom_ms, dom_launches = prof[dom]
@@ -324,6 +354,16 @@ def main():
                 "bytes_per_launch": int(bytes_per_launch), "avg_launch_us": round(avg_launch_s * 1e6, 2),
                 "measured": f"HIP events around every launch of {psteps} profiled steps issued in order "
                             f"({ms_seq:.3f} ms/step)"}
+    # the same launches against the FP64 vector issue rate: the mid kernel's
+    # model intensity (FP64 instructions per algorithmic byte) is above the
+    # chip's balance point, so FP64 issue, not HBM, is its compute roofline
+    instr_per_launch = fp64_kind[dom] / launches_per_step
+    roofline["fp64"] = {"instr_per_launch_model": int(instr_per_launch),
+                        "achieved_instr_per_s": round(instr_per_launch / avg_launch_s / 1e12, 3) * 1e12,
+                        "peak_instr_per_s": FP64_INSTR_PEAK,
+                        "frac": round(instr_per_launch / avg_launch_s / FP64_INSTR_PEAK, 4),
+                        "intensity_instr_per_byte": round(instr_per_launch / bytes_per_launch, 2),
+                        "balance_instr_per_byte": round(FP64_INSTR_PEAK / (HBM_PEAK_GBS * 1e9), 2)}
     valu = {"fp64_instr_per_s_model": flops / (ms * 1e-3), "peak_instr_per_s": FP64_INSTR_PEAK,
             "frac": round(flops / (ms * 1e-3) / FP64_INSTR_PEAK, 4),
             "step_hbm_GBps_model": round(step_bytes / (ms * 1e-3) / 1e9, 1)}

@@ -90,6 +90,9 @@ struct TileGen {
     int32_t prime0;
 };
 
+// LDS of one gfx950 CU (bytes).
+constexpr size_t LDS_PER_CU = 160 * 1024;
+
 // The mid kernel's block-size bound (its __launch_bounds__): one thread per
 // 16 points of a mid tile of at most 8192 points.
 constexpr int MID_MAX_THREADS = 512;

@@ -940,6 +940,23 @@ __global__ void __launch_bounds__(1024) k_group_inv(const DevJob *__restrict__ j
     group_kernel<false>(jobs, tiles, gen, lds);
 }
 
+// The same kernels held to six waves per SIMD (at most 80 VGPRs, no spills)
+// for launches whose tiles are small enough in LDS that more than the four
+// waves per SIMD of the unrestricted build fit on a CU: those are bound by
+// memory latency, and two more waves per SIMD hide more of it.  Launches of
+// large tiles are LDS-bound at four waves and keep the unrestricted build.
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
+k_group_fwd_w6(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles, TileGen gen) {
+    extern __shared__ double lds[];
+    group_kernel<true>(jobs, tiles, gen, lds);
+}
+
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
+k_group_inv_w6(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles, TileGen gen) {
+    extern __shared__ double lds[];
+    group_kernel<false>(jobs, tiles, gen, lds);
+}
+
 // ---------------------------------------------------------------------------
 // Mid pass: last forward group on both operands, pointwise product
 // (src/fpntt.jl:770-785), first inverse group; for a single-group transform
@@ -1667,8 +1684,17 @@ static inline unsigned grid_for(int64_t n, int threads, int64_t cap = 1 << 16) {
 hipError_t launch_group(const DevJob *jobs, const Tile *tiles, const TileGen &gen, int64_t ntiles, int fwd,
                         size_t lds_bytes, int threads, hipStream_t s) {
     if (ntiles <= 0) return hipSuccess;
-    if (fwd) hipLaunchKernelGGL(k_group_fwd, dim3((unsigned)ntiles), dim3(threads), lds_bytes, s, jobs, tiles, gen);
-    else hipLaunchKernelGGL(k_group_inv, dim3((unsigned)ntiles), dim3(threads), lds_bytes, s, jobs, tiles, gen);
+    // waves per CU that LDS admits: above 16 (four per SIMD) the six-wave build pays
+    const int64_t lds_waves = (int64_t)(LDS_PER_CU / std::max<size_t>(lds_bytes, 1)) * (threads / 64);
+    const bool w6 = lds_waves > 16;
+    const dim3 grid((unsigned)ntiles), block(threads);
+    if (fwd) {
+        if (w6) hipLaunchKernelGGL(k_group_fwd_w6, grid, block, lds_bytes, s, jobs, tiles, gen);
+        else hipLaunchKernelGGL(k_group_fwd, grid, block, lds_bytes, s, jobs, tiles, gen);
+    } else {
+        if (w6) hipLaunchKernelGGL(k_group_inv_w6, grid, block, lds_bytes, s, jobs, tiles, gen);
+        else hipLaunchKernelGGL(k_group_inv, grid, block, lds_bytes, s, jobs, tiles, gen);
+    }
     return hipGetLastError();
 }
 
@@ -1771,6 +1797,10 @@ hipError_t set_lds_limit(size_t bytes) {
                                        (int)bytes);
     if (e != hipSuccess) return e;
     e = hipFuncSetAttribute((const void *)k_group_inv, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void *)k_group_fwd_w6, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void *)k_group_inv_w6, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (e != hipSuccess) return e;
     return hipFuncSetAttribute((const void *)k_mid, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }

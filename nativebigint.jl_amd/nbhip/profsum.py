@@ -102,8 +102,8 @@ def main():
     ap.add_argument("--traffic-write-db", default="",
                     help="traffic mode: db is the FETCH_SIZE run, this the WRITE_SIZE run; prints JSON")
     ap.add_argument("--copy-bytes", type=int, default=256 << 20)
-    ap.add_argument("--kernels", default="k_mid,k_group_fwd,k_group_inv,k_unpack,k_carry_fix,k_combine,"
-                                          "k_barrett_fix")
+    ap.add_argument("--kernels", default="k_mid,k_group_fwd,k_group_inv,k_group_fwd_w6,k_group_inv_w6,k_unpack,"
+                                          "k_carry_fix,k_combine,k_barrett_fix")
     a = ap.parse_args()
     if a.traffic_write_db:
         import json
