@@ -883,7 +883,7 @@ __device__ __forceinline__ void store_tile(double *__restrict__ x, const double 
 // Group pass: one tile of one group of one transform, forward or inverse.
 
 template <int K>
-__device__ void group_body(const DevJob *__restrict__ Jp, const Tile &t, bool fwd, double *lds) {
+__device__ __forceinline__ void group_body(const DevJob *__restrict__ Jp, const Tile &t, bool fwd, double *lds) {
     const auto *J = cst(Jp);
     const DevPlan *__restrict__ P = unip(J->plan[K]);
     const auto *G = &cst(P)->g[t.group];
@@ -965,7 +965,7 @@ k_group_inv_w6(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles, 
 // spectrum and transforms a only.
 
 template <int K>
-__device__ void mid_body(const DevJob *__restrict__ Jp, const Tile &t, double *lds) {
+__device__ __forceinline__ void mid_body(const DevJob *__restrict__ Jp, const Tile &t, double *lds) {
     const auto *J = cst(Jp);
     const DevPlan *__restrict__ P = unip(J->plan[K]);
     const auto *G = &cst(P)->g[uni(cst(P)->ngroups) - 1];
