@@ -358,12 +358,13 @@ def main():
     # model intensity (FP64 instructions per algorithmic byte) is above the
     # chip's balance point, so FP64 issue, not HBM, is its compute roofline
     instr_per_launch = fp64_kind[dom] / launches_per_step
-    roofline["fp64"] = {"instr_per_launch_model": int(instr_per_launch),
-                        "achieved_instr_per_s": round(instr_per_launch / avg_launch_s / 1e12, 3) * 1e12,
-                        "peak_instr_per_s": FP64_INSTR_PEAK,
-                        "frac": round(instr_per_launch / avg_launch_s / FP64_INSTR_PEAK, 4),
-                        "intensity_instr_per_byte": round(instr_per_launch / bytes_per_launch, 2),
-                        "balance_instr_per_byte": round(FP64_INSTR_PEAK / (HBM_PEAK_GBS * 1e9), 2)}
+    if instr_per_launch > 0 and bytes_per_launch > 0:
+        roofline["fp64"] = {"instr_per_launch_model": int(instr_per_launch),
+                            "achieved_instr_per_s": round(instr_per_launch / avg_launch_s / 1e12, 3) * 1e12,
+                            "peak_instr_per_s": FP64_INSTR_PEAK,
+                            "frac": round(instr_per_launch / avg_launch_s / FP64_INSTR_PEAK, 4),
+                            "intensity_instr_per_byte": round(instr_per_launch / bytes_per_launch, 2),
+                            "balance_instr_per_byte": round(FP64_INSTR_PEAK / (HBM_PEAK_GBS * 1e9), 2)}
     valu = {"fp64_instr_per_s_model": flops / (ms * 1e-3), "peak_instr_per_s": FP64_INSTR_PEAK,
             "frac": round(flops / (ms * 1e-3) / FP64_INSTR_PEAK, 4),
             "step_hbm_GBps_model": round(step_bytes / (ms * 1e-3) / 1e9, 1)}

@@ -69,6 +69,12 @@ def counter_per_dispatch(db: str, counter: str):
     return out
 
 
+def _names(full: str, k: str) -> bool:
+    """Whether a demangled kernel name is kernel k itself (k_group_fwd does
+    not name k_group_fwd_w6)."""
+    return re.search(r"(^|::|\s)" + re.escape(k) + r"(?![\w])", full) is not None
+
+
 def traffic(fetch_db: str, write_db: str, kernels, copy_bytes: int, copy_kernel: str = "copyBuffer"):
     """HBM bytes per launch of each kernel from FETCH_SIZE and WRITE_SIZE
     (KiB per dispatch), each scaled by the factor that makes the calibration
@@ -82,8 +88,8 @@ def traffic(fetch_db: str, write_db: str, kernels, copy_bytes: int, copy_kernel:
     w_scale = copy_bytes / (big_w * 1024)
     out = {"copy_bytes": copy_bytes, "fetch_scale": f_scale, "write_scale": w_scale, "kernels": {}}
     for k in kernels:
-        fk = [v for name, vs in f.items() if k in name for v in vs]
-        wk = [v for name, vs in w.items() if k in name for v in vs]
+        fk = [v for name, vs in f.items() if _names(name, k) for v in vs]
+        wk = [v for name, vs in w.items() if _names(name, k) for v in vs]
         if not fk or not wk:
             continue
         fetch = sum(fk) * 1024 * f_scale / len(fk)
