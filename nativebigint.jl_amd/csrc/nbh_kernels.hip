@@ -729,6 +729,32 @@ __device__ __forceinline__ double pack_chunk(const uint64_t *__restrict__ limbs,
     return pack_bits(limbs, n, (1ull << b) - 1, i < nch, i * (int64_t)b);
 }
 
+// Operands of at most this many limbs (2 GiB) are read through a buffer
+// resource: 32-bit byte offsets, and a dword past the operand reads as zero
+// (range checking is per dword), as the bits past the operand are.
+constexpr int64_t BUF_MAX_LIMBS = (int64_t)1 << 28;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t limb_rsrc(const uint64_t *limbs, int64_t n) {
+    // word 3 of a gfx9-family raw buffer descriptor: 32-bit data, no swizzle
+    return __builtin_amdgcn_make_buffer_rsrc((void *)limbs, (short)0, (int)(8 * n), 0x00020000);
+}
+
+// The chunk at `bit` through the buffer: a chunk of at most 49 bits starting
+// at bit (bit mod 32) of its first dword lies in that dword and the next two,
+// and two funnel shifts extract it.  A lane that is not `in` reads far past
+// the operand and yields 0.
+__device__ __forceinline__ double pack_buf(__amdgpu_buffer_rsrc_t r, uint32_t lmask, uint32_t hmask, bool in,
+                                           int64_t bit) {
+    const uint32_t off = in ? (uint32_t)(bit >> 5) << 2 : 0x80000000u;
+    const uint32_t sh = (uint32_t)bit & 31u;
+    const uint32_t x0 = __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+    const uint32_t x1 = __builtin_amdgcn_raw_buffer_load_b32(r, off + 4, 0, 0);
+    const uint32_t x2 = __builtin_amdgcn_raw_buffer_load_b32(r, off + 8, 0, 0);
+    const uint32_t lo = __builtin_amdgcn_alignbit(x1, x0, sh) & lmask;
+    const uint32_t hi = __builtin_amdgcn_alignbit(x2, x1, sh) & hmask;
+    return __builtin_fma((double)hi, 4294967296.0, (double)lo);
+}
+
 // Where a tile's points come from: spectra x, or (pack) the b-bit chunks of
 // an operand's limbs.
 struct PointSrc {
@@ -752,11 +778,20 @@ __device__ __forceinline__ void load_round(double (&v)[NU], const PointSrc &src,
         const int64_t i0 = gbase + (int64_t)(e0 >> g.logC) * g.Lb + (e0 & cmask);
         const int64_t step = (int64_t)(T >> g.logC) * g.Lb;
         if (PACK) {
-            const uint64_t bmask = (1ull << src.b) - 1;
             const int64_t bit0 = i0 * src.b, bstep = step * src.b;
+            if (src.n <= BUF_MAX_LIMBS) {
+                const __amdgpu_buffer_rsrc_t r = limb_rsrc(src.limbs, src.n);
+                const uint32_t lmask = src.b >= 32 ? ~0u : (1u << src.b) - 1u;
+                const uint32_t hmask = src.b >= 32 ? (1u << (src.b - 32)) - 1u : 0u;
 #pragma unroll
-            for (int k = 0; k < NU; k++)
-                v[k] = pack_bits(src.limbs, src.n, bmask, i0 + k * step < src.nch, bit0 + k * bstep);
+                for (int k = 0; k < NU; k++)
+                    v[k] = pack_buf(r, lmask, hmask, i0 + k * step < src.nch, bit0 + k * bstep);
+            } else {
+                const uint64_t bmask = (1ull << src.b) - 1;
+#pragma unroll
+                for (int k = 0; k < NU; k++)
+                    v[k] = pack_bits(src.limbs, src.n, bmask, i0 + k * step < src.nch, bit0 + k * bstep);
+            }
         } else {
 #pragma unroll
             for (int k = 0; k < NU; k++) v[k] = gld(src.x, i0 + k * step);

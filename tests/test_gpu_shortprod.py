@@ -167,3 +167,21 @@ def test_errors():
         D.sqrlo_dev(r, a, -3)
     with pytest.raises(NbhError):
         D.Batch([(r, a, a, -1, -1, -1)])
+
+
+@pytest.mark.parametrize("k", [300, 3000, 20000])
+def test_limbs_at_or_above_k_are_not_read(k):
+    """Operands longer than k with all-ones limbs above k: the chunk pack
+    reads whole dwords past the clamped operand, which must count as zero."""
+    rng = np.random.default_rng(k + 1)
+    a = np.concatenate([_rand(rng, k), np.full(64, 2**64 - 1, dtype=np.uint64)])
+    b = np.concatenate([_rand(rng, k - 5), np.full(70, 2**64 - 1, dtype=np.uint64)])
+    for x, y in ((a, b), (b, a)):
+        r = D.empty_limbs(k)
+        D.mullo_dev(r, D.to_dev(x), D.to_dev(y), k)
+        torch.cuda.synchronize()
+        assert ints(D.to_host(r)) == (ints(x[:k]) * ints(y[:k])) % BETA**k
+    r = D.empty_limbs(k)
+    D.sqrlo_dev(r, D.to_dev(a), k)
+    torch.cuda.synchronize()
+    assert ints(D.to_host(r)) == (ints(a[:k]) ** 2) % BETA**k
