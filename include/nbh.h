@@ -135,16 +135,27 @@ int32_t nbh_mul_batch_dev(const nbh_job *jobs, int64_t count, void *stream);
  *                     application of hgcd_matrix_adjust! (src/gcd.jl:398-437)
  *                     without the ra*2^(64p) term and for any matrix, so the
  *                     results may be negative.
+ *   NBH_HGCD_ADJUST_FULL: hgcd_matrix_adjust! (src/gcd.jl:398-437) itself,
+ *                     with u, v the low p = n limbs and ra, rb the nn-limb
+ *                     reduced tops: r0 = ra*2^(64p) + m11*u - m01*v,
+ *                     r1 = rb*2^(64p) + m00*v - m10*u, written zero-padded to
+ *                     L = p + max(mn+1, nn) + 1 limbs.  The reference asserts
+ *                     both are nonnegative; s0/s1 are 1 (positive), 0 (zero)
+ *                     or -1 where that assertion would fail (r then holds the
+ *                     magnitude).
  * The four products of every instance run as one planned product batch, the
  * adds/subtractions as one more launch.  No reference counterpart as a batch. */
 #define NBH_HGCD_FORWARD 0
 #define NBH_HGCD_ADJUST 1
+#define NBH_HGCD_ADJUST_FULL 2
 typedef struct {
     const uint64_t *m00, *m01, *m10, *m11;
     const uint64_t *u, *v;
     int64_t mn, n;
     uint64_t *r0, *r1;
     int32_t *s0, *s1;
+    const uint64_t *ra, *rb;  /* NBH_HGCD_ADJUST_FULL only */
+    int64_t nn;
 } nbh_hgcd_job;
 typedef struct nbh_hgcd_s *nbh_hgcd;
 int32_t nbh_hgcd_create(const nbh_hgcd_job *jobs, int64_t count, int32_t mode, nbh_hgcd *out);

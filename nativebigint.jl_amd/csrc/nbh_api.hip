@@ -1158,14 +1158,15 @@ void hgcd_free(nbh_hgcd_s *H) {
 }
 
 int hgcd_create(const nbh_hgcd_job *jobs, int64_t count, int mode, nbh_hgcd_s **out) {
-    if (!jobs || count < 1 || (mode != NBH_HGCD_FORWARD && mode != NBH_HGCD_ADJUST))
+    if (!jobs || count < 1 || (mode != NBH_HGCD_FORWARD && mode != NBH_HGCD_ADJUST && mode != NBH_HGCD_ADJUST_FULL))
         return fail(NBH_ERR_ARG, "bad cofactor-step batch");
+    const bool full = mode == NBH_HGCD_ADJUST_FULL;
     std::unique_ptr<nbh_hgcd_s, void (*)(nbh_hgcd_s *)> H(new nbh_hgcd_s, hgcd_free);
     size_t words = 0;
     for (int64_t i = 0; i < count; i++) {
         const nbh_hgcd_job &q = jobs[i];
         if (q.mn < 1 || q.n < 1 || !q.m00 || !q.m01 || !q.m10 || !q.m11 || !q.u || !q.v || !q.r0 || !q.r1 ||
-            (mode == NBH_HGCD_ADJUST && (!q.s0 || !q.s1)))
+            (mode != NBH_HGCD_FORWARD && (!q.s0 || !q.s1)) || (full && (!q.ra || !q.rb || q.nn < 1)))
             return fail(NBH_ERR_ARG, "bad cofactor-step instance " + std::to_string(i));
         words += 4 * (size_t)(q.n + q.mn);
     }
@@ -1181,17 +1182,24 @@ int hgcd_create(const nbh_hgcd_job *jobs, int64_t count, int mode, nbh_hgcd_s **
         // forward: m00 u, m01 v, m10 u, m11 v; adjust: m11 u, m01 v, m00 v, m10 u
         const uint64_t *mat[4] = {q.m00, q.m01, q.m10, q.m11};
         const uint64_t *vec[4] = {q.u, q.v, q.u, q.v};
-        if (mode == NBH_HGCD_ADJUST) {
+        if (mode != NBH_HGCD_FORWARD) {
             mat[0] = q.m11; vec[0] = q.u;
             mat[1] = q.m01; vec[1] = q.v;
             mat[2] = q.m00; vec[2] = q.v;
             mat[3] = q.m10; vec[3] = q.u;
         }
-        for (int k = 0; k < 4; k++) prods.push_back(nbh_job{P[k], vec[k], mat[k], q.n, q.mn, -1, -1});
-        const int sub = mode == NBH_HGCD_ADJUST;
-        const int64_t lr = sub ? lp : lp + 1;
-        comb.push_back(CombineJob{P[0], P[1], lp, lp, q.r0, lr, q.s0, sub, 0});
-        comb.push_back(CombineJob{P[2], P[3], lp, lp, q.r1, lr, q.s1, sub, 0});
+        for (int k = 0; k < 4; k++) prods.push_back(nbh_job{P[k], vec[k], mat[k], q.n, q.mn, -1, -1, 0});
+        const int sub = mode != NBH_HGCD_FORWARD;
+        if (full) {
+            // hgcd_matrix_adjust! (src/gcd.jl:398-437): p = n, L = p + max(mn + 1, nn) + 1
+            const int64_t L = q.n + std::max(q.mn + 1, q.nn) + 1;
+            comb.push_back(CombineJob{P[0], P[1], lp, lp, q.r0, L, q.s0, 1, 0, q.ra, q.nn, q.n});
+            comb.push_back(CombineJob{P[2], P[3], lp, lp, q.r1, L, q.s1, 1, 0, q.rb, q.nn, q.n});
+        } else {
+            const int64_t lr = sub ? lp : lp + 1;
+            comb.push_back(CombineJob{P[0], P[1], lp, lp, q.r0, lr, q.s0, sub, 0, nullptr, 0, 0});
+            comb.push_back(CombineJob{P[2], P[3], lp, lp, q.r1, lr, q.s1, sub, 0, nullptr, 0, 0});
+        }
     }
     RCCHK(batch_create(prods.data(), (int64_t)prods.size(), &H->products));
     H->ncomb = (int64_t)comb.size();

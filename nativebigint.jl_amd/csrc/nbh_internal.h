@@ -134,6 +134,8 @@ struct CombineJob {
     int64_t lr;
     int32_t *sign;  // sub only: -1, 0, +1 (device)
     int32_t sub, pad;
+    const uint64_t *C;  // when set: A is first replaced by A + C * beta^cshift, written to r
+    int64_t lc, cshift;
 };
 
 // Barrett correction of one reduction (barrett_reduce!, src/barrett.jl:

@@ -1359,15 +1359,80 @@ __device__ __forceinline__ uint32_t block_carry(bool gen, bool prop, uint32_t ci
     return (uint32_t)((cv >> lane) & 1);
 }
 
+// r[0..lr) = X + Y*beta^ysh (add) or X - Y*beta^ysh (sub, X >= Y*beta^ysh):
+// one block walks the limbs in chunks of UNPACK_LPC, the carry (borrow)
+// between threads from the ballot prefix and between chunks in a register.
+// A thread reads limbs k0..k0+3 of X and Y before it writes those of r, so r
+// may be X or Y.
+__device__ __forceinline__ void combine_pass(const uint64_t *X, int64_t lx, const uint64_t *Y, int64_t ly,
+                                             int64_t ysh, uint64_t *r, int64_t lr, bool sub, uint32_t *wsum) {
+    const int tid = threadIdx.x;
+    uint32_t carry = 0;
+    for (int64_t c0 = 0; c0 < lr; c0 += UNPACK_LPC) {
+        const int64_t k0 = c0 + UNPACK_LPT * tid;
+        uint64_t d[UNPACK_LPT];
+        bool prop = true;
+        // segment result with carry (borrow) in 0, and its generate/propagate
+        uint64_t cc = 0;
+#pragma unroll
+        for (int w = 0; w < UNPACK_LPT; w++) {
+            const int64_t k = k0 + w;
+            const uint64_t x = k < lx ? gldu(X, k) : 0;
+            const uint64_t y = k >= ysh && k - ysh < ly ? gldu(Y, k - ysh) : 0;
+            if (!sub) {
+                const uint64_t s1 = x + y;
+                const uint64_t c1 = s1 < x;
+                const uint64_t s2 = s1 + cc;
+                cc = c1 | (s2 < s1);
+                d[w] = s2;
+                prop = prop && s2 == ~0ull;
+            } else {
+                const uint64_t s1 = x - y;
+                const uint64_t b1 = x < y;
+                const uint64_t s2 = s1 - cc;
+                cc = b1 | (s1 < cc);
+                d[w] = s2;
+                prop = prop && s2 == 0;
+            }
+        }
+        uint32_t cout = 0;
+        const uint32_t cin = block_carry(cc != 0, prop, carry, wsum, cout);
+        uint64_t cv = cin;
+#pragma unroll
+        for (int w = 0; w < UNPACK_LPT; w++) {
+            const int64_t k = k0 + w;
+            uint64_t v;
+            if (!sub) {
+                v = d[w] + cv;
+                cv = v < cv;
+            } else {
+                v = d[w] - cv;
+                cv = d[w] < cv;
+            }
+            if (k < lr) r[k] = v;
+        }
+        carry = cout;
+    }
+}
+
 __global__ void __launch_bounds__(UNPACK_THREADS) k_combine(const CombineJob *__restrict__ jobs) {
     __shared__ uint32_t wsum[UNPACK_THREADS / 64];
     __shared__ unsigned long long top;  // 1 + highest differing limb, 0 if none
     const auto *J = cst(jobs + blockIdx.x);
     const uint64_t *A = unip(J->A), *B = unip(J->B);
-    const int64_t la = uni64(J->la), lb = uni64(J->lb), lr = uni64(J->lr);
+    int64_t la = uni64(J->la);
+    const int64_t lb = uni64(J->lb), lr = uni64(J->lr);
     uint64_t *__restrict__ r = unip(J->r);
     const bool sub = uni(J->sub) != 0;
     const int tid = threadIdx.x;
+    const uint64_t *C = unip(J->C);
+    if (C) {
+        // hgcd_matrix_adjust!'s shifted top (src/gcd.jl:415-419): A + C*beta^cshift into r first
+        combine_pass(A, la, C, uni64(J->lc), uni64(J->cshift), r, lr, false, wsum);
+        __syncthreads();  // r, written by the whole block, is read below by other threads
+        A = r;
+        la = lr;
+    }
     bool swap = false;
     if (sub) {
         // highest limb where A and B differ decides the sign
@@ -1389,55 +1454,11 @@ __global__ void __launch_bounds__(UNPACK_THREADS) k_combine(const CombineJob *__
         }
         if (tid == 0) *unip(J->sign) = sg;
         swap = sg < 0;
+        __syncthreads();  // every thread has read A before r (which may be A) is rewritten
     }
     const uint64_t *X = swap ? B : A, *Y = swap ? A : B;
     const int64_t lx = swap ? lb : la, ly = swap ? la : lb;
-    uint32_t carry = 0;
-    for (int64_t c0 = 0; c0 < lr; c0 += UNPACK_LPC) {
-        const int64_t k0 = c0 + UNPACK_LPT * tid;
-        uint64_t d[UNPACK_LPT];
-        bool gen = false, prop = true;
-        // segment result with carry (borrow) in 0, and its generate/propagate
-        uint64_t cc = 0;
-#pragma unroll
-        for (int w = 0; w < UNPACK_LPT; w++) {
-            const int64_t k = k0 + w;
-            const uint64_t x = k < lx ? gldu(X, k) : 0, y = k < ly ? gldu(Y, k) : 0;
-            if (!sub) {
-                const uint64_t s1 = x + y;
-                const uint64_t c1 = s1 < x;
-                const uint64_t s2 = s1 + cc;
-                cc = c1 | (s2 < s1);
-                d[w] = s2;
-                prop = prop && s2 == ~0ull;
-            } else {
-                const uint64_t s1 = x - y;
-                const uint64_t b1 = x < y;
-                const uint64_t s2 = s1 - cc;
-                cc = b1 | (s1 < cc);
-                d[w] = s2;
-                prop = prop && s2 == 0;
-            }
-        }
-        gen = cc != 0;
-        uint32_t cout = 0;
-        const uint32_t cin = block_carry(gen, prop, carry, wsum, cout);
-        uint64_t cv = cin;
-#pragma unroll
-        for (int w = 0; w < UNPACK_LPT; w++) {
-            const int64_t k = k0 + w;
-            uint64_t v;
-            if (!sub) {
-                v = d[w] + cv;
-                cv = v < cv;
-            } else {
-                v = d[w] - cv;
-                cv = d[w] < cv;
-            }
-            if (k < lr) r[k] = v;
-        }
-        carry = cout;
-    }
+    combine_pass(X, lx, Y, ly, 0, r, lr, sub, wsum);
 }
 
 // One workgroup per reduction.  Thread t owns limbs c0 + 4t .. c0 + 4t + 3 of

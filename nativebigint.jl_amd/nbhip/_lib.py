@@ -45,7 +45,8 @@ class nbh_hgcd_job(ctypes.Structure):
                 ("m11", ctypes.c_void_p), ("u", ctypes.c_void_p), ("v", ctypes.c_void_p),
                 ("mn", ctypes.c_int64), ("n", ctypes.c_int64),
                 ("r0", ctypes.c_void_p), ("r1", ctypes.c_void_p),
-                ("s0", ctypes.c_void_p), ("s1", ctypes.c_void_p)]
+                ("s0", ctypes.c_void_p), ("s1", ctypes.c_void_p),
+                ("ra", ctypes.c_void_p), ("rb", ctypes.c_void_p), ("nn", ctypes.c_int64)]
 
 
 class nbh_spec_job(ctypes.Structure):
@@ -59,6 +60,7 @@ class nbh_barrett_job(ctypes.Structure):
 
 NBH_HGCD_FORWARD = 0
 NBH_HGCD_ADJUST = 1
+NBH_HGCD_ADJUST_FULL = 2
 
 _i32, _i64, _vp, _d = ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p, ctypes.c_double
 _P = ctypes.POINTER

@@ -11,7 +11,8 @@ import ctypes
 import numpy as np
 import torch
 
-from ._lib import NBH_HGCD_ADJUST, NBH_HGCD_FORWARD, check, init, lib, nbh_hgcd_job, nbh_job, nbh_spec_job
+from ._lib import (NBH_HGCD_ADJUST, NBH_HGCD_ADJUST_FULL, NBH_HGCD_FORWARD, check, init, lib, nbh_hgcd_job,
+                   nbh_job, nbh_spec_job)
 
 
 def _stream():
@@ -122,16 +123,29 @@ class HgcdStep:
     instances: list of dicts with tensors m00, m01, m10, m11 (mn limbs each),
     u, v (n limbs each).  mode "forward": r0 = m00 u + m01 v, r1 = m10 u + m11 v
     (n+mn+1 limbs); mode "adjust": r0 = |m11 u - m01 v|, r1 = |m00 v - m10 u|
-    (n+mn limbs) with signs s0, s1.  Outputs are allocated here."""
+    (n+mn limbs) with signs s0, s1; mode "adjust_full": hgcd_matrix_adjust!
+    (src/gcd.jl:398-437) with u, v the low p = n limbs and ra, rb (nn limbs
+    each, also in the dict) the reduced tops: r0 = ra*2^(64p) + m11 u - m01 v,
+    r1 = rb*2^(64p) + m00 v - m10 u, p + max(mn+1, nn) + 1 limbs, signs -1
+    where the reference's nonnegativity assertion would fail.  Outputs are
+    allocated here."""
+
+    MODES = {"forward": NBH_HGCD_FORWARD, "adjust": NBH_HGCD_ADJUST, "adjust_full": NBH_HGCD_ADJUST_FULL}
 
     def __init__(self, instances, mode: str = "forward"):
         init()
-        self.mode = NBH_HGCD_FORWARD if mode == "forward" else NBH_HGCD_ADJUST
+        self.mode = self.MODES[mode]
         arr = (nbh_hgcd_job * len(instances))()
         self.out = []
         for i, q in enumerate(instances):
             mn, n = q["m00"].numel(), q["u"].numel()
-            lr = n + mn + (1 if self.mode == NBH_HGCD_FORWARD else 0)
+            if self.mode == NBH_HGCD_ADJUST_FULL:
+                nn = q["ra"].numel()
+                assert q["rb"].numel() == nn
+                lr = n + max(mn + 1, nn) + 1
+                arr[i].ra, arr[i].rb, arr[i].nn = q["ra"].data_ptr(), q["rb"].data_ptr(), nn
+            else:
+                lr = n + mn + (1 if self.mode == NBH_HGCD_FORWARD else 0)
             r0 = torch.empty(lr, dtype=torch.int64, device="cuda")
             r1 = torch.empty(lr, dtype=torch.int64, device="cuda")
             sg = torch.zeros(2, dtype=torch.int32, device="cuda")

@@ -99,3 +99,49 @@ def test_matrix_mul():
         n, n1 = M["m00"].size, M1["m00"].size
         assert R["m00"].numel() == n + n1 + 1
     H.close()
+
+
+# (seed, p, mn, nn, force the first result negative); with nn = 1 the top
+# term is small and the results come out negative on their own
+FULL_CASES = [(21, 600, 300, 320, False), (22, 1000, 500, 480, False), (23, 333, 200, 1, False),
+              (24, 2500, 1250, 1300, False), (25, 700, 350, 351, True), (26, 160, 80, 90, False),
+              (27, 12000, 6000, 6100, False), (28, 5, 3, 2, False)]
+
+
+@pytest.mark.parametrize("seed,p,mn,nn,neg", FULL_CASES)
+def test_adjust_full(seed, p, mn, nn, neg):
+    """hgcd_matrix_adjust! (src/gcd.jl:398-437): u' = ra*2^(64p) + m11*ulow
+    - m01*vlow, v' = rb*2^(64p) + m00*vlow - m10*ulow, zero-padded to
+    L = p + max(mn+1, nn) + 1 limbs; a result the reference would assert on
+    (negative) comes back with sign -1 and its magnitude."""
+    engine.init()
+    rng = np.random.default_rng(seed)
+    q = {k: rng.integers(0, 2**64, mn, dtype=np.uint64) for k in ("m00", "m01", "m10", "m11")}
+    q["u"] = rng.integers(0, 2**64, p, dtype=np.uint64)
+    q["v"] = rng.integers(0, 2**64, p, dtype=np.uint64)
+    q["ra"] = rng.integers(0, 2**64, nn, dtype=np.uint64)
+    q["rb"] = rng.integers(0, 2**64, nn, dtype=np.uint64)
+    q["rb"][-1] |= np.uint64(1 << 63)
+    if neg:
+        q["ra"][:] = 0                # m01*vlow > m11*ulow makes u' negative
+        q["m11"][-1] = np.uint64(1)
+        q["m01"][-1] = np.uint64(2**64 - 1)
+    M = {k: ints(v) for k, v in q.items()}
+    L = p + max(mn + 1, nn) + 1
+    e0 = (M["ra"] << (64 * p)) + M["m11"] * M["u"] - M["m01"] * M["v"]
+    e1 = (M["rb"] << (64 * p)) + M["m00"] * M["v"] - M["m10"] * M["u"]
+    H = D.HgcdStep([{k: D.to_dev(v) for k, v in q.items()}], "adjust_full")
+    for rep in range(2):
+        for t in H.out[0][:2]:
+            t.fill_(-1)
+        H.run()
+        torch.cuda.synchronize()
+        r0, r1, sg, _ = H.out[0]
+        assert r0.numel() == L and r1.numel() == L
+        s = sg.cpu().numpy()
+        for r, e, sign in ((r0, e0, int(s[0])), (r1, e1, int(s[1]))):
+            assert sign == (1 if e > 0 else 0 if e == 0 else -1)
+            assert ints(D.to_host(r)) == abs(e)
+    if neg:
+        assert e0 < 0               # the case meant to exercise the assertion's sign
+    H.close()
