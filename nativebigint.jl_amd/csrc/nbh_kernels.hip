@@ -372,8 +372,14 @@ __device__ void run_stages(double *x0, double *x1, int narr, const DevPlan *__re
 // first inverse pass run on registers: one LDS round trip instead of three.
 // The butterflies are exactly those of pass44 / pass1.
 
+// x times the twiddle w^0 = 1, which is every twiddle of a transform's last
+// stage (span = radix): fp_mulmod3(x, 1) = reduce(x) + fma(x, 1, -x) =
+// reduce(x) + (+0.0), bit for bit, without the product and the fma.
+template <int K> __device__ __forceinline__ double fp_mulmod_one(double x) { return fp_reduce<K>(x) + 0.0; }
+
 // One radix-M stage on registers y[0..M) with its twiddles w[0..M-2].
-template <int K, bool FWD, int M>
+// UNIT: the transform's last stage, whose twiddles are all 1 (w unread).
+template <int K, bool FWD, int M, bool UNIT = false>
 __device__ __forceinline__ void stage_regs(double *y, const double *w, const StageU &s) {
     if constexpr (M == 2) {
         const double u = y[0], v = y[1];
@@ -382,9 +388,8 @@ __device__ __forceinline__ void stage_regs(double *y, const double *w, const Sta
     } else if constexpr (M == 4) {
         if (FWD) dft4<K>(y[0], y[1], y[2], y[3], s.rot[0]);
         y[0] = fp_reduce<K>(y[0]);
-        y[1] = fp_mulmod3<K>(y[1], w[0]);
-        y[2] = fp_mulmod3<K>(y[2], w[1]);
-        y[3] = fp_mulmod3<K>(y[3], w[2]);
+#pragma unroll
+        for (int t = 1; t < 4; t++) y[t] = UNIT ? fp_mulmod_one<K>(y[t]) : fp_mulmod3<K>(y[t], w[t - 1]);
         if (!FWD) dft4<K>(y[0], y[1], y[2], y[3], s.rot[0]);
     } else if constexpr (M == 3) {
         if (FWD) dft3<K>(y[0], y[1], y[2], s.rot[0]);
@@ -449,7 +454,7 @@ __device__ __forceinline__ int ix44(const Sub44 &u, int a, int b) {
 }
 
 // Forward stages s0 (over b) and s1 (over a) of one sub-problem on registers.
-template <int K, bool FWD>
+template <int K, bool FWD, bool UNIT1 = false>
 __device__ __forceinline__ void regs44(double (&y)[4][4], const double (&w0)[4][3], const double (&w1)[3],
                                        const StageU &s0, const StageU &s1) {
     if (FWD) {
@@ -458,7 +463,7 @@ __device__ __forceinline__ void regs44(double (&y)[4][4], const double (&w0)[4][
 #pragma unroll
         for (int b = 0; b < 4; b++) {
             double t[4] = {y[0][b], y[1][b], y[2][b], y[3][b]};
-            stage_regs<K, true, 4>(t, w1, s1);
+            stage_regs<K, true, 4, UNIT1>(t, w1, s1);
 #pragma unroll
             for (int a = 0; a < 4; a++) y[a][b] = t[a];
         }
@@ -466,7 +471,7 @@ __device__ __forceinline__ void regs44(double (&y)[4][4], const double (&w0)[4][
 #pragma unroll
         for (int b = 0; b < 4; b++) {
             double t[4] = {y[0][b], y[1][b], y[2][b], y[3][b]};
-            stage_regs<K, false, 4>(t, w1, s1);
+            stage_regs<K, false, 4, UNIT1>(t, w1, s1);
 #pragma unroll
             for (int a = 0; a < 4; a++) y[a][b] = t[a];
         }
@@ -475,12 +480,16 @@ __device__ __forceinline__ void regs44(double (&y)[4][4], const double (&w0)[4][
     }
 }
 
-template <bool LIN>
+template <bool LIN, bool UNIT1 = false>
 __device__ __forceinline__ void tw44(double (&w0)[4][3], double (&w1)[3], const Sub44 &u, const StageU &s0,
                                      const StageU &s1) {
 #pragma unroll
     for (int a = 0; a < 4; a++) load_tw<4>(w0[a], s0, u.jt + a * s1.Q);
-    load_tw<4>(w1, s1, u.jt);
+    if (UNIT1) {
+        w1[0] = w1[1] = w1[2] = 1.0;  // unread
+    } else {
+        load_tw<4>(w1, s1, u.jt);
+    }
 }
 
 // Forward radix-4 pair on NA arrays (a, and b when NA == 2).
@@ -621,9 +630,10 @@ __device__ void last44(double *__restrict__ x, double (&hb)[HOLD], const double 
     for (int a = 0; a < 4; a++)
 #pragma unroll
         for (int b = 0; b < 4; b++) y[a][b] = x[ix44<LIN>(u, a, b)];
+    // s1 is the transform's last stage: its twiddles are all 1
     double w0[4][3], w1[3];
-    tw44<LIN>(w0, w1, u, s0, s1);
-    regs44<K, true>(y, w0, w1, s0, s1);
+    tw44<LIN, true>(w0, w1, u, s0, s1);
+    regs44<K, true, true>(y, w0, w1, s0, s1);
     if (HOLDB) {
 #pragma unroll
         for (int a = 0; a < 4; a++)
@@ -640,7 +650,7 @@ __device__ void last44(double *__restrict__ x, double (&hb)[HOLD], const double 
                                                : gld(gb, gbase + u.base + a * u.sa + b * u.sb);
             y[a][b] = fp_mulmod3<K>(y[a][b], v);
         }
-    regs44<K, false>(y, w0, w1, s0, s1);
+    regs44<K, false, true>(y, w0, w1, s0, s1);
 #pragma unroll
     for (int a = 0; a < 4; a++)
 #pragma unroll
@@ -659,12 +669,12 @@ __device__ void last1(double *__restrict__ x, double (&hb)[HOLD], const double *
         const int beta = threadIdx.x + it * (int)blockDim.x;
         if (beta >= nb) break;
         const Sub1 u = sub1<M>(beta, s, g);
-        double w[M > 1 ? M - 1 : 1];
-        load_tw<M>(w, s, u.j);
+        // s is the transform's last stage: its twiddles are all 1 (w unread)
+        const double w[M > 1 ? M - 1 : 1] = {};
         double y[M];
 #pragma unroll
         for (int t = 0; t < M; t++) y[t] = x[ix1<LIN>(u, t)];
-        stage_regs<K, true, M>(y, w, s);
+        stage_regs<K, true, M, true>(y, w, s);
         if (HOLDB) {
 #pragma unroll
             for (int t = 0; t < M; t++) hb[it * M + t] = y[t];
@@ -677,7 +687,7 @@ __device__ void last1(double *__restrict__ x, double (&hb)[HOLD], const double *
                                                : gld(gb, gbase + u.base + t * u.strideC);
             y[t] = fp_mulmod3<K>(y[t], v);
         }
-        stage_regs<K, false, M>(y, w, s);
+        stage_regs<K, false, M, true>(y, w, s);
 #pragma unroll
         for (int t = 0; t < M; t++) x[ix1<LIN>(u, t)] = y[t];
     }
