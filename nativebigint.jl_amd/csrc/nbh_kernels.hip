@@ -365,10 +365,9 @@ __device__ void run_stages(double *x0, double *x1, int narr, const DevPlan *__re
 }
 
 // ---------------------------------------------------------------------------
-// Mid-kernel passes with both operands in one thread.  Thread t owns the same
-// sub-problem of a and of b, so a forward pass loads each twiddle once for
-// both operands, every thread has work in the single-array inverse, and the
-// last forward pass, the pointwise product (src/fpntt.jl:770-785) and the
+// Mid-kernel passes.  The forward passes below take NA arrays and load each
+// twiddle once for all of them (the single-array mid kernel passes NA = 1);
+// the last forward pass, the pointwise product (src/fpntt.jl:770-785) and the
 // first inverse pass run on registers: one LDS round trip instead of three.
 // The butterflies are exactly those of pass44 / pass1.
 
