@@ -24,6 +24,11 @@ cached spectra; limb-products counts the result limbs of all three products.
 workload "serve": Poisson arrivals into the continuous-batching server.
 workload "small": 4096 products below the NTT thresholds (limbs log-uniform
 in [1, 151], 25% squares): the batch's schoolbook kernel.
+workload "single": the plain single product of 4096 x 4096 limbs and the
+square of 4096 limbs, one call at a time through the drop-in boundary
+(nbh_mul/nbh_sqr on host buffers: copies in and out included) and on device
+buffers (nbh_mul_dev/nbh_sqr_dev), beside the oracle on one host core;
+limb-products/s counts the 8192 result limbs of each call.
 workload "shortprod": 1024 low short products k x k -> k limbs (mullo!/sqrlo!,
 src/mul.jl:305-378), k log-uniform in [224, 65536] where the reference falls
 back to a full product, 25% sqrlo, in one planned batch, against the same
@@ -63,7 +68,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", choices=["batch", "large", "hgcd", "spectra", "powermod", "serve", "small",
-                                           "shortprod"],
+                                           "shortprod", "single"],
                     default="batch")
     ap.add_argument("--bases", type=int, default=64, help="powermod: bases sharing the exponent and modulus")
     ap.add_argument("--exp-bits", type=int, default=64, help="powermod: exponent bits")
@@ -177,6 +182,8 @@ def main():
         return bench_powermod(args, D, engine, wl, info, stream)
     if args.workload == "shortprod":
         return bench_shortprod(args, D, engine, wl, info, stream)
+    if args.workload == "single":
+        return bench_single(args, D, engine, wl, info, stream)
 
     # ---- build the job list --------------------------------------------------
     if args.workload == "batch":
@@ -622,6 +629,86 @@ def bench_spectra(args, D, engine, wl, info, stream):
     BP.close()
     S_mu.close()
     S_m.close()
+
+
+def bench_single(args, D, engine, wl, info, stream):
+    """One product at a time: the drop-in host path, the device path and the
+    oracle on one core, for a 4096 x 4096 product and a 4096-limb square."""
+    import torch
+    n = 4096
+    a, b = wl.operand("single", 0, 0, n), wl.operand("single", 0, 1, n)
+    reps = max(args.steps, 1) * 20
+    r = np.empty(2 * n, dtype=np.uint64)
+    res = {}
+    for name, call in (("mul", lambda: engine.mul_fpntt2(r, 0, a, 0, n, b, 0, n)),
+                       ("sqr", lambda: engine.sqr_fpntt2(r, 0, a, 0, n))):
+        for _ in range(max(args.warmup, 1) * 5):
+            call()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            call()
+        res[name + "_host_us"] = (time.perf_counter() - t0) / reps * 1e6
+    host_mul = r.copy()
+    engine.mul_fpntt2(host_mul, 0, a, 0, n, b, 0, n)
+    da, db = D.to_dev(a), D.to_dev(b)
+    dr = D.empty_limbs(2 * n)
+    with torch.cuda.stream(stream):
+        for name, call in (("mul", lambda: D.mul_dev(dr, da, db, 64 * n, 64 * n)),
+                           ("sqr", lambda: D.sqr_dev(dr, da, 64 * n))):
+            for _ in range(max(args.warmup, 1) * 5):
+                call()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(reps):
+                call()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            res[name + "_dev_us"] = e0.elapsed_time(e1) / reps * 1e3
+    with torch.cuda.stream(stream):
+        D.mul_dev(dr, da, db, 64 * n, 64 * n)
+    torch.cuda.synchronize()
+    dev_mul = D.to_host(dr)
+    verified = None
+    if not args.no_verify:
+        from tests import gmp_ref
+        want = gmp_ref.mul(a, b)
+        sq = np.empty(2 * n, dtype=np.uint64)
+        engine.sqr_fpntt2(sq, 0, a, 0, n)
+        verified = bool(np.array_equal(host_mul, want) and np.array_equal(dev_mul, want)
+                        and np.array_equal(sq, gmp_ref.sqr(a)))
+        log(f"verified: product (host and device paths) and square equal to libgmp: {verified}")
+        if not verified:
+            print(json.dumps({"error": "single product differs"}))
+            sys.exit(1)
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        O.mul(a, b)
+        creps = 20
+        c0 = time.perf_counter()
+        for _ in range(creps):
+            O.mul(a, b)
+        cmul = (time.perf_counter() - c0) / creps
+        cpu = {"value": 2 * n / cmul, "unit": "limb-products/s", "cores": 1, "kind": "port",
+               "sample": f"{creps} products of 4096 x 4096 limbs on one host core, C restatement of "
+                         f"src/fpntt.jl (oracle/), {cmul * 1e6:.0f} us each"}
+        log(f"cpu (oracle, one core): {cmul * 1e6:.0f} us per 4096 x 4096 product")
+    log(f"single 4096 x 4096: host path {res['mul_host_us']:.0f} us, device path {res['mul_dev_us']:.0f} us; "
+        f"square: host {res['sqr_host_us']:.0f} us, device {res['sqr_dev_us']:.0f} us")
+    value = 2 * n / (res["mul_host_us"] * 1e-6)
+    out = {
+        "metric": "bit-exact limb-products/s", "value": value, "unit": "limb-products/s", "n_gpus": 1,
+        "steps": reps, "warmup": args.warmup, "ms_per_step": res["mul_host_us"] * 1e-3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (SplitMix64 uniform limbs, nonzero top limb)",
+        "config": {"workload": "single: one 4096 x 4096 product per call through nbh_mul (host buffers, "
+                               "copies included)", "parallelism": "single GPU"},
+        "latency_us": {k: round(v, 1) for k, v in res.items() if k.endswith("_us")},
+        "verified_bit_exact": verified,
+        "cpu_baseline": cpu,
+        "device": info,
+    }
+    print(json.dumps(out))
 
 
 def bench_shortprod(args, D, engine, wl, info, stream):
