@@ -78,6 +78,8 @@ def parse():
     ap.add_argument("--requests", type=int, default=20000, help="serve: requests per offered load")
     ap.add_argument("--budget-gib", type=float, default=8.0, help="serve: HBM budget")
     ap.add_argument("--jobs", type=int, default=4096)
+    ap.add_argument("--single-sizes", default="152,256,512,1024,2048,4096,8192,16384,65536",
+                    help="single: balanced sizes (limbs) of the drop-in latency sweep")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-one-per-launch", action="store_true")
@@ -695,6 +697,33 @@ def bench_single(args, D, engine, wl, info, stream):
         log(f"cpu (oracle, one core): {cmul * 1e6:.0f} us per 4096 x 4096 product")
     log(f"single 4096 x 4096: host path {res['mul_host_us']:.0f} us, device path {res['mul_dev_us']:.0f} us; "
         f"square: host {res['sqr_host_us']:.0f} us, device {res['sqr_dev_us']:.0f} us")
+    # drop-in latency against the oracle on one core across sizes: where the
+    # GPU call starts to pay for a caller that multiplies one product at a time
+    sweep = []
+    if not args.no_cpu:
+        from oracle import oracle as O
+        for k, sz in enumerate(int(v) for v in args.single_sizes.split(",") if v):
+            x, y = wl.operand("single", 1 + k, 0, sz), wl.operand("single", 1 + k, 1, sz)
+            rr = np.empty(2 * sz, dtype=np.uint64)
+            engine.mul_fpntt2(rr, 0, x, 0, sz, y, 0, sz)
+            nrep = max(3, min(200, int(2e7 // (sz * 64))))
+            t0 = time.perf_counter()
+            for _ in range(nrep):
+                engine.mul_fpntt2(rr, 0, x, 0, sz, y, 0, sz)
+            gpu_us = (time.perf_counter() - t0) / nrep * 1e6
+            O.mul(x, y)
+            t0 = time.perf_counter()
+            for _ in range(nrep):
+                ref = O.mul(x, y)
+            cpu_us = (time.perf_counter() - t0) / nrep * 1e6
+            ok = bool(np.array_equal(rr, ref))
+            sweep.append({"limbs": sz, "drop_in_us": round(gpu_us, 1), "oracle_1core_us": round(cpu_us, 1),
+                          "equal": ok})
+            log(f"  {sz:>6} x {sz:<6} drop-in {gpu_us:9.1f} us   oracle (1 core) {cpu_us:10.1f} us   "
+                f"x{cpu_us / gpu_us:5.2f}{'' if ok else '   DIFFERENT'}")
+            if not ok:
+                print(json.dumps({"error": f"sweep product at {sz} limbs differs"}))
+                sys.exit(1)
     value = 2 * n / (res["mul_host_us"] * 1e-6)
     out = {
         "metric": "bit-exact limb-products/s", "value": value, "unit": "limb-products/s", "n_gpus": 1,
@@ -704,6 +733,7 @@ def bench_single(args, D, engine, wl, info, stream):
         "config": {"workload": "single: one 4096 x 4096 product per call through nbh_mul (host buffers, "
                                "copies included)", "parallelism": "single GPU"},
         "latency_us": {k: round(v, 1) for k, v in res.items() if k.endswith("_us")},
+        "drop_in_sweep": sweep,
         "verified_bit_exact": verified,
         "cpu_baseline": cpu,
         "device": info,

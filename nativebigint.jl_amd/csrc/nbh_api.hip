@@ -228,6 +228,8 @@ struct Lib {
     size_t ws_size = 0;
     void *io = nullptr;  // host-pointer API staging of a, b, r
     size_t io_size = 0;
+    void *pin = nullptr;  // pinned host bounce buffer of the host-pointer API
+    size_t pin_size = 0;
     DevJob *slots = nullptr;
     int slot_next = 0;
     hipStream_t side[8] = {};  // batch phases fork onto these (default priority)
@@ -528,6 +530,23 @@ int get_plan(int64_t N, int k, PlanEntry **out) {
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// Host-pointer products at most this large (operands + result) go through
+// the pinned bounce buffer: a host memcpy plus a DMA from pinned memory beats
+// a copy from pageable memory at these sizes; larger ones copy directly.
+constexpr size_t PIN_MAX_BYTES = (size_t)64 << 20;
+
+int ensure_pinned(size_t need) {
+    if (g.pin_size >= need) return NBH_OK;
+    HIPCHK(hipStreamSynchronize(g.stream));
+    if (g.pin) HIPCHK(hipHostFree(g.pin));
+    g.pin = nullptr;
+    g.pin_size = 0;
+    const size_t sz = std::max(need, (size_t)1 << 20);
+    HIPCHK(hipHostMalloc(&g.pin, sz, hipHostMallocDefault));
+    g.pin_size = sz;
+    return NBH_OK;
+}
+
 int ensure_buf(void **buf, size_t *cur, size_t need) {
     if (*cur >= need) return NBH_OK;
     HIPCHK(hipStreamSynchronize(g.stream));
@@ -750,12 +769,29 @@ int host_product(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, i
     uint64_t *da = (uint64_t *)g.io;
     uint64_t *db = (uint64_t *)((char *)g.io + sa);
     uint64_t *dr = (uint64_t *)((char *)g.io + sa + sb);
-    HIPCHK(hipMemcpyAsync(da, a, 8 * (size_t)m, hipMemcpyHostToDevice, g.stream));
-    if (!sqr) HIPCHK(hipMemcpyAsync(db, b, 8 * (size_t)n, hipMemcpyHostToDevice, g.stream));
+    const bool pinned = sa + sb + sr <= PIN_MAX_BYTES;
+    if (pinned) {
+        // operands into the pinned buffer with the device layout, one DMA in, one out
+        RCCHK(ensure_pinned(sa + sb + sr));
+        char *p = (char *)g.pin;
+        std::memcpy(p, a, 8 * (size_t)m);
+        if (!sqr) std::memcpy(p + sa, b, 8 * (size_t)n);
+        HIPCHK(hipMemcpyAsync(da, p, sa + sb, hipMemcpyHostToDevice, g.stream));
+    } else {
+        HIPCHK(hipMemcpyAsync(da, a, 8 * (size_t)m, hipMemcpyHostToDevice, g.stream));
+        if (!sqr) HIPCHK(hipMemcpyAsync(db, b, 8 * (size_t)n, hipMemcpyHostToDevice, g.stream));
+    }
     if (keep) RCCHK(run_lo(dr, da, m, sqr ? nullptr : db, n, keep, bits_a, bits_b, sqr));
     else RCCHK(run_single(dr, da, m, sqr ? nullptr : db, n, bits_a, bits_b, sqr));
-    HIPCHK(hipMemcpyAsync(r, dr, 8 * (size_t)rl, hipMemcpyDeviceToHost, g.stream));
-    HIPCHK(hipStreamSynchronize(g.stream));
+    if (pinned) {
+        char *pr = (char *)g.pin + sa + sb;
+        HIPCHK(hipMemcpyAsync(pr, dr, 8 * (size_t)rl, hipMemcpyDeviceToHost, g.stream));
+        HIPCHK(hipStreamSynchronize(g.stream));
+        std::memcpy(r, pr, 8 * (size_t)rl);
+    } else {
+        HIPCHK(hipMemcpyAsync(r, dr, 8 * (size_t)rl, hipMemcpyDeviceToHost, g.stream));
+        HIPCHK(hipStreamSynchronize(g.stream));
+    }
     return NBH_OK;
 }
 
@@ -1260,9 +1296,11 @@ void nbh_shutdown(void) {
     g.stages.clear();
     if (g.ws) (void)hipFree(g.ws);
     if (g.io) (void)hipFree(g.io);
+    if (g.pin) (void)hipHostFree(g.pin);
     if (g.slots) (void)hipFree(g.slots);
     g.ws = g.io = nullptr;
-    g.ws_size = g.io_size = 0;
+    g.pin = nullptr;
+    g.ws_size = g.io_size = g.pin_size = 0;
     g.slots = nullptr;
     for (int i = 0; i < NSIDE; i++) (void)hipStreamDestroy(g.side[i]);
     (void)hipEventDestroy(g.ev_in);
