@@ -194,7 +194,27 @@ class PowermodBatch:
                     self.red_acc.run(stream)
         return self.acc
 
+    def capture(self) -> None:
+        """Record one whole run (table set-up, every square, multiply and
+        reduction, the accumulator loads) as a single HIP graph: a run is
+        hundreds of short launches, and replay() submits them at once.  The
+        graph holds kernels, events and device copies only."""
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        self.run(s)                     # eager once: every plan and table exists before capture
+        s.synchronize()
+        self._graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self._graph, stream=s):
+            self.run(s)
+        torch.cuda.current_stream().wait_stream(s)
+
+    def replay(self) -> list:
+        """The captured run on torch's current stream (capture() first)."""
+        self._graph.replay()
+        return self.acc
+
     def close(self) -> None:
+        self._graph = None
         for b in [self.sq_acc, self.red_acc, *self.mul_tab, *self._setup]:
             b.close()
         self.ctx.close()

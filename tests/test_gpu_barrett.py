@@ -92,6 +92,35 @@ def test_powermod_batch(k, nbits, count):
     P.close()
 
 
+@pytest.mark.parametrize("k,nbits,count", [(200, 40, 3), (1024, 70, 4)])
+def test_powermod_graph_replays(k, nbits, count):
+    """The whole exponentiation captured as one graph: every replay, with
+    eager device work and a cleared accumulator between replays, gives the
+    eager run's limbs and pow()'s value."""
+    rng = np.random.default_rng(2000 + k)
+    m = _modulus(rng, k)
+    M = ints(m)
+    e = (1 << (nbits - 1)) | (int.from_bytes(rng.bytes(16), "little") % (1 << (nbits - 1)))
+    bases = [wl.int_to_limbs(max(2, int.from_bytes(rng.bytes(8 * k), "little") % M), k) for _ in range(count)]
+    P = BR.PowermodBatch(m, bases, e)
+    P.run()
+    torch.cuda.synchronize()
+    eager = [D.to_host(r) for r in P.acc]
+    for b, r in zip(bases, eager):
+        assert ints(r) == pow(ints(b), e, M)
+    P.capture()
+    for rep in range(3):
+        for r in P.acc:
+            r.fill_(-1)
+        x = torch.arange(1 << 16, device="cuda", dtype=torch.int64).sum()   # eager work between replays
+        out = P.replay()
+        torch.cuda.synchronize()
+        assert int(x) == (1 << 16) * ((1 << 16) - 1) // 2
+        for want, r in zip(eager, out):
+            assert np.array_equal(D.to_host(r), want), rep
+    P.close()
+
+
 def test_barrett_arguments_are_checked():
     import ctypes
     from nbhip._lib import NBH_ERR_ARG, lib
