@@ -926,7 +926,7 @@ __device__ __forceinline__ void store_tile(double *__restrict__ x, const double 
 // ---------------------------------------------------------------------------
 // Group pass: one tile of one group of one transform, forward or inverse.
 
-template <int K>
+template <int K, int NU>
 __device__ __forceinline__ void group_body(const DevJob *__restrict__ Jp, const Tile &t, bool fwd, double *lds) {
     const auto *J = cst(Jp);
     const DevPlan *__restrict__ P = unip(J->plan[K]);
@@ -953,7 +953,7 @@ __device__ __forceinline__ void group_body(const DevJob *__restrict__ Jp, const 
     src.n = uni64(t.op ? J->n : J->m);
     src.nch = uni64(t.op ? J->ncb : J->nca);
     src.b = uni(J->bch);
-    load_tile<LU>(lds, src, (t.flags & TILE_PACK) != 0, total, g, gbase);
+    load_tile<NU>(lds, src, (t.flags & TILE_PACK) != 0, total, g, gbase);
     __syncthreads();
     if (fwd) run_stages<K, true>(lds, nullptr, 1, P, s0, s1, g);
     else run_stages<K, false>(lds, nullptr, 1, P, s0, s1, g);
@@ -962,26 +962,27 @@ __device__ __forceinline__ void group_body(const DevJob *__restrict__ Jp, const 
 
 // Forward and inverse group passes are separate kernels (one code path each,
 // and separate names in a kernel trace).
-template <bool FWD>
+// NU: points each thread loads per round (see load_tile).
+template <bool FWD, int NU>
 __device__ __forceinline__ void group_kernel(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles,
                                              const TileGen &gen, double *lds) {
     const Tile t = fetch_tile(tiles, gen);
     const DevJob *J = jobs + t.job;
     if (uni(cst(J)->zero)) return;
-    if (t.prime == 0) group_body<0>(J, t, FWD, lds);
-    else group_body<1>(J, t, FWD, lds);
+    if (t.prime == 0) group_body<0, NU>(J, t, FWD, lds);
+    else group_body<1, NU>(J, t, FWD, lds);
 }
 
 __global__ void __launch_bounds__(1024) k_group_fwd(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles,
                                                     TileGen gen) {
     extern __shared__ double lds[];
-    group_kernel<true>(jobs, tiles, gen, lds);
+    group_kernel<true, LU>(jobs, tiles, gen, lds);
 }
 
 __global__ void __launch_bounds__(1024) k_group_inv(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles,
                                                     TileGen gen) {
     extern __shared__ double lds[];
-    group_kernel<false>(jobs, tiles, gen, lds);
+    group_kernel<false, LU>(jobs, tiles, gen, lds);
 }
 
 // The same kernels held to six waves per SIMD (at most 80 VGPRs, no spills)
@@ -989,16 +990,18 @@ __global__ void __launch_bounds__(1024) k_group_inv(const DevJob *__restrict__ j
 // waves per SIMD of the unrestricted build fit on a CU: those are bound by
 // memory latency, and two more waves per SIMD hide more of it.  Launches of
 // large tiles are LDS-bound at four waves and keep the unrestricted build.
+// With more waves in flight, rounds of 8 loads per thread (instead of 16)
+// measured faster here, and slower in the large-tile build.
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
 k_group_fwd_w6(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles, TileGen gen) {
     extern __shared__ double lds[];
-    group_kernel<true>(jobs, tiles, gen, lds);
+    group_kernel<true, LU / 2>(jobs, tiles, gen, lds);
 }
 
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
 k_group_inv_w6(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles, TileGen gen) {
     extern __shared__ double lds[];
-    group_kernel<false>(jobs, tiles, gen, lds);
+    group_kernel<false, LU / 2>(jobs, tiles, gen, lds);
 }
 
 // ---------------------------------------------------------------------------
