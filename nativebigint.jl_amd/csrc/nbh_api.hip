@@ -564,7 +564,7 @@ int64_t bits_from_top(int64_t n, uint64_t top) { return 64 * (n - 1) + top_set_b
 // Fill a DevJob's sizing fields; returns NBH_OK with J.zero set for a zero operand.
 int size_job(DevJob &J, int64_t bits_a, int64_t bits_b, int64_t *N_out) {
     J.rn = J.m + J.n;
-    J.nchunks = (J.rn + UNPACK_LPC - 1) / UNPACK_LPC;
+    J.nchunks = (J.rn + UNPACK_CHUNK - 1) / UNPACK_CHUNK;
     if (bits_a == 0 || bits_b == 0) {
         J.zero = 1;
         *N_out = 0;
@@ -587,7 +587,7 @@ int size_job(DevJob &J, int64_t bits_a, int64_t bits_b, int64_t *N_out) {
 // and length serve every a of up to bits_a_max bits.
 int size_job_spec(DevJob &J, int64_t bits_a, const nbh_spectrum_s &S, int64_t *N_out) {
     J.rn = J.m + J.n;
-    J.nchunks = (J.rn + UNPACK_LPC - 1) / UNPACK_LPC;
+    J.nchunks = (J.rn + UNPACK_CHUNK - 1) / UNPACK_CHUNK;
     if (bits_a > S.bits_a_max) return fail(NBH_ERR_ARG, "operand longer than the spectrum's bits_a_max");
     if (bits_a == 0 || S.zero) {
         J.zero = 1;
@@ -609,7 +609,7 @@ int size_job_spec(DevJob &J, int64_t bits_a, const nbh_spectrum_s &S, int64_t *N
 void keep_limbs(DevJob &J, int64_t keep) {
     if (keep <= 0) return;
     J.rn = keep;
-    J.nchunks = (keep + UNPACK_LPC - 1) / UNPACK_LPC;
+    J.nchunks = (keep + UNPACK_CHUNK - 1) / UNPACK_CHUNK;
 }
 
 // The products mul!/sqr! keep out of the NTT engine (Karatsuba and
@@ -1804,7 +1804,7 @@ int32_t nbh_unpack_dev(uint64_t *r, int64_t rn, const double *x1, const double *
     J.nconv = nconv;
     J.bch = b;
     J.bmagic = ~0ull / (uint64_t)b + 1;
-    J.nchunks = (rn + UNPACK_LPC - 1) / UNPACK_LPC;
+    J.nchunks = (rn + UNPACK_CHUNK - 1) / UNPACK_CHUNK;
     J.plan[0] = P[0]->dev;
     J.plan[1] = P[1]->dev;
     J.xa[0] = const_cast<double *>(x1);

@@ -104,7 +104,11 @@ NBH_HDI constexpr int lds_doubles(int n) { return n + (n >> 4) + 1; }
 // Unpack geometry: each thread owns UNPACK_LPT consecutive output limbs.
 constexpr int UNPACK_THREADS = 256;
 constexpr int UNPACK_LPT = 4;
-constexpr int UNPACK_LPC = UNPACK_THREADS * UNPACK_LPT;  // limbs per chunk
+constexpr int UNPACK_LPC = UNPACK_THREADS * UNPACK_LPT;  // limbs per block-wide pass (combine, basecase, Barrett)
+// Limbs per unpack chunk: thread 0 of an unpack block takes the 4 limbs just
+// below the chunk (the previous chunk's top segment, for its spill only) and
+// threads 1..255 the chunk's own limbs.
+constexpr int UNPACK_CHUNK = (UNPACK_THREADS - 1) * UNPACK_LPT;
 // Garner values staged per chunk: (LPC + LPT) * 64 / b + 2 with b >= 32, the
 // smallest chunk width of any product whose spectra fit in device memory.
 constexpr int UNPACK_MIN_B = 32;

@@ -1171,7 +1171,10 @@ __device__ __forceinline__ Seg segment_sum(const UnpackU &U, int64_t k0, int64_t
 
 // Each chunk first stages the Garner values of every coefficient starting in
 // its limbs (and in the 4 limbs before, for the first thread's spill) in LDS:
-// coalesced reads of both spectra, each coefficient computed once.
+// coalesced reads of both spectra, each coefficient computed once.  Thread t
+// sums the segment of limbs [L0 + 4(t-1), L0 + 4t): thread 0's segment is the
+// previous chunk's top one, summed only for the spill it passes to thread 1,
+// so every thread sums exactly one segment.
 __global__ void __launch_bounds__(UNPACK_THREADS) k_unpack(const DevJob *__restrict__ jobs,
                                                            const Tile *__restrict__ tiles, TileGen gen) {
     __shared__ uint64_t clo[UNPACK_STAGE], chi[UNPACK_STAGE];
@@ -1181,21 +1184,23 @@ __global__ void __launch_bounds__(UNPACK_THREADS) k_unpack(const DevJob *__restr
     const DevJob *__restrict__ Jp = jobs + t.job;
     const auto *J = cst(Jp);
     const int tid = threadIdx.x;
-    const int64_t L0 = (int64_t)t.tile * UNPACK_LPC;
-    const int64_t k0 = L0 + UNPACK_LPT * tid;
+    const int64_t L0 = (int64_t)t.tile * UNPACK_CHUNK;
+    const int64_t k0 = L0 + UNPACK_LPT * (tid - 1);
+    const bool own = tid > 0;  // thread 0's limbs belong to the previous chunk
     uint64_t *__restrict__ r = unip(J->r);
     const int64_t rn = uni64(J->rn);
     uint32_t *__restrict__ flags = unip(J->chunk_flags);
     if (uni(J->zero)) {
-        for (int w = 0; w < UNPACK_LPT; w++)
-            if (k0 + w < rn) r[k0 + w] = 0;
+        if (own)
+            for (int w = 0; w < UNPACK_LPT; w++)
+                if (k0 + w < rn) r[k0 + w] = 0;
         if (tid == 0) flags[t.tile] = 0;
         return;
     }
     const UnpackU U = unpack_u(Jp);
     const int64_t kstart = L0 >= UNPACK_LPT ? L0 - UNPACK_LPT : 0;
     const int64_t i_first = (int64_t)__umul64hi((uint64_t)(64 * kstart + U.b - 1), U.bmagic);
-    const int64_t i_end = (int64_t)__umul64hi((uint64_t)(64 * (L0 + UNPACK_LPC) + U.b - 1), U.bmagic);
+    const int64_t i_end = (int64_t)__umul64hi((uint64_t)(64 * (L0 + UNPACK_CHUNK) + U.b - 1), U.bmagic);
     const int cnt = (int)(i_end - i_first);
     for (int l0 = tid; l0 < cnt; l0 += GU * UNPACK_THREADS) {
         double a1[GU], a2[GU];
@@ -1220,17 +1225,13 @@ __global__ void __launch_bounds__(UNPACK_THREADS) k_unpack(const DevJob *__restr
         }
     }
     __syncthreads();
-    const Seg sg = segment_sum(U, k0, i_first, clo, chi);
+    // chunk 0 has nothing below it: thread 0's segment is empty there
+    const Seg sg = k0 >= 0 ? segment_sum(U, k0, i_first, clo, chi) : Seg{0, 0, 0, 0, 0, 0};
     spill0[tid] = sg.s0;
     spill1[tid] = sg.s1;
-    uint64_t p0 = 0, p1 = 0;
-    if (tid == 0 && k0 >= UNPACK_LPT) {
-        const Seg pv = segment_sum(U, k0 - UNPACK_LPT, i_first, clo, chi);
-        p0 = pv.s0;
-        p1 = pv.s1;
-    }
     __syncthreads();
-    if (tid > 0) {
+    uint64_t p0 = 0, p1 = 0;
+    if (own) {
         p0 = spill0[tid - 1];
         p1 = spill1[tid - 1];
     }
@@ -1245,8 +1246,9 @@ __global__ void __launch_bounds__(UNPACK_THREADS) k_unpack(const DevJob *__restr
     c = q2 < c1;
     uint64_t q3 = sg.o3 + c;
     c = q3 < c;
-    const bool G = c != 0;
-    const bool Pp = (q0 & q1 & q2 & q3) == ~0ull;
+    // thread 0 neither generates nor stops a carry into the chunk
+    const bool G = own && c != 0;
+    const bool Pp = !own || (q0 & q1 & q2 & q3) == ~0ull;
 
     const int lane = tid & 63, wid = tid >> 6;
     const uint64_t gm = __ballot(G);
@@ -1272,10 +1274,12 @@ __global__ void __launch_bounds__(UNPACK_THREADS) k_unpack(const DevJob *__restr
     q2 += c;
     c = q2 < c;
     q3 += c;
-    if (k0 + 0 < rn) r[k0 + 0] = q0;
-    if (k0 + 1 < rn) r[k0 + 1] = q1;
-    if (k0 + 2 < rn) r[k0 + 2] = q2;
-    if (k0 + 3 < rn) r[k0 + 3] = q3;
+    if (own) {
+        if (k0 + 0 < rn) r[k0 + 0] = q0;
+        if (k0 + 1 < rn) r[k0 + 1] = q1;
+        if (k0 + 2 < rn) r[k0 + 2] = q2;
+        if (k0 + 3 < rn) r[k0 + 3] = q3;
+    }
     if (tid == UNPACK_THREADS - 1) {
         // carry out of the chunk with carry-in 0, and whether all limbs propagate
         const uint32_t cg = (G || (Pp && my)) ? CHUNK_G : 0u;
@@ -1303,21 +1307,25 @@ __global__ void __launch_bounds__(UNPACK_THREADS) k_carry_fix(const DevJob *__re
     }
     if (!cin) return;
     const int tid = threadIdx.x;
-    const int64_t k0 = (int64_t)t.tile * UNPACK_LPC + UNPACK_LPT * tid;
+    // the unpack's layout: thread t >= 1 owns limbs [L0 + 4(t-1), L0 + 4t), thread 0 none
+    const int64_t k0 = (int64_t)t.tile * UNPACK_CHUNK + UNPACK_LPT * (tid - 1);
+    const bool own = tid > 0;
     uint64_t *__restrict__ r = unip(J->r);
     const int64_t rn = uni64(J->rn);
     uint64_t v[UNPACK_LPT];
     bool ones = true;
 #pragma unroll
     for (int w = 0; w < UNPACK_LPT; w++) {
-        v[w] = (k0 + w < rn) ? r[k0 + w] : 0;
-        ones = ones && (k0 + w < rn) && v[w] == ~0ull;
+        v[w] = (own && k0 + w < rn) ? r[k0 + w] : 0;
+        ones = ones && (!own || ((k0 + w < rn) && v[w] == ~0ull));
     }
     if (tid == 0) first = UNPACK_THREADS;
     __syncthreads();
     if (!ones) atomicMin(&first, tid);
     __syncthreads();
-    if (tid < first) {
+    if (!own) {
+        // thread 0 owns no limbs (and is never the first that stops the carry)
+    } else if (tid < first) {
 #pragma unroll
         for (int w = 0; w < UNPACK_LPT; w++)
             if (k0 + w < rn) r[k0 + w] = 0;
