@@ -703,7 +703,7 @@ int run_single(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int
         ProfScope ps(3, g.stream);
         HIPCHK(launch_unpack(slot, nullptr, ug, J.nchunks, g.stream));
     }
-    {
+    if (J.nchunks > 1) {  // one chunk has no carry to pass on
         ProfScope ps(4, g.stream);
         HIPCHK(launch_carry_fix(slot, nullptr, ug, J.nchunks, g.stream));
     }
@@ -1041,9 +1041,13 @@ int batch_plan(const JobIn *jobs, int64_t count, nbh_batch_s **out, size_t *need
         }
         push_phase(0, 0, list);
     }
-    for (int64_t i = 0; i < count; i++)
-        list.push_back({0, UNPACK_THREADS, Tile{(uint32_t)i, 0, 0, 0, 0, 0, 0}, (uint32_t)J[(size_t)i].nchunks});
-    std::vector<TileReq> list2 = list;
+    std::vector<TileReq> list2;
+    for (int64_t i = 0; i < count; i++) {
+        const uint32_t nch = (uint32_t)J[(size_t)i].nchunks;
+        list.push_back({0, UNPACK_THREADS, Tile{(uint32_t)i, 0, 0, 0, 0, 0, 0}, nch});
+        // carries pass into chunks 1.. only; a batch of one-chunk products has no carry launch
+        if (nch > 1) list2.push_back({0, UNPACK_THREADS, Tile{(uint32_t)i, 0, 0, 0, 0, 1, 0}, nch - 1});
+    }
     push_phase(2, 0, list);
     push_phase(3, 0, list2);
     // the schoolbook products: one launch, beside the first NTT step
