@@ -1,0 +1,34 @@
+"""The shipped schedule table (tuning_gfx950.txt beside libnbh.so) is one the
+library accepts: 'Nmax grp_lds mid_max cmin cmax' per line, Nmax increasing,
+every knob a power of two, cmin <= cmax, tiles within a gfx950 block's LDS
+(160 KiB: 8 B per point plus one pad double per 16)."""
+import os
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+TABLE = os.path.join(ROOT, "nativebigint.jl_amd", "tuning_gfx950.txt")
+LDS = 160 * 1024
+
+
+def _pow2(x):
+    return x > 0 and x & (x - 1) == 0
+
+
+def test_table_is_well_formed():
+    if not os.path.exists(TABLE):
+        pytest.skip("no table shipped")
+    rows = []
+    for line in open(TABLE):
+        if not line.strip() or line.startswith("#"):
+            continue
+        v = [int(x) for x in line.split()]
+        assert len(v) == 5, line
+        rows.append(v)
+    assert rows
+    assert [r[0] for r in rows] == sorted({r[0] for r in rows})
+    for nmax, grp, mid, cmin, cmax in rows:
+        assert all(_pow2(x) for x in (grp, mid, cmin, cmax)), (nmax, grp, mid, cmin, cmax)
+        assert cmin <= cmax
+        for pts in (grp, mid):
+            assert 8 * (pts + pts // 16 + 1) <= LDS
