@@ -701,7 +701,7 @@ int run_single(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int
     TileGen ug{0, 1, J.nchunks, 0, 0};
     {
         ProfScope ps(3, g.stream);
-        HIPCHK(launch_unpack(slot, nullptr, ug, J.nchunks, g.stream));
+        HIPCHK(launch_unpack(slot, nullptr, ug, J.nchunks, J.bch, g.stream));
     }
     if (J.nchunks > 1) {  // one chunk has no carry to pass on
         ProfScope ps(4, g.stream);
@@ -1000,7 +1000,7 @@ int batch_plan(const JobIn *jobs, int64_t count, nbh_batch_s **out, size_t *need
                 ph.count += list[i].count;
                 i++;
             }
-            if (kind >= 2) ph.lds = 0;
+            if (kind >= 3) ph.lds = 0;
             B->phases.push_back(ph);
         }
         list.clear();
@@ -1042,9 +1042,14 @@ int batch_plan(const JobIn *jobs, int64_t count, nbh_batch_s **out, size_t *need
         push_phase(0, 0, list);
     }
     std::vector<TileReq> list2;
+    // one unpack launch: its field `lds` carries the smallest chunk width (zero
+    // products and schoolbook jobs have none)
+    int bmin = 64;
+    for (int64_t i = 0; i < count; i++)
+        if (!J[(size_t)i].zero) bmin = std::min(bmin, (int)J[(size_t)i].bch);
     for (int64_t i = 0; i < count; i++) {
         const uint32_t nch = (uint32_t)J[(size_t)i].nchunks;
-        list.push_back({0, UNPACK_THREADS, Tile{(uint32_t)i, 0, 0, 0, 0, 0, 0}, nch});
+        list.push_back({(size_t)bmin, UNPACK_THREADS, Tile{(uint32_t)i, 0, 0, 0, 0, 0, 0}, nch});
         // carries pass into chunks 1.. only; a batch of one-chunk products has no carry launch
         if (nch > 1) list2.push_back({0, UNPACK_THREADS, Tile{(uint32_t)i, 0, 0, 0, 0, 1, 0}, nch - 1});
     }
@@ -1140,7 +1145,7 @@ int launch_phase(nbh_batch_s *B, const nbh_batch_s::Phase &ph, hipStream_t s) {
     case 5: HIPCHK(launch_basecase(B->dbase, ph.count, s)); break;
     case 0: HIPCHK(launch_group(B->djobs, t, none, ph.count, ph.fwd, ph.lds, ph.threads, s)); break;
     case 1: HIPCHK(launch_mid(B->djobs, t, none, ph.count, ph.lds, ph.threads, s)); break;
-    case 2: HIPCHK(launch_unpack(B->djobs, t, none, ph.count, s)); break;
+    case 2: HIPCHK(launch_unpack(B->djobs, t, none, ph.count, (int)ph.lds, s)); break;  // lds: smallest b
     default: HIPCHK(launch_carry_fix(B->djobs, t, none, ph.count, s)); break;
     }
     return NBH_OK;
@@ -1818,7 +1823,7 @@ int32_t nbh_unpack_dev(uint64_t *r, int64_t rn, const double *x1, const double *
     DevJob *slot = g.slots + (g.slot_next++ % NSLOTS);
     HIPCHK(launch_store_job(J, slot, g.stream));
     TileGen ug{0, 1, J.nchunks, 0, 0};
-    HIPCHK(launch_unpack(slot, nullptr, ug, J.nchunks, g.stream));
+    HIPCHK(launch_unpack(slot, nullptr, ug, J.nchunks, b, g.stream));
     HIPCHK(launch_carry_fix(slot, nullptr, ug, J.nchunks, g.stream));
     return stream_leave(stream);
 }

@@ -113,6 +113,11 @@ constexpr int UNPACK_CHUNK = (UNPACK_THREADS - 1) * UNPACK_LPT;
 // smallest chunk width of any product whose spectra fit in device memory.
 constexpr int UNPACK_MIN_B = 32;
 constexpr int UNPACK_STAGE = (UNPACK_LPC + UNPACK_LPT) * 64 / UNPACK_MIN_B + 2;
+// Staged coefficients of one unpack chunk at chunk width b (>= the count any
+// chunk starts, 64 (UNPACK_CHUNK + 4) / b + 1); the launch's LDS holds two
+// arrays of this many words for the smallest b among its jobs.
+NBH_HDI constexpr int unpack_stage(int b) { return (UNPACK_CHUNK + UNPACK_LPT) * 64 / b + 2; }
+NBH_HDI constexpr size_t unpack_lds_bytes(int bmin) { return 2 * sizeof(uint64_t) * (size_t)unpack_stage(bmin); }
 constexpr int GU = 8;  // coefficients per thread per staging round
 
 constexpr uint32_t CHUNK_G = 1;  // carry out with carry-in 0

@@ -1175,9 +1175,11 @@ __device__ __forceinline__ Seg segment_sum(const UnpackU &U, int64_t k0, int64_t
 // sums the segment of limbs [L0 + 4(t-1), L0 + 4t): thread 0's segment is the
 // previous chunk's top one, summed only for the spill it passes to thread 1,
 // so every thread sums exactly one segment.
-__global__ void __launch_bounds__(UNPACK_THREADS) k_unpack(const DevJob *__restrict__ jobs,
-                                                           const Tile *__restrict__ tiles, TileGen gen) {
-    __shared__ uint64_t clo[UNPACK_STAGE], chi[UNPACK_STAGE];
+// Held to five waves per SIMD (at most 96 VGPRs): with the staging sized to
+// the chunk width, five blocks' LDS fit a CU for b >= 40.
+__global__ void __launch_bounds__(UNPACK_THREADS) __attribute__((amdgpu_waves_per_eu(5)))
+k_unpack(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles, TileGen gen) {
+    extern __shared__ uint64_t ustage[];
     __shared__ uint64_t spill0[UNPACK_THREADS], spill1[UNPACK_THREADS];
     __shared__ uint32_t wsum[UNPACK_THREADS / 64];
     const Tile t = fetch_tile(tiles, gen);
@@ -1198,6 +1200,8 @@ __global__ void __launch_bounds__(UNPACK_THREADS) k_unpack(const DevJob *__restr
         return;
     }
     const UnpackU U = unpack_u(Jp);
+    uint64_t *__restrict__ clo = ustage;
+    uint64_t *__restrict__ chi = ustage + unpack_stage(U.b);  // the launch's LDS fits its smallest b
     const int64_t kstart = L0 >= UNPACK_LPT ? L0 - UNPACK_LPT : 0;
     const int64_t i_first = (int64_t)__umul64hi((uint64_t)(64 * kstart + U.b - 1), U.bmagic);
     const int64_t i_end = (int64_t)__umul64hi((uint64_t)(64 * (L0 + UNPACK_CHUNK) + U.b - 1), U.bmagic);
@@ -1781,9 +1785,11 @@ hipError_t launch_mid(const DevJob *jobs, const Tile *tiles, const TileGen &gen,
     return hipGetLastError();
 }
 
-hipError_t launch_unpack(const DevJob *jobs, const Tile *tiles, const TileGen &gen, int64_t ntiles, hipStream_t s) {
+hipError_t launch_unpack(const DevJob *jobs, const Tile *tiles, const TileGen &gen, int64_t ntiles, int bmin,
+                         hipStream_t s) {
     if (ntiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_unpack, dim3((unsigned)ntiles), dim3(UNPACK_THREADS), 0, s, jobs, tiles, gen);
+    hipLaunchKernelGGL(k_unpack, dim3((unsigned)ntiles), dim3(UNPACK_THREADS), unpack_lds_bytes(bmin < UNPACK_MIN_B ? UNPACK_MIN_B : bmin), s,
+                       jobs, tiles, gen);
     return hipGetLastError();
 }
 

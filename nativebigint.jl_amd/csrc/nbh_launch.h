@@ -20,7 +20,9 @@ hipError_t launch_group(const DevJob *jobs, const Tile *tiles, const TileGen &ge
                         size_t lds_bytes, int threads, hipStream_t s);
 hipError_t launch_mid(const DevJob *jobs, const Tile *tiles, const TileGen &gen, int64_t ntiles, size_t lds_bytes,
                       int threads, hipStream_t s);
-hipError_t launch_unpack(const DevJob *jobs, const Tile *tiles, const TileGen &gen, int64_t ntiles, hipStream_t s);
+// bmin: the smallest chunk width among the launch's jobs (sizes the staging LDS)
+hipError_t launch_unpack(const DevJob *jobs, const Tile *tiles, const TileGen &gen, int64_t ntiles, int bmin,
+                         hipStream_t s);
 hipError_t launch_carry_fix(const DevJob *jobs, const Tile *tiles, const TileGen &gen, int64_t ntiles,
                             hipStream_t s);
 hipError_t launch_store_job(const DevJob &j, DevJob *dst, hipStream_t s);
