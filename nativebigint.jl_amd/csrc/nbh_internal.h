@@ -109,10 +109,8 @@ constexpr int UNPACK_LPC = UNPACK_THREADS * UNPACK_LPT;  // limbs per block-wide
 // below the chunk (the previous chunk's top segment, for its spill only) and
 // threads 1..255 the chunk's own limbs.
 constexpr int UNPACK_CHUNK = (UNPACK_THREADS - 1) * UNPACK_LPT;
-// Garner values staged per chunk: (LPC + LPT) * 64 / b + 2 with b >= 32, the
-// smallest chunk width of any product whose spectra fit in device memory.
+// The smallest chunk width of any product whose spectra fit in device memory.
 constexpr int UNPACK_MIN_B = 32;
-constexpr int UNPACK_STAGE = (UNPACK_LPC + UNPACK_LPT) * 64 / UNPACK_MIN_B + 2;
 // Staged coefficients of one unpack chunk at chunk width b (>= the count any
 // chunk starts, 64 (UNPACK_CHUNK + 4) / b + 1); the launch's LDS holds two
 // arrays of this many words for the smallest b among its jobs.
