@@ -124,6 +124,12 @@ int32_t nbh_batch_create_in(const nbh_job *jobs, int64_t count, void *arena, int
 void nbh_batch_release(nbh_batch b);
 /* One-shot: create, run, synchronise, destroy. */
 int32_t nbh_mul_batch_dev(const nbh_job *jobs, int64_t count, void *stream);
+/* The same on host buffers, synchronous: every job is one mul!/sqr!/mullo!/
+ * sqrlo! call of the reference (src/mul.jl:127-164,305-389, engine
+ * src/fpntt.jl:982,1016) on its Memory{UInt64} limbs.  bits_* are ignored
+ * (read from the top limbs).  Jobs are independent: no r may overlap another
+ * job's operands or result. */
+int32_t nbh_mul_batch(const nbh_job *jobs, int64_t count);
 
 /* Batched 2x2 cofactor step of HGCD (device pointers): per instance a 2x2
  * matrix of mn-limb cofactors (zero-padded) and a 2-vector of n-limb operands.

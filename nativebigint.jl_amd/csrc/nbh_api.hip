@@ -1504,6 +1504,123 @@ int32_t nbh_mul_batch_dev(const nbh_job *jobs, int64_t count, void *stream) {
     return NBH_OK;
 }
 
+// Host-pointer batch.  Jobs are staged through the device I/O buffer in
+// groups of at most PIN_MAX_BYTES (operands + results): per group one host
+// copy of the operands into the pinned buffer, one DMA in, one planned batch,
+// one DMA of the results out and one host copy out.  A job larger than that is
+// a group of its own with direct copies.  Bit lengths come from the operands'
+// top limbs on the host; a product with a zero operand is written on the host.
+int32_t nbh_mul_batch(const nbh_job *jobs, int64_t count) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    if (count < 1 || !jobs) return fail(NBH_ERR_ARG, "empty batch");
+    struct Staged {
+        nbh_job d;          // sizes and bit lengths; pointers set when the group is placed
+        size_t sa, sb, sr;  // aligned bytes of a, b and r in the staging buffer
+        int64_t rl;         // result limbs
+        bool zero;
+    };
+    std::vector<Staged> S((size_t)count);
+    for (int64_t i = 0; i < count; i++) {
+        const nbh_job &q = jobs[i];
+        const bool sqr = q.b == nullptr;
+        if (!q.r || !q.a || q.m < 1 || (!sqr && q.n < 1) || q.k < 0)
+            return fail(NBH_ERR_ARG, "bad job " + std::to_string(i));
+        Staged &s = S[(size_t)i];
+        // a short product reads only the operands' low k limbs (mullo!, src/mul.jl:308-309)
+        const int64_t m = q.k ? std::min(q.m, q.k) : q.m;
+        const int64_t n = sqr ? m : (q.k ? std::min(q.n, q.k) : q.n);
+        const int64_t ba = bits_from_top(m, q.a[m - 1]);
+        const int64_t bb = sqr ? ba : bits_from_top(n, q.b[n - 1]);
+        s.d = nbh_job{nullptr, nullptr, nullptr, m, n, ba, bb, q.k};
+        s.rl = q.k ? q.k : m + n;
+        s.zero = ba == 0 || bb == 0;
+        s.sa = s.zero ? 0 : align256(8 * (size_t)m);
+        s.sb = s.zero || sqr ? 0 : align256(8 * (size_t)n);
+        s.sr = s.zero ? 0 : align256(8 * (size_t)s.rl);
+    }
+    int64_t i0 = 0;
+    while (i0 < count) {
+        // the group [i0, i1): operands first, then the results, each back to back
+        size_t ops = 0, res = 0;
+        int64_t i1 = i0;
+        while (i1 < count) {
+            const Staged &s = S[(size_t)i1];
+            if (i1 > i0 && ops + res + s.sa + s.sb + s.sr > PIN_MAX_BYTES) break;
+            ops += s.sa + s.sb;
+            res += s.sr;
+            i1++;
+        }
+        const bool pinned = ops + res <= PIN_MAX_BYTES;
+        std::vector<nbh_job> dj;
+        std::vector<int64_t> idx;
+        if (ops + res > 0) {
+            RCCHK(ensure_buf(&g.io, &g.io_size, ops + res));
+            if (pinned) RCCHK(ensure_pinned(ops + res));
+        }
+        char *dev = (char *)g.io;
+        size_t oa = 0, orr = ops;
+        for (int64_t i = i0; i < i1; i++) {
+            Staged &s = S[(size_t)i];
+            if (s.zero) {
+                std::memset(jobs[i].r, 0, 8 * (size_t)s.rl);
+                continue;
+            }
+            nbh_job d = s.d;
+            d.a = (const uint64_t *)(dev + oa);
+            d.b = jobs[i].b ? (const uint64_t *)(dev + oa + s.sa) : nullptr;
+            d.r = (uint64_t *)(dev + orr);
+            oa += s.sa + s.sb;
+            orr += s.sr;
+            dj.push_back(d);
+            idx.push_back(i);
+        }
+        if (dj.empty()) {
+            i0 = i1;
+            continue;
+        }
+        // plan before any copy (planning reads no operand: every bit length is known)
+        nbh_batch_s *B = nullptr;
+        RCCHK(batch_create(dj.data(), (int64_t)dj.size(), &B));
+        std::unique_ptr<nbh_batch_s, void (*)(nbh_batch_s *)> hold(B, batch_free);
+        for (size_t j = 0; j < dj.size(); j++) {
+            const nbh_job &q = jobs[idx[j]];
+            const nbh_job &d = dj[j];
+            if (pinned) {
+                char *p = (char *)g.pin + ((const char *)d.a - dev);
+                std::memcpy(p, q.a, 8 * (size_t)d.m);
+                if (q.b) std::memcpy(p + S[(size_t)idx[j]].sa, q.b, 8 * (size_t)d.n);
+            } else {
+                HIPCHK(hipMemcpyAsync((void *)d.a, q.a, 8 * (size_t)d.m, hipMemcpyHostToDevice, g.stream));
+                if (q.b) HIPCHK(hipMemcpyAsync((void *)d.b, q.b, 8 * (size_t)d.n, hipMemcpyHostToDevice, g.stream));
+            }
+        }
+        if (pinned) HIPCHK(hipMemcpyAsync(dev, g.pin, ops, hipMemcpyHostToDevice, g.stream));
+        int rc = batch_run(B, g.stream);
+        if (rc == NBH_OK) {
+            hipError_t e = hipSuccess;
+            if (pinned) {
+                e = hipMemcpyAsync((char *)g.pin + ops, dev + ops, res, hipMemcpyDeviceToHost, g.stream);
+            } else {
+                for (size_t j = 0; j < dj.size() && e == hipSuccess; j++)
+                    e = hipMemcpyAsync(jobs[idx[j]].r, dj[j].r, 8 * (size_t)S[(size_t)idx[j]].rl,
+                                       hipMemcpyDeviceToHost, g.stream);
+            }
+            if (e != hipSuccess) rc = fail(NBH_ERR_HIP, std::string("copy out: ") + hipGetErrorString(e));
+        }
+        // wait even after a failure: the pinned and staging buffers are reused
+        const hipError_t es = hipStreamSynchronize(g.stream);
+        RCCHK(rc);
+        HIPCHK(es);
+        if (pinned)
+            for (size_t j = 0; j < dj.size(); j++)
+                std::memcpy(jobs[idx[j]].r, (char *)g.pin + ((const char *)dj[j].r - dev),
+                            8 * (size_t)S[(size_t)idx[j]].rl);
+        i0 = i1;
+    }
+    return NBH_OK;
+}
+
 int32_t nbh_spectrum_create(const uint64_t *b, int64_t n, int64_t bits_b, int64_t bits_a_max, void *stream,
                             nbh_spectrum *out) {
     std::lock_guard<std::recursive_mutex> lk(g.mu);

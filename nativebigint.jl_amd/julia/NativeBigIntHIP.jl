@@ -68,6 +68,37 @@ function sqrlo!(r::Memory{UInt64}, ro::Int, a::Memory{UInt64}, ao::Int, la::Int,
     return nothing
 end
 
+# Many independent products in one call (nbh_mul_batch): each element of
+# `jobs` is (r, ro, a, ao, m, b, bo, n, k) as in mul_fpntt2!/mullo!, with
+# b === nothing for a square and k == 0 for the full product.  Results must
+# not overlap each other or any operand.
+struct NbhJob
+    r::Ptr{UInt64}
+    a::Ptr{UInt64}
+    b::Ptr{UInt64}
+    m::Int64
+    n::Int64
+    bits_a::Int64
+    bits_b::Int64
+    k::Int64
+end
+
+function mul_batch!(jobs::AbstractVector)
+    bufs = Any[]
+    recs = Vector{NbhJob}(undef, length(jobs))
+    for (i, (r, ro, a, ao, m, b, bo, n, k)) in enumerate(jobs)
+        push!(bufs, r, a)
+        b === nothing || push!(bufs, b)
+        pb = b === nothing ? Ptr{UInt64}(0) : pointer(b, bo + 1)
+        recs[i] = NbhJob(pointer(r, ro + 1), pointer(a, ao + 1), pb,
+                         m, b === nothing ? m : n, 0, 0, k)
+    end
+    GC.@preserve bufs begin
+        check(ccall((:nbh_mul_batch, libnbh), Int32, (Ptr{NbhJob}, Int64), recs, length(recs)))
+    end
+    return nothing
+end
+
 function ntt_len(T::Int)
     N = Ref{Int64}(0)
     check(ccall((:nbh_ntt_len, libnbh), Int32, (Int64, Ptr{Int64}), T, N))

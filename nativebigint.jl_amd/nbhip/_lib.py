@@ -83,6 +83,7 @@ SIGNATURES = {
     "nbh_batch_run": (_i32, [_vp, _vp]),
     "nbh_batch_destroy": (None, [_vp]),
     "nbh_mul_batch_dev": (_i32, [_P(nbh_job), _i64, _vp]),
+    "nbh_mul_batch": (_i32, [_P(nbh_job), _i64]),
     "nbh_batch_arena_bytes": (_i32, [_P(nbh_job), _i64, _P(_i64)]),
     "nbh_batch_create_in": (_i32, [_P(nbh_job), _i64, _vp, _i64, _vp, _P(_vp)]),
     "nbh_batch_release": (None, [_vp]),

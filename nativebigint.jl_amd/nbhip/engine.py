@@ -23,7 +23,7 @@ SQR_FPNTT_THRESHOLD = 160
 
 __all__ = ["MUL_FPNTT_MIN", "MUL_FPNTT_THRESHOLD", "SQR_FPNTT_THRESHOLD", "ArgumentError", "NbhError",
            "magnitude_bits", "ntt_len", "fp_ntt_params", "mul_fpntt2", "sqr_fpntt2", "mul", "sqr", "mul_bal",
-           "mullo", "sqrlo", "init"]
+           "mullo", "sqrlo", "mul_batch", "init"]
 
 
 def _u64(buf) -> np.ndarray:
@@ -129,3 +129,54 @@ def sqrlo(r, ro: int, a, ao: int, la: int, k: int) -> None:
     _check_ranges(r, ro, k, a, ao, la)
     init()
     check(lib().nbh_sqrlo(_ptr(r, ro), _ptr(a, ao), la, k))
+
+
+def mul_batch(jobs) -> None:
+    """Many independent products on host limb buffers in one call
+    (nbh_mul_batch): each job is one mul!/sqr!/mullo!/sqrlo! of the reference
+    (src/mul.jl:127-164, 305-389).  A job is the tuple
+    (r, ro, a, ao, m, b, bo, n, k): b None for a square (bo, n ignored), k 0
+    for the full product r[ro:ro+m+n] (2m for a square), k >= 1 for the low
+    short product r[ro:ro+k] with the operands clamped to k limbs.  Every size
+    runs on the GPU (schoolbook below the mul!/sqr! NTT thresholds).  No
+    result may overlap another job's result or any operand."""
+    recs = []
+    spans = []   # (start, end, is_result) byte ranges
+    for j, job in enumerate(jobs):
+        r, ro, a, ao, m, b, bo, n, k = job
+        r, a = _u64(r), _u64(a)
+        sqr = b is None
+        if k < 0:
+            raise ValueError(f"job {j}: k >= 0")
+        if not sqr:
+            b = _u64(b)
+        mm = min(m, k) if k else m
+        nn = mm if sqr else (min(n, k) if k else n)
+        rl = k if k else mm + nn
+        if sqr:
+            _check_ranges(r, ro, rl, a, ao, mm)
+        else:
+            _check_ranges(r, ro, rl, a, ao, mm, b, bo, nn)
+        pa = _ptr(a, ao)
+        pb = 0 if sqr else _ptr(b, bo)
+        pr = _ptr(r, ro)
+        recs.append((pr, pa, pb, mm, nn, 0, 0, k))
+        spans.append((pa, pa + 8 * mm, False))
+        if not sqr:
+            spans.append((pb, pb + 8 * nn, False))
+        spans.append((pr, pr + 8 * rl, True))
+    if not recs:
+        return
+    spans.sort()
+    end_all = end_res = 0
+    for s, e, is_res in spans:
+        if s < end_res or (is_res and s < end_all):
+            raise ValueError("a result overlaps another job's result or an operand")
+        end_all = max(end_all, e)
+        if is_res:
+            end_res = max(end_res, e)
+    arr = (_lib.nbh_job * len(recs))()
+    for i, (pr, pa, pb, mm, nn, _, _, k) in enumerate(recs):
+        arr[i] = _lib.nbh_job(pr, pa, pb or None, mm, nn, 0, 0, k)
+    init()
+    check(lib().nbh_mul_batch(arr, len(recs)))
