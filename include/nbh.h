@@ -131,6 +131,34 @@ int32_t nbh_mul_batch_dev(const nbh_job *jobs, int64_t count, void *stream);
  * job's operands or result. */
 int32_t nbh_mul_batch(const nbh_job *jobs, int64_t count);
 
+/* Serving queue (no reference counterpart: the reference multiplies one
+ * product per call).  Continuous batching inside an HBM budget of its own,
+ * on nstreams HIP streams of default priority.  nbh_server_submit queues a
+ * job on device pointers (bits_* <= 0 reads the top limb, with a host
+ * synchronisation) and returns its ticket; NBH_ERR_BUDGET if the job can
+ * never fit the budget.  Each nbh_server_poll retires the micro-batches that
+ * have completed, admits waiting jobs (oldest first, at most max_batch per
+ * micro-batch, while their workspace fits the free arena) and launches them
+ * without device allocation or host synchronisation, then reports up to cap
+ * completed tickets: their results are in r.  Operands and r must stay valid
+ * until the ticket is reported.  nbh_server_drain blocks until every
+ * submitted job has completed (poll afterwards to collect the tickets). */
+typedef struct nbh_server_s *nbh_server;
+int32_t nbh_server_create(int64_t budget_bytes, int32_t nstreams, int32_t max_batch, nbh_server *out);
+int32_t nbh_server_submit(nbh_server s, const nbh_job *job, int64_t *ticket);
+int32_t nbh_server_poll(nbh_server s, int64_t *done, int64_t cap, int64_t *ndone);
+int32_t nbh_server_drain(nbh_server s);
+int32_t nbh_server_stats(nbh_server s, int64_t *pending, int64_t *in_flight, int64_t *batches);
+void nbh_server_destroy(nbh_server s);
+
+/* Device memory for callers with no GPU array type of their own (e.g. plain
+ * Julia): allocate, free (after the device is idle), and synchronous copies
+ * (a download waits for all device work first). */
+int32_t nbh_alloc(int64_t bytes, void **ptr);
+void nbh_free(void *ptr);
+int32_t nbh_upload(void *dst_dev, const void *src_host, int64_t bytes);
+int32_t nbh_download(void *dst_host, const void *src_dev, int64_t bytes);
+
 /* Batched 2x2 cofactor step of HGCD (device pointers): per instance a 2x2
  * matrix of mn-limb cofactors (zero-padded) and a 2-vector of n-limb operands.
  *   NBH_HGCD_FORWARD: r0 = m00*u + m01*v, r1 = m10*u + m11*v, n+mn+1 limbs

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -1488,6 +1489,286 @@ void nbh_batch_destroy(nbh_batch b) {
         (void)hipDeviceSynchronize();
         batch_free(b);
     }
+}
+
+// ---------------------------------------------------------------------------
+// Serving queue: continuous batching inside an HBM budget.
+//
+// Requests (device operands and results) wait in arrival order.  Each poll
+// retires the micro-batches whose completion event has fired (their arena
+// block is freed and their tickets reported), then, for every idle stream,
+// admits the oldest requests while their arena bytes fit the largest free
+// block (at most max_batch per micro-batch), plans them inside that block
+// (batch_plan's arena form: no device allocation, no host synchronisation)
+// and launches them.  So requests that arrive while others run start as soon
+// as a stream and memory free up, and micro-batches overlap on the device.
+
+struct nbh_server_s {
+    char *arena = nullptr;
+    size_t budget = 0;
+    int max_batch = 256;
+    std::vector<std::pair<size_t, size_t>> free_list;  // (offset, length), sorted, coalesced
+    struct Req {
+        nbh_job job;
+        int64_t ticket;
+        size_t need;
+    };
+    std::deque<Req> pending;
+    struct Flight {
+        hipStream_t stream = nullptr;
+        hipEvent_t ev = nullptr;
+        nbh_batch_s *B = nullptr;  // null: the stream is idle
+        std::vector<int64_t> tickets;
+        size_t off = 0, len = 0;
+    };
+    std::vector<Flight> flights;
+    std::map<std::tuple<int64_t, int64_t, int64_t, int64_t, int64_t, bool>, size_t> need_cache;
+    std::deque<int64_t> done;
+    int64_t next_ticket = 0, batches = 0;
+};
+
+namespace {
+
+size_t srv_largest(const nbh_server_s &S) {
+    size_t m = 0;
+    for (const auto &f : S.free_list) m = std::max(m, f.second);
+    return m;
+}
+
+bool srv_alloc(nbh_server_s &S, size_t n, size_t *off) {
+    for (size_t i = 0; i < S.free_list.size(); i++) {
+        auto &f = S.free_list[i];
+        if (f.second < n) continue;
+        *off = f.first;
+        if (f.second == n) S.free_list.erase(S.free_list.begin() + (std::ptrdiff_t)i);
+        else f = {f.first + n, f.second - n};
+        return true;
+    }
+    return false;
+}
+
+void srv_release(nbh_server_s &S, size_t off, size_t n) {
+    auto it = std::lower_bound(S.free_list.begin(), S.free_list.end(), std::make_pair(off, (size_t)0));
+    it = S.free_list.insert(it, {off, n});
+    // merge with the next block, then with the previous one
+    if (it + 1 != S.free_list.end() && it->first + it->second == (it + 1)->first) {
+        it->second += (it + 1)->second;
+        S.free_list.erase(it + 1);
+    }
+    if (it != S.free_list.begin() && (it - 1)->first + (it - 1)->second == it->first) {
+        (it - 1)->second += it->second;
+        S.free_list.erase(it);
+    }
+}
+
+// Arena bytes of one request alone (workspace + tables, 256-aligned); a
+// micro-batch needs at most the sum over its requests.
+int srv_need(nbh_server_s &S, const nbh_job &j, size_t *need) {
+    const auto key = std::make_tuple(j.m, j.n, j.bits_a, j.bits_b, j.k, j.b == nullptr);
+    auto it = S.need_cache.find(key);
+    if (it != S.need_cache.end()) {
+        *need = it->second;
+        return NBH_OK;
+    }
+    size_t n = 0;
+    RCCHK(batch_create(&j, 1, nullptr, &n));
+    n = align256(n);
+    S.need_cache[key] = n;
+    *need = n;
+    return NBH_OK;
+}
+
+int srv_retire(nbh_server_s &S) {
+    for (auto &f : S.flights) {
+        if (!f.B) continue;
+        const hipError_t e = hipEventQuery(f.ev);
+        if (e == hipErrorNotReady) continue;
+        HIPCHK(e);
+        for (int64_t t : f.tickets) S.done.push_back(t);
+        f.tickets.clear();
+        batch_free(f.B);  // host side only: the batch lives in the arena
+        f.B = nullptr;
+        srv_release(S, f.off, f.len);
+    }
+    return NBH_OK;
+}
+
+int srv_admit(nbh_server_s &S) {
+    for (auto &f : S.flights) {
+        if (f.B || S.pending.empty()) continue;
+        const size_t room = srv_largest(S);
+        size_t need = 0;
+        size_t cnt = 0;
+        while (cnt < S.pending.size() && (int)cnt < S.max_batch && need + S.pending[cnt].need <= room)
+            need += S.pending[cnt++].need;
+        if (cnt == 0) return NBH_OK;  // wait for memory
+        size_t off = 0;
+        if (!srv_alloc(S, need, &off)) return fail(NBH_ERR_BUDGET, "server arena: no block");
+        std::vector<nbh_job> jobs(cnt);
+        for (size_t i = 0; i < cnt; i++) jobs[i] = S.pending[i].job;
+        nbh_batch_s *B = nullptr;
+        const int rc = batch_create(jobs.data(), (int64_t)cnt, &B, nullptr, S.arena + off, need, f.stream);
+        if (rc != NBH_OK) {
+            srv_release(S, off, need);
+            return rc;
+        }
+        B->nside = 0;  // micro-batches already overlap on the server's streams
+        f.B = B;
+        f.off = off;
+        f.len = need;
+        for (size_t i = 0; i < cnt; i++) f.tickets.push_back(S.pending[i].ticket);
+        S.pending.erase(S.pending.begin(), S.pending.begin() + (std::ptrdiff_t)cnt);
+        RCCHK(batch_run(B, f.stream));
+        HIPCHK(hipEventRecord(f.ev, f.stream));
+        S.batches++;
+    }
+    return NBH_OK;
+}
+
+void srv_free(nbh_server_s *S) {
+    if (!S) return;
+    for (auto &f : S->flights) {
+        if (f.stream) (void)hipStreamSynchronize(f.stream);
+        if (f.B) batch_free(f.B);
+        if (f.ev) (void)hipEventDestroy(f.ev);
+        if (f.stream) (void)hipStreamDestroy(f.stream);
+    }
+    if (S->arena) (void)hipFree(S->arena);
+    delete S;
+}
+
+}  // namespace
+
+int32_t nbh_server_create(int64_t budget_bytes, int32_t nstreams, int32_t max_batch, nbh_server *out) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    if (!out || budget_bytes < 4096 || nstreams < 1 || nstreams > 16 || max_batch < 1)
+        return fail(NBH_ERR_ARG, "bad server parameters");
+    *out = nullptr;
+    std::unique_ptr<nbh_server_s, void (*)(nbh_server_s *)> S(new nbh_server_s, srv_free);
+    S->budget = (size_t)budget_bytes & ~(size_t)255;
+    S->max_batch = max_batch;
+    HIPCHK(hipMalloc((void **)&S->arena, S->budget));
+    S->free_list.push_back({0, S->budget});
+    S->flights.resize((size_t)nstreams);
+    for (auto &f : S->flights) {
+        HIPCHK(hipStreamCreateWithFlags(&f.stream, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&f.ev, hipEventDisableTiming));
+    }
+    *out = S.release();
+    return NBH_OK;
+}
+
+int32_t nbh_server_submit(nbh_server s, const nbh_job *job, int64_t *ticket) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    if (!s || !job || !ticket) return fail(NBH_ERR_ARG, "null argument");
+    nbh_job j = *job;
+    const bool sqr = j.b == nullptr;
+    if (!j.r || !j.a || j.m < 1 || (!sqr && j.n < 1) || j.k < 0) return fail(NBH_ERR_ARG, "bad job");
+    if (sqr) j.n = j.m;
+    // admission sizes a request from its bit lengths: read the top limbs if not given
+    if (j.bits_a <= 0) RCCHK(read_bits(j.a, j.m, &j.bits_a));
+    if (sqr) j.bits_b = j.bits_a;
+    else if (j.bits_b <= 0) RCCHK(read_bits(j.b, j.n, &j.bits_b));
+    // bit lengths are upper bounds: a zero operand is sized as one bit (the
+    // engine still computes its zero product exactly)
+    j.bits_a = std::max<int64_t>(j.bits_a, 1);
+    j.bits_b = std::max<int64_t>(j.bits_b, 1);
+    size_t need = 0;
+    RCCHK(srv_need(*s, j, &need));
+    if (need > s->budget) return fail(NBH_ERR_BUDGET, "request exceeds the server's HBM budget");
+    *ticket = s->next_ticket++;
+    s->pending.push_back({j, *ticket, need});
+    return NBH_OK;
+}
+
+int32_t nbh_server_poll(nbh_server s, int64_t *done, int64_t cap, int64_t *ndone) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    if (!s || !ndone || cap < 0 || (cap > 0 && !done)) return fail(NBH_ERR_ARG, "bad argument");
+    RCCHK(srv_retire(*s));
+    RCCHK(srv_admit(*s));
+    int64_t n = 0;
+    while (n < cap && !s->done.empty()) {
+        done[n++] = s->done.front();
+        s->done.pop_front();
+    }
+    *ndone = n;
+    return NBH_OK;
+}
+
+int32_t nbh_server_drain(nbh_server s) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    if (!s) return fail(NBH_ERR_ARG, "null handle");
+    for (;;) {
+        RCCHK(srv_retire(*s));
+        RCCHK(srv_admit(*s));
+        bool busy = false;
+        for (auto &f : s->flights)
+            if (f.B) {
+                busy = true;
+                HIPCHK(hipEventSynchronize(f.ev));
+                break;
+            }
+        if (!busy && s->pending.empty()) return NBH_OK;
+    }
+}
+
+int32_t nbh_server_stats(nbh_server s, int64_t *pending, int64_t *in_flight, int64_t *batches) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    if (!s) return fail(NBH_ERR_ARG, "null handle");
+    int64_t fl = 0;
+    for (auto &f : s->flights) fl += f.B ? (int64_t)f.tickets.size() : 0;
+    if (pending) *pending = (int64_t)s->pending.size();
+    if (in_flight) *in_flight = fl;
+    if (batches) *batches = s->batches;
+    return NBH_OK;
+}
+
+void nbh_server_destroy(nbh_server s) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    srv_free(s);
+}
+
+// Device memory and copies for callers with no GPU array type of their own.
+int32_t nbh_alloc(int64_t bytes, void **ptr) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    if (!ptr || bytes < 1) return fail(NBH_ERR_ARG, "bad argument");
+    *ptr = nullptr;
+    if (hipMalloc(ptr, (size_t)bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        *ptr = nullptr;
+        return fail(NBH_ERR_OOM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+    }
+    return NBH_OK;
+}
+
+void nbh_free(void *ptr) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    if (ptr) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(ptr);
+    }
+}
+
+int32_t nbh_upload(void *dst, const void *src, int64_t bytes) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    if (!dst || !src || bytes < 0) return fail(NBH_ERR_ARG, "bad argument");
+    HIPCHK(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice));
+    return NBH_OK;
+}
+
+int32_t nbh_download(void *dst, const void *src, int64_t bytes) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    if (!dst || !src || bytes < 0) return fail(NBH_ERR_ARG, "bad argument");
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost));
+    return NBH_OK;
 }
 
 int32_t nbh_mul_batch_dev(const nbh_job *jobs, int64_t count, void *stream) {

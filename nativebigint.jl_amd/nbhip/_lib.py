@@ -84,6 +84,16 @@ SIGNATURES = {
     "nbh_batch_destroy": (None, [_vp]),
     "nbh_mul_batch_dev": (_i32, [_P(nbh_job), _i64, _vp]),
     "nbh_mul_batch": (_i32, [_P(nbh_job), _i64]),
+    "nbh_server_create": (_i32, [_i64, _i32, _i32, _P(_vp)]),
+    "nbh_server_submit": (_i32, [_vp, _P(nbh_job), _P(_i64)]),
+    "nbh_server_poll": (_i32, [_vp, _P(_i64), _i64, _P(_i64)]),
+    "nbh_server_drain": (_i32, [_vp]),
+    "nbh_server_stats": (_i32, [_vp, _P(_i64), _P(_i64), _P(_i64)]),
+    "nbh_server_destroy": (None, [_vp]),
+    "nbh_alloc": (_i32, [_i64, _P(_vp)]),
+    "nbh_free": (None, [_vp]),
+    "nbh_upload": (_i32, [_vp, _vp, _i64]),
+    "nbh_download": (_i32, [_vp, _vp, _i64]),
     "nbh_batch_arena_bytes": (_i32, [_P(nbh_job), _i64, _P(_i64)]),
     "nbh_batch_create_in": (_i32, [_P(nbh_job), _i64, _vp, _i64, _vp, _P(_vp)]),
     "nbh_batch_release": (None, [_vp]),

@@ -251,3 +251,80 @@ def latency_stats(reqs) -> dict:
         return {}
     return {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "mean_ms": float(lat.mean()), "max_ms": float(lat.max()), "requests": int(lat.size)}
+
+
+class NativeServer:
+    """The serving loop of the library itself (nbh_server_*): queueing,
+    admission by HBM budget, planning inside the arena and launches all run
+    in C++; this class only maps tickets to requests.  Each request's result
+    goes to its own device buffer `req.out` (int64 limbs, allocated by the
+    caller before submit; at least m + n limbs)."""
+
+    def __init__(self, budget_bytes: int, nstreams: int = 4, max_batch: int = 256, cap: int = 4096):
+        init()
+        self.h = ctypes.c_void_p()
+        check(lib().nbh_server_create(int(budget_bytes), nstreams, max_batch, ctypes.byref(self.h)))
+        self.cap = cap
+        self._buf = (ctypes.c_int64 * cap)()
+        self._n = ctypes.c_int64()
+        self._tk = ctypes.c_int64()
+        self.inflight = {}
+        self.done = []
+        self.rejected = []
+        self.host_s = 0.0   # host time in submit and poll calls
+
+    def submit(self, req: Request, t: float | None = None) -> Request:
+        t0 = time.perf_counter()
+        req.t_arrival = t0 if t is None else t
+        job = self.job(req)
+        rc = lib().nbh_server_submit(self.h, ctypes.byref(job), ctypes.byref(self._tk))
+        if rc == NBH_ERR_BUDGET:
+            self.rejected.append(req)
+        else:
+            check(rc)
+            req.rid = self._tk.value
+            self.inflight[req.rid] = req
+        self.host_s += time.perf_counter() - t0
+        return req
+
+    @staticmethod
+    def job(req: Request) -> nbh_job:
+        """The request's nbh_job record (built once and kept on the request)."""
+        job = getattr(req, "_job", None)
+        if job is None:
+            if req.out is None or req.out.numel() < req.limbs:
+                raise ValueError("req.out must hold m + n limbs on the device")
+            job = nbh_job(req.out.data_ptr(), req.dev_a, req.dev_b or None, req.m, req.n, req.bits_a, req.bits_b, 0)
+            req._job = job
+        return job
+
+    def poll(self) -> None:
+        t0 = time.perf_counter()
+        check(lib().nbh_server_poll(self.h, self._buf, self.cap, ctypes.byref(self._n)))
+        n = self._n.value
+        if n:
+            now = time.perf_counter()
+            for i in range(n):
+                r = self.inflight.pop(self._buf[i])
+                r.t_done = now
+                self.done.append(r)
+        self.host_s += time.perf_counter() - t0
+
+    def idle(self) -> bool:
+        return not self.inflight
+
+    def drain(self) -> None:
+        check(lib().nbh_server_drain(self.h))
+        while self.inflight:
+            self.poll()
+
+    @property
+    def batches(self) -> int:
+        b = ctypes.c_int64()
+        check(lib().nbh_server_stats(self.h, None, None, ctypes.byref(b)))
+        return b.value
+
+    def close(self) -> None:
+        if self.h:
+            lib().nbh_server_destroy(self.h)
+            self.h = None
