@@ -99,6 +99,51 @@ function mul_batch!(jobs::AbstractVector)
     return nothing
 end
 
+# Device buffers for callers with no GPU array type (nbh_alloc & co.).
+function dev_alloc(nlimbs::Int)
+    p = Ref{Ptr{UInt64}}(C_NULL)
+    check(ccall((:nbh_alloc, libnbh), Int32, (Int64, Ref{Ptr{UInt64}}), 8 * nlimbs, p))
+    return p[]
+end
+dev_free(p::Ptr{UInt64}) = ccall((:nbh_free, libnbh), Cvoid, (Ptr{UInt64},), p)
+function upload!(d::Ptr{UInt64}, x::Memory{UInt64}, n::Int=length(x))
+    GC.@preserve x check(ccall((:nbh_upload, libnbh), Int32, (Ptr{UInt64}, Ptr{UInt64}, Int64),
+                               d, pointer(x), 8 * n))
+end
+function download!(x::Memory{UInt64}, d::Ptr{UInt64}, n::Int=length(x))
+    GC.@preserve x check(ccall((:nbh_download, libnbh), Int32, (Ptr{UInt64}, Ptr{UInt64}, Int64),
+                               pointer(x), d, 8 * n))
+end
+
+# The serving queue (nbh_server_*): submit device-resident products as they
+# arrive, poll for finished tickets.  b == C_NULL is a square; bits <= 0
+# reads the top limb.
+mutable struct Server
+    h::Ptr{Cvoid}
+end
+function Server(budget_bytes::Int; nstreams::Int=4, max_batch::Int=256)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:nbh_server_create, libnbh), Int32, (Int64, Int32, Int32, Ref{Ptr{Cvoid}}),
+                budget_bytes, nstreams, max_batch, h))
+    s = Server(h[])
+    finalizer(x -> ccall((:nbh_server_destroy, libnbh), Cvoid, (Ptr{Cvoid},), x.h), s)
+    return s
+end
+function submit!(s::Server, r::Ptr{UInt64}, a::Ptr{UInt64}, m::Int, b::Ptr{UInt64}, n::Int;
+                 bits_a::Int=0, bits_b::Int=0)
+    t = Ref{Int64}(0)
+    job = Ref(NbhJob(r, a, b, m, b == C_NULL ? m : n, bits_a, bits_b, 0))
+    check(ccall((:nbh_server_submit, libnbh), Int32, (Ptr{Cvoid}, Ref{NbhJob}, Ref{Int64}), s.h, job, t))
+    return t[]
+end
+function poll!(s::Server, done::Vector{Int64})
+    n = Ref{Int64}(0)
+    check(ccall((:nbh_server_poll, libnbh), Int32, (Ptr{Cvoid}, Ptr{Int64}, Int64, Ref{Int64}),
+                s.h, done, length(done), n))
+    return Int(n[])
+end
+drain!(s::Server) = check(ccall((:nbh_server_drain, libnbh), Int32, (Ptr{Cvoid},), s.h))
+
 function ntt_len(T::Int)
     N = Ref{Int64}(0)
     check(ccall((:nbh_ntt_len, libnbh), Int32, (Int64, Ptr{Int64}), T, N))
