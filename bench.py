@@ -77,6 +77,8 @@ def parse():
                     help="side streams a batch step forks its launches onto (0: in order)")
     ap.add_argument("--requests", type=int, default=20000, help="serve: requests per offered load")
     ap.add_argument("--budget-gib", type=float, default=8.0, help="serve: HBM budget")
+    ap.add_argument("--serve-streams", type=int, default=4, help="serve: streams (micro-batches in flight)")
+    ap.add_argument("--serve-max-batch", type=int, default=256, help="serve: requests per micro-batch at most")
     ap.add_argument("--jobs", type=int, default=4096)
     ap.add_argument("--single-sizes", default="152,256,512,1024,2048,4096,8192,16384,65536",
                     help="single: balanced sizes (limbs) of the drop-in latency sweep")
@@ -970,7 +972,7 @@ def bench_serve(args, D, engine, wl, info, stream):
     results, native = {}, {}
     for load, pick, arrivals in streams:
         # the Python loop over the arena planner (results in the arena, sample copied out)
-        srv = serve.Server(budget_bytes=budget, nstreams=4, max_batch=256)
+        srv = serve.Server(budget_bytes=budget, nstreams=args.serve_streams, max_batch=args.serve_max_batch)
         reqs = [serve.Request(dev[p][0], dev[p][1], *bits[p], keep=i in keep) for i, p in enumerate(pick)]
         srv.prepare(reqs)                 # start-up: plans of every request shape
         t0, t_end = drive(srv, reqs, arrivals)
@@ -985,7 +987,7 @@ def bench_serve(args, D, engine, wl, info, stream):
         for r, o in zip(reqs, offs.tolist()):
             r.out = out[o:o + r.limbs]
             serve.NativeServer.job(r)
-        srv = serve.NativeServer(budget_bytes=budget, nstreams=4, max_batch=256)
+        srv = serve.NativeServer(budget_bytes=budget, nstreams=args.serve_streams, max_batch=args.serve_max_batch)
         warm = {}                         # start-up: one request of every shape through the queue
         for p in sorted(set(pick.tolist())):
             w = serve.Request(dev[p][0], dev[p][1], *bits[p])
@@ -1008,7 +1010,8 @@ def bench_serve(args, D, engine, wl, info, stream):
            "config": {"workload": "serve: Poisson arrivals, limbs log-uniform [152, 262144], 30% squares, "
                                   "20% unbalanced (ratio <= 8), operands resident in HBM",
                       "capacity_limb_products_per_s": cap, "budget_gib": args.budget_gib,
-                      "parallelism": "single GPU, 4 streams",
+                      "parallelism": f"single GPU, {args.serve_streams} streams",
+                      "max_batch": args.serve_max_batch,
                       "server": "nbh_server_* (the library's queue); latency_python_loop: nbhip.serve.Server"},
            "latency": native, "latency_python_loop": results, "device": info}
     print(json.dumps(out))
