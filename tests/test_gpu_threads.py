@@ -1,0 +1,61 @@
+"""The C ABI is safe to call from several host threads at once (the
+reference's engine is reentrant and its plan cache thread-safe,
+src/fpntt.jl:273-293): threads issuing single products, squares, short
+products and host batches on their own buffers, with plans for new lengths
+built while the others run, all get libgmp's limbs."""
+import threading
+
+import numpy as np
+import pytest
+
+from tests import gmp_ref
+from tests.helpers import rand_big_style
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+from nbhip import engine  # noqa: E402
+
+
+def test_concurrent_calls():
+    if not gmp_ref.available():
+        pytest.skip("libgmp not found")
+    engine.init()
+    errors = []
+
+    def worker(seed):
+        try:
+            rng = np.random.default_rng(seed)
+            for it in range(12):
+                m = int(rng.integers(160, 30000))
+                n = int(rng.integers(152, m + 1))
+                a, b = rand_big_style(rng, m, 1), rand_big_style(rng, n, 1)
+                kind = (seed + it) % 4
+                if kind == 0:
+                    r = np.zeros(m + n, dtype=np.uint64)
+                    engine.mul_fpntt2(r, 0, a, 0, m, b, 0, n)
+                    ok = np.array_equal(r, gmp_ref.mul(a, b))
+                elif kind == 1:
+                    r = np.zeros(2 * m, dtype=np.uint64)
+                    engine.sqr_fpntt2(r, 0, a, 0, m)
+                    ok = np.array_equal(r, gmp_ref.sqr(a))
+                elif kind == 2:
+                    k = int(rng.integers(1, m + n))
+                    r = np.zeros(k, dtype=np.uint64)
+                    engine.mullo(r, 0, a, 0, m, b, 0, n, k)
+                    ok = np.array_equal(r, gmp_ref.mul(a, b)[:k])
+                else:
+                    r = np.zeros(m + n + 2 * n, dtype=np.uint64)
+                    engine.mul_batch([(r, 0, a, 0, m, b, 0, n, 0), (r, m + n, b, 0, n, None, 0, 0, 0)])
+                    ok = np.array_equal(r[:m + n], gmp_ref.mul(a, b)) and np.array_equal(r[m + n:], gmp_ref.sqr(b))
+                if not ok:
+                    errors.append((seed, it, kind, m, n))
+        except Exception as e:  # surfaced below
+            errors.append((seed, repr(e)))
+
+    threads = [threading.Thread(target=worker, args=(0x7A + s,)) for s in range(6)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=100)
+    assert not any(t.is_alive() for t in threads)
+    assert not errors, errors
