@@ -187,7 +187,7 @@ void dft7_consts(uint64_t c, int k, double *out) {
 
 struct Tuning {
     int grp_lds = 16384;  // points per strided group tile
-    int mid_max = 8192;   // largest contiguous last group (the mid kernel holds two)
+    int mid_max = 8192;   // largest contiguous last group (one LDS array in the mid kernel)
     int cmin = 8;
     int cmax = 64;
 };
@@ -468,7 +468,8 @@ bool tuning_ok(const Tuning &t) {
     auto pow2 = [](int32_t x) { return x > 0 && (x & (x - 1)) == 0; };
     return pow2(t.grp_lds) && pow2(t.mid_max) && pow2(t.cmin) && pow2(t.cmax) && t.cmin <= t.cmax &&
            t.mid_max >= 4 && t.grp_lds >= 16 && t.mid_max <= (1 << 16) && t.grp_lds <= (1 << 16) &&
-           8 * (size_t)lds_doubles(t.grp_lds) <= g.lds_block && 16 * (size_t)lds_doubles(t.mid_max) <= g.lds_block;
+           8 * (size_t)lds_doubles(t.grp_lds) <= g.lds_block && 8 * (size_t)lds_doubles(t.mid_max) <= g.lds_block &&
+           t.mid_max <= 16 * MID_MAX_THREADS;
 }
 
 // The autotuner's table, "Nmax grp_lds mid_max cmin cmax" per line, read at
@@ -1290,7 +1291,7 @@ int32_t nbh_init(int32_t device) {
     HIPCHK(set_lds_limit(g.lds_block));
     load_tuning_file();
     // keep the default tiles inside the device's per-block LDS
-    while (16 * (size_t)lds_doubles(g.tun.mid_max) > g.lds_block && g.tun.mid_max > 256) g.tun.mid_max >>= 1;
+    while (8 * (size_t)lds_doubles(g.tun.mid_max) > g.lds_block && g.tun.mid_max > 256) g.tun.mid_max >>= 1;
     while (8 * (size_t)lds_doubles(g.tun.grp_lds) > g.lds_block && g.tun.grp_lds > 256) g.tun.grp_lds >>= 1;
     g.init = true;
     return NBH_OK;

@@ -94,8 +94,9 @@ struct TileGen {
 constexpr size_t LDS_PER_CU = 160 * 1024;
 
 // The mid kernel's block-size bound (its __launch_bounds__): one thread per
-// 16 points of a mid tile of at most 8192 points.
-constexpr int MID_MAX_THREADS = 512;
+// 16 points of a mid tile of at most 16384 points (one LDS array of 8.5
+// bytes per point: 139 KB).
+constexpr int MID_MAX_THREADS = 1024;
 
 // Doubles of LDS that an n-point tile occupies with one pad double per 16
 // (the kernels index LDS as i + i/16).

@@ -94,7 +94,8 @@ def test_transform_bit_exact(k, N):
     _transform_case(N, k, N * 3 + k)
 
 
-@pytest.mark.parametrize("tuning", [(256, 64, 1, 4), (64, 16, 2, 2), (1024, 256, 4, 16), (16384, 8192, 8, 64)])
+@pytest.mark.parametrize("tuning", [(256, 64, 1, 4), (64, 16, 2, 2), (1024, 256, 4, 16), (16384, 8192, 8, 64),
+                                    (4096, 16384, 8, 64)])
 @pytest.mark.parametrize("N", [12288, 7 * 2**12, 315 * 2**8, 105 * 2**10, 2**16, 9 * 2**12, 5 * 2**13])
 def test_transform_other_schedules(tuning, N):
     """Several strided groups (odd radices inside them, narrow and wide column

@@ -32,3 +32,4 @@ def test_table_is_well_formed():
         assert cmin <= cmax
         for pts in (grp, mid):
             assert 8 * (pts + pts // 16 + 1) <= LDS
+        assert mid <= 16 * 1024          # one mid thread per 16 points, at most 1024 threads
