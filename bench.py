@@ -406,10 +406,12 @@ def main():
     if not args.no_cpu and wname != "small":
         from oracle import oracle as O
         ncores = min(16, os.cpu_count() or 1)
-        # the whole job list once: ~20 s of CPU work over the threads
+        # the whole job list, three times over: about 13 s of CPU work for the
+        # batch list (4 s per pass at ~1.7e7 limb-products/s per core)
+        passes = 3
         sample = list(range(len(jobs)))
         lock = threading.Lock()
-        work = list(sample)
+        work = sample * passes
         done_limbs = [0]
 
         def worker():
@@ -430,7 +432,7 @@ def main():
             t.join()
         cdt = time.perf_counter() - c0
         cpu = {"value": done_limbs[0] / cdt, "unit": "limb-products/s", "cores": ncores, "kind": "port",
-               "sample": f"all {len(sample)} {wname} products once, C restatement of src/fpntt.jl "
+               "sample": f"all {len(sample)} {wname} products, {passes} passes, C restatement of src/fpntt.jl "
                          f"(oracle/, gcc -O3 -march=x86-64-v3), one product per thread, {cdt:.1f}s wall"}
         log(f"cpu baseline {cpu['value']:.4g} limb-products/s on {ncores} threads")
 
