@@ -285,6 +285,13 @@ int32_t nbh_plan_groups(int64_t N, int32_t *ngroups, int64_t *la_lb_s_c /* 4 per
  * clears the table. */
 int32_t nbh_set_tuning_table(const int64_t *nmax, const int32_t *knobs, int32_t count);
 int32_t nbh_get_tuning_for(int64_t N, int32_t *knobs4);
+/* The same for products run alone (nbh_mul, nbh_sqr, nbh_mullo, nbh_sqrlo
+ * and their _dev forms): one product has to spread its transforms over the
+ * whole chip, so it may want smaller tiles than a batch of many.  Lengths
+ * past this table use the batch table.  The schedule file's lines that start
+ * with "single" fill it at init. */
+int32_t nbh_set_tuning_table_single(const int64_t *nmax, const int32_t *knobs, int32_t count);
+int32_t nbh_get_tuning_for_single(int64_t N, int32_t *knobs4);
 
 /* Per-kernel timing: when enabled, every launch of the product pipeline is
  * bracketed by HIP events on the stream it is launched on.  nbh_profile_read

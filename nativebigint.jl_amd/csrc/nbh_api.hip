@@ -223,6 +223,9 @@ struct Lib {
     std::string name;
     Tuning tun;                                    // default schedule knobs
     std::vector<std::pair<int64_t, Tuning>> table;  // per-length knobs: first entry with N <= Nmax
+    // knobs for products run alone (nbh_mul & co.): their blocks must spread one
+    // transform over the chip, so they favour smaller tiles than batches do
+    std::vector<std::pair<int64_t, Tuning>> table_single;
     std::map<std::pair<int64_t, int>, std::unique_ptr<StageList>> stages;
     std::map<std::tuple<int64_t, int, int, int, int, int>, std::unique_ptr<PlanEntry>> plans;
     void *ws = nullptr;  // single-product workspace
@@ -458,7 +461,10 @@ int make_schedule(const StageList &L, const Tuning &T, DevPlan &P, PlanEntry &E)
     return NBH_OK;
 }
 
-Tuning tuning_for(int64_t N) {
+Tuning tuning_for(int64_t N, bool single = false) {
+    if (single)
+        for (const auto &e : g.table_single)
+            if (N <= e.first) return e.second;
     for (const auto &e : g.table)
         if (N <= e.first) return e.second;
     return g.tun;
@@ -483,26 +489,34 @@ void load_tuning_file() {
     std::string arch = g.name.substr(0, g.name.find(':'));
     FILE *f = std::fopen((dir + "/tuning_" + arch + ".txt").c_str(), "r");
     if (!f) return;
-    std::vector<std::pair<int64_t, Tuning>> tab;
+    // "Nmax grp_lds mid_max cmin cmax" for batches; the same after "single"
+    // for products run alone
+    std::vector<std::pair<int64_t, Tuning>> tab, tab1;
     char line[256];
     bool ok = true;
     while (std::fgets(line, sizeof line, f)) {
         if (line[0] == '#' || line[0] == '\n') continue;
+        const bool single = std::strncmp(line, "single ", 7) == 0;
+        auto &t_ = single ? tab1 : tab;
         long long nmax;
         Tuning t;
-        if (std::sscanf(line, "%lld %d %d %d %d", &nmax, &t.grp_lds, &t.mid_max, &t.cmin, &t.cmax) != 5 ||
-            !tuning_ok(t) || (!tab.empty() && nmax <= tab.back().first)) {
+        if (std::sscanf(line + (single ? 7 : 0), "%lld %d %d %d %d", &nmax, &t.grp_lds, &t.mid_max, &t.cmin,
+                        &t.cmax) != 5 ||
+            !tuning_ok(t) || (!t_.empty() && nmax <= t_.back().first)) {
             ok = false;
             break;
         }
-        tab.emplace_back((int64_t)nmax, t);
+        t_.emplace_back((int64_t)nmax, t);
     }
     std::fclose(f);
-    if (ok) g.table = tab;
+    if (ok) {
+        g.table = tab;
+        g.table_single = tab1;
+    }
 }
 
-int get_plan(int64_t N, int k, PlanEntry **out) {
-    const Tuning T = tuning_for(N);
+int get_plan(int64_t N, int k, PlanEntry **out, bool single = false) {
+    const Tuning T = tuning_for(N, single);
     auto key = std::make_tuple(N, k, T.grp_lds, T.mid_max, T.cmin, T.cmax);
     auto it = g.plans.find(key);
     if (it != g.plans.end()) {
@@ -656,8 +670,8 @@ int run_single(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int
         return NBH_OK;
     }
     PlanEntry *P[2];
-    RCCHK(get_plan(N, 0, &P[0]));
-    RCCHK(get_plan(N, 1, &P[1]));
+    RCCHK(get_plan(N, 0, &P[0], true));
+    RCCHK(get_plan(N, 1, &P[1], true));
     // operands transformed here: a, and b unless it is a's square or cached
     const int nops = sqr || spec ? 1 : 2;
     const size_t xbytes = align256(sizeof(double) * (size_t)N);
@@ -2282,9 +2296,9 @@ int32_t nbh_set_tuning(int32_t grp_lds_doubles, int32_t mid_max, int32_t cmin, i
     return NBH_OK;
 }
 
-int32_t nbh_set_tuning_table(const int64_t *nmax, const int32_t *knobs, int32_t count) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
+namespace {
+int set_table(std::vector<std::pair<int64_t, Tuning>> &dst, const int64_t *nmax, const int32_t *knobs,
+              int32_t count) {
     if (count < 0 || (count > 0 && (!nmax || !knobs))) return fail(NBH_ERR_ARG, "bad table");
     std::vector<std::pair<int64_t, Tuning>> tab;
     for (int32_t i = 0; i < count; i++) {
@@ -2293,19 +2307,43 @@ int32_t nbh_set_tuning_table(const int64_t *nmax, const int32_t *knobs, int32_t 
         if (i > 0 && nmax[i] <= nmax[i - 1]) return fail(NBH_ERR_ARG, "table lengths must increase");
         tab.emplace_back(nmax[i], t);
     }
-    g.table = tab;
+    dst = tab;
     return NBH_OK;
 }
 
-int32_t nbh_get_tuning_for(int64_t N, int32_t *knobs4) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    const Tuning t = tuning_for(N);
+int get_for(int64_t N, int32_t *knobs4, bool single) {
+    if (!knobs4) return fail(NBH_ERR_ARG, "null output");
+    const Tuning t = tuning_for(N, single);
     knobs4[0] = t.grp_lds;
     knobs4[1] = t.mid_max;
     knobs4[2] = t.cmin;
     knobs4[3] = t.cmax;
     return NBH_OK;
+}
+}  // namespace
+
+int32_t nbh_set_tuning_table(const int64_t *nmax, const int32_t *knobs, int32_t count) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    return set_table(g.table, nmax, knobs, count);
+}
+
+int32_t nbh_set_tuning_table_single(const int64_t *nmax, const int32_t *knobs, int32_t count) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    return set_table(g.table_single, nmax, knobs, count);
+}
+
+int32_t nbh_get_tuning_for(int64_t N, int32_t *knobs4) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    return get_for(N, knobs4, false);
+}
+
+int32_t nbh_get_tuning_for_single(int64_t N, int32_t *knobs4) {
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    RCCHK(ensure_init());
+    return get_for(N, knobs4, true);
 }
 
 int32_t nbh_get_tuning(int32_t *grp_lds_doubles, int32_t *mid_max, int32_t *cmin, int32_t *cmax) {

@@ -127,6 +127,8 @@ SIGNATURES = {
     "nbh_get_tuning": (_i32, [_P(_i32), _P(_i32), _P(_i32), _P(_i32)]),
     "nbh_plan_groups": (_i32, [_i64, _P(_i32), _vp]),
     "nbh_set_tuning_table": (_i32, [_vp, _vp, _i32]),
+    "nbh_set_tuning_table_single": (_i32, [_vp, _vp, _i32]),
+    "nbh_get_tuning_for_single": (_i32, [_i64, _vp]),
     "nbh_get_tuning_for": (_i32, [_i64, _vp]),
     "nbh_profile_enable": (_i32, [_i32]),
     "nbh_profile_read": (_i32, [_vp, _vp, _i32]),

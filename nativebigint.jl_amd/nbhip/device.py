@@ -339,18 +339,21 @@ def get_tuning():
     return tuple(x.value for x in v)
 
 
-def set_tuning_table(entries) -> None:
-    """entries: [(Nmax, (grp_lds, mid_max, cmin, cmax)), ...] with increasing Nmax."""
+def set_tuning_table(entries, single: bool = False) -> None:
+    """entries: [(Nmax, (grp_lds, mid_max, cmin, cmax)), ...] with increasing Nmax;
+    single=True sets the table of products run alone."""
     init()
     nmax = np.array([e[0] for e in entries], dtype=np.int64)
     knobs = np.array([list(e[1]) for e in entries], dtype=np.int32).reshape(-1)
-    check(lib().nbh_set_tuning_table(nmax.ctypes.data_as(ctypes.c_void_p) if len(entries) else None,
-                                     knobs.ctypes.data_as(ctypes.c_void_p) if len(entries) else None,
-                                     len(entries)))
+    f = lib().nbh_set_tuning_table_single if single else lib().nbh_set_tuning_table
+    check(f(nmax.ctypes.data_as(ctypes.c_void_p) if len(entries) else None,
+            knobs.ctypes.data_as(ctypes.c_void_p) if len(entries) else None, len(entries)))
 
 
-def tuning_file_entries():
-    """The table the library loaded at init (tuning_<arch>.txt beside libnbh.so)."""
+def tuning_file_entries(single: bool = False):
+    """The table the library loaded at init (tuning_<arch>.txt beside
+    libnbh.so): the batch entries, or with single=True those of products run
+    alone (the lines that start with "single")."""
     import os
     from ._lib import PKG_DIR
     path = os.path.join(PKG_DIR, f"tuning_{device_info()['arch'].split(':')[0]}.txt")
@@ -358,10 +361,21 @@ def tuning_file_entries():
         return []
     out = []
     for line in open(path):
-        if line.strip() and not line.startswith("#"):
-            v = [int(x) for x in line.split()]
-            out.append((v[0], tuple(v[1:5])))
+        if not line.strip() or line.startswith("#"):
+            continue
+        w = line.split()
+        if (w[0] == "single") != single:
+            continue
+        v = [int(x) for x in w[1:] if single] if single else [int(x) for x in w]
+        out.append((v[0], tuple(v[1:5])))
     return out
+
+
+def get_tuning_for_single(N: int):
+    init()
+    out = np.zeros(4, dtype=np.int32)
+    check(lib().nbh_get_tuning_for_single(N, out.ctypes.data_as(ctypes.c_void_p)))
+    return tuple(int(v) for v in out)
 
 
 def get_tuning_for(N: int):

@@ -329,3 +329,25 @@ def test_large_by_residues():
 def test_length_error_is_argument_error():
     with pytest.raises(engine.ArgumentError):
         engine.ntt_len((315 << 34) + 1)
+
+
+def test_single_product_table():
+    """Products run alone read the 'single' lines of the schedule table (and
+    the batch lines past them); batches keep the batch lines.  Both give the
+    same limbs."""
+    single = D.tuning_file_entries(single=True)
+    if not single:
+        pytest.skip("no single-product entries shipped")
+    for nmax, knobs in single:
+        assert D.get_tuning_for_single(nmax) == knobs
+    past = single[-1][0] * 2
+    assert D.get_tuning_for_single(past) == D.get_tuning_for(past)
+    rng = np.random.default_rng(0x51)
+    for n in (2048, 4096, 9000):
+        a, b = rng.integers(0, 2**64, n, dtype=np.uint64), rng.integers(0, 2**64, n - 7, dtype=np.uint64)
+        r1 = np.zeros(2 * n - 7, dtype=np.uint64)
+        engine.mul_fpntt2(r1, 0, a, 0, n, b, 0, n - 7)
+        r2 = np.zeros(2 * n - 7, dtype=np.uint64)
+        engine.mul_batch([(r2, 0, a, 0, n, b, 0, n - 7, 0)])
+        assert np.array_equal(r1, r2)
+        assert np.array_equal(r1, O.mul(a, b))
