@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -217,6 +218,11 @@ struct Lib {
     int device = -1;
     hipStream_t stream = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    // end of the last call that used the shared single-product resources
+    // (workspace, job slots, I/O buffers), on whatever stream it ran
+    hipEvent_t last = nullptr;
+    bool last_valid = false;
+    bool last_on_lib = false;  // `last` was recorded on g.stream
     int cus = 0;
     size_t lds_block = 65536;
     size_t hbm = 0;
@@ -560,7 +566,7 @@ constexpr size_t ZERO_COPY_MIN_BYTES = (size_t)32 << 10;
 
 int ensure_pinned(size_t need) {
     if (g.pin_size >= need) return NBH_OK;
-    HIPCHK(hipStreamSynchronize(g.stream));
+    HIPCHK(hipDeviceSynchronize());
     if (g.pin) HIPCHK(hipHostFree(g.pin));
     g.pin = nullptr;
     g.pin_size = 0;
@@ -577,7 +583,7 @@ int ensure_pinned(size_t need) {
 
 int ensure_buf(void **buf, size_t *cur, size_t need) {
     if (*cur >= need) return NBH_OK;
-    HIPCHK(hipStreamSynchronize(g.stream));
+    HIPCHK(hipDeviceSynchronize());
     if (*buf) HIPCHK(hipFree(*buf));
     *buf = nullptr;
     *cur = 0;
@@ -647,24 +653,28 @@ bool schoolbook_route(int64_t m, int64_t n, bool sqr) {
     return sqr ? m < SQR_FPNTT_THRESHOLD : std::min(m, n) < MUL_FPNTT_MIN || m + n < 2 * MUL_FPNTT_THRESHOLD;
 }
 
-int read_bits(const uint64_t *a, int64_t n, int64_t *bits) {
+int read_bits(const uint64_t *a, int64_t n, int64_t *bits, hipStream_t s = nullptr) {
+    if (!s) s = g.stream;
     uint64_t top = 0;
-    HIPCHK(hipMemcpyAsync(&top, a + (n - 1), sizeof top, hipMemcpyDeviceToHost, g.stream));
-    HIPCHK(hipStreamSynchronize(g.stream));
+    HIPCHK(hipMemcpyAsync(&top, a + (n - 1), sizeof top, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
     *bits = bits_from_top(n, top);
     return NBH_OK;
 }
 
-// One product on g.stream: pack+forward groups, mid, inverse groups, unpack.
-// keep > 0: only r[0..keep) is written, the product mod beta^keep.
+// One product on stream s (null: g.stream): pack+forward groups, mid,
+// inverse groups, unpack.  keep > 0: only r[0..keep) is written, the product
+// mod beta^keep.  The caller orders it with lib_begin / lib_end.
 int run_single(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int64_t n, int64_t bits_a,
-               int64_t bits_b, bool sqr, const nbh_spectrum_s *spec = nullptr, int64_t keep = 0) {
+               int64_t bits_b, bool sqr, const nbh_spectrum_s *spec = nullptr, int64_t keep = 0,
+               hipStream_t s = nullptr) {
+    if (!s) s = g.stream;
     if (spec) { b = spec->b; n = spec->n; bits_b = spec->bits_b; }
     if (!r || !a || m < 1 || (!sqr && (!b || n < 1))) return fail(NBH_ERR_ARG, "null buffer or empty operand");
     if (sqr) { b = a; n = m; }
-    if (bits_a < 0) RCCHK(read_bits(a, m, &bits_a));
+    if (bits_a < 0) RCCHK(read_bits(a, m, &bits_a, s));
     if (sqr) bits_b = bits_a;
-    else if (bits_b < 0) RCCHK(read_bits(b, n, &bits_b));
+    else if (bits_b < 0) RCCHK(read_bits(b, n, &bits_b, s));
     DevJob J;
     std::memset(&J, 0, sizeof J);
     J.a = a;
@@ -678,7 +688,7 @@ int run_single(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int
     else RCCHK(size_job(J, bits_a, bits_b, &N));
     keep_limbs(J, keep);
     if (J.zero) {
-        HIPCHK(launch_fill_u64(r, J.rn, 0, g.stream));
+        HIPCHK(launch_fill_u64(r, J.rn, 0, s));
         return NBH_OK;
     }
     PlanEntry *P[2];
@@ -704,47 +714,48 @@ int run_single(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int
         }
     }
     DevJob *slot = g.slots + (g.slot_next++ % NSLOTS);
-    HIPCHK(launch_store_job(J, slot, g.stream));
+    HIPCHK(launch_store_job(J, slot, s));
     const DevPlan &H = P[0]->host;
     const int K = H.ngroups;
     for (int gi = 0; gi + 1 < K; gi++) {
         TileGen tg{gi, nops, H.g[gi].tiles, gi == 0 ? TILE_PACK : 0, 0};
-        ProfScope ps(0, g.stream);
+        ProfScope ps(0, s);
         HIPCHK(launch_group(slot, nullptr, tg, 2 * nops * H.g[gi].tiles, 1, P[0]->grp_lds_bytes[gi],
-                            P[0]->grp_threads[gi], g.stream));
+                            P[0]->grp_threads[gi], s));
     }
     {
         TileGen tg{K - 1, 1, H.g[K - 1].tiles, K == 1 ? TILE_PACK : 0, 0};
-        ProfScope ps(1, g.stream);
+        ProfScope ps(1, s);
         // a cached b still takes an LDS array, but only a is transformed
         HIPCHK(launch_mid(slot, nullptr, tg, 2 * H.g[K - 1].tiles, P[0]->mid_lds_bytes[sqr ? 1 : 0],
-                          P[0]->mid_threads[sqr || spec ? 1 : 0], g.stream));
+                          P[0]->mid_threads[sqr || spec ? 1 : 0], s));
     }
     for (int gi = K - 2; gi >= 0; gi--) {
         TileGen tg{gi, 1, H.g[gi].tiles, 0, 0};
-        ProfScope ps(2, g.stream);
+        ProfScope ps(2, s);
         HIPCHK(launch_group(slot, nullptr, tg, 2 * H.g[gi].tiles, 0, P[0]->grp_lds_bytes[gi], P[0]->grp_threads[gi],
-                            g.stream));
+                            s));
     }
     TileGen ug{0, 1, J.nchunks, 0, 0};
     {
-        ProfScope ps(3, g.stream);
-        HIPCHK(launch_unpack(slot, nullptr, ug, J.nchunks, J.bch, g.stream));
+        ProfScope ps(3, s);
+        HIPCHK(launch_unpack(slot, nullptr, ug, J.nchunks, J.bch, s));
     }
     if (J.nchunks > 1) {  // one chunk has no carry to pass on
-        ProfScope ps(4, g.stream);
-        HIPCHK(launch_carry_fix(slot, nullptr, ug, J.nchunks, g.stream));
+        ProfScope ps(4, s);
+        HIPCHK(launch_carry_fix(slot, nullptr, ug, J.nchunks, s));
     }
     return NBH_OK;
 }
 
-// Low short product on g.stream (mullo!/sqrlo!, src/mul.jl:305-378):
+// Low short product on stream s (mullo!/sqrlo!, src/mul.jl:305-378):
 // r[0..k) = (a mod beta^k) * (b mod beta^k) mod beta^k.  The clamped product
 // takes the engine mul!/sqr! would give it (every engine is exact, so the low
 // limbs are the reference's whichever it takes), with only its low k limbs
 // unpacked or summed.
 int run_lo(uint64_t *r, const uint64_t *a, int64_t la, const uint64_t *b, int64_t lb, int64_t k, int64_t bits_a,
-           int64_t bits_b, bool sqr) {
+           int64_t bits_b, bool sqr, hipStream_t s = nullptr) {
+    if (!s) s = g.stream;
     if (!r || !a || la < 1 || k < 1 || (!sqr && (!b || lb < 1)))
         return fail(NBH_ERR_ARG, "null buffer, empty operand or k < 1");
     if (sqr) { b = a; lb = la; bits_b = bits_a; }
@@ -756,26 +767,56 @@ int run_lo(uint64_t *r, const uint64_t *a, int64_t la, const uint64_t *b, int64_
     if (schoolbook_route(la, lb, sqr)) {
         const bool swap = lb > la;
         const BaseJob j{swap ? b : a, swap ? a : b, r, std::max(la, lb), std::min(la, lb), k};
-        HIPCHK(launch_basecase_one(j, g.stream));
+        HIPCHK(launch_basecase_one(j, s));
         return NBH_OK;
     }
-    return run_single(r, a, la, sqr ? nullptr : b, lb, bits_a, bits_b, sqr, nullptr, k);
+    return run_single(r, a, la, sqr ? nullptr : b, lb, bits_a, bits_b, sqr, nullptr, k, s);
 }
 
-// Make the library stream wait for `user`'s prior work; after the call, make
-// `user` wait for ours.  The single-product workspace is shared, so every
-// single-product call is serialised on g.stream.
+// The single-product resources (workspace, job slots, I/O and pinned
+// buffers) serve one call at a time, in submission order, whatever stream a
+// call runs on: each call's stream first waits for the event recorded at the
+// end of the previous call (lib_begin) and records it again at its own end
+// (lib_end).  Single products on a caller's stream run on that stream, so
+// the caller's own work before and after them needs no cross-stream wait.
+int lib_begin(hipStream_t s) {
+    // the library's own stream is already ordered after its own last call
+    // (a caller's stream could be a new one at a destroyed one's address, so
+    // it always waits)
+    if (g.last_valid && !(s == g.stream && g.last_on_lib)) HIPCHK(hipStreamWaitEvent(s, g.last, 0));
+    return NBH_OK;
+}
+int lib_end(hipStream_t s) {
+    HIPCHK(hipEventRecord(g.last, s));
+    g.last_valid = true;
+    g.last_on_lib = s == g.stream;
+    return NBH_OK;
+}
+
+// For entry points that run on g.stream: order it after `user`'s prior work
+// and the previous call; after the call, make `user` wait for ours.
 int stream_enter(void *user) {
+    RCCHK(lib_begin(g.stream));
     if (!user) return NBH_OK;
     HIPCHK(hipEventRecord(g.ev_in, (hipStream_t)user));
     HIPCHK(hipStreamWaitEvent(g.stream, g.ev_in, 0));
     return NBH_OK;
 }
 int stream_leave(void *user) {
+    RCCHK(lib_end(g.stream));
     if (!user) return NBH_OK;
     HIPCHK(hipEventRecord(g.ev_out, g.stream));
     HIPCHK(hipStreamWaitEvent((hipStream_t)user, g.ev_out, 0));
     return NBH_OK;
+}
+
+// A single product (or short product) on the caller's stream.
+int run_on(void *stream, const std::function<int(hipStream_t)> &f) {
+    const hipStream_t s = stream ? (hipStream_t)stream : g.stream;
+    RCCHK(lib_begin(s));
+    const int rc = f(s);
+    const int rc2 = lib_end(s);
+    return rc != NBH_OK ? rc : rc2;
 }
 
 // Host buffers in and out; keep > 0 is the low short product r[0..keep).
@@ -788,6 +829,7 @@ int host_product(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, i
         m = std::min(m, keep);
         n = std::min(n, keep);
     }
+    RCCHK(lib_begin(g.stream));
     const int64_t bits_a = bits_from_top(m, a[m - 1]);
     const int64_t bits_b = sqr ? bits_a : bits_from_top(n, b[n - 1]);
     const int64_t rl = keep ? keep : m + n;
@@ -810,6 +852,7 @@ int host_product(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, i
             uint64_t *za = (uint64_t *)d, *zb = (uint64_t *)(d + sa), *zr = (uint64_t *)(d + sa + sb);
             if (keep) RCCHK(run_lo(zr, za, m, sqr ? nullptr : zb, n, keep, bits_a, bits_b, sqr));
             else RCCHK(run_single(zr, za, m, sqr ? nullptr : zb, n, bits_a, bits_b, sqr));
+            RCCHK(lib_end(g.stream));
             HIPCHK(hipStreamSynchronize(g.stream));
             std::memcpy(r, p + sa + sb, 8 * (size_t)rl);
             return NBH_OK;
@@ -831,10 +874,12 @@ int host_product(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, i
     if (pinned) {
         char *pr = (char *)g.pin + sa + sb;
         HIPCHK(hipMemcpyAsync(pr, dr, 8 * (size_t)rl, hipMemcpyDeviceToHost, g.stream));
+        RCCHK(lib_end(g.stream));
         HIPCHK(hipStreamSynchronize(g.stream));
         std::memcpy(r, pr, 8 * (size_t)rl);
     } else {
         HIPCHK(hipMemcpyAsync(r, dr, 8 * (size_t)rl, hipMemcpyDeviceToHost, g.stream));
+        RCCHK(lib_end(g.stream));
         HIPCHK(hipStreamSynchronize(g.stream));
     }
     return NBH_OK;
@@ -1329,6 +1374,8 @@ int32_t nbh_init(int32_t device) {
     HIPCHK(hipStreamCreate(&g.stream));
     HIPCHK(hipEventCreateWithFlags(&g.ev_in, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&g.ev_out, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&g.last, hipEventDisableTiming));
+    g.last_valid = g.last_on_lib = false;
     HIPCHK(hipMalloc((void **)&g.slots, sizeof(DevJob) * NSLOTS));
     for (int i = 0; i < NSIDE; i++) HIPCHK(hipStreamCreateWithFlags(&g.side[i], hipStreamNonBlocking));
     HIPCHK(set_lds_limit(g.lds_block));
@@ -1359,6 +1406,8 @@ void nbh_shutdown(void) {
     for (int i = 0; i < NSIDE; i++) (void)hipStreamDestroy(g.side[i]);
     (void)hipEventDestroy(g.ev_in);
     (void)hipEventDestroy(g.ev_out);
+    (void)hipEventDestroy(g.last);
+    g.last_valid = false;
     (void)hipStreamDestroy(g.stream);
     g.init = false;
     g.device = -1;
@@ -1398,17 +1447,14 @@ int32_t nbh_mul_dev(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b
                     int64_t bits_b, void *stream) {
     std::lock_guard<std::recursive_mutex> lk(g.mu);
     RCCHK(ensure_init());
-    RCCHK(stream_enter(stream));
-    RCCHK(run_single(r, a, m, b, n, bits_a, bits_b, false));
-    return stream_leave(stream);
+    return run_on(stream, [&](hipStream_t s) { return run_single(r, a, m, b, n, bits_a, bits_b, false, nullptr, 0, s); });
 }
 
 int32_t nbh_sqr_dev(uint64_t *r, const uint64_t *a, int64_t n, int64_t bits_a, void *stream) {
     std::lock_guard<std::recursive_mutex> lk(g.mu);
     RCCHK(ensure_init());
-    RCCHK(stream_enter(stream));
-    RCCHK(run_single(r, a, n, nullptr, n, bits_a, bits_a, true));
-    return stream_leave(stream);
+    return run_on(stream,
+                  [&](hipStream_t s) { return run_single(r, a, n, nullptr, n, bits_a, bits_a, true, nullptr, 0, s); });
 }
 
 int32_t nbh_mullo(uint64_t *r, const uint64_t *a, int64_t la, const uint64_t *b, int64_t lb, int64_t k) {
@@ -1429,17 +1475,13 @@ int32_t nbh_mullo_dev(uint64_t *r, const uint64_t *a, int64_t la, const uint64_t
                       int64_t bits_a, int64_t bits_b, void *stream) {
     std::lock_guard<std::recursive_mutex> lk(g.mu);
     RCCHK(ensure_init());
-    RCCHK(stream_enter(stream));
-    RCCHK(run_lo(r, a, la, b, lb, k, bits_a, bits_b, false));
-    return stream_leave(stream);
+    return run_on(stream, [&](hipStream_t s) { return run_lo(r, a, la, b, lb, k, bits_a, bits_b, false, s); });
 }
 
 int32_t nbh_sqrlo_dev(uint64_t *r, const uint64_t *a, int64_t la, int64_t k, int64_t bits_a, void *stream) {
     std::lock_guard<std::recursive_mutex> lk(g.mu);
     RCCHK(ensure_init());
-    RCCHK(stream_enter(stream));
-    RCCHK(run_lo(r, a, la, nullptr, la, k, bits_a, bits_a, true));
-    return stream_leave(stream);
+    return run_on(stream, [&](hipStream_t s) { return run_lo(r, a, la, nullptr, la, k, bits_a, bits_a, true, s); });
 }
 
 int32_t nbh_batch_create(const nbh_job *jobs, int64_t count, nbh_batch *out) {
@@ -2027,9 +2069,8 @@ int32_t nbh_mul_spec_dev(uint64_t *r, const uint64_t *a, int64_t m, int64_t bits
     std::lock_guard<std::recursive_mutex> lk(g.mu);
     RCCHK(ensure_init());
     if (!s) return fail(NBH_ERR_ARG, "null spectrum");
-    RCCHK(stream_enter(stream));
-    RCCHK(run_single(r, a, m, nullptr, 0, bits_a, 0, false, s));
-    return stream_leave(stream);
+    const nbh_spectrum_s *sp = s;
+    return run_on(stream, [&](hipStream_t st) { return run_single(r, a, m, nullptr, 0, bits_a, 0, false, sp, 0, st); });
 }
 
 int32_t nbh_spec_batch_create(const nbh_spec_job *jobs, int64_t count, nbh_batch *out) {
