@@ -59,3 +59,33 @@ def test_concurrent_calls():
         t.join(timeout=100)
     assert not any(t.is_alive() for t in threads)
     assert not errors, errors
+
+
+def test_single_products_across_streams():
+    """Device-path single products share one workspace: calls issued on three
+    different streams without host synchronisation, with host-path calls in
+    between, must still each give their own product."""
+    if not gmp_ref.available():
+        pytest.skip("libgmp not found")
+    from nbhip import device as D
+    engine.init()
+    rng = np.random.default_rng(0x5717)
+    streams = [torch.cuda.Stream() for _ in range(3)]
+    cases = []
+    for i in range(24):
+        m = int(rng.integers(200, 20000))
+        n = int(rng.integers(152, m + 1))
+        a, b = rand_big_style(rng, m, 1), rand_big_style(rng, n, 1)
+        da, db = D.to_dev(a), D.to_dev(b)
+        torch.cuda.synchronize()
+        dr = D.empty_limbs(m + n)
+        with torch.cuda.stream(streams[i % 3]):
+            D.mul_dev(dr, da, db)
+        if i % 5 == 4:                      # a host-path call on the library's stream in between
+            r = np.zeros(m + n, dtype=np.uint64)
+            engine.mul_fpntt2(r, 0, a, 0, m, b, 0, n)
+            assert np.array_equal(r, gmp_ref.mul(a, b))
+        cases.append((dr, a, b))
+    torch.cuda.synchronize()
+    for dr, a, b in cases:
+        assert np.array_equal(D.to_host(dr), gmp_ref.mul(a, b))
