@@ -2081,7 +2081,7 @@ int32_t nbh_spec_batch_create(const nbh_spec_job *jobs, int64_t count, nbh_batch
     for (int64_t i = 0; i < count; i++) {
         const nbh_spec_job &q = jobs[i];
         if (!q.s) return fail(NBH_ERR_ARG, "job " + std::to_string(i) + " has no spectrum");
-        in[(size_t)i] = JobIn{q.r, q.a, q.s->b, q.m, q.s->n, q.bits_a, q.s->bits_b, q.s};
+        in[(size_t)i] = JobIn{q.r, q.a, q.s->b, q.m, q.s->n, q.bits_a, q.s->bits_b, q.s, 0};
     }
     nbh_batch_s *B = nullptr;
     RCCHK(batch_plan(in.data(), count, &B, nullptr, nullptr, 0, nullptr));
@@ -2156,7 +2156,7 @@ int32_t nbh_barrett_batch_create(nbh_barrett b, const nbh_barrett_job *jobs, int
         uint64_t *p = (uint64_t *)((char *)q2 + align256(8 * q2_limbs));
         // operands are taken at their full lengths: the values change from run
         // to run, and a length is an upper bound of the magnitude
-        j1[(size_t)i] = JobIn{q2, jobs[i].t + (k - 1), b->smu->b, k + 1, lmu, 64 * (k + 1), b->smu->bits_b, b->smu};
+        j1[(size_t)i] = JobIn{q2, jobs[i].t + (k - 1), b->smu->b, k + 1, lmu, 64 * (k + 1), b->smu->bits_b, b->smu, 0};
         // r = T - q3*m is taken mod beta^(k+1): q3*m's low k+1 limbs suffice
         j2[(size_t)i] = JobIn{p, q2 + (k + 1), b->m, lmu, k, 64 * lmu, b->sm->bits_b, b->sm, k + 1};
         fx[(size_t)i] = BarrettFixJob{jobs[i].t, p, p, jobs[i].r, b->m, k};
