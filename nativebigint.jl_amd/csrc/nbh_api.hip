@@ -568,12 +568,11 @@ int ensure_pinned(size_t need) {
     if (g.pin_size >= need) return NBH_OK;
     HIPCHK(hipDeviceSynchronize());
     if (g.pin) HIPCHK(hipHostFree(g.pin));
-    g.pin = nullptr;
+    g.pin = g.pin_dev = nullptr;
     g.pin_size = 0;
     const size_t sz = std::max(need, (size_t)1 << 20);
     HIPCHK(hipHostMalloc(&g.pin, sz, hipHostMallocDefault));
     g.pin_size = sz;
-    g.pin_dev = nullptr;
     if (hipHostGetDevicePointer(&g.pin_dev, g.pin, 0) != hipSuccess) {
         (void)hipGetLastError();
         g.pin_dev = nullptr;
@@ -1400,7 +1399,7 @@ void nbh_shutdown(void) {
     if (g.pin) (void)hipHostFree(g.pin);
     if (g.slots) (void)hipFree(g.slots);
     g.ws = g.io = nullptr;
-    g.pin = nullptr;
+    g.pin = g.pin_dev = nullptr;
     g.ws_size = g.io_size = g.pin_size = 0;
     g.slots = nullptr;
     for (int i = 0; i < NSIDE; i++) (void)hipStreamDestroy(g.side[i]);
