@@ -20,6 +20,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -1870,8 +1871,31 @@ int32_t nbh_mul_batch_dev(const nbh_job *jobs, int64_t count, void *stream) {
     return NBH_OK;
 }
 
+// f(j) for j in [0, n) on up to 8 host threads (j interleaved over them) when
+// the work is large, else on the calling thread.
+static void host_parallel_for(size_t n, bool large, const std::function<void(size_t)> &f) {
+    const unsigned hw = std::thread::hardware_concurrency();
+    const unsigned nt = large ? std::max(1u, std::min(8u, hw)) : 1u;
+    if (nt == 1 || n < 2) {
+        for (size_t j = 0; j < n; j++) f(j);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < nt; t++)
+        th.emplace_back([&, t] {
+            for (size_t j = t; j < n; j += nt) f(j);
+        });
+    for (size_t j = 0; j < n; j += nt) f(j);
+    for (auto &x : th) x.join();
+}
+
+// Groups of a host-pointer batch (operands + results) and the size from which
+// its host staging copies run on several threads.
+constexpr size_t HOST_BATCH_GROUP_BYTES = (size_t)256 << 20;
+constexpr size_t HOST_PARALLEL_COPY_BYTES = (size_t)8 << 20;
+
 // Host-pointer batch.  Jobs are staged through the device I/O buffer in
-// groups of at most PIN_MAX_BYTES (operands + results): per group one host
+// groups of at most HOST_BATCH_GROUP_BYTES (operands + results): per group one host
 // copy of the operands into the pinned buffer, one DMA in, one planned batch,
 // one DMA of the results out and one host copy out.  A job larger than that is
 // a group of its own with direct copies.  Bit lengths come from the operands'
@@ -1912,12 +1936,13 @@ int32_t nbh_mul_batch(const nbh_job *jobs, int64_t count) {
         int64_t i1 = i0;
         while (i1 < count) {
             const Staged &s = S[(size_t)i1];
-            if (i1 > i0 && ops + res + s.sa + s.sb + s.sr > PIN_MAX_BYTES) break;
+            if (i1 > i0 && ops + res + s.sa + s.sb + s.sr > HOST_BATCH_GROUP_BYTES) break;
             ops += s.sa + s.sb;
             res += s.sr;
             i1++;
         }
-        const bool pinned = ops + res <= PIN_MAX_BYTES;
+        const bool pinned = ops + res <= HOST_BATCH_GROUP_BYTES;
+        const bool large = ops + res >= HOST_PARALLEL_COPY_BYTES;
         std::vector<nbh_job> dj;
         std::vector<int64_t> idx;
         if (ops + res > 0) {
@@ -1949,14 +1974,18 @@ int32_t nbh_mul_batch(const nbh_job *jobs, int64_t count) {
         nbh_batch_s *B = nullptr;
         RCCHK(batch_create(dj.data(), (int64_t)dj.size(), &B));
         std::unique_ptr<nbh_batch_s, void (*)(nbh_batch_s *)> hold(B, batch_free);
-        for (size_t j = 0; j < dj.size(); j++) {
-            const nbh_job &q = jobs[idx[j]];
-            const nbh_job &d = dj[j];
-            if (pinned) {
+        if (pinned) {
+            host_parallel_for(dj.size(), large, [&](size_t j) {
+                const nbh_job &q = jobs[idx[j]];
+                const nbh_job &d = dj[j];
                 char *p = (char *)g.pin + ((const char *)d.a - dev);
                 std::memcpy(p, q.a, 8 * (size_t)d.m);
                 if (q.b) std::memcpy(p + S[(size_t)idx[j]].sa, q.b, 8 * (size_t)d.n);
-            } else {
+            });
+        } else {
+            for (size_t j = 0; j < dj.size(); j++) {
+                const nbh_job &q = jobs[idx[j]];
+                const nbh_job &d = dj[j];
                 HIPCHK(hipMemcpyAsync((void *)d.a, q.a, 8 * (size_t)d.m, hipMemcpyHostToDevice, g.stream));
                 if (q.b) HIPCHK(hipMemcpyAsync((void *)d.b, q.b, 8 * (size_t)d.n, hipMemcpyHostToDevice, g.stream));
             }
@@ -1979,9 +2008,10 @@ int32_t nbh_mul_batch(const nbh_job *jobs, int64_t count) {
         RCCHK(rc);
         HIPCHK(es);
         if (pinned)
-            for (size_t j = 0; j < dj.size(); j++)
+            host_parallel_for(dj.size(), large, [&](size_t j) {
                 std::memcpy(jobs[idx[j]].r, (char *)g.pin + ((const char *)dj[j].r - dev),
                             8 * (size_t)S[(size_t)idx[j]].rl);
+            });
         i0 = i1;
     }
     return NBH_OK;

@@ -23,7 +23,7 @@ SQR_FPNTT_THRESHOLD = 160
 
 __all__ = ["MUL_FPNTT_MIN", "MUL_FPNTT_THRESHOLD", "SQR_FPNTT_THRESHOLD", "ArgumentError", "NbhError",
            "magnitude_bits", "ntt_len", "fp_ntt_params", "mul_fpntt2", "sqr_fpntt2", "mul", "sqr", "mul_bal",
-           "mullo", "sqrlo", "mul_batch", "init"]
+           "mullo", "sqrlo", "mul_batch", "mul_batch_records", "init"]
 
 
 def _u64(buf) -> np.ndarray:
@@ -131,10 +131,11 @@ def sqrlo(r, ro: int, a, ao: int, la: int, k: int) -> None:
     check(lib().nbh_sqrlo(_ptr(r, ro), _ptr(a, ao), la, k))
 
 
-def mul_batch(jobs) -> None:
-    """Many independent products on host limb buffers in one call
-    (nbh_mul_batch): each job is one mul!/sqr!/mullo!/sqrlo! of the reference
-    (src/mul.jl:127-164, 305-389).  A job is the tuple
+def mul_batch_records(jobs):
+    """The checked nbh_job records of a host-buffer batch (None if empty),
+    for callers that issue the same batch repeatedly (nbh_mul_batch): each
+    job is one mul!/sqr!/mullo!/sqrlo! of the reference (src/mul.jl:127-164,
+    305-389).  A job is the tuple
     (r, ro, a, ao, m, b, bo, n, k): b None for a square (bo, n ignored), k 0
     for the full product r[ro:ro+m+n] (2m for a square), k >= 1 for the low
     short product r[ro:ro+k] with the operands clamped to k limbs.  Every size
@@ -166,7 +167,7 @@ def mul_batch(jobs) -> None:
             spans.append((pb, pb + 8 * nn, False))
         spans.append((pr, pr + 8 * rl, True))
     if not recs:
-        return
+        return None
     spans.sort()
     end_all = end_res = 0
     for s, e, is_res in spans:
@@ -178,5 +179,14 @@ def mul_batch(jobs) -> None:
     arr = (_lib.nbh_job * len(recs))()
     for i, (pr, pa, pb, mm, nn, _, _, k) in enumerate(recs):
         arr[i] = _lib.nbh_job(pr, pa, pb or None, mm, nn, 0, 0, k)
+    return arr
+
+
+def mul_batch(jobs) -> None:
+    """Many independent products on host limb buffers in one call
+    (nbh_mul_batch); see mul_batch_records for the job tuples."""
+    arr = mul_batch_records(jobs)
+    if arr is None:
+        return
     init()
-    check(lib().nbh_mul_batch(arr, len(recs)))
+    check(lib().nbh_mul_batch(arr, len(arr)))

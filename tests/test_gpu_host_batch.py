@@ -104,10 +104,11 @@ def test_matches_single_products():
 
 
 def test_several_groups_and_an_oversized_job():
-    """Six 2^19 x 2^19 products (16 MiB of operands and result each) need
-    two staging groups; a 2^21+4096 square (96 MiB) is a group of its own."""
+    """Eighteen 2^19 x 2^19 products (16 MiB of operands and result each)
+    need two 256 MiB staging groups; a 12M-limb square (288 MiB) is a group
+    of its own, copied directly."""
     rng = np.random.default_rng(0x6B)
-    specs = [(1 << 19, 1 << 19, 0, 1)] * 6 + [((1 << 21) + 4096, None, 0, 2), (300, 200, 0, 1)]
+    specs = [(1 << 19, 1 << 19, 0, 1)] * 18 + [(12 << 20, None, 0, 2), (300, 200, 0, 1)]
     _run(specs, rng)
 
 
