@@ -1,6 +1,7 @@
 """Benchmark: bit-exact limb-products/s of the MI355X NTT product engine.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batch|large|hgcd|spectra|powermod|serve]
+python bench.py [--gpus 1] [--steps K] [--warmup W]
+                [--workload batch|large|hgcd|spectra|powermod|serve|small|single|shortprod|hostbatch]
 
 workload "batch" (default): 4096 independent products, larger size
 log-uniform in [152, 65536] limbs, 25% squarings, size ratio uniform in
@@ -21,7 +22,9 @@ workload "powermod": 64 bases raised to one 64-bit exponent modulo a fixed
 2048-limb modulus (powermod_limbs' Barrett branch, src/algorithms.jl:17-107):
 per modular product a product batch and a Barrett reduction batch against the
 cached spectra; limb-products counts the result limbs of all three products.
-workload "serve": Poisson arrivals into the continuous-batching server.
+workload "serve": Poisson arrivals into the library's serving queue
+(nbh_server_*) and, on the same arrivals, the Python loop over the arena
+planner.
 workload "small": 4096 products below the NTT thresholds (limbs log-uniform
 in [1, 151], 25% squares): the batch's schoolbook kernel.
 workload "single": the plain single product of 4096 x 4096 limbs and the
@@ -33,6 +36,9 @@ workload "shortprod": 1024 low short products k x k -> k limbs (mullo!/sqrlo!,
 src/mul.jl:305-378), k log-uniform in [224, 65536] where the reference falls
 back to a full product, 25% sqrlo, in one planned batch, against the same
 operands as full products.  limb-products counts the k kept limbs.
+workload "hostbatch": the batch list with operands and results in host
+memory, one nbh_mul_batch call per step: the PCIe-inclusive rate (never the
+headline: that one has its inputs resident in HBM).
 
 limb-products/s = sum over completed bit-exact products of (m + n) result
 limbs / wall time, inputs already resident in HBM.  Every product of the
