@@ -242,6 +242,14 @@ struct Lib {
     void *pin = nullptr;  // pinned host bounce buffer of the host-pointer API
     void *pin_dev = nullptr;  // its device address (null: kernels do not read it directly)
     size_t pin_size = 0;
+    // host-pointer batches: two staging slots (pinned operands + results, and
+    // the device copy with the batch's arena), so one group's host copies
+    // overlap the device work of the one before
+    struct HostSlot {
+        void *pin = nullptr, *dev = nullptr;
+        size_t pin_size = 0, dev_size = 0;
+        hipEvent_t ev = nullptr;
+    } hs[2];
     DevJob *slots = nullptr;
     int slot_next = 0;
     hipStream_t side[8] = {};  // batch phases fork onto these (default priority)
@@ -1398,6 +1406,12 @@ void nbh_shutdown(void) {
     if (g.ws) (void)hipFree(g.ws);
     if (g.io) (void)hipFree(g.io);
     if (g.pin) (void)hipHostFree(g.pin);
+    for (auto &h : g.hs) {
+        if (h.pin) (void)hipHostFree(h.pin);
+        if (h.dev) (void)hipFree(h.dev);
+        if (h.ev) (void)hipEventDestroy(h.ev);
+        h = Lib::HostSlot{};
+    }
     if (g.slots) (void)hipFree(g.slots);
     g.ws = g.io = nullptr;
     g.pin = g.pin_dev = nullptr;
@@ -1891,15 +1905,19 @@ static void host_parallel_for(size_t n, bool large, const std::function<void(siz
 
 // Groups of a host-pointer batch (operands + results) and the size from which
 // its host staging copies run on several threads.
-constexpr size_t HOST_BATCH_GROUP_BYTES = (size_t)256 << 20;
+constexpr size_t HOST_BATCH_GROUP_BYTES = (size_t)128 << 20;
 constexpr size_t HOST_PARALLEL_COPY_BYTES = (size_t)8 << 20;
 
-// Host-pointer batch.  Jobs are staged through the device I/O buffer in
-// groups of at most HOST_BATCH_GROUP_BYTES (operands + results): per group one host
-// copy of the operands into the pinned buffer, one DMA in, one planned batch,
-// one DMA of the results out and one host copy out.  A job larger than that is
-// a group of its own with direct copies.  Bit lengths come from the operands'
-// top limbs on the host; a product with a zero operand is written on the host.
+// Host-pointer batch.  Jobs are staged in groups of at most
+// HOST_BATCH_GROUP_BYTES (operands + results) through two staging slots: per
+// group a host copy of the operands into the slot's pinned buffer, one DMA
+// in, one batch planned inside the slot's device buffer (no allocation, no
+// host synchronisation), one DMA of the results out and an event; the host
+// copies the results out when the slot comes round again, so one group's
+// host copies overlap the device work of the group before.  A job larger
+// than a group runs on its own with direct copies.  Bit lengths come from the
+// operands' top limbs on the host; a product with a zero operand is written
+// on the host.
 int32_t nbh_mul_batch(const nbh_job *jobs, int64_t count) {
     std::lock_guard<std::recursive_mutex> lk(g.mu);
     RCCHK(ensure_init());
@@ -1929,7 +1947,41 @@ int32_t nbh_mul_batch(const nbh_job *jobs, int64_t count) {
         s.sb = s.zero || sqr ? 0 : align256(8 * (size_t)n);
         s.sr = s.zero ? 0 : align256(8 * (size_t)s.rl);
     }
+    // A group in flight in a staging slot: its batch, its jobs (device
+    // records and caller indices) and where its results sit in the slot.
+    struct Pending {
+        nbh_batch_s *B = nullptr;
+        std::vector<nbh_job> dj;
+        std::vector<int64_t> idx;
+        char *dev = nullptr;
+        bool busy = false, large = false;
+    } pend[2];
+    // wait for slot k's group, copy its results to the caller, release it
+    auto finish = [&](int k) -> int {
+        Pending &P = pend[k];
+        if (!P.busy) return NBH_OK;
+        P.busy = false;
+        const hipError_t e = hipEventSynchronize(g.hs[k].ev);
+        batch_free(P.B);  // host side only: the batch lives in the slot's arena
+        P.B = nullptr;
+        HIPCHK(e);
+        char *pin = (char *)g.hs[k].pin;
+        host_parallel_for(P.dj.size(), P.large, [&](size_t j) {
+            std::memcpy(jobs[P.idx[j]].r, pin + ((const char *)P.dj[j].r - P.dev), 8 * (size_t)S[(size_t)P.idx[j]].rl);
+        });
+        return NBH_OK;
+    };
+    // on an error: let both slots' work finish before the buffers can be reused
+    auto drain = [&]() {
+        (void)hipStreamSynchronize(g.stream);
+        for (auto &P : pend)
+            if (P.busy) {
+                batch_free(P.B);
+                P = Pending{};
+            }
+    };
     int64_t i0 = 0;
+    int slot = 0;
     while (i0 < count) {
         // the group [i0, i1): operands first, then the results, each back to back
         size_t ops = 0, res = 0;
@@ -1941,78 +1993,140 @@ int32_t nbh_mul_batch(const nbh_job *jobs, int64_t count) {
             res += s.sr;
             i1++;
         }
-        const bool pinned = ops + res <= HOST_BATCH_GROUP_BYTES;
-        const bool large = ops + res >= HOST_PARALLEL_COPY_BYTES;
         std::vector<nbh_job> dj;
         std::vector<int64_t> idx;
-        if (ops + res > 0) {
-            RCCHK(ensure_buf(&g.io, &g.io_size, ops + res));
-            if (pinned) RCCHK(ensure_pinned(ops + res));
-        }
-        char *dev = (char *)g.io;
-        size_t oa = 0, orr = ops;
         for (int64_t i = i0; i < i1; i++) {
-            Staged &s = S[(size_t)i];
-            if (s.zero) {
-                std::memset(jobs[i].r, 0, 8 * (size_t)s.rl);
+            if (S[(size_t)i].zero) {
+                std::memset(jobs[i].r, 0, 8 * (size_t)S[(size_t)i].rl);
                 continue;
             }
-            nbh_job d = s.d;
-            d.a = (const uint64_t *)(dev + oa);
-            d.b = jobs[i].b ? (const uint64_t *)(dev + oa + s.sa) : nullptr;
-            d.r = (uint64_t *)(dev + orr);
-            oa += s.sa + s.sb;
-            orr += s.sr;
-            dj.push_back(d);
+            dj.push_back(S[(size_t)i].d);
             idx.push_back(i);
         }
         if (dj.empty()) {
             i0 = i1;
             continue;
         }
-        // plan before any copy (planning reads no operand: every bit length is known)
-        nbh_batch_s *B = nullptr;
-        RCCHK(batch_create(dj.data(), (int64_t)dj.size(), &B));
-        std::unique_ptr<nbh_batch_s, void (*)(nbh_batch_s *)> hold(B, batch_free);
-        if (pinned) {
-            host_parallel_for(dj.size(), large, [&](size_t j) {
-                const nbh_job &q = jobs[idx[j]];
-                const nbh_job &d = dj[j];
-                char *p = (char *)g.pin + ((const char *)d.a - dev);
-                std::memcpy(p, q.a, 8 * (size_t)d.m);
-                if (q.b) std::memcpy(p + S[(size_t)idx[j]].sa, q.b, 8 * (size_t)d.n);
-            });
-        } else {
+        if (ops + res > HOST_BATCH_GROUP_BYTES) {
+            // one job larger than a group: on its own, copied directly
+            for (int k = 0; k < 2; k++) {
+                const int rc = finish(k);
+                if (rc != NBH_OK) { drain(); return rc; }
+            }
+            RCCHK(ensure_buf(&g.io, &g.io_size, ops + res));
+            char *dev = (char *)g.io;
+            const int64_t i = idx[0];
+            nbh_job d = dj[0];
+            const Staged &st = S[(size_t)i];
+            d.a = (const uint64_t *)dev;
+            d.b = jobs[i].b ? (const uint64_t *)(dev + st.sa) : nullptr;
+            d.r = (uint64_t *)(dev + ops);
+            nbh_batch_s *B = nullptr;
+            RCCHK(batch_create(&d, 1, &B));
+            std::unique_ptr<nbh_batch_s, void (*)(nbh_batch_s *)> hold(B, batch_free);
+            HIPCHK(hipMemcpyAsync((void *)d.a, jobs[i].a, 8 * (size_t)d.m, hipMemcpyHostToDevice, g.stream));
+            if (jobs[i].b) HIPCHK(hipMemcpyAsync((void *)d.b, jobs[i].b, 8 * (size_t)d.n, hipMemcpyHostToDevice, g.stream));
+            int rc = batch_run(B, g.stream);
+            if (rc == NBH_OK) {
+                const hipError_t e = hipMemcpyAsync(jobs[i].r, d.r, 8 * (size_t)st.rl, hipMemcpyDeviceToHost, g.stream);
+                if (e != hipSuccess) rc = fail(NBH_ERR_HIP, std::string("copy out: ") + hipGetErrorString(e));
+            }
+            const hipError_t es = hipStreamSynchronize(g.stream);
+            RCCHK(rc);
+            HIPCHK(es);
+            i0 = i1;
+            continue;
+        }
+        // a staging slot: wait for the group it held two groups ago
+        {
+            const int rc = finish(slot);
+            if (rc != NBH_OK) { drain(); return rc; }
+        }
+        Lib::HostSlot &H = g.hs[slot];
+        // lay the group out: operands, results, then the batch's arena
+        std::vector<size_t> offa(dj.size()), offr(dj.size());
+        size_t oa = 0, orr = ops;
+        for (size_t j = 0; j < dj.size(); j++) {
+            const Staged &st = S[(size_t)idx[j]];
+            offa[j] = oa;
+            offr[j] = orr;
+            oa += st.sa + st.sb;
+            orr += st.sr;
+        }
+        auto place = [&](char *base) {
             for (size_t j = 0; j < dj.size(); j++) {
-                const nbh_job &q = jobs[idx[j]];
-                const nbh_job &d = dj[j];
-                HIPCHK(hipMemcpyAsync((void *)d.a, q.a, 8 * (size_t)d.m, hipMemcpyHostToDevice, g.stream));
-                if (q.b) HIPCHK(hipMemcpyAsync((void *)d.b, q.b, 8 * (size_t)d.n, hipMemcpyHostToDevice, g.stream));
+                dj[j].a = (const uint64_t *)(base + offa[j]);
+                dj[j].b = jobs[idx[j]].b ? (const uint64_t *)(base + offa[j] + S[(size_t)idx[j]].sa) : nullptr;
+                dj[j].r = (uint64_t *)(base + offr[j]);
+            }
+        };
+        size_t need = 0;
+        {
+            // sizing reads no operand: any non-null base will do
+            place(reinterpret_cast<char *>((uintptr_t)4096));
+            const int rc = batch_create(dj.data(), (int64_t)dj.size(), nullptr, &need);
+            if (rc != NBH_OK) { drain(); return rc; }
+        }
+        need = align256(need);
+        if (H.dev_size < ops + res + need || H.pin_size < ops + res || !H.ev) {
+            HIPCHK(hipDeviceSynchronize());
+            if (!H.ev) HIPCHK(hipEventCreateWithFlags(&H.ev, hipEventDisableTiming));
+            if (H.dev_size < ops + res + need) {
+                if (H.dev) HIPCHK(hipFree(H.dev));
+                H.dev = nullptr;
+                H.dev_size = 0;
+                HIPCHK(hipMalloc(&H.dev, ops + res + need));
+                H.dev_size = ops + res + need;
+            }
+            if (H.pin_size < ops + res) {
+                if (H.pin) HIPCHK(hipHostFree(H.pin));
+                H.pin = nullptr;
+                H.pin_size = 0;
+                HIPCHK(hipHostMalloc(&H.pin, ops + res, hipHostMallocDefault));
+                H.pin_size = ops + res;
             }
         }
-        if (pinned) HIPCHK(hipMemcpyAsync(dev, g.pin, ops, hipMemcpyHostToDevice, g.stream));
-        int rc = batch_run(B, g.stream);
-        if (rc == NBH_OK) {
-            hipError_t e = hipSuccess;
-            if (pinned) {
-                e = hipMemcpyAsync((char *)g.pin + ops, dev + ops, res, hipMemcpyDeviceToHost, g.stream);
-            } else {
-                for (size_t j = 0; j < dj.size() && e == hipSuccess; j++)
-                    e = hipMemcpyAsync(jobs[idx[j]].r, dj[j].r, 8 * (size_t)S[(size_t)idx[j]].rl,
-                                       hipMemcpyDeviceToHost, g.stream);
-            }
-            if (e != hipSuccess) rc = fail(NBH_ERR_HIP, std::string("copy out: ") + hipGetErrorString(e));
+        char *dev = (char *)H.dev;
+        place(dev);
+        const bool large = ops + res >= HOST_PARALLEL_COPY_BYTES;
+        char *pin = (char *)H.pin;
+        host_parallel_for(dj.size(), large, [&](size_t j) {
+            const nbh_job &q = jobs[idx[j]];
+            char *p = pin + ((const char *)dj[j].a - dev);
+            std::memcpy(p, q.a, 8 * (size_t)dj[j].m);
+            if (q.b) std::memcpy(p + S[(size_t)idx[j]].sa, q.b, 8 * (size_t)dj[j].n);
+        });
+        // tables into the arena, operands in, the batch, results out, then
+        // the slot's event, all in order on g.stream
+        nbh_batch_s *B = nullptr;
+        int rc = batch_create(dj.data(), (int64_t)dj.size(), &B, nullptr, dev + ops + res, need, g.stream);
+        auto hip_ok = [&](hipError_t e, const char *what) {
+            if (e != hipSuccess && rc == NBH_OK) rc = fail(NBH_ERR_HIP, std::string(what) + hipGetErrorString(e));
+        };
+        if (rc == NBH_OK) hip_ok(hipMemcpyAsync(dev, pin, ops, hipMemcpyHostToDevice, g.stream), "copy in: ");
+        if (rc == NBH_OK) rc = batch_run(B, g.stream);
+        if (rc == NBH_OK) hip_ok(hipMemcpyAsync(pin + ops, dev + ops, res, hipMemcpyDeviceToHost, g.stream), "copy out: ");
+        if (rc == NBH_OK) hip_ok(hipEventRecord(H.ev, g.stream), "event: ");
+        if (rc != NBH_OK) {
+            drain();
+            if (B) batch_free(B);
+            return rc;
         }
-        // wait even after a failure: the pinned and staging buffers are reused
-        const hipError_t es = hipStreamSynchronize(g.stream);
-        RCCHK(rc);
-        HIPCHK(es);
-        if (pinned)
-            host_parallel_for(dj.size(), large, [&](size_t j) {
-                std::memcpy(jobs[idx[j]].r, (char *)g.pin + ((const char *)dj[j].r - dev),
-                            8 * (size_t)S[(size_t)idx[j]].rl);
-            });
+        Pending &P = pend[slot];
+        P.B = B;
+        P.dj = std::move(dj);
+        P.idx = std::move(idx);
+        P.dev = dev;
+        P.busy = true;
+        P.large = large;
+        slot ^= 1;
         i0 = i1;
+    }
+    // the last two groups, oldest first
+    for (int k = 0; k < 2; k++) {
+        const int rc = finish(slot);
+        if (rc != NBH_OK) { drain(); return rc; }
+        slot ^= 1;
     }
     return NBH_OK;
 }

@@ -105,8 +105,8 @@ def test_matches_single_products():
 
 def test_several_groups_and_an_oversized_job():
     """Eighteen 2^19 x 2^19 products (16 MiB of operands and result each)
-    need two 256 MiB staging groups; a 12M-limb square (288 MiB) is a group
-    of its own, copied directly."""
+    need three 128 MiB staging groups, so a staging slot is reused; a
+    12M-limb square (288 MiB) runs on its own, copied directly."""
     rng = np.random.default_rng(0x6B)
     specs = [(1 << 19, 1 << 19, 0, 1)] * 18 + [(12 << 20, None, 0, 2), (300, 200, 0, 1)]
     _run(specs, rng)
