@@ -99,9 +99,69 @@ def traffic(fetch_db: str, write_db: str, kernels, copy_bytes: int, copy_kernel:
     return out
 
 
+def _md_table(path: str) -> dict:
+    """The per-dispatch table of a counter summary written by this tool."""
+    txt = open(path).read().split("per dispatch")[1]
+    lines = txt.split("\n")
+    hdr = [x for x in lines if x.startswith("| kernel")][0]
+    cols = [c.strip() for c in hdr.strip("|").split("|")]
+    out = {}
+    for row in (x for x in lines if x.startswith("| nbh::")):
+        v = [c.strip() for c in row.strip("|").split("|")]
+        out[v[0]] = {cols[i]: float(v[i]) for i in range(1, len(v))}
+    return out
+
+
+def counters_report(pass1: str, pass2: str, trace: str, commands: list[str], kernels: list[str],
+                    cus: int = 256, ghz: float = 2.4) -> str:
+    """Derived SQ-counter figures per kernel from two counter summaries of
+    this tool (pass1: SQ_INSTS_VALU, its F64 parts, SQ_INSTS_SALU,
+    SQ_INSTS_LDS; pass2: SQ_WAVE_CYCLES) and the kernel-trace summary of the
+    same command (average launch time)."""
+    t1, t2 = _md_table(pass1), _md_table(pass2)
+    avg = {}
+    for line in open(trace):
+        m = re.match(r"\| (nbh::\S+) \| (\d+) \| [\d.]+ \| ([\d.]+) \|", line)
+        if m:
+            avg[m.group(1)] = float(m.group(3))
+    out = ["# SQ counters: batch workload, launches in order", "",
+           "Commands (one pass each, summarised per dispatch by `python3 -m nbhip.profsum <results.db>`, "
+           "then by `python3 -m nbhip.profsum --counters pass1.md pass2.md kernel_trace_batch.md`):", "",
+           "```", *commands, "```", "",
+           "Average launch time from `kernel_trace_batch.md` (same workload and flags). Derived columns:",
+           "VALU wave-instructions per dispatch; FP64 share of them; VALU issue use = VALU wave-instructions",
+           f"/ ({cus} CUs x 4 SIMDs x launch cycles at {ghz} GHz / 4 cycles per wave64 instruction); resident "
+           "waves", f"per CU = SQ_WAVE_CYCLES (quad-cycles) x 4 / launch cycles / {cus}.", "",
+           "| kernel | avg us | VALU instr | FP64 share | VALU issue use | FP64 issue use | SALU instr | LDS instr | "
+           "waves/CU |", "|---|---|---|---|---|---|---|---|---|"]
+    for k in kernels:
+        if k not in t1 or k not in t2 or k not in avg:
+            continue
+        a, b, us = t1[k], t2[k], avg[k]
+        cyc = us * 1e-6 * ghz * 1e9
+        cap = cus * 4 * cyc / 4
+        valu = a["SQ_INSTS_VALU"]
+        f64 = a["SQ_INSTS_VALU_ADD_F64"] + a["SQ_INSTS_VALU_MUL_F64"] + a["SQ_INSTS_VALU_FMA_F64"]
+        waves = b["SQ_WAVE_CYCLES"] * 4 / cyc / cus
+        out.append(f"| {k[5:]} | {us:.1f} | {valu:.3g} | {f64 / valu:.0%} | {valu / cap:.0%} | {f64 / cap:.0%} | "
+                   f"{a['SQ_INSTS_SALU']:.3g} | {a['SQ_INSTS_LDS']:.3g} | {waves:.1f} |")
+    out += ["", "Per-dispatch counter values:", ""]
+    for path in (pass1, pass2):
+        txt = open(path).read().split("per dispatch (sum / dispatches):")[1]
+        listed = {k.rsplit("_w6", 1)[0] for k in kernels}     # both builds of the group kernels
+        out += [x for x in txt.strip().split("\n")
+                if x.startswith("|") and ("kernel" in x or "---" in x or any(k in x for k in listed))]
+        out.append("")
+    return "\n".join(out)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("db")
+    ap.add_argument("db", nargs="?", default="")
+    ap.add_argument("--counters", nargs=3, metavar=("PASS1_MD", "PASS2_MD", "TRACE_MD"),
+                    help="derived SQ-counter table from two counter summaries and a kernel-trace summary")
+    ap.add_argument("--counter-commands", default="",
+                    help="counters mode: the rocprofv3 commands of the two passes, separated by ';;'")
     ap.add_argument("--title", default="rocprofv3 summary")
     ap.add_argument("--command", default="")
     ap.add_argument("--dispatches", default="", help="also list every dispatch of kernels matching this")
@@ -111,6 +171,12 @@ def main():
     ap.add_argument("--kernels", default="k_mid,k_group_fwd,k_group_inv,k_group_fwd_w6,k_group_inv_w6,k_unpack,"
                                           "k_carry_fix,k_combine,k_barrett_fix")
     a = ap.parse_args()
+    if a.counters:
+        ks = ["nbh::k_mid", "nbh::k_group_fwd_w6", "nbh::k_group_inv_w6", "nbh::k_unpack"]
+        print(counters_report(*a.counters, [c for c in a.counter_commands.split(";;") if c], ks))
+        return
+    if not a.db:
+        ap.error("a results database is needed")
     if a.traffic_write_db:
         import json
         res = traffic(a.db, a.traffic_write_db, a.kernels.split(","), a.copy_bytes)
