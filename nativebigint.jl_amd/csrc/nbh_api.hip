@@ -1895,11 +1895,18 @@ static void host_parallel_for(size_t n, bool large, const std::function<void(siz
         return;
     }
     std::vector<std::thread> th;
-    for (unsigned t = 1; t < nt; t++)
-        th.emplace_back([&, t] {
+    unsigned started = 1;  // thread 0 is the caller
+    try {
+        for (unsigned t = 1; t < nt; t++, started++)
+            th.emplace_back([&, t] {
+                for (size_t j = t; j < n; j += nt) f(j);
+            });
+    } catch (...) {
+        // no thread could be started: the caller takes the remaining residues
+    }
+    for (unsigned t = 0; t < nt; t++)
+        if (t == 0 || t >= started)
             for (size_t j = t; j < n; j += nt) f(j);
-        });
-    for (size_t j = 0; j < n; j += nt) f(j);
     for (auto &x : th) x.join();
 }
 
