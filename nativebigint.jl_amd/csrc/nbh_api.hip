@@ -19,6 +19,8 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <thread>
 #include <tuple>
@@ -58,6 +60,21 @@ int fail(int code, const std::string &msg) {
             return fail(e_ == hipErrorOutOfMemory ? NBH_ERR_OOM : NBH_ERR_HIP,              \
                         std::string(#expr) + ": " + hipGetErrorString(e_));                 \
     } while (0)
+
+// Entry points return an error code for every failure: no C++ exception
+// (a failed host allocation) crosses the C ABI.
+#define NBH_GUARD_BEGIN try {
+#define NBH_GUARD_END                                                          \
+    }                                                                          \
+    catch (const std::bad_alloc &) {                                           \
+        return fail(NBH_ERR_OOM, "host memory allocation failed");             \
+    }                                                                          \
+    catch (const std::exception &ex_) {                                        \
+        return fail(NBH_ERR_ARG, std::string("unexpected failure: ") + ex_.what()); \
+    }                                                                          \
+    catch (...) {                                                              \
+        return fail(NBH_ERR_ARG, "unexpected failure");                        \
+    }
 
 #define RCCHK(expr)                \
     do {                           \
@@ -1360,39 +1377,41 @@ int hgcd_create(const nbh_hgcd_job *jobs, int64_t count, int mode, nbh_hgcd_s **
 extern "C" {
 
 int32_t nbh_init(int32_t device) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    if (g.init) {
-        if (device != g.device) return fail(NBH_ERR_ARG, "already initialised on another device");
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        if (g.init) {
+            if (device != g.device) return fail(NBH_ERR_ARG, "already initialised on another device");
+            return NBH_OK;
+        }
+        int n = 0;
+        HIPCHK(hipGetDeviceCount(&n));
+        if (device < 0 || device >= n) return fail(NBH_ERR_ARG, "no such HIP device");
+        HIPCHK(hipSetDevice(device));
+        hipDeviceProp_t prop;
+        HIPCHK(hipGetDeviceProperties(&prop, device));
+        g.device = device;
+        g.cus = prop.multiProcessorCount;
+        g.lds_block = prop.sharedMemPerBlock;
+        g.hbm = prop.totalGlobalMem;
+        g.name = prop.gcnArchName;
+        if (1.0 / P1_D != P1_INV || 1.0 / P2_D != P2_INV) return fail(NBH_ERR_HIP, "fl(1/p) constant mismatch");
+        // A blocking stream: it orders with the legacy default stream (handle 0),
+        // which is where callers such as torch enqueue by default.
+        HIPCHK(hipStreamCreate(&g.stream));
+        HIPCHK(hipEventCreateWithFlags(&g.ev_in, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&g.ev_out, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&g.last, hipEventDisableTiming));
+        g.last_valid = g.last_on_lib = false;
+        HIPCHK(hipMalloc((void **)&g.slots, sizeof(DevJob) * NSLOTS));
+        for (int i = 0; i < NSIDE; i++) HIPCHK(hipStreamCreateWithFlags(&g.side[i], hipStreamNonBlocking));
+        HIPCHK(set_lds_limit(g.lds_block));
+        load_tuning_file();
+        // keep the default tiles inside the device's per-block LDS
+        while (8 * (size_t)lds_doubles(g.tun.mid_max) > g.lds_block && g.tun.mid_max > 256) g.tun.mid_max >>= 1;
+        while (8 * (size_t)lds_doubles(g.tun.grp_lds) > g.lds_block && g.tun.grp_lds > 256) g.tun.grp_lds >>= 1;
+        g.init = true;
         return NBH_OK;
-    }
-    int n = 0;
-    HIPCHK(hipGetDeviceCount(&n));
-    if (device < 0 || device >= n) return fail(NBH_ERR_ARG, "no such HIP device");
-    HIPCHK(hipSetDevice(device));
-    hipDeviceProp_t prop;
-    HIPCHK(hipGetDeviceProperties(&prop, device));
-    g.device = device;
-    g.cus = prop.multiProcessorCount;
-    g.lds_block = prop.sharedMemPerBlock;
-    g.hbm = prop.totalGlobalMem;
-    g.name = prop.gcnArchName;
-    if (1.0 / P1_D != P1_INV || 1.0 / P2_D != P2_INV) return fail(NBH_ERR_HIP, "fl(1/p) constant mismatch");
-    // A blocking stream: it orders with the legacy default stream (handle 0),
-    // which is where callers such as torch enqueue by default.
-    HIPCHK(hipStreamCreate(&g.stream));
-    HIPCHK(hipEventCreateWithFlags(&g.ev_in, hipEventDisableTiming));
-    HIPCHK(hipEventCreateWithFlags(&g.ev_out, hipEventDisableTiming));
-    HIPCHK(hipEventCreateWithFlags(&g.last, hipEventDisableTiming));
-    g.last_valid = g.last_on_lib = false;
-    HIPCHK(hipMalloc((void **)&g.slots, sizeof(DevJob) * NSLOTS));
-    for (int i = 0; i < NSIDE; i++) HIPCHK(hipStreamCreateWithFlags(&g.side[i], hipStreamNonBlocking));
-    HIPCHK(set_lds_limit(g.lds_block));
-    load_tuning_file();
-    // keep the default tiles inside the device's per-block LDS
-    while (8 * (size_t)lds_doubles(g.tun.mid_max) > g.lds_block && g.tun.mid_max > 256) g.tun.mid_max >>= 1;
-    while (8 * (size_t)lds_doubles(g.tun.grp_lds) > g.lds_block && g.tun.grp_lds > 256) g.tun.grp_lds >>= 1;
-    g.init = true;
-    return NBH_OK;
+    NBH_GUARD_END
 }
 
 void nbh_shutdown(void) {
@@ -1430,89 +1449,113 @@ void nbh_shutdown(void) {
 const char *nbh_last_error_string(void) { return t_err.c_str(); }
 
 int32_t nbh_ntt_len(int64_t T, int64_t *N) {
-    if (!N || T < 1) return fail(NBH_ERR_ARG, "bad argument");
-    const int64_t n = ntt_len(T);
-    if (n < 0) return fail(NBH_ERR_LENGTH, "fp NTT length exceeds the 315*2^34 the primes support");
-    *N = n;
-    return NBH_OK;
+    NBH_GUARD_BEGIN
+        if (!N || T < 1) return fail(NBH_ERR_ARG, "bad argument");
+        const int64_t n = ntt_len(T);
+        if (n < 0) return fail(NBH_ERR_LENGTH, "fp NTT length exceeds the 315*2^34 the primes support");
+        *N = n;
+        return NBH_OK;
+    NBH_GUARD_END
 }
 
 int32_t nbh_ntt_params(int64_t bits_a, int64_t bits_b, int32_t *b, int64_t *N) {
-    if (!b || !N || bits_a < 1 || bits_b < 1) return fail(NBH_ERR_ARG, "bad argument");
-    int bb = 0;
-    RCCHK(ntt_params(bits_a, bits_b, &bb, N));
-    *b = bb;
-    return NBH_OK;
+    NBH_GUARD_BEGIN
+        if (!b || !N || bits_a < 1 || bits_b < 1) return fail(NBH_ERR_ARG, "bad argument");
+        int bb = 0;
+        RCCHK(ntt_params(bits_a, bits_b, &bb, N));
+        *b = bb;
+        return NBH_OK;
+    NBH_GUARD_END
 }
 
 int32_t nbh_mul(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int64_t n) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    return host_product(r, a, m, b, n, false);
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        return host_product(r, a, m, b, n, false);
+    NBH_GUARD_END
 }
 
 int32_t nbh_sqr(uint64_t *r, const uint64_t *a, int64_t n) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    return host_product(r, a, n, nullptr, n, true);
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        return host_product(r, a, n, nullptr, n, true);
+    NBH_GUARD_END
 }
 
 int32_t nbh_mul_dev(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int64_t n, int64_t bits_a,
                     int64_t bits_b, void *stream) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    return run_on(stream, [&](hipStream_t s) { return run_single(r, a, m, b, n, bits_a, bits_b, false, nullptr, 0, s); });
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        return run_on(stream, [&](hipStream_t s) { return run_single(r, a, m, b, n, bits_a, bits_b, false, nullptr, 0, s); });
+    NBH_GUARD_END
 }
 
 int32_t nbh_sqr_dev(uint64_t *r, const uint64_t *a, int64_t n, int64_t bits_a, void *stream) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    return run_on(stream,
-                  [&](hipStream_t s) { return run_single(r, a, n, nullptr, n, bits_a, bits_a, true, nullptr, 0, s); });
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        return run_on(stream,
+                      [&](hipStream_t s) { return run_single(r, a, n, nullptr, n, bits_a, bits_a, true, nullptr, 0, s); });
+    NBH_GUARD_END
 }
 
 int32_t nbh_mullo(uint64_t *r, const uint64_t *a, int64_t la, const uint64_t *b, int64_t lb, int64_t k) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    if (k < 1) return fail(NBH_ERR_ARG, "k < 1");
-    return host_product(r, a, la, b, lb, false, k);
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        if (k < 1) return fail(NBH_ERR_ARG, "k < 1");
+        return host_product(r, a, la, b, lb, false, k);
+    NBH_GUARD_END
 }
 
 int32_t nbh_sqrlo(uint64_t *r, const uint64_t *a, int64_t la, int64_t k) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    if (k < 1) return fail(NBH_ERR_ARG, "k < 1");
-    return host_product(r, a, la, nullptr, la, true, k);
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        if (k < 1) return fail(NBH_ERR_ARG, "k < 1");
+        return host_product(r, a, la, nullptr, la, true, k);
+    NBH_GUARD_END
 }
 
 int32_t nbh_mullo_dev(uint64_t *r, const uint64_t *a, int64_t la, const uint64_t *b, int64_t lb, int64_t k,
                       int64_t bits_a, int64_t bits_b, void *stream) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    return run_on(stream, [&](hipStream_t s) { return run_lo(r, a, la, b, lb, k, bits_a, bits_b, false, s); });
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        return run_on(stream, [&](hipStream_t s) { return run_lo(r, a, la, b, lb, k, bits_a, bits_b, false, s); });
+    NBH_GUARD_END
 }
 
 int32_t nbh_sqrlo_dev(uint64_t *r, const uint64_t *a, int64_t la, int64_t k, int64_t bits_a, void *stream) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    return run_on(stream, [&](hipStream_t s) { return run_lo(r, a, la, nullptr, la, k, bits_a, bits_a, true, s); });
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        return run_on(stream, [&](hipStream_t s) { return run_lo(r, a, la, nullptr, la, k, bits_a, bits_a, true, s); });
+    NBH_GUARD_END
 }
 
 int32_t nbh_batch_create(const nbh_job *jobs, int64_t count, nbh_batch *out) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    if (!out) return fail(NBH_ERR_ARG, "null handle");
-    nbh_batch_s *B = nullptr;
-    RCCHK(batch_create(jobs, count, &B));
-    *out = B;
-    return NBH_OK;
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        if (!out) return fail(NBH_ERR_ARG, "null handle");
+        nbh_batch_s *B = nullptr;
+        RCCHK(batch_create(jobs, count, &B));
+        *out = B;
+        return NBH_OK;
+    NBH_GUARD_END
 }
 
 int32_t nbh_batch_run(nbh_batch b, void *stream) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    if (!b) return fail(NBH_ERR_ARG, "null handle");
-    return batch_run(b, stream ? (hipStream_t)stream : g.stream);
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        if (!b) return fail(NBH_ERR_ARG, "null handle");
+        return batch_run(b, stream ? (hipStream_t)stream : g.stream);
+    NBH_GUARD_END
 }
 
 // Capture the batch's whole launch sequence -- every phase, and the fork /
@@ -1520,62 +1563,70 @@ int32_t nbh_batch_run(nbh_batch b, void *stream) {
 // stream, so a run is one graph launch.  The sequence records kernels and
 // events only (no memset nodes).  Per-launch profiling is off while capturing.
 int32_t nbh_batch_capture(nbh_batch b) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    if (!b) return fail(NBH_ERR_ARG, "null handle");
-    if (b->gexec) return NBH_OK;
-    hipStream_t cs = nullptr;
-    HIPCHK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-    const bool prof = g.prof;
-    g.prof = false;
-    hipGraph_t graph = nullptr;
-    int rc = NBH_OK;
-    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
-    if (e == hipSuccess) {
-        rc = batch_run(b, cs);
-        e = hipStreamEndCapture(cs, &graph);
-    }
-    g.prof = prof;
-    if (e == hipSuccess && rc == NBH_OK) e = hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0);
-    (void)hipStreamDestroy(cs);
-    if (rc != NBH_OK || e != hipSuccess) {
-        if (graph) (void)hipGraphDestroy(graph);
-        b->gexec = nullptr;
-        return rc != NBH_OK ? rc : fail(NBH_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-    }
-    b->graph = graph;
-    return NBH_OK;
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        if (!b) return fail(NBH_ERR_ARG, "null handle");
+        if (b->gexec) return NBH_OK;
+        hipStream_t cs = nullptr;
+        HIPCHK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        const bool prof = g.prof;
+        g.prof = false;
+        hipGraph_t graph = nullptr;
+        int rc = NBH_OK;
+        hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            rc = batch_run(b, cs);
+            e = hipStreamEndCapture(cs, &graph);
+        }
+        g.prof = prof;
+        if (e == hipSuccess && rc == NBH_OK) e = hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0);
+        (void)hipStreamDestroy(cs);
+        if (rc != NBH_OK || e != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            b->gexec = nullptr;
+            return rc != NBH_OK ? rc : fail(NBH_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        }
+        b->graph = graph;
+        return NBH_OK;
+    NBH_GUARD_END
 }
 
 int32_t nbh_batch_replay(nbh_batch b, void *stream) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    if (!b) return fail(NBH_ERR_ARG, "null handle");
-    if (!b->gexec) RCCHK(nbh_batch_capture(b));
-    HIPCHK(hipGraphLaunch(b->gexec, stream ? (hipStream_t)stream : g.stream));
-    return NBH_OK;
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        if (!b) return fail(NBH_ERR_ARG, "null handle");
+        if (!b->gexec) RCCHK(nbh_batch_capture(b));
+        HIPCHK(hipGraphLaunch(b->gexec, stream ? (hipStream_t)stream : g.stream));
+        return NBH_OK;
+    NBH_GUARD_END
 }
 
 int32_t nbh_batch_arena_bytes(const nbh_job *jobs, int64_t count, int64_t *bytes) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    if (!bytes) return fail(NBH_ERR_ARG, "null output");
-    size_t need = 0;
-    RCCHK(batch_create(jobs, count, nullptr, &need));
-    *bytes = (int64_t)need;
-    return NBH_OK;
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        if (!bytes) return fail(NBH_ERR_ARG, "null output");
+        size_t need = 0;
+        RCCHK(batch_create(jobs, count, nullptr, &need));
+        *bytes = (int64_t)need;
+        return NBH_OK;
+    NBH_GUARD_END
 }
 
 int32_t nbh_batch_create_in(const nbh_job *jobs, int64_t count, void *arena, int64_t arena_bytes, void *stream,
                             nbh_batch *out) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    if (!out || !arena || arena_bytes <= 0) return fail(NBH_ERR_ARG, "bad arena");
-    nbh_batch_s *B = nullptr;
-    RCCHK(batch_create(jobs, count, &B, nullptr, arena, (size_t)arena_bytes,
-                       stream ? (hipStream_t)stream : g.stream));
-    *out = B;
-    return NBH_OK;
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        if (!out || !arena || arena_bytes <= 0) return fail(NBH_ERR_ARG, "bad arena");
+        nbh_batch_s *B = nullptr;
+        RCCHK(batch_create(jobs, count, &B, nullptr, arena, (size_t)arena_bytes,
+                           stream ? (hipStream_t)stream : g.stream));
+        *out = B;
+        return NBH_OK;
+    NBH_GUARD_END
 }
 
 void nbh_batch_release(nbh_batch b) {
@@ -1740,91 +1791,101 @@ void srv_free(nbh_server_s *S) {
 }  // namespace
 
 int32_t nbh_server_create(int64_t budget_bytes, int32_t nstreams, int32_t max_batch, nbh_server *out) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    if (!out || budget_bytes < 4096 || nstreams < 1 || nstreams > 16 || max_batch < 1)
-        return fail(NBH_ERR_ARG, "bad server parameters");
-    *out = nullptr;
-    std::unique_ptr<nbh_server_s, void (*)(nbh_server_s *)> S(new nbh_server_s, srv_free);
-    S->budget = (size_t)budget_bytes & ~(size_t)255;
-    S->max_batch = max_batch;
-    HIPCHK(hipMalloc((void **)&S->arena, S->budget));
-    S->free_list.push_back({0, S->budget});
-    S->flights.resize((size_t)nstreams);
-    for (auto &f : S->flights) {
-        HIPCHK(hipStreamCreateWithFlags(&f.stream, hipStreamNonBlocking));
-        HIPCHK(hipEventCreateWithFlags(&f.ev, hipEventDisableTiming));
-    }
-    *out = S.release();
-    return NBH_OK;
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        if (!out || budget_bytes < 4096 || nstreams < 1 || nstreams > 16 || max_batch < 1)
+            return fail(NBH_ERR_ARG, "bad server parameters");
+        *out = nullptr;
+        std::unique_ptr<nbh_server_s, void (*)(nbh_server_s *)> S(new nbh_server_s, srv_free);
+        S->budget = (size_t)budget_bytes & ~(size_t)255;
+        S->max_batch = max_batch;
+        HIPCHK(hipMalloc((void **)&S->arena, S->budget));
+        S->free_list.push_back({0, S->budget});
+        S->flights.resize((size_t)nstreams);
+        for (auto &f : S->flights) {
+            HIPCHK(hipStreamCreateWithFlags(&f.stream, hipStreamNonBlocking));
+            HIPCHK(hipEventCreateWithFlags(&f.ev, hipEventDisableTiming));
+        }
+        *out = S.release();
+        return NBH_OK;
+    NBH_GUARD_END
 }
 
 int32_t nbh_server_submit(nbh_server s, const nbh_job *job, int64_t *ticket) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    if (!s || !job || !ticket) return fail(NBH_ERR_ARG, "null argument");
-    nbh_job j = *job;
-    const bool sqr = j.b == nullptr;
-    if (!j.r || !j.a || j.m < 1 || (!sqr && j.n < 1) || j.k < 0) return fail(NBH_ERR_ARG, "bad job");
-    if (sqr) j.n = j.m;
-    // admission sizes a request from its bit lengths: read the top limbs if not given
-    if (j.bits_a <= 0) RCCHK(read_bits(j.a, j.m, &j.bits_a));
-    if (sqr) j.bits_b = j.bits_a;
-    else if (j.bits_b <= 0) RCCHK(read_bits(j.b, j.n, &j.bits_b));
-    // bit lengths are upper bounds: a zero operand is sized as one bit (the
-    // engine still computes its zero product exactly)
-    j.bits_a = std::max<int64_t>(j.bits_a, 1);
-    j.bits_b = std::max<int64_t>(j.bits_b, 1);
-    size_t need = 0;
-    RCCHK(srv_need(*s, j, &need));
-    if (need > s->budget) return fail(NBH_ERR_BUDGET, "request exceeds the server's HBM budget");
-    *ticket = s->next_ticket++;
-    s->pending.push_back({j, *ticket, need});
-    return NBH_OK;
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        if (!s || !job || !ticket) return fail(NBH_ERR_ARG, "null argument");
+        nbh_job j = *job;
+        const bool sqr = j.b == nullptr;
+        if (!j.r || !j.a || j.m < 1 || (!sqr && j.n < 1) || j.k < 0) return fail(NBH_ERR_ARG, "bad job");
+        if (sqr) j.n = j.m;
+        // admission sizes a request from its bit lengths: read the top limbs if not given
+        if (j.bits_a <= 0) RCCHK(read_bits(j.a, j.m, &j.bits_a));
+        if (sqr) j.bits_b = j.bits_a;
+        else if (j.bits_b <= 0) RCCHK(read_bits(j.b, j.n, &j.bits_b));
+        // bit lengths are upper bounds: a zero operand is sized as one bit (the
+        // engine still computes its zero product exactly)
+        j.bits_a = std::max<int64_t>(j.bits_a, 1);
+        j.bits_b = std::max<int64_t>(j.bits_b, 1);
+        size_t need = 0;
+        RCCHK(srv_need(*s, j, &need));
+        if (need > s->budget) return fail(NBH_ERR_BUDGET, "request exceeds the server's HBM budget");
+        *ticket = s->next_ticket++;
+        s->pending.push_back({j, *ticket, need});
+        return NBH_OK;
+    NBH_GUARD_END
 }
 
 int32_t nbh_server_poll(nbh_server s, int64_t *done, int64_t cap, int64_t *ndone) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    if (!s || !ndone || cap < 0 || (cap > 0 && !done)) return fail(NBH_ERR_ARG, "bad argument");
-    RCCHK(srv_retire(*s));
-    RCCHK(srv_admit(*s));
-    int64_t n = 0;
-    while (n < cap && !s->done.empty()) {
-        done[n++] = s->done.front();
-        s->done.pop_front();
-    }
-    *ndone = n;
-    return NBH_OK;
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        if (!s || !ndone || cap < 0 || (cap > 0 && !done)) return fail(NBH_ERR_ARG, "bad argument");
+        RCCHK(srv_retire(*s));
+        RCCHK(srv_admit(*s));
+        int64_t n = 0;
+        while (n < cap && !s->done.empty()) {
+            done[n++] = s->done.front();
+            s->done.pop_front();
+        }
+        *ndone = n;
+        return NBH_OK;
+    NBH_GUARD_END
 }
 
 int32_t nbh_server_drain(nbh_server s) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    if (!s) return fail(NBH_ERR_ARG, "null handle");
-    for (;;) {
-        RCCHK(srv_retire(*s));
-        RCCHK(srv_admit(*s));
-        bool busy = false;
-        for (auto &f : s->flights)
-            if (f.B) {
-                busy = true;
-                HIPCHK(hipEventSynchronize(f.ev));
-                break;
-            }
-        if (!busy && s->pending.empty()) return NBH_OK;
-    }
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        if (!s) return fail(NBH_ERR_ARG, "null handle");
+        for (;;) {
+            RCCHK(srv_retire(*s));
+            RCCHK(srv_admit(*s));
+            bool busy = false;
+            for (auto &f : s->flights)
+                if (f.B) {
+                    busy = true;
+                    HIPCHK(hipEventSynchronize(f.ev));
+                    break;
+                }
+            if (!busy && s->pending.empty()) return NBH_OK;
+        }
+    NBH_GUARD_END
 }
 
 int32_t nbh_server_stats(nbh_server s, int64_t *pending, int64_t *in_flight, int64_t *batches) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    if (!s) return fail(NBH_ERR_ARG, "null handle");
-    int64_t fl = 0;
-    for (auto &f : s->flights) fl += f.B ? (int64_t)f.tickets.size() : 0;
-    if (pending) *pending = (int64_t)s->pending.size();
-    if (in_flight) *in_flight = fl;
-    if (batches) *batches = s->batches;
-    return NBH_OK;
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        if (!s) return fail(NBH_ERR_ARG, "null handle");
+        int64_t fl = 0;
+        for (auto &f : s->flights) fl += f.B ? (int64_t)f.tickets.size() : 0;
+        if (pending) *pending = (int64_t)s->pending.size();
+        if (in_flight) *in_flight = fl;
+        if (batches) *batches = s->batches;
+        return NBH_OK;
+    NBH_GUARD_END
 }
 
 void nbh_server_destroy(nbh_server s) {
@@ -1834,16 +1895,18 @@ void nbh_server_destroy(nbh_server s) {
 
 // Device memory and copies for callers with no GPU array type of their own.
 int32_t nbh_alloc(int64_t bytes, void **ptr) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    if (!ptr || bytes < 1) return fail(NBH_ERR_ARG, "bad argument");
-    *ptr = nullptr;
-    if (hipMalloc(ptr, (size_t)bytes) != hipSuccess) {
-        (void)hipGetLastError();
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        if (!ptr || bytes < 1) return fail(NBH_ERR_ARG, "bad argument");
         *ptr = nullptr;
-        return fail(NBH_ERR_OOM, "device allocation of " + std::to_string(bytes) + " bytes failed");
-    }
-    return NBH_OK;
+        if (hipMalloc(ptr, (size_t)bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            *ptr = nullptr;
+            return fail(NBH_ERR_OOM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+        }
+        return NBH_OK;
+    NBH_GUARD_END
 }
 
 void nbh_free(void *ptr) {
@@ -1855,34 +1918,40 @@ void nbh_free(void *ptr) {
 }
 
 int32_t nbh_upload(void *dst, const void *src, int64_t bytes) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    if (!dst || !src || bytes < 0) return fail(NBH_ERR_ARG, "bad argument");
-    HIPCHK(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice));
-    return NBH_OK;
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        if (!dst || !src || bytes < 0) return fail(NBH_ERR_ARG, "bad argument");
+        HIPCHK(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice));
+        return NBH_OK;
+    NBH_GUARD_END
 }
 
 int32_t nbh_download(void *dst, const void *src, int64_t bytes) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    if (!dst || !src || bytes < 0) return fail(NBH_ERR_ARG, "bad argument");
-    HIPCHK(hipDeviceSynchronize());
-    HIPCHK(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost));
-    return NBH_OK;
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        if (!dst || !src || bytes < 0) return fail(NBH_ERR_ARG, "bad argument");
+        HIPCHK(hipDeviceSynchronize());
+        HIPCHK(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost));
+        return NBH_OK;
+    NBH_GUARD_END
 }
 
 int32_t nbh_mul_batch_dev(const nbh_job *jobs, int64_t count, void *stream) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    nbh_batch_s *B = nullptr;
-    RCCHK(batch_create(jobs, count, &B));
-    hipStream_t s = stream ? (hipStream_t)stream : g.stream;
-    int rc = batch_run(B, s);
-    hipError_t e = hipStreamSynchronize(s);
-    batch_free(B);
-    if (rc) return rc;
-    HIPCHK(e);
-    return NBH_OK;
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        nbh_batch_s *B = nullptr;
+        RCCHK(batch_create(jobs, count, &B));
+        hipStream_t s = stream ? (hipStream_t)stream : g.stream;
+        int rc = batch_run(B, s);
+        hipError_t e = hipStreamSynchronize(s);
+        batch_free(B);
+        if (rc) return rc;
+        HIPCHK(e);
+        return NBH_OK;
+    NBH_GUARD_END
 }
 
 // f(j) for j in [0, n) on up to 8 host threads (j interleaved over them) when
@@ -1926,274 +1995,278 @@ constexpr size_t HOST_PARALLEL_COPY_BYTES = (size_t)8 << 20;
 // operands' top limbs on the host; a product with a zero operand is written
 // on the host.
 int32_t nbh_mul_batch(const nbh_job *jobs, int64_t count) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    if (count < 1 || !jobs) return fail(NBH_ERR_ARG, "empty batch");
-    struct Staged {
-        nbh_job d;          // sizes and bit lengths; pointers set when the group is placed
-        size_t sa, sb, sr;  // aligned bytes of a, b and r in the staging buffer
-        int64_t rl;         // result limbs
-        bool zero;
-    };
-    std::vector<Staged> S((size_t)count);
-    for (int64_t i = 0; i < count; i++) {
-        const nbh_job &q = jobs[i];
-        const bool sqr = q.b == nullptr;
-        if (!q.r || !q.a || q.m < 1 || (!sqr && q.n < 1) || q.k < 0)
-            return fail(NBH_ERR_ARG, "bad job " + std::to_string(i));
-        Staged &s = S[(size_t)i];
-        // a short product reads only the operands' low k limbs (mullo!, src/mul.jl:308-309)
-        const int64_t m = q.k ? std::min(q.m, q.k) : q.m;
-        const int64_t n = sqr ? m : (q.k ? std::min(q.n, q.k) : q.n);
-        const int64_t ba = bits_from_top(m, q.a[m - 1]);
-        const int64_t bb = sqr ? ba : bits_from_top(n, q.b[n - 1]);
-        s.d = nbh_job{nullptr, nullptr, nullptr, m, n, ba, bb, q.k};
-        s.rl = q.k ? q.k : m + n;
-        s.zero = ba == 0 || bb == 0;
-        s.sa = s.zero ? 0 : align256(8 * (size_t)m);
-        s.sb = s.zero || sqr ? 0 : align256(8 * (size_t)n);
-        s.sr = s.zero ? 0 : align256(8 * (size_t)s.rl);
-    }
-    // A group in flight in a staging slot: its batch, its jobs (device
-    // records and caller indices) and where its results sit in the slot.
-    struct Pending {
-        nbh_batch_s *B = nullptr;
-        std::vector<nbh_job> dj;
-        std::vector<int64_t> idx;
-        char *dev = nullptr;
-        bool busy = false, large = false;
-    } pend[2];
-    // wait for slot k's group, copy its results to the caller, release it
-    auto finish = [&](int k) -> int {
-        Pending &P = pend[k];
-        if (!P.busy) return NBH_OK;
-        P.busy = false;
-        const hipError_t e = hipEventSynchronize(g.hs[k].ev);
-        batch_free(P.B);  // host side only: the batch lives in the slot's arena
-        P.B = nullptr;
-        HIPCHK(e);
-        char *pin = (char *)g.hs[k].pin;
-        host_parallel_for(P.dj.size(), P.large, [&](size_t j) {
-            std::memcpy(jobs[P.idx[j]].r, pin + ((const char *)P.dj[j].r - P.dev), 8 * (size_t)S[(size_t)P.idx[j]].rl);
-        });
-        return NBH_OK;
-    };
-    // on an error: let both slots' work finish before the buffers can be reused
-    auto drain = [&]() {
-        (void)hipStreamSynchronize(g.stream);
-        for (auto &P : pend)
-            if (P.busy) {
-                batch_free(P.B);
-                P = Pending{};
-            }
-    };
-    int64_t i0 = 0;
-    int slot = 0;
-    while (i0 < count) {
-        // the group [i0, i1): operands first, then the results, each back to back
-        size_t ops = 0, res = 0;
-        int64_t i1 = i0;
-        while (i1 < count) {
-            const Staged &s = S[(size_t)i1];
-            if (i1 > i0 && ops + res + s.sa + s.sb + s.sr > HOST_BATCH_GROUP_BYTES) break;
-            ops += s.sa + s.sb;
-            res += s.sr;
-            i1++;
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        if (count < 1 || !jobs) return fail(NBH_ERR_ARG, "empty batch");
+        struct Staged {
+            nbh_job d;          // sizes and bit lengths; pointers set when the group is placed
+            size_t sa, sb, sr;  // aligned bytes of a, b and r in the staging buffer
+            int64_t rl;         // result limbs
+            bool zero;
+        };
+        std::vector<Staged> S((size_t)count);
+        for (int64_t i = 0; i < count; i++) {
+            const nbh_job &q = jobs[i];
+            const bool sqr = q.b == nullptr;
+            if (!q.r || !q.a || q.m < 1 || (!sqr && q.n < 1) || q.k < 0)
+                return fail(NBH_ERR_ARG, "bad job " + std::to_string(i));
+            Staged &s = S[(size_t)i];
+            // a short product reads only the operands' low k limbs (mullo!, src/mul.jl:308-309)
+            const int64_t m = q.k ? std::min(q.m, q.k) : q.m;
+            const int64_t n = sqr ? m : (q.k ? std::min(q.n, q.k) : q.n);
+            const int64_t ba = bits_from_top(m, q.a[m - 1]);
+            const int64_t bb = sqr ? ba : bits_from_top(n, q.b[n - 1]);
+            s.d = nbh_job{nullptr, nullptr, nullptr, m, n, ba, bb, q.k};
+            s.rl = q.k ? q.k : m + n;
+            s.zero = ba == 0 || bb == 0;
+            s.sa = s.zero ? 0 : align256(8 * (size_t)m);
+            s.sb = s.zero || sqr ? 0 : align256(8 * (size_t)n);
+            s.sr = s.zero ? 0 : align256(8 * (size_t)s.rl);
         }
-        std::vector<nbh_job> dj;
-        std::vector<int64_t> idx;
-        for (int64_t i = i0; i < i1; i++) {
-            if (S[(size_t)i].zero) {
-                std::memset(jobs[i].r, 0, 8 * (size_t)S[(size_t)i].rl);
+        // A group in flight in a staging slot: its batch, its jobs (device
+        // records and caller indices) and where its results sit in the slot.
+        struct Pending {
+            nbh_batch_s *B = nullptr;
+            std::vector<nbh_job> dj;
+            std::vector<int64_t> idx;
+            char *dev = nullptr;
+            bool busy = false, large = false;
+        } pend[2];
+        // wait for slot k's group, copy its results to the caller, release it
+        auto finish = [&](int k) -> int {
+            Pending &P = pend[k];
+            if (!P.busy) return NBH_OK;
+            P.busy = false;
+            const hipError_t e = hipEventSynchronize(g.hs[k].ev);
+            batch_free(P.B);  // host side only: the batch lives in the slot's arena
+            P.B = nullptr;
+            HIPCHK(e);
+            char *pin = (char *)g.hs[k].pin;
+            host_parallel_for(P.dj.size(), P.large, [&](size_t j) {
+                std::memcpy(jobs[P.idx[j]].r, pin + ((const char *)P.dj[j].r - P.dev), 8 * (size_t)S[(size_t)P.idx[j]].rl);
+            });
+            return NBH_OK;
+        };
+        // on an error: let both slots' work finish before the buffers can be reused
+        auto drain = [&]() {
+            (void)hipStreamSynchronize(g.stream);
+            for (auto &P : pend)
+                if (P.busy) {
+                    batch_free(P.B);
+                    P = Pending{};
+                }
+        };
+        int64_t i0 = 0;
+        int slot = 0;
+        while (i0 < count) {
+            // the group [i0, i1): operands first, then the results, each back to back
+            size_t ops = 0, res = 0;
+            int64_t i1 = i0;
+            while (i1 < count) {
+                const Staged &s = S[(size_t)i1];
+                if (i1 > i0 && ops + res + s.sa + s.sb + s.sr > HOST_BATCH_GROUP_BYTES) break;
+                ops += s.sa + s.sb;
+                res += s.sr;
+                i1++;
+            }
+            std::vector<nbh_job> dj;
+            std::vector<int64_t> idx;
+            for (int64_t i = i0; i < i1; i++) {
+                if (S[(size_t)i].zero) {
+                    std::memset(jobs[i].r, 0, 8 * (size_t)S[(size_t)i].rl);
+                    continue;
+                }
+                dj.push_back(S[(size_t)i].d);
+                idx.push_back(i);
+            }
+            if (dj.empty()) {
+                i0 = i1;
                 continue;
             }
-            dj.push_back(S[(size_t)i].d);
-            idx.push_back(i);
-        }
-        if (dj.empty()) {
-            i0 = i1;
-            continue;
-        }
-        if (ops + res > HOST_BATCH_GROUP_BYTES) {
-            // one job larger than a group: on its own, copied directly
-            for (int k = 0; k < 2; k++) {
-                const int rc = finish(k);
+            if (ops + res > HOST_BATCH_GROUP_BYTES) {
+                // one job larger than a group: on its own, copied directly
+                for (int k = 0; k < 2; k++) {
+                    const int rc = finish(k);
+                    if (rc != NBH_OK) { drain(); return rc; }
+                }
+                RCCHK(ensure_buf(&g.io, &g.io_size, ops + res));
+                char *dev = (char *)g.io;
+                const int64_t i = idx[0];
+                nbh_job d = dj[0];
+                const Staged &st = S[(size_t)i];
+                d.a = (const uint64_t *)dev;
+                d.b = jobs[i].b ? (const uint64_t *)(dev + st.sa) : nullptr;
+                d.r = (uint64_t *)(dev + ops);
+                nbh_batch_s *B = nullptr;
+                RCCHK(batch_create(&d, 1, &B));
+                std::unique_ptr<nbh_batch_s, void (*)(nbh_batch_s *)> hold(B, batch_free);
+                HIPCHK(hipMemcpyAsync((void *)d.a, jobs[i].a, 8 * (size_t)d.m, hipMemcpyHostToDevice, g.stream));
+                if (jobs[i].b) HIPCHK(hipMemcpyAsync((void *)d.b, jobs[i].b, 8 * (size_t)d.n, hipMemcpyHostToDevice, g.stream));
+                int rc = batch_run(B, g.stream);
+                if (rc == NBH_OK) {
+                    const hipError_t e = hipMemcpyAsync(jobs[i].r, d.r, 8 * (size_t)st.rl, hipMemcpyDeviceToHost, g.stream);
+                    if (e != hipSuccess) rc = fail(NBH_ERR_HIP, std::string("copy out: ") + hipGetErrorString(e));
+                }
+                const hipError_t es = hipStreamSynchronize(g.stream);
+                RCCHK(rc);
+                HIPCHK(es);
+                i0 = i1;
+                continue;
+            }
+            // a staging slot: wait for the group it held two groups ago
+            {
+                const int rc = finish(slot);
                 if (rc != NBH_OK) { drain(); return rc; }
             }
-            RCCHK(ensure_buf(&g.io, &g.io_size, ops + res));
-            char *dev = (char *)g.io;
-            const int64_t i = idx[0];
-            nbh_job d = dj[0];
-            const Staged &st = S[(size_t)i];
-            d.a = (const uint64_t *)dev;
-            d.b = jobs[i].b ? (const uint64_t *)(dev + st.sa) : nullptr;
-            d.r = (uint64_t *)(dev + ops);
-            nbh_batch_s *B = nullptr;
-            RCCHK(batch_create(&d, 1, &B));
-            std::unique_ptr<nbh_batch_s, void (*)(nbh_batch_s *)> hold(B, batch_free);
-            HIPCHK(hipMemcpyAsync((void *)d.a, jobs[i].a, 8 * (size_t)d.m, hipMemcpyHostToDevice, g.stream));
-            if (jobs[i].b) HIPCHK(hipMemcpyAsync((void *)d.b, jobs[i].b, 8 * (size_t)d.n, hipMemcpyHostToDevice, g.stream));
-            int rc = batch_run(B, g.stream);
-            if (rc == NBH_OK) {
-                const hipError_t e = hipMemcpyAsync(jobs[i].r, d.r, 8 * (size_t)st.rl, hipMemcpyDeviceToHost, g.stream);
-                if (e != hipSuccess) rc = fail(NBH_ERR_HIP, std::string("copy out: ") + hipGetErrorString(e));
+            Lib::HostSlot &H = g.hs[slot];
+            // lay the group out: operands, results, then the batch's arena
+            std::vector<size_t> offa(dj.size()), offr(dj.size());
+            size_t oa = 0, orr = ops;
+            for (size_t j = 0; j < dj.size(); j++) {
+                const Staged &st = S[(size_t)idx[j]];
+                offa[j] = oa;
+                offr[j] = orr;
+                oa += st.sa + st.sb;
+                orr += st.sr;
             }
-            const hipError_t es = hipStreamSynchronize(g.stream);
-            RCCHK(rc);
-            HIPCHK(es);
+            auto place = [&](char *base) {
+                for (size_t j = 0; j < dj.size(); j++) {
+                    dj[j].a = (const uint64_t *)(base + offa[j]);
+                    dj[j].b = jobs[idx[j]].b ? (const uint64_t *)(base + offa[j] + S[(size_t)idx[j]].sa) : nullptr;
+                    dj[j].r = (uint64_t *)(base + offr[j]);
+                }
+            };
+            size_t need = 0;
+            {
+                // sizing reads no operand: any non-null base will do
+                place(reinterpret_cast<char *>((uintptr_t)4096));
+                const int rc = batch_create(dj.data(), (int64_t)dj.size(), nullptr, &need);
+                if (rc != NBH_OK) { drain(); return rc; }
+            }
+            need = align256(need);
+            if (H.dev_size < ops + res + need || H.pin_size < ops + res || !H.ev) {
+                HIPCHK(hipDeviceSynchronize());
+                if (!H.ev) HIPCHK(hipEventCreateWithFlags(&H.ev, hipEventDisableTiming));
+                if (H.dev_size < ops + res + need) {
+                    if (H.dev) HIPCHK(hipFree(H.dev));
+                    H.dev = nullptr;
+                    H.dev_size = 0;
+                    HIPCHK(hipMalloc(&H.dev, ops + res + need));
+                    H.dev_size = ops + res + need;
+                }
+                if (H.pin_size < ops + res) {
+                    if (H.pin) HIPCHK(hipHostFree(H.pin));
+                    H.pin = nullptr;
+                    H.pin_size = 0;
+                    HIPCHK(hipHostMalloc(&H.pin, ops + res, hipHostMallocDefault));
+                    H.pin_size = ops + res;
+                }
+            }
+            char *dev = (char *)H.dev;
+            place(dev);
+            const bool large = ops + res >= HOST_PARALLEL_COPY_BYTES;
+            char *pin = (char *)H.pin;
+            host_parallel_for(dj.size(), large, [&](size_t j) {
+                const nbh_job &q = jobs[idx[j]];
+                char *p = pin + ((const char *)dj[j].a - dev);
+                std::memcpy(p, q.a, 8 * (size_t)dj[j].m);
+                if (q.b) std::memcpy(p + S[(size_t)idx[j]].sa, q.b, 8 * (size_t)dj[j].n);
+            });
+            // tables into the arena, operands in, the batch, results out, then
+            // the slot's event, all in order on g.stream
+            nbh_batch_s *B = nullptr;
+            int rc = batch_create(dj.data(), (int64_t)dj.size(), &B, nullptr, dev + ops + res, need, g.stream);
+            auto hip_ok = [&](hipError_t e, const char *what) {
+                if (e != hipSuccess && rc == NBH_OK) rc = fail(NBH_ERR_HIP, std::string(what) + hipGetErrorString(e));
+            };
+            if (rc == NBH_OK) hip_ok(hipMemcpyAsync(dev, pin, ops, hipMemcpyHostToDevice, g.stream), "copy in: ");
+            if (rc == NBH_OK) rc = batch_run(B, g.stream);
+            if (rc == NBH_OK) hip_ok(hipMemcpyAsync(pin + ops, dev + ops, res, hipMemcpyDeviceToHost, g.stream), "copy out: ");
+            if (rc == NBH_OK) hip_ok(hipEventRecord(H.ev, g.stream), "event: ");
+            if (rc != NBH_OK) {
+                drain();
+                if (B) batch_free(B);
+                return rc;
+            }
+            Pending &P = pend[slot];
+            P.B = B;
+            P.dj = std::move(dj);
+            P.idx = std::move(idx);
+            P.dev = dev;
+            P.busy = true;
+            P.large = large;
+            slot ^= 1;
             i0 = i1;
-            continue;
         }
-        // a staging slot: wait for the group it held two groups ago
-        {
+        // the last two groups, oldest first
+        for (int k = 0; k < 2; k++) {
             const int rc = finish(slot);
             if (rc != NBH_OK) { drain(); return rc; }
+            slot ^= 1;
         }
-        Lib::HostSlot &H = g.hs[slot];
-        // lay the group out: operands, results, then the batch's arena
-        std::vector<size_t> offa(dj.size()), offr(dj.size());
-        size_t oa = 0, orr = ops;
-        for (size_t j = 0; j < dj.size(); j++) {
-            const Staged &st = S[(size_t)idx[j]];
-            offa[j] = oa;
-            offr[j] = orr;
-            oa += st.sa + st.sb;
-            orr += st.sr;
-        }
-        auto place = [&](char *base) {
-            for (size_t j = 0; j < dj.size(); j++) {
-                dj[j].a = (const uint64_t *)(base + offa[j]);
-                dj[j].b = jobs[idx[j]].b ? (const uint64_t *)(base + offa[j] + S[(size_t)idx[j]].sa) : nullptr;
-                dj[j].r = (uint64_t *)(base + offr[j]);
-            }
-        };
-        size_t need = 0;
-        {
-            // sizing reads no operand: any non-null base will do
-            place(reinterpret_cast<char *>((uintptr_t)4096));
-            const int rc = batch_create(dj.data(), (int64_t)dj.size(), nullptr, &need);
-            if (rc != NBH_OK) { drain(); return rc; }
-        }
-        need = align256(need);
-        if (H.dev_size < ops + res + need || H.pin_size < ops + res || !H.ev) {
-            HIPCHK(hipDeviceSynchronize());
-            if (!H.ev) HIPCHK(hipEventCreateWithFlags(&H.ev, hipEventDisableTiming));
-            if (H.dev_size < ops + res + need) {
-                if (H.dev) HIPCHK(hipFree(H.dev));
-                H.dev = nullptr;
-                H.dev_size = 0;
-                HIPCHK(hipMalloc(&H.dev, ops + res + need));
-                H.dev_size = ops + res + need;
-            }
-            if (H.pin_size < ops + res) {
-                if (H.pin) HIPCHK(hipHostFree(H.pin));
-                H.pin = nullptr;
-                H.pin_size = 0;
-                HIPCHK(hipHostMalloc(&H.pin, ops + res, hipHostMallocDefault));
-                H.pin_size = ops + res;
-            }
-        }
-        char *dev = (char *)H.dev;
-        place(dev);
-        const bool large = ops + res >= HOST_PARALLEL_COPY_BYTES;
-        char *pin = (char *)H.pin;
-        host_parallel_for(dj.size(), large, [&](size_t j) {
-            const nbh_job &q = jobs[idx[j]];
-            char *p = pin + ((const char *)dj[j].a - dev);
-            std::memcpy(p, q.a, 8 * (size_t)dj[j].m);
-            if (q.b) std::memcpy(p + S[(size_t)idx[j]].sa, q.b, 8 * (size_t)dj[j].n);
-        });
-        // tables into the arena, operands in, the batch, results out, then
-        // the slot's event, all in order on g.stream
-        nbh_batch_s *B = nullptr;
-        int rc = batch_create(dj.data(), (int64_t)dj.size(), &B, nullptr, dev + ops + res, need, g.stream);
-        auto hip_ok = [&](hipError_t e, const char *what) {
-            if (e != hipSuccess && rc == NBH_OK) rc = fail(NBH_ERR_HIP, std::string(what) + hipGetErrorString(e));
-        };
-        if (rc == NBH_OK) hip_ok(hipMemcpyAsync(dev, pin, ops, hipMemcpyHostToDevice, g.stream), "copy in: ");
-        if (rc == NBH_OK) rc = batch_run(B, g.stream);
-        if (rc == NBH_OK) hip_ok(hipMemcpyAsync(pin + ops, dev + ops, res, hipMemcpyDeviceToHost, g.stream), "copy out: ");
-        if (rc == NBH_OK) hip_ok(hipEventRecord(H.ev, g.stream), "event: ");
-        if (rc != NBH_OK) {
-            drain();
-            if (B) batch_free(B);
-            return rc;
-        }
-        Pending &P = pend[slot];
-        P.B = B;
-        P.dj = std::move(dj);
-        P.idx = std::move(idx);
-        P.dev = dev;
-        P.busy = true;
-        P.large = large;
-        slot ^= 1;
-        i0 = i1;
-    }
-    // the last two groups, oldest first
-    for (int k = 0; k < 2; k++) {
-        const int rc = finish(slot);
-        if (rc != NBH_OK) { drain(); return rc; }
-        slot ^= 1;
-    }
-    return NBH_OK;
+        return NBH_OK;
+    NBH_GUARD_END
 }
 
 int32_t nbh_spectrum_create(const uint64_t *b, int64_t n, int64_t bits_b, int64_t bits_a_max, void *stream,
                             nbh_spectrum *out) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    if (!out || !b || n < 1 || bits_a_max < 1 || bits_b > 64 * n) return fail(NBH_ERR_ARG, "bad argument");
-    *out = nullptr;
-    RCCHK(stream_enter(stream));
-    if (bits_b < 0) RCCHK(read_bits(b, n, &bits_b));
-    std::unique_ptr<nbh_spectrum_s> S(new nbh_spectrum_s);
-    S->b = b;
-    S->n = n;
-    S->bits_b = bits_b;
-    S->bits_a_max = bits_a_max;
-    if (bits_b == 0) {
-        S->zero = 1;
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        if (!out || !b || n < 1 || bits_a_max < 1 || bits_b > 64 * n) return fail(NBH_ERR_ARG, "bad argument");
+        *out = nullptr;
+        RCCHK(stream_enter(stream));
+        if (bits_b < 0) RCCHK(read_bits(b, n, &bits_b));
+        std::unique_ptr<nbh_spectrum_s> S(new nbh_spectrum_s);
+        S->b = b;
+        S->n = n;
+        S->bits_b = bits_b;
+        S->bits_a_max = bits_a_max;
+        if (bits_b == 0) {
+            S->zero = 1;
+            *out = S.release();
+            return stream_leave(stream);
+        }
+        int bch = 0;
+        int64_t N = 0;
+        RCCHK(ntt_params(bits_a_max, bits_b, &bch, &N));
+        if (bch < UNPACK_MIN_B) return fail(NBH_ERR_LENGTH, "product too large for device memory (chunk width below 32)");
+        S->bch = bch;
+        S->N = N;
+        S->ncb = (bits_b + bch - 1) / bch;
+        PlanEntry *P[2];
+        RCCHK(get_plan(N, 0, &P[0]));
+        RCCHK(get_plan(N, 1, &P[1]));
+        for (int k = 0; k < 2; k++) {
+            if (hipMalloc((void **)&S->x[k], sizeof(double) * (size_t)N) != hipSuccess) {
+                if (S->x[0]) (void)hipFree(S->x[0]);
+                return fail(NBH_ERR_OOM, "spectrum allocation failed");
+            }
+        }
+        // b as operand a of a one-operand job: every forward group, the first
+        // one packing the chunks, both primes per launch
+        DevJob J;
+        std::memset(&J, 0, sizeof J);
+        J.a = b;
+        J.m = n;
+        J.nca = S->ncb;
+        J.bch = bch;
+        for (int k = 0; k < 2; k++) {
+            J.plan[k] = P[k]->dev;
+            J.xa[k] = S->x[k];
+        }
+        DevJob *slot = g.slots + (g.slot_next++ % NSLOTS);
+        HIPCHK(launch_store_job(J, slot, g.stream));
+        const DevPlan &H = P[0]->host;
+        for (int gi = 0; gi < H.ngroups; gi++) {
+            TileGen tg{gi, 1, H.g[gi].tiles, gi == 0 ? TILE_PACK : 0, 0};
+            HIPCHK(launch_group(slot, nullptr, tg, 2 * H.g[gi].tiles, 1, P[0]->grp_lds_bytes[gi], P[0]->grp_threads[gi],
+                                g.stream));
+        }
         *out = S.release();
         return stream_leave(stream);
-    }
-    int bch = 0;
-    int64_t N = 0;
-    RCCHK(ntt_params(bits_a_max, bits_b, &bch, &N));
-    if (bch < UNPACK_MIN_B) return fail(NBH_ERR_LENGTH, "product too large for device memory (chunk width below 32)");
-    S->bch = bch;
-    S->N = N;
-    S->ncb = (bits_b + bch - 1) / bch;
-    PlanEntry *P[2];
-    RCCHK(get_plan(N, 0, &P[0]));
-    RCCHK(get_plan(N, 1, &P[1]));
-    for (int k = 0; k < 2; k++) {
-        if (hipMalloc((void **)&S->x[k], sizeof(double) * (size_t)N) != hipSuccess) {
-            if (S->x[0]) (void)hipFree(S->x[0]);
-            return fail(NBH_ERR_OOM, "spectrum allocation failed");
-        }
-    }
-    // b as operand a of a one-operand job: every forward group, the first
-    // one packing the chunks, both primes per launch
-    DevJob J;
-    std::memset(&J, 0, sizeof J);
-    J.a = b;
-    J.m = n;
-    J.nca = S->ncb;
-    J.bch = bch;
-    for (int k = 0; k < 2; k++) {
-        J.plan[k] = P[k]->dev;
-        J.xa[k] = S->x[k];
-    }
-    DevJob *slot = g.slots + (g.slot_next++ % NSLOTS);
-    HIPCHK(launch_store_job(J, slot, g.stream));
-    const DevPlan &H = P[0]->host;
-    for (int gi = 0; gi < H.ngroups; gi++) {
-        TileGen tg{gi, 1, H.g[gi].tiles, gi == 0 ? TILE_PACK : 0, 0};
-        HIPCHK(launch_group(slot, nullptr, tg, 2 * H.g[gi].tiles, 1, P[0]->grp_lds_bytes[gi], P[0]->grp_threads[gi],
-                            g.stream));
-    }
-    *out = S.release();
-    return stream_leave(stream);
+    NBH_GUARD_END
 }
 
 void nbh_spectrum_destroy(nbh_spectrum s) {
@@ -2207,36 +2280,42 @@ void nbh_spectrum_destroy(nbh_spectrum s) {
 }
 
 int32_t nbh_spectrum_info(nbh_spectrum s, int64_t *N, int32_t *b, int64_t *n, int64_t *bits_a_max) {
-    if (!s) return fail(NBH_ERR_ARG, "null handle");
-    if (N) *N = s->N;
-    if (b) *b = s->bch;
-    if (n) *n = s->n;
-    if (bits_a_max) *bits_a_max = s->bits_a_max;
-    return NBH_OK;
+    NBH_GUARD_BEGIN
+        if (!s) return fail(NBH_ERR_ARG, "null handle");
+        if (N) *N = s->N;
+        if (b) *b = s->bch;
+        if (n) *n = s->n;
+        if (bits_a_max) *bits_a_max = s->bits_a_max;
+        return NBH_OK;
+    NBH_GUARD_END
 }
 
 int32_t nbh_mul_spec_dev(uint64_t *r, const uint64_t *a, int64_t m, int64_t bits_a, nbh_spectrum s, void *stream) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    if (!s) return fail(NBH_ERR_ARG, "null spectrum");
-    const nbh_spectrum_s *sp = s;
-    return run_on(stream, [&](hipStream_t st) { return run_single(r, a, m, nullptr, 0, bits_a, 0, false, sp, 0, st); });
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        if (!s) return fail(NBH_ERR_ARG, "null spectrum");
+        const nbh_spectrum_s *sp = s;
+        return run_on(stream, [&](hipStream_t st) { return run_single(r, a, m, nullptr, 0, bits_a, 0, false, sp, 0, st); });
+    NBH_GUARD_END
 }
 
 int32_t nbh_spec_batch_create(const nbh_spec_job *jobs, int64_t count, nbh_batch *out) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    if (!out || !jobs || count < 1) return fail(NBH_ERR_ARG, "bad argument");
-    std::vector<JobIn> in((size_t)count);
-    for (int64_t i = 0; i < count; i++) {
-        const nbh_spec_job &q = jobs[i];
-        if (!q.s) return fail(NBH_ERR_ARG, "job " + std::to_string(i) + " has no spectrum");
-        in[(size_t)i] = JobIn{q.r, q.a, q.s->b, q.m, q.s->n, q.bits_a, q.s->bits_b, q.s, 0};
-    }
-    nbh_batch_s *B = nullptr;
-    RCCHK(batch_plan(in.data(), count, &B, nullptr, nullptr, 0, nullptr));
-    *out = B;
-    return NBH_OK;
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        if (!out || !jobs || count < 1) return fail(NBH_ERR_ARG, "bad argument");
+        std::vector<JobIn> in((size_t)count);
+        for (int64_t i = 0; i < count; i++) {
+            const nbh_spec_job &q = jobs[i];
+            if (!q.s) return fail(NBH_ERR_ARG, "job " + std::to_string(i) + " has no spectrum");
+            in[(size_t)i] = JobIn{q.r, q.a, q.s->b, q.m, q.s->n, q.bits_a, q.s->bits_b, q.s, 0};
+        }
+        nbh_batch_s *B = nullptr;
+        RCCHK(batch_plan(in.data(), count, &B, nullptr, nullptr, 0, nullptr));
+        *out = B;
+        return NBH_OK;
+    NBH_GUARD_END
 }
 
 // ---------------------------------------------------------------------------
@@ -2260,23 +2339,25 @@ void nbh_barrett_batch_destroy(nbh_barrett_batch rb);
 
 int32_t nbh_barrett_create(const uint64_t *m, int64_t k, const uint64_t *mu, int64_t lmu, void *stream,
                            nbh_barrett *out) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    if (!out || !m || !mu || k < 1 || lmu < 1 || lmu > k + 2) return fail(NBH_ERR_ARG, "bad argument");
-    *out = nullptr;
-    std::unique_ptr<nbh_barrett_s> B(new nbh_barrett_s);
-    B->m = m;
-    B->k = k;
-    B->lmu = lmu;
-    // q1 = floor(T / beta^(k-1)) has k+1 limbs; q3 = floor(q1 mu / beta^(k+1)) has lmu
-    int32_t rc = nbh_spectrum_create(mu, lmu, -1, 64 * (k + 1), stream, &B->smu);
-    if (rc == NBH_OK) rc = nbh_spectrum_create(m, k, -1, 64 * lmu, stream, &B->sm);
-    if (rc != NBH_OK) {
-        nbh_spectrum_destroy(B->smu);
-        return rc;
-    }
-    *out = B.release();
-    return NBH_OK;
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        if (!out || !m || !mu || k < 1 || lmu < 1 || lmu > k + 2) return fail(NBH_ERR_ARG, "bad argument");
+        *out = nullptr;
+        std::unique_ptr<nbh_barrett_s> B(new nbh_barrett_s);
+        B->m = m;
+        B->k = k;
+        B->lmu = lmu;
+        // q1 = floor(T / beta^(k-1)) has k+1 limbs; q3 = floor(q1 mu / beta^(k+1)) has lmu
+        int32_t rc = nbh_spectrum_create(mu, lmu, -1, 64 * (k + 1), stream, &B->smu);
+        if (rc == NBH_OK) rc = nbh_spectrum_create(m, k, -1, 64 * lmu, stream, &B->sm);
+        if (rc != NBH_OK) {
+            nbh_spectrum_destroy(B->smu);
+            return rc;
+        }
+        *out = B.release();
+        return NBH_OK;
+    NBH_GUARD_END
 }
 
 void nbh_barrett_destroy(nbh_barrett b) {
@@ -2287,47 +2368,51 @@ void nbh_barrett_destroy(nbh_barrett b) {
 }
 
 int32_t nbh_barrett_batch_create(nbh_barrett b, const nbh_barrett_job *jobs, int64_t count, nbh_barrett_batch *out) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    if (!b || !jobs || count < 1 || !out) return fail(NBH_ERR_ARG, "bad argument");
-    *out = nullptr;
-    const int64_t k = b->k, lmu = b->lmu;
-    const size_t q2_limbs = (size_t)(k + 1 + lmu), p_limbs = (size_t)(lmu + k);
-    const size_t per = align256(8 * q2_limbs) + align256(8 * p_limbs);
-    std::unique_ptr<nbh_barrett_batch_s, void (*)(nbh_barrett_batch)> R(new nbh_barrett_batch_s,
-                                                                       nbh_barrett_batch_destroy);
-    R->count = count;
-    HIPCHK(hipMalloc(&R->ws, per * (size_t)count));
-    std::vector<JobIn> j1((size_t)count), j2((size_t)count);
-    std::vector<BarrettFixJob> fx((size_t)count);
-    for (int64_t i = 0; i < count; i++) {
-        if (!jobs[i].t || !jobs[i].r) return fail(NBH_ERR_ARG, "bad reduction " + std::to_string(i));
-        uint64_t *q2 = (uint64_t *)((char *)R->ws + per * (size_t)i);
-        uint64_t *p = (uint64_t *)((char *)q2 + align256(8 * q2_limbs));
-        // operands are taken at their full lengths: the values change from run
-        // to run, and a length is an upper bound of the magnitude
-        j1[(size_t)i] = JobIn{q2, jobs[i].t + (k - 1), b->smu->b, k + 1, lmu, 64 * (k + 1), b->smu->bits_b, b->smu, 0};
-        // r = T - q3*m is taken mod beta^(k+1): q3*m's low k+1 limbs suffice
-        j2[(size_t)i] = JobIn{p, q2 + (k + 1), b->m, lmu, k, 64 * lmu, b->sm->bits_b, b->sm, k + 1};
-        fx[(size_t)i] = BarrettFixJob{jobs[i].t, p, p, jobs[i].r, b->m, k};
-    }
-    RCCHK(batch_plan(j1.data(), count, &R->p1, nullptr, nullptr, 0, nullptr));
-    RCCHK(batch_plan(j2.data(), count, &R->p2, nullptr, nullptr, 0, nullptr));
-    HIPCHK(hipMalloc((void **)&R->dfix, sizeof(BarrettFixJob) * (size_t)count));
-    HIPCHK(hipMemcpy(R->dfix, fx.data(), sizeof(BarrettFixJob) * (size_t)count, hipMemcpyHostToDevice));
-    *out = R.release();
-    return NBH_OK;
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        if (!b || !jobs || count < 1 || !out) return fail(NBH_ERR_ARG, "bad argument");
+        *out = nullptr;
+        const int64_t k = b->k, lmu = b->lmu;
+        const size_t q2_limbs = (size_t)(k + 1 + lmu), p_limbs = (size_t)(lmu + k);
+        const size_t per = align256(8 * q2_limbs) + align256(8 * p_limbs);
+        std::unique_ptr<nbh_barrett_batch_s, void (*)(nbh_barrett_batch)> R(new nbh_barrett_batch_s,
+                                                                           nbh_barrett_batch_destroy);
+        R->count = count;
+        HIPCHK(hipMalloc(&R->ws, per * (size_t)count));
+        std::vector<JobIn> j1((size_t)count), j2((size_t)count);
+        std::vector<BarrettFixJob> fx((size_t)count);
+        for (int64_t i = 0; i < count; i++) {
+            if (!jobs[i].t || !jobs[i].r) return fail(NBH_ERR_ARG, "bad reduction " + std::to_string(i));
+            uint64_t *q2 = (uint64_t *)((char *)R->ws + per * (size_t)i);
+            uint64_t *p = (uint64_t *)((char *)q2 + align256(8 * q2_limbs));
+            // operands are taken at their full lengths: the values change from run
+            // to run, and a length is an upper bound of the magnitude
+            j1[(size_t)i] = JobIn{q2, jobs[i].t + (k - 1), b->smu->b, k + 1, lmu, 64 * (k + 1), b->smu->bits_b, b->smu, 0};
+            // r = T - q3*m is taken mod beta^(k+1): q3*m's low k+1 limbs suffice
+            j2[(size_t)i] = JobIn{p, q2 + (k + 1), b->m, lmu, k, 64 * lmu, b->sm->bits_b, b->sm, k + 1};
+            fx[(size_t)i] = BarrettFixJob{jobs[i].t, p, p, jobs[i].r, b->m, k};
+        }
+        RCCHK(batch_plan(j1.data(), count, &R->p1, nullptr, nullptr, 0, nullptr));
+        RCCHK(batch_plan(j2.data(), count, &R->p2, nullptr, nullptr, 0, nullptr));
+        HIPCHK(hipMalloc((void **)&R->dfix, sizeof(BarrettFixJob) * (size_t)count));
+        HIPCHK(hipMemcpy(R->dfix, fx.data(), sizeof(BarrettFixJob) * (size_t)count, hipMemcpyHostToDevice));
+        *out = R.release();
+        return NBH_OK;
+    NBH_GUARD_END
 }
 
 int32_t nbh_barrett_batch_run(nbh_barrett_batch rb, void *stream) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    if (!rb) return fail(NBH_ERR_ARG, "null handle");
-    hipStream_t s = stream ? (hipStream_t)stream : g.stream;
-    RCCHK(batch_run(rb->p1, s));
-    RCCHK(batch_run(rb->p2, s));
-    HIPCHK(launch_barrett_fix(rb->dfix, rb->count, s));
-    return NBH_OK;
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        if (!rb) return fail(NBH_ERR_ARG, "null handle");
+        hipStream_t s = stream ? (hipStream_t)stream : g.stream;
+        RCCHK(batch_run(rb->p1, s));
+        RCCHK(batch_run(rb->p2, s));
+        HIPCHK(launch_barrett_fix(rb->dfix, rb->count, s));
+        return NBH_OK;
+    NBH_GUARD_END
 }
 
 void nbh_barrett_batch_destroy(nbh_barrett_batch rb) {
@@ -2340,24 +2425,28 @@ void nbh_barrett_batch_destroy(nbh_barrett_batch rb) {
 }
 
 int32_t nbh_hgcd_create(const nbh_hgcd_job *jobs, int64_t count, int32_t mode, nbh_hgcd *out) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    if (!out) return fail(NBH_ERR_ARG, "null handle");
-    nbh_hgcd_s *H = nullptr;
-    RCCHK(hgcd_create(jobs, count, mode, &H));
-    *out = H;
-    return NBH_OK;
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        if (!out) return fail(NBH_ERR_ARG, "null handle");
+        nbh_hgcd_s *H = nullptr;
+        RCCHK(hgcd_create(jobs, count, mode, &H));
+        *out = H;
+        return NBH_OK;
+    NBH_GUARD_END
 }
 
 int32_t nbh_hgcd_run(nbh_hgcd h, void *stream) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    if (!h) return fail(NBH_ERR_ARG, "null handle");
-    hipStream_t s = stream ? (hipStream_t)stream : g.stream;
-    RCCHK(batch_run(h->products, s));
-    ProfScope ps(5, s);
-    HIPCHK(launch_combine(h->dcomb, h->ncomb, s));
-    return NBH_OK;
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        if (!h) return fail(NBH_ERR_ARG, "null handle");
+        hipStream_t s = stream ? (hipStream_t)stream : g.stream;
+        RCCHK(batch_run(h->products, s));
+        ProfScope ps(5, s);
+        HIPCHK(launch_combine(h->dcomb, h->ncomb, s));
+        return NBH_OK;
+    NBH_GUARD_END
 }
 
 void nbh_hgcd_destroy(nbh_hgcd h) {
@@ -2369,151 +2458,171 @@ void nbh_hgcd_destroy(nbh_hgcd h) {
 }
 
 int32_t nbh_ntt_fwd_dev(double *x, int64_t N, int32_t prime, void *stream) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    if (!x || (prime != 0 && prime != 1)) return fail(NBH_ERR_ARG, "bad argument");
-    PlanEntry *P = nullptr;
-    RCCHK(get_plan(N, prime, &P));
-    RCCHK(stream_enter(stream));
-    DevJob J;
-    std::memset(&J, 0, sizeof J);
-    J.plan[prime] = P->dev;
-    J.xa[prime] = x;
-    DevJob *slot = g.slots + (g.slot_next++ % NSLOTS);
-    HIPCHK(launch_store_job(J, slot, g.stream));
-    const DevPlan &H = P->host;
-    for (int gi = 0; gi < H.ngroups; gi++) {
-        TileGen tg{gi, 1, H.g[gi].tiles, 0, prime};
-        HIPCHK(launch_group(slot, nullptr, tg, H.g[gi].tiles, 1, P->grp_lds_bytes[gi], P->grp_threads[gi], g.stream));
-    }
-    return stream_leave(stream);
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        if (!x || (prime != 0 && prime != 1)) return fail(NBH_ERR_ARG, "bad argument");
+        PlanEntry *P = nullptr;
+        RCCHK(get_plan(N, prime, &P));
+        RCCHK(stream_enter(stream));
+        DevJob J;
+        std::memset(&J, 0, sizeof J);
+        J.plan[prime] = P->dev;
+        J.xa[prime] = x;
+        DevJob *slot = g.slots + (g.slot_next++ % NSLOTS);
+        HIPCHK(launch_store_job(J, slot, g.stream));
+        const DevPlan &H = P->host;
+        for (int gi = 0; gi < H.ngroups; gi++) {
+            TileGen tg{gi, 1, H.g[gi].tiles, 0, prime};
+            HIPCHK(launch_group(slot, nullptr, tg, H.g[gi].tiles, 1, P->grp_lds_bytes[gi], P->grp_threads[gi], g.stream));
+        }
+        return stream_leave(stream);
+    NBH_GUARD_END
 }
 
 int32_t nbh_ntt_rev_dev(double *x, int64_t N, int32_t prime, void *stream) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    if (!x || (prime != 0 && prime != 1)) return fail(NBH_ERR_ARG, "bad argument");
-    PlanEntry *P = nullptr;
-    RCCHK(get_plan(N, prime, &P));
-    RCCHK(stream_enter(stream));
-    DevJob J;
-    std::memset(&J, 0, sizeof J);
-    J.plan[prime] = P->dev;
-    J.xa[prime] = x;
-    DevJob *slot = g.slots + (g.slot_next++ % NSLOTS);
-    HIPCHK(launch_store_job(J, slot, g.stream));
-    const DevPlan &H = P->host;
-    for (int gi = H.ngroups - 1; gi >= 0; gi--) {
-        TileGen tg{gi, 1, H.g[gi].tiles, 0, prime};
-        HIPCHK(launch_group(slot, nullptr, tg, H.g[gi].tiles, 0, P->grp_lds_bytes[gi], P->grp_threads[gi], g.stream));
-    }
-    return stream_leave(stream);
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        if (!x || (prime != 0 && prime != 1)) return fail(NBH_ERR_ARG, "bad argument");
+        PlanEntry *P = nullptr;
+        RCCHK(get_plan(N, prime, &P));
+        RCCHK(stream_enter(stream));
+        DevJob J;
+        std::memset(&J, 0, sizeof J);
+        J.plan[prime] = P->dev;
+        J.xa[prime] = x;
+        DevJob *slot = g.slots + (g.slot_next++ % NSLOTS);
+        HIPCHK(launch_store_job(J, slot, g.stream));
+        const DevPlan &H = P->host;
+        for (int gi = H.ngroups - 1; gi >= 0; gi--) {
+            TileGen tg{gi, 1, H.g[gi].tiles, 0, prime};
+            HIPCHK(launch_group(slot, nullptr, tg, H.g[gi].tiles, 0, P->grp_lds_bytes[gi], P->grp_threads[gi], g.stream));
+        }
+        return stream_leave(stream);
+    NBH_GUARD_END
 }
 
 int32_t nbh_pack_dev(double *x, int64_t N, const uint64_t *limbs, int64_t n, int32_t b, int64_t nch, void *stream) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    if (!x || !limbs || n < 1 || b < 1 || b > 52 || nch < 0 || nch > N || (nch - 1) * b >= 64 * n)
-        return fail(NBH_ERR_ARG, "bad argument");
-    RCCHK(stream_enter(stream));
-    HIPCHK(launch_pack(x, N, limbs, n, b, nch, g.stream));
-    return stream_leave(stream);
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        if (!x || !limbs || n < 1 || b < 1 || b > 52 || nch < 0 || nch > N || (nch - 1) * b >= 64 * n)
+            return fail(NBH_ERR_ARG, "bad argument");
+        RCCHK(stream_enter(stream));
+        HIPCHK(launch_pack(x, N, limbs, n, b, nch, g.stream));
+        return stream_leave(stream);
+    NBH_GUARD_END
 }
 
 int32_t nbh_pointwise_dev(double *xa, const double *xb, int64_t N, int32_t prime, void *stream) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    if (!xa || !xb || N < 0 || (prime != 0 && prime != 1)) return fail(NBH_ERR_ARG, "bad argument");
-    RCCHK(stream_enter(stream));
-    HIPCHK(launch_pointwise(prime, xa, xb, N, g.stream));
-    return stream_leave(stream);
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        if (!xa || !xb || N < 0 || (prime != 0 && prime != 1)) return fail(NBH_ERR_ARG, "bad argument");
+        RCCHK(stream_enter(stream));
+        HIPCHK(launch_pointwise(prime, xa, xb, N, g.stream));
+        return stream_leave(stream);
+    NBH_GUARD_END
 }
 
 int32_t nbh_unpack_dev(uint64_t *r, int64_t rn, const double *x1, const double *x2, int64_t N, int64_t nconv,
                        int32_t b, void *stream) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    if (!r || !x1 || !x2 || rn < 1 || nconv < 1 || nconv > N || b < UNPACK_MIN_B || b > 49)
-        return fail(NBH_ERR_ARG, "bad argument");
-    PlanEntry *P[2];
-    RCCHK(get_plan(N, 0, &P[0]));
-    RCCHK(get_plan(N, 1, &P[1]));
-    RCCHK(stream_enter(stream));
-    DevJob J;
-    std::memset(&J, 0, sizeof J);
-    J.r = r;
-    J.rn = rn;
-    J.nconv = nconv;
-    J.bch = b;
-    J.bmagic = ~0ull / (uint64_t)b + 1;
-    J.nchunks = (rn + UNPACK_CHUNK - 1) / UNPACK_CHUNK;
-    J.plan[0] = P[0]->dev;
-    J.plan[1] = P[1]->dev;
-    J.xa[0] = const_cast<double *>(x1);
-    J.xa[1] = const_cast<double *>(x2);
-    RCCHK(ensure_buf(&g.ws, &g.ws_size, align256(sizeof(uint32_t) * (size_t)J.nchunks)));
-    J.chunk_flags = (uint32_t *)g.ws;
-    DevJob *slot = g.slots + (g.slot_next++ % NSLOTS);
-    HIPCHK(launch_store_job(J, slot, g.stream));
-    TileGen ug{0, 1, J.nchunks, 0, 0};
-    HIPCHK(launch_unpack(slot, nullptr, ug, J.nchunks, b, g.stream));
-    HIPCHK(launch_carry_fix(slot, nullptr, ug, J.nchunks, g.stream));
-    return stream_leave(stream);
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        if (!r || !x1 || !x2 || rn < 1 || nconv < 1 || nconv > N || b < UNPACK_MIN_B || b > 49)
+            return fail(NBH_ERR_ARG, "bad argument");
+        PlanEntry *P[2];
+        RCCHK(get_plan(N, 0, &P[0]));
+        RCCHK(get_plan(N, 1, &P[1]));
+        RCCHK(stream_enter(stream));
+        DevJob J;
+        std::memset(&J, 0, sizeof J);
+        J.r = r;
+        J.rn = rn;
+        J.nconv = nconv;
+        J.bch = b;
+        J.bmagic = ~0ull / (uint64_t)b + 1;
+        J.nchunks = (rn + UNPACK_CHUNK - 1) / UNPACK_CHUNK;
+        J.plan[0] = P[0]->dev;
+        J.plan[1] = P[1]->dev;
+        J.xa[0] = const_cast<double *>(x1);
+        J.xa[1] = const_cast<double *>(x2);
+        RCCHK(ensure_buf(&g.ws, &g.ws_size, align256(sizeof(uint32_t) * (size_t)J.nchunks)));
+        J.chunk_flags = (uint32_t *)g.ws;
+        DevJob *slot = g.slots + (g.slot_next++ % NSLOTS);
+        HIPCHK(launch_store_job(J, slot, g.stream));
+        TileGen ug{0, 1, J.nchunks, 0, 0};
+        HIPCHK(launch_unpack(slot, nullptr, ug, J.nchunks, b, g.stream));
+        HIPCHK(launch_carry_fix(slot, nullptr, ug, J.nchunks, g.stream));
+        return stream_leave(stream);
+    NBH_GUARD_END
 }
 
 int32_t nbh_plan_nstages(int64_t N, int32_t prime, int32_t *nst) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    PlanEntry *P = nullptr;
-    RCCHK(get_plan(N, prime, &P));
-    *nst = P->host.nst;
-    return NBH_OK;
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        PlanEntry *P = nullptr;
+        RCCHK(get_plan(N, prime, &P));
+        *nst = P->host.nst;
+        return NBH_OK;
+    NBH_GUARD_END
 }
 
 int32_t nbh_plan_stage(int64_t N, int32_t prime, int32_t i, int32_t *m, int64_t *span, int64_t *Q, double *rot8) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    PlanEntry *P = nullptr;
-    RCCHK(get_plan(N, prime, &P));
-    if (i < 0 || i >= P->host.nst) return fail(NBH_ERR_ARG, "no such stage");
-    const StageDesc &s = P->host.st[i];
-    *m = s.m;
-    *span = s.L;
-    *Q = s.Q;
-    std::memcpy(rot8, s.rot, sizeof s.rot);
-    return NBH_OK;
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        PlanEntry *P = nullptr;
+        RCCHK(get_plan(N, prime, &P));
+        if (i < 0 || i >= P->host.nst) return fail(NBH_ERR_ARG, "no such stage");
+        const StageDesc &s = P->host.st[i];
+        *m = s.m;
+        *span = s.L;
+        *Q = s.Q;
+        std::memcpy(rot8, s.rot, sizeof s.rot);
+        return NBH_OK;
+    NBH_GUARD_END
 }
 
 int32_t nbh_plan_twiddle_row(int64_t N, int32_t prime, int32_t i, int32_t t, double *out) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    PlanEntry *P = nullptr;
-    RCCHK(get_plan(N, prime, &P));
-    if (i < 0 || i >= P->host.nst) return fail(NBH_ERR_ARG, "no such stage");
-    const StageDesc &s = P->host.st[i];
-    if (t < 1 || t >= s.m || s.m == 2) return fail(NBH_ERR_ARG, "no such twiddle row");
-    HIPCHK(hipMemcpy(out, P->host.tw + s.tw_off + (int64_t)(t - 1) * s.Q, sizeof(double) * (size_t)s.Q,
-                     hipMemcpyDeviceToHost));
-    return NBH_OK;
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        PlanEntry *P = nullptr;
+        RCCHK(get_plan(N, prime, &P));
+        if (i < 0 || i >= P->host.nst) return fail(NBH_ERR_ARG, "no such stage");
+        const StageDesc &s = P->host.st[i];
+        if (t < 1 || t >= s.m || s.m == 2) return fail(NBH_ERR_ARG, "no such twiddle row");
+        HIPCHK(hipMemcpy(out, P->host.tw + s.tw_off + (int64_t)(t - 1) * s.Q, sizeof(double) * (size_t)s.Q,
+                         hipMemcpyDeviceToHost));
+        return NBH_OK;
+    NBH_GUARD_END
 }
 
 int32_t nbh_plan_ninv(int64_t N, int32_t prime, double *ninv, double *ninvp) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    PlanEntry *P = nullptr;
-    RCCHK(get_plan(N, prime, &P));
-    *ninv = P->host.ninv;
-    *ninvp = P->host.ninvp;
-    return NBH_OK;
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        PlanEntry *P = nullptr;
+        RCCHK(get_plan(N, prime, &P));
+        *ninv = P->host.ninv;
+        *ninvp = P->host.ninvp;
+        return NBH_OK;
+    NBH_GUARD_END
 }
 
 int32_t nbh_set_tuning(int32_t grp_lds_doubles, int32_t mid_max, int32_t cmin, int32_t cmax) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    const Tuning t{grp_lds_doubles, mid_max, cmin, cmax};
-    if (!tuning_ok(t)) return fail(NBH_ERR_ARG, "tuning outside the device limits");
-    g.tun = t;
-    return NBH_OK;
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        const Tuning t{grp_lds_doubles, mid_max, cmin, cmax};
+        if (!tuning_ok(t)) return fail(NBH_ERR_ARG, "tuning outside the device limits");
+        g.tun = t;
+        return NBH_OK;
+    NBH_GUARD_END
 }
 
 namespace {
@@ -2543,121 +2652,145 @@ int get_for(int64_t N, int32_t *knobs4, bool single) {
 }  // namespace
 
 int32_t nbh_set_tuning_table(const int64_t *nmax, const int32_t *knobs, int32_t count) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    return set_table(g.table, nmax, knobs, count);
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        return set_table(g.table, nmax, knobs, count);
+    NBH_GUARD_END
 }
 
 int32_t nbh_set_tuning_table_single(const int64_t *nmax, const int32_t *knobs, int32_t count) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    return set_table(g.table_single, nmax, knobs, count);
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        return set_table(g.table_single, nmax, knobs, count);
+    NBH_GUARD_END
 }
 
 int32_t nbh_get_tuning_for(int64_t N, int32_t *knobs4) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    return get_for(N, knobs4, false);
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        return get_for(N, knobs4, false);
+    NBH_GUARD_END
 }
 
 int32_t nbh_get_tuning_for_single(int64_t N, int32_t *knobs4) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    return get_for(N, knobs4, true);
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        return get_for(N, knobs4, true);
+    NBH_GUARD_END
 }
 
 int32_t nbh_get_tuning(int32_t *grp_lds_doubles, int32_t *mid_max, int32_t *cmin, int32_t *cmax) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    *grp_lds_doubles = g.tun.grp_lds;
-    *mid_max = g.tun.mid_max;
-    *cmin = g.tun.cmin;
-    *cmax = g.tun.cmax;
-    return NBH_OK;
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        *grp_lds_doubles = g.tun.grp_lds;
+        *mid_max = g.tun.mid_max;
+        *cmin = g.tun.cmin;
+        *cmax = g.tun.cmax;
+        return NBH_OK;
+    NBH_GUARD_END
 }
 
 int32_t nbh_plan_groups(int64_t N, int32_t *ngroups, int64_t *out) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    PlanEntry *P = nullptr;
-    RCCHK(get_plan(N, 0, &P));
-    *ngroups = P->host.ngroups;
-    for (int i = 0; i < P->host.ngroups; i++) {
-        out[4 * i + 0] = P->host.g[i].La;
-        out[4 * i + 1] = P->host.g[i].Lb;
-        out[4 * i + 2] = P->host.g[i].S;
-        out[4 * i + 3] = P->host.g[i].C;
-    }
-    return NBH_OK;
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        PlanEntry *P = nullptr;
+        RCCHK(get_plan(N, 0, &P));
+        *ngroups = P->host.ngroups;
+        for (int i = 0; i < P->host.ngroups; i++) {
+            out[4 * i + 0] = P->host.g[i].La;
+            out[4 * i + 1] = P->host.g[i].Lb;
+            out[4 * i + 2] = P->host.g[i].S;
+            out[4 * i + 3] = P->host.g[i].C;
+        }
+        return NBH_OK;
+    NBH_GUARD_END
 }
 
 int32_t nbh_set_streams(int32_t nside) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    if (nside < 0 || nside > NSIDE) return fail(NBH_ERR_ARG, "side streams must be 0.." + std::to_string(NSIDE));
-    g.nside = nside;
-    return NBH_OK;
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        if (nside < 0 || nside > NSIDE) return fail(NBH_ERR_ARG, "side streams must be 0.." + std::to_string(NSIDE));
+        g.nside = nside;
+        return NBH_OK;
+    NBH_GUARD_END
 }
 
 int32_t nbh_batch_set_streams(nbh_batch b, int32_t nside) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    if (!b) return fail(NBH_ERR_ARG, "null handle");
-    if (nside < -1 || nside > NSIDE) return fail(NBH_ERR_ARG, "side streams must be -1.." + std::to_string(NSIDE));
-    b->nside = nside;
-    return NBH_OK;
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        if (!b) return fail(NBH_ERR_ARG, "null handle");
+        if (nside < -1 || nside > NSIDE) return fail(NBH_ERR_ARG, "side streams must be -1.." + std::to_string(NSIDE));
+        b->nside = nside;
+        return NBH_OK;
+    NBH_GUARD_END
 }
 
 int32_t nbh_profile_enable(int32_t on) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    g.prof = on != 0;
-    return NBH_OK;
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        g.prof = on != 0;
+        return NBH_OK;
+    NBH_GUARD_END
 }
 
 int32_t nbh_profile_read(double *ms, int64_t *launches, int32_t nkinds) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    for (int i = 0; i < nkinds; i++) {
-        if (ms) ms[i] = 0;
-        if (launches) launches[i] = 0;
-    }
-    int rc = NBH_OK;
-    for (auto &e : g.prof_open) {
-        const int kind = std::get<0>(e);
-        hipEvent_t a = std::get<1>(e), b = std::get<2>(e);
-        if (hipEventSynchronize(b) != hipSuccess) rc = fail(NBH_ERR_HIP, "profile event");
-        float t = 0;
-        if (rc == NBH_OK && hipEventElapsedTime(&t, a, b) != hipSuccess) rc = fail(NBH_ERR_HIP, "elapsed time");
-        if (kind < nkinds) {
-            if (ms) ms[kind] += t;
-            if (launches) launches[kind] += 1;
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        for (int i = 0; i < nkinds; i++) {
+            if (ms) ms[i] = 0;
+            if (launches) launches[i] = 0;
         }
-        g.prof_pool.push_back(a);
-        g.prof_pool.push_back(b);
-    }
-    g.prof_open.clear();
-    return rc;
+        int rc = NBH_OK;
+        for (auto &e : g.prof_open) {
+            const int kind = std::get<0>(e);
+            hipEvent_t a = std::get<1>(e), b = std::get<2>(e);
+            if (hipEventSynchronize(b) != hipSuccess) rc = fail(NBH_ERR_HIP, "profile event");
+            float t = 0;
+            if (rc == NBH_OK && hipEventElapsedTime(&t, a, b) != hipSuccess) rc = fail(NBH_ERR_HIP, "elapsed time");
+            if (kind < nkinds) {
+                if (ms) ms[kind] += t;
+                if (launches) launches[kind] += 1;
+            }
+            g.prof_pool.push_back(a);
+            g.prof_pool.push_back(b);
+        }
+        g.prof_open.clear();
+        return rc;
+    NBH_GUARD_END
 }
 
 int32_t nbh_synchronize(void *stream) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    HIPCHK(hipStreamSynchronize(stream ? (hipStream_t)stream : g.stream));
-    return NBH_OK;
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        HIPCHK(hipStreamSynchronize(stream ? (hipStream_t)stream : g.stream));
+        return NBH_OK;
+    NBH_GUARD_END
 }
 
 int32_t nbh_device_info(int32_t *cus, int64_t *lds_per_block, int64_t *hbm_bytes, char *name, int32_t name_len) {
-    std::lock_guard<std::recursive_mutex> lk(g.mu);
-    RCCHK(ensure_init());
-    if (cus) *cus = g.cus;
-    if (lds_per_block) *lds_per_block = (int64_t)g.lds_block;
-    if (hbm_bytes) *hbm_bytes = (int64_t)g.hbm;
-    if (name && name_len > 0) {
-        std::strncpy(name, g.name.c_str(), (size_t)name_len - 1);
-        name[name_len - 1] = 0;
-    }
-    return NBH_OK;
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        if (cus) *cus = g.cus;
+        if (lds_per_block) *lds_per_block = (int64_t)g.lds_block;
+        if (hbm_bytes) *hbm_bytes = (int64_t)g.hbm;
+        if (name && name_len > 0) {
+            std::strncpy(name, g.name.c_str(), (size_t)name_len - 1);
+            name[name_len - 1] = 0;
+        }
+        return NBH_OK;
+    NBH_GUARD_END
 }
 
 }  // extern "C"
