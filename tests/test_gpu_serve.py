@@ -158,3 +158,36 @@ def test_native_memory_helpers():
         assert np.array_equal(x, y)
     finally:
         lib().nbh_free(p)
+
+
+def test_native_server_short_products():
+    """Low short products (nbh_job.k > 0) through the library's queue: exactly
+    k limbs written, equal to libgmp's low limbs."""
+    import ctypes
+    from nbhip._lib import check, lib, nbh_job
+    engine.init()
+    srv = serve.NativeServer(budget_bytes=256 << 20, nstreams=2, max_batch=8)
+    try:
+        rng = np.random.default_rng(0x5407)
+        cases = []
+        for i in range(12):
+            m = int(rng.integers(100, 9000))
+            n = int(rng.integers(50, m + 1))
+            k = int(rng.integers(1, m + n + 1))
+            a, b = wl.operand("serve", 700 + i, 0, m), wl.operand("serve", 700 + i, 1, n)
+            da, db = D.to_dev(a), D.to_dev(b)
+            r = torch.full((k + 2,), -1, dtype=torch.int64, device="cuda")
+            job = nbh_job(r.data_ptr(), da.data_ptr(), db.data_ptr(), m, n, 0, 0, k)
+            t = ctypes.c_int64()
+            check(lib().nbh_server_submit(srv.h, ctypes.byref(job), ctypes.byref(t)))
+            cases.append((a, b, k, r, da, db))
+        check(lib().nbh_server_drain(srv.h))
+        for a, b, k, r, _, _ in cases:
+            got = D.to_host(r)
+            want = np.zeros(k, dtype=np.uint64)
+            full = gmp_ref.mul(a, b)
+            want[:min(k, full.size)] = full[:k]
+            assert np.array_equal(got[:k], want), k
+            assert (got[k:] == np.uint64(0xFFFFFFFFFFFFFFFF)).all()
+    finally:
+        srv.close()
