@@ -325,6 +325,13 @@ class NativeServer:
         return b.value
 
     def close(self) -> None:
-        if self.h:
+        """Destroy the queue (waits for its work); idempotent."""
+        if getattr(self, "h", None):
             lib().nbh_server_destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
