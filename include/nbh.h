@@ -36,7 +36,13 @@ extern "C" {
  * reference's engine is a pure function with a lazily built plan cache,
  * src/fpntt.jl:273-293); the plan/twiddle cache here is per device. */
 int32_t nbh_init(int32_t device);
-void nbh_shutdown(void);
+/* Waits for all device work, then frees the plan and twiddle caches, the
+ * single-product workspace and the library's streams; nbh_init may be called
+ * again afterwards.  Every handle (nbh_batch, nbh_hgcd, nbh_spectrum,
+ * nbh_barrett, nbh_barrett_batch, nbh_server) points into those caches, so
+ * while any handle is live the call does nothing and returns NBH_ERR_ARG:
+ * destroy (or release) the handles first. */
+int32_t nbh_shutdown(void);
 const char *nbh_last_error_string(void);
 
 /* ntt_len (src/fpntt.jl:936-951): smallest admissible m*2^k >= T. */

@@ -276,6 +276,12 @@ struct Lib {
     std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> prof_open;
     std::vector<hipEvent_t> prof_pool;
     std::vector<hipEvent_t> ev_pool;  // fork/join events of released batches
+    // handles given to callers and not yet destroyed (batches, cofactor steps,
+    // spectra, Barrett contexts and batches, servers): they point into the plan
+    // cache and twiddle tables, so nbh_shutdown refuses while any is live
+    int64_t live = 0;
+    // bumped whenever a schedule knob changes: arena sizes depend on it
+    uint64_t tuning_gen = 0;
 };
 Lib g;
 
@@ -1414,10 +1420,14 @@ int32_t nbh_init(int32_t device) {
     NBH_GUARD_END
 }
 
-void nbh_shutdown(void) {
+int32_t nbh_shutdown(void) {
     std::lock_guard<std::recursive_mutex> lk(g.mu);
-    if (!g.init) return;
-    (void)hipStreamSynchronize(g.stream);
+    if (!g.init) return NBH_OK;
+    if (g.live > 0)
+        return fail(NBH_ERR_ARG, std::to_string(g.live) + " live handle(s) still point into the plan cache: "
+                                 "destroy them before nbh_shutdown");
+    // single products may have run on callers' streams: wait for all of them
+    (void)hipDeviceSynchronize();
     for (auto &e : g.plans) (void)hipFree(e.second->dev);
     for (auto &e : g.stages) (void)hipFree(e.second->tw);
     g.plans.clear();
@@ -1444,6 +1454,7 @@ void nbh_shutdown(void) {
     (void)hipStreamDestroy(g.stream);
     g.init = false;
     g.device = -1;
+    return NBH_OK;
 }
 
 const char *nbh_last_error_string(void) { return t_err.c_str(); }
@@ -1545,6 +1556,7 @@ int32_t nbh_batch_create(const nbh_job *jobs, int64_t count, nbh_batch *out) {
         nbh_batch_s *B = nullptr;
         RCCHK(batch_create(jobs, count, &B));
         *out = B;
+        g.live++;
         return NBH_OK;
     NBH_GUARD_END
 }
@@ -1568,26 +1580,37 @@ int32_t nbh_batch_capture(nbh_batch b) {
         RCCHK(ensure_init());
         if (!b) return fail(NBH_ERR_ARG, "null handle");
         if (b->gexec) return NBH_OK;
-        hipStream_t cs = nullptr;
-        HIPCHK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-        const bool prof = g.prof;
+        // Restores the profiling flag, ends an open capture and frees the
+        // private stream and a partial graph on every way out, an exception
+        // included.
+        struct CaptureScope {
+            hipStream_t cs = nullptr;
+            bool capturing = false;
+            bool prof = g.prof;
+            hipGraph_t graph = nullptr;
+            ~CaptureScope() {
+                g.prof = prof;
+                if (capturing) {
+                    hipGraph_t partial = nullptr;
+                    if (hipStreamEndCapture(cs, &partial) == hipSuccess && partial) (void)hipGraphDestroy(partial);
+                    (void)hipGetLastError();
+                }
+                if (graph) (void)hipGraphDestroy(graph);
+                if (cs) (void)hipStreamDestroy(cs);
+            }
+        } sc;
+        HIPCHK(hipStreamCreateWithFlags(&sc.cs, hipStreamNonBlocking));
         g.prof = false;
-        hipGraph_t graph = nullptr;
-        int rc = NBH_OK;
-        hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
-        if (e == hipSuccess) {
-            rc = batch_run(b, cs);
-            e = hipStreamEndCapture(cs, &graph);
-        }
-        g.prof = prof;
-        if (e == hipSuccess && rc == NBH_OK) e = hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0);
-        (void)hipStreamDestroy(cs);
-        if (rc != NBH_OK || e != hipSuccess) {
-            if (graph) (void)hipGraphDestroy(graph);
-            b->gexec = nullptr;
-            return rc != NBH_OK ? rc : fail(NBH_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-        }
-        b->graph = graph;
+        HIPCHK(hipStreamBeginCapture(sc.cs, hipStreamCaptureModeThreadLocal));
+        sc.capturing = true;
+        const int rc = batch_run(b, sc.cs);
+        sc.capturing = false;
+        const hipError_t e = hipStreamEndCapture(sc.cs, &sc.graph);
+        RCCHK(rc);
+        if (e != hipSuccess) return fail(NBH_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        HIPCHK(hipGraphInstantiate(&b->gexec, sc.graph, nullptr, nullptr, 0));
+        b->graph = sc.graph;
+        sc.graph = nullptr;  // owned by the batch now
         return NBH_OK;
     NBH_GUARD_END
 }
@@ -1625,13 +1648,16 @@ int32_t nbh_batch_create_in(const nbh_job *jobs, int64_t count, void *arena, int
         RCCHK(batch_create(jobs, count, &B, nullptr, arena, (size_t)arena_bytes,
                            stream ? (hipStream_t)stream : g.stream));
         *out = B;
+        g.live++;
         return NBH_OK;
     NBH_GUARD_END
 }
 
 void nbh_batch_release(nbh_batch b) {
     std::lock_guard<std::recursive_mutex> lk(g.mu);
+    if (!b) return;
     batch_free(b);
+    g.live--;
 }
 
 void nbh_batch_destroy(nbh_batch b) {
@@ -1639,6 +1665,7 @@ void nbh_batch_destroy(nbh_batch b) {
     if (b) {
         (void)hipDeviceSynchronize();
         batch_free(b);
+        g.live--;
     }
 }
 
@@ -1674,6 +1701,8 @@ struct nbh_server_s {
     };
     std::vector<Flight> flights;
     std::map<std::tuple<int64_t, int64_t, int64_t, int64_t, int64_t, bool>, size_t> need_cache;
+    uint64_t cache_gen = 0;    // g.tuning_gen the cached sizes were computed under
+    uint64_t pending_gen = 0;  // ... and the waiting requests' sizes
     std::deque<int64_t> done;
     int64_t next_ticket = 0, batches = 0;
 };
@@ -1715,6 +1744,11 @@ void srv_release(nbh_server_s &S, size_t off, size_t n) {
 // Arena bytes of one request alone (workspace + tables, 256-aligned); a
 // micro-batch needs at most the sum over its requests.
 int srv_need(nbh_server_s &S, const nbh_job &j, size_t *need) {
+    if (S.cache_gen != g.tuning_gen) {
+        // tile counts, and so the tables' bytes, follow the schedule knobs
+        S.need_cache.clear();
+        S.cache_gen = g.tuning_gen;
+    }
     const auto key = std::make_tuple(j.m, j.n, j.bits_a, j.bits_b, j.k, j.b == nullptr);
     auto it = S.need_cache.find(key);
     if (it != S.need_cache.end()) {
@@ -1744,7 +1778,30 @@ int srv_retire(nbh_server_s &S) {
     return NBH_OK;
 }
 
+// The schedule knobs changed after the waiting requests were sized: size them
+// again, so an admitted micro-batch always fits the block it is given.  A
+// request that no longer fits the whole budget leaves the queue with an error
+// naming its ticket (it would otherwise block every request behind it).
+int srv_resize(nbh_server_s &S) {
+    if (S.pending_gen == g.tuning_gen) return NBH_OK;
+    for (auto it = S.pending.begin(); it != S.pending.end();) {
+        size_t need = 0;
+        RCCHK(srv_need(S, it->job, &need));
+        if (need > S.budget) {
+            const int64_t t = it->ticket;
+            S.pending.erase(it);
+            return fail(NBH_ERR_BUDGET, "ticket " + std::to_string(t) +
+                                            " exceeds the server's HBM budget under the new schedule; dropped");
+        }
+        it->need = need;
+        ++it;
+    }
+    S.pending_gen = g.tuning_gen;
+    return NBH_OK;
+}
+
 int srv_admit(nbh_server_s &S) {
+    RCCHK(srv_resize(S));
     for (auto &f : S.flights) {
         if (f.B || S.pending.empty()) continue;
         const size_t room = srv_largest(S);
@@ -1799,6 +1856,7 @@ int32_t nbh_server_create(int64_t budget_bytes, int32_t nstreams, int32_t max_ba
         *out = nullptr;
         std::unique_ptr<nbh_server_s, void (*)(nbh_server_s *)> S(new nbh_server_s, srv_free);
         S->budget = (size_t)budget_bytes & ~(size_t)255;
+        S->cache_gen = S->pending_gen = g.tuning_gen;
         S->max_batch = max_batch;
         HIPCHK(hipMalloc((void **)&S->arena, S->budget));
         S->free_list.push_back({0, S->budget});
@@ -1808,6 +1866,7 @@ int32_t nbh_server_create(int64_t budget_bytes, int32_t nstreams, int32_t max_ba
             HIPCHK(hipEventCreateWithFlags(&f.ev, hipEventDisableTiming));
         }
         *out = S.release();
+        g.live++;
         return NBH_OK;
     NBH_GUARD_END
 }
@@ -1890,7 +1949,9 @@ int32_t nbh_server_stats(nbh_server s, int64_t *pending, int64_t *in_flight, int
 
 void nbh_server_destroy(nbh_server s) {
     std::lock_guard<std::recursive_mutex> lk(g.mu);
+    if (!s) return;
     srv_free(s);
+    g.live--;
 }
 
 // Device memory and copies for callers with no GPU array type of their own.
@@ -2226,6 +2287,7 @@ int32_t nbh_spectrum_create(const uint64_t *b, int64_t n, int64_t bits_b, int64_
         if (bits_b == 0) {
             S->zero = 1;
             *out = S.release();
+            g.live++;
             return stream_leave(stream);
         }
         int bch = 0;
@@ -2265,6 +2327,7 @@ int32_t nbh_spectrum_create(const uint64_t *b, int64_t n, int64_t bits_b, int64_
                                 g.stream));
         }
         *out = S.release();
+        g.live++;
         return stream_leave(stream);
     NBH_GUARD_END
 }
@@ -2277,6 +2340,7 @@ void nbh_spectrum_destroy(nbh_spectrum s) {
     for (int k = 0; k < 2; k++)
         if (s->x[k]) (void)hipFree(s->x[k]);
     delete s;
+    g.live--;
 }
 
 int32_t nbh_spectrum_info(nbh_spectrum s, int64_t *N, int32_t *b, int64_t *n, int64_t *bits_a_max) {
@@ -2314,6 +2378,7 @@ int32_t nbh_spec_batch_create(const nbh_spec_job *jobs, int64_t count, nbh_batch
         nbh_batch_s *B = nullptr;
         RCCHK(batch_plan(in.data(), count, &B, nullptr, nullptr, 0, nullptr));
         *out = B;
+        g.live++;
         return NBH_OK;
     NBH_GUARD_END
 }
@@ -2335,7 +2400,18 @@ struct nbh_barrett_batch_s {
     int64_t count = 0;
 };
 
-void nbh_barrett_batch_destroy(nbh_barrett_batch rb);
+namespace {
+// Host and device resources of a Barrett batch; the caller holds g.mu and has
+// made sure no queued work still uses them.
+void barrett_batch_free(nbh_barrett_batch_s *rb) {
+    if (!rb) return;
+    batch_free(rb->p1);
+    batch_free(rb->p2);
+    if (rb->ws) (void)hipFree(rb->ws);
+    if (rb->dfix) (void)hipFree(rb->dfix);
+    delete rb;
+}
+}  // namespace
 
 int32_t nbh_barrett_create(const uint64_t *m, int64_t k, const uint64_t *mu, int64_t lmu, void *stream,
                            nbh_barrett *out) {
@@ -2356,15 +2432,18 @@ int32_t nbh_barrett_create(const uint64_t *m, int64_t k, const uint64_t *mu, int
             return rc;
         }
         *out = B.release();
+        g.live++;
         return NBH_OK;
     NBH_GUARD_END
 }
 
 void nbh_barrett_destroy(nbh_barrett b) {
     if (!b) return;
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
     nbh_spectrum_destroy(b->smu);
     nbh_spectrum_destroy(b->sm);
     delete b;
+    g.live--;
 }
 
 int32_t nbh_barrett_batch_create(nbh_barrett b, const nbh_barrett_job *jobs, int64_t count, nbh_barrett_batch *out) {
@@ -2376,8 +2455,8 @@ int32_t nbh_barrett_batch_create(nbh_barrett b, const nbh_barrett_job *jobs, int
         const int64_t k = b->k, lmu = b->lmu;
         const size_t q2_limbs = (size_t)(k + 1 + lmu), p_limbs = (size_t)(lmu + k);
         const size_t per = align256(8 * q2_limbs) + align256(8 * p_limbs);
-        std::unique_ptr<nbh_barrett_batch_s, void (*)(nbh_barrett_batch)> R(new nbh_barrett_batch_s,
-                                                                           nbh_barrett_batch_destroy);
+        std::unique_ptr<nbh_barrett_batch_s, void (*)(nbh_barrett_batch_s *)> R(new nbh_barrett_batch_s,
+                                                                               barrett_batch_free);
         R->count = count;
         HIPCHK(hipMalloc(&R->ws, per * (size_t)count));
         std::vector<JobIn> j1((size_t)count), j2((size_t)count);
@@ -2398,6 +2477,7 @@ int32_t nbh_barrett_batch_create(nbh_barrett b, const nbh_barrett_job *jobs, int
         HIPCHK(hipMalloc((void **)&R->dfix, sizeof(BarrettFixJob) * (size_t)count));
         HIPCHK(hipMemcpy(R->dfix, fx.data(), sizeof(BarrettFixJob) * (size_t)count, hipMemcpyHostToDevice));
         *out = R.release();
+        g.live++;
         return NBH_OK;
     NBH_GUARD_END
 }
@@ -2417,11 +2497,11 @@ int32_t nbh_barrett_batch_run(nbh_barrett_batch rb, void *stream) {
 
 void nbh_barrett_batch_destroy(nbh_barrett_batch rb) {
     if (!rb) return;
-    batch_free(rb->p1);
-    batch_free(rb->p2);
-    if (rb->ws) (void)hipFree(rb->ws);
-    if (rb->dfix) (void)hipFree(rb->dfix);
-    delete rb;
+    // shares g.ev_pool with every other batch, and queued work may still use it
+    std::lock_guard<std::recursive_mutex> lk(g.mu);
+    (void)hipDeviceSynchronize();
+    barrett_batch_free(rb);
+    g.live--;
 }
 
 int32_t nbh_hgcd_create(const nbh_hgcd_job *jobs, int64_t count, int32_t mode, nbh_hgcd *out) {
@@ -2432,6 +2512,7 @@ int32_t nbh_hgcd_create(const nbh_hgcd_job *jobs, int64_t count, int32_t mode, n
         nbh_hgcd_s *H = nullptr;
         RCCHK(hgcd_create(jobs, count, mode, &H));
         *out = H;
+        g.live++;
         return NBH_OK;
     NBH_GUARD_END
 }
@@ -2454,6 +2535,7 @@ void nbh_hgcd_destroy(nbh_hgcd h) {
     if (h) {
         (void)hipDeviceSynchronize();
         hgcd_free(h);
+        g.live--;
     }
 }
 
@@ -2621,6 +2703,7 @@ int32_t nbh_set_tuning(int32_t grp_lds_doubles, int32_t mid_max, int32_t cmin, i
         const Tuning t{grp_lds_doubles, mid_max, cmin, cmax};
         if (!tuning_ok(t)) return fail(NBH_ERR_ARG, "tuning outside the device limits");
         g.tun = t;
+        g.tuning_gen++;
         return NBH_OK;
     NBH_GUARD_END
 }
@@ -2637,6 +2720,7 @@ int set_table(std::vector<std::pair<int64_t, Tuning>> &dst, const int64_t *nmax,
         tab.emplace_back(nmax[i], t);
     }
     dst = tab;
+    g.tuning_gen++;
     return NBH_OK;
 }
 

@@ -67,7 +67,7 @@ _P = ctypes.POINTER
 
 SIGNATURES = {
     "nbh_init": (_i32, [_i32]),
-    "nbh_shutdown": (None, []),
+    "nbh_shutdown": (_i32, []),
     "nbh_last_error_string": (ctypes.c_char_p, []),
     "nbh_ntt_len": (_i32, [_i64, _P(_i64)]),
     "nbh_ntt_params": (_i32, [_i64, _i64, _P(_i32), _P(_i64)]),
@@ -177,3 +177,10 @@ def init(device: int = 0) -> None:
     if not _initialised:
         check(lib().nbh_init(device))
         _initialised = True
+
+
+def shutdown() -> None:
+    """nbh_shutdown: raises NbhError (NBH_ERR_ARG) while handles are live."""
+    global _initialised
+    check(lib().nbh_shutdown())
+    _initialised = False

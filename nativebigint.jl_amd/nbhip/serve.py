@@ -245,6 +245,28 @@ def budget_error(req: Request) -> NbhError:
     return NbhError(NBH_ERR_BUDGET, f"request {req.rid} ({req.limbs} limbs) exceeds the HBM budget")
 
 
+def poisson_arrivals(rate_per_s: float, count: int, seed: int) -> np.ndarray:
+    """Arrival times (seconds from the start) of a Poisson stream."""
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate_per_s, count))
+
+
+def drive(srv, reqs, arrivals) -> tuple[float, float]:
+    """Submit each request at its arrival time, polling in between, until
+    every request has completed; returns (start, last completion) in
+    perf_counter seconds.  Works with Server and NativeServer."""
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    i, count = 0, len(reqs)
+    while i < count or not srv.idle():
+        now = time.perf_counter() - t0
+        while i < count and arrivals[i] <= now:
+            srv.submit(reqs[i], t=t0 + arrivals[i])
+            i += 1
+        srv.poll()
+    return t0, max(r.t_done for r in srv.done)
+
+
 def latency_stats(reqs) -> dict:
     lat = np.array([r.t_done - r.t_arrival for r in reqs]) * 1e3
     if lat.size == 0:

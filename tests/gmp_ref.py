@@ -6,6 +6,8 @@ Returns None from `available()` when libgmp is absent.
 """
 import ctypes
 import ctypes.util
+import os
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -55,3 +57,11 @@ def sqr(a: np.ndarray) -> np.ndarray:
     r = np.empty(2 * a.size, dtype=np.uint64)
     _load().__gmpn_sqr(_p(r), _p(a), a.size)
     return r
+
+
+def products(pairs, threads: int | None = None) -> list:
+    """libgmp products of many (a, b) pairs (b None: a square) on a thread
+    pool; ctypes releases the GIL during each call."""
+    threads = threads or min(16, os.cpu_count() or 1)
+    with ThreadPoolExecutor(threads) as ex:
+        return list(ex.map(lambda ab: sqr(ab[0]) if ab[1] is None else mul(ab[0], ab[1]), pairs))
