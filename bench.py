@@ -74,7 +74,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", choices=["batch", "large", "hgcd", "spectra", "powermod", "serve", "small",
-                                           "shortprod", "single", "hostbatch"],
+                                           "shortprod", "single", "hostbatch", "hgcdlevel"],
                     default="batch")
     ap.add_argument("--bases", type=int, default=64, help="powermod: bases sharing the exponent and modulus")
     ap.add_argument("--exp-bits", type=int, default=64, help="powermod: exponent bits")
@@ -184,6 +184,8 @@ def main():
 
     if args.workload == "hgcd":
         return bench_hgcd(args, D, engine, wl, info, stream)
+    if args.workload == "hgcdlevel":
+        return bench_hgcd_level(args, D, engine, wl, info, stream)
     if args.workload == "serve":
         return bench_serve(args, D, engine, wl, info, stream)
     if args.workload == "spectra":
@@ -540,6 +542,56 @@ def bench_hgcd(args, D, engine, wl, info, stream):
                                "[600, 32768], cofactors n/2 limbs", "instances": count,
                    "product_limbs_per_step": total_limbs, "parallelism": "single GPU"},
         "verified_bit_exact": verified,
+        "kernels": {k: {"ms_per_step": prof[k][0] / args.steps, "launches_per_step": prof[k][1] / args.steps}
+                    for k in D.PROF_KINDS},
+        "device": info,
+    }
+    print(json.dumps(out))
+    H.close()
+
+
+def bench_hgcd_level(args, D, engine, wl, info, stream):
+    """One hgcd! level (src/gcd.jl:546-554) per instance on resident buffers:
+    hgcd_matrix_adjust! by M1 and hgcd_matrix_mul! M <- M * M1, 12 products
+    and 6 sums / signed differences per instance, updated in place."""
+    import torch
+    count = min(args.jobs, 1024)
+    sizes = wl.hgcd_sizes(count)
+    dev = []
+    total_limbs = 0
+    for k, n0 in enumerate(sizes):
+        q = wl.hgcd_level_instance(k, n0)
+        sh = wl.hgcd_level_shape(n0)
+        d = {}
+        for name in ("m00", "m01", "m10", "m11"):
+            x = np.zeros(sh["Lm"], dtype=np.uint64)
+            x[:q[name].size] = q[name]
+            d[name] = D.to_dev(x)
+        for name in ("n00", "n01", "n10", "n11", "ra", "rb"):
+            d[name] = D.to_dev(q[name])
+        for name in ("a", "b"):
+            x = np.zeros(max(sh["L"], q[name].size), dtype=np.uint64)
+            x[:q[name].size] = q[name]
+            d[name] = D.to_dev(x)
+        d["mn"], d["p"] = sh["mn"], sh["p"]
+        dev.append(d)
+        total_limbs += 4 * (sh["p"] + sh["m1n"]) + 8 * (sh["mn"] + sh["m1n"])
+    torch.cuda.synchronize()
+    log(f"{count} hgcd levels, {total_limbs} product limbs per step")
+    with torch.cuda.stream(stream):
+        H = D.HgcdLevel(dev)
+    ms, prof = _timed_runs(H, stream, args.steps, args.warmup, D)
+    value = total_limbs / (ms * 1e-3)
+    log(f"timed: {ms:.3f} ms/step, {value:.4g} limb-products/s")
+    out = {
+        "metric": "bit-exact limb-products/s", "value": value, "unit": "limb-products/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (SplitMix64 uniform limbs, nonzero top limb)",
+        "config": {"workload": "hgcdlevel: 1024 hgcd! levels (hgcd_matrix_adjust! by M1, then M <- M*M1) on "
+                               "resident buffers, n0 log-uniform [600, 32768]", "instances": count,
+                   "product_limbs_per_step": total_limbs, "parallelism": "single GPU"},
+        "verified_bit_exact": "tests/test_gpu_hgcd_level.py (in place, two runs, libgmp and Python ints)",
         "kernels": {k: {"ms_per_step": prof[k][0] / args.steps, "launches_per_step": prof[k][1] / args.steps}
                     for k in D.PROF_KINDS},
         "device": info,

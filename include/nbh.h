@@ -202,6 +202,41 @@ int32_t nbh_hgcd_create(const nbh_hgcd_job *jobs, int64_t count, int32_t mode, n
 int32_t nbh_hgcd_run(nbh_hgcd h, void *stream);
 void nbh_hgcd_destroy(nbh_hgcd h);
 
+/* One level of hgcd! on the device, for many instances at once: the second
+ * half of a recursion level (src/gcd.jl:546-554), which applies
+ *   hgcd_matrix_adjust!(M1, a, b, p, ra, rb, nn)   (src/gcd.jl:398-437)
+ *   hgcd_matrix_mul!(M, M1)                        (src/gcd.jl:372-391)
+ * to buffers that stay in device memory, updated in place:
+ *   a <- ra*2^(64p) + n11*a[0..p) - n01*b[0..p),  b <- rb*2^(64p) + n00*b[0..p) - n10*a[0..p),
+ *       each zero-padded to L = p + max(m1n+1, nn) + 1 limbs (a and b must hold L),
+ *       with sa/sb = 1 (positive), 0 (zero) or -1 where the reference's
+ *       nonnegativity assertion would fail (the buffer then holds the magnitude);
+ *   M <- M * M1 (hgcd_mul2! per entry), zero-padded to mn + m1n + 1 limbs
+ *       (each M entry must hold that many);
+ *   *n_ab = max(normlen(a), normlen(b), 1) (the adjust's return value) and
+ *   *n_m = max(normlen of the M entries, 1) (the new M.n), device int64s.
+ * (n00, n01, n10, n11) is M1 (m1n limbs each), (m00 .. m11) is M (mn limbs
+ * each, zero-padded entries allowed).  The two updates read disjoint inputs,
+ * so the level's 12 products run as one planned product batch and the six
+ * sums / signed differences as one more launch.  Operands are sized by their
+ * full lengths (an upper bound of their magnitude), so the buffers' contents
+ * may change between runs.  The handle is an nbh_hgcd: run it with
+ * nbh_hgcd_run, free it with nbh_hgcd_destroy.  No reference counterpart as
+ * a batch. */
+typedef struct {
+    uint64_t *m00, *m01, *m10, *m11;
+    int64_t mn;
+    const uint64_t *n00, *n01, *n10, *n11;
+    int64_t m1n;
+    uint64_t *a, *b;
+    int64_t p;
+    const uint64_t *ra, *rb;
+    int64_t nn;
+    int32_t *sa, *sb;
+    int64_t *n_ab, *n_m;
+} nbh_hgcd_level_job;
+int32_t nbh_hgcd_level_create(const nbh_hgcd_level_job *jobs, int64_t count, nbh_hgcd *out);
+
 /* Cached forward spectra of a fixed operand.  barrett_reduce!
  * (src/barrett.jl:44-81), which powermod calls once per step
  * (src/algorithms.jl:17-107), multiplies by the same modulus m and reciprocal

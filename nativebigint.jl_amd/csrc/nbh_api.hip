@@ -1311,6 +1311,7 @@ struct nbh_hgcd_s {
     void *tmp = nullptr;
     CombineJob *dcomb = nullptr;
     int64_t ncomb = 0;
+    bool lens = false;  // combine jobs carry normlen outputs (a level)
 };
 
 namespace {
@@ -1369,6 +1370,69 @@ int hgcd_create(const nbh_hgcd_job *jobs, int64_t count, int mode, nbh_hgcd_s **
     }
     RCCHK(batch_create(prods.data(), (int64_t)prods.size(), &H->products));
     H->ncomb = (int64_t)comb.size();
+    HIPCHK(hipMalloc((void **)&H->dcomb, sizeof(CombineJob) * comb.size()));
+    HIPCHK(hipMemcpy(H->dcomb, comb.data(), sizeof(CombineJob) * comb.size(), hipMemcpyHostToDevice));
+    *out = H.release();
+    return NBH_OK;
+}
+
+// One hgcd! level (src/gcd.jl:546-554) per instance: hgcd_matrix_adjust!
+// (src/gcd.jl:398-437) on a, b with M1 and hgcd_matrix_mul! (src/gcd.jl:
+// 372-391) M <- M * M1.  The adjust reads M1 and the low p limbs of a, b; the
+// matrix product reads M and M1: disjoint from what the other writes, so all
+// twelve products of an instance go into one product batch (into a private
+// workspace) and the six outputs are written in place by one combine launch
+// after every product is done.
+int hgcd_level_create(const nbh_hgcd_level_job *jobs, int64_t count, nbh_hgcd_s **out) {
+    if (!jobs || count < 1) return fail(NBH_ERR_ARG, "bad hgcd level batch");
+    std::unique_ptr<nbh_hgcd_s, void (*)(nbh_hgcd_s *)> H(new nbh_hgcd_s, hgcd_free);
+    size_t words = 0;
+    for (int64_t i = 0; i < count; i++) {
+        const nbh_hgcd_level_job &q = jobs[i];
+        if (q.mn < 1 || q.m1n < 1 || q.p < 1 || q.nn < 1 || !q.m00 || !q.m01 || !q.m10 || !q.m11 || !q.n00 ||
+            !q.n01 || !q.n10 || !q.n11 || !q.a || !q.b || !q.ra || !q.rb || !q.sa || !q.sb || !q.n_ab || !q.n_m)
+            return fail(NBH_ERR_ARG, "bad hgcd level instance " + std::to_string(i));
+        words += 4 * (size_t)(q.p + q.m1n) + 8 * (size_t)(q.mn + q.m1n);
+    }
+    HIPCHK(hipMalloc(&H->tmp, 8 * words));
+    std::vector<nbh_job> prods;
+    std::vector<CombineJob> comb;
+    uint64_t *t = (uint64_t *)H->tmp;
+    // full lengths as bit bounds: the contents may change between runs
+    auto job = [&](uint64_t *r, const uint64_t *x, int64_t lx, const uint64_t *y, int64_t ly) {
+        prods.push_back(nbh_job{r, x, y, lx, ly, 64 * lx, 64 * ly, 0});
+    };
+    for (int64_t i = 0; i < count; i++) {
+        const nbh_hgcd_level_job &q = jobs[i];
+        // hgcd_matrix_adjust!(M1, a, b, p, ra, rb, nn): n11 a, n01 b, n00 b, n10 a
+        const int64_t lp = q.p + q.m1n;
+        uint64_t *P[4] = {t, t + lp, t + 2 * lp, t + 3 * lp};
+        t += 4 * lp;
+        job(P[0], q.a, q.p, q.n11, q.m1n);
+        job(P[1], q.b, q.p, q.n01, q.m1n);
+        job(P[2], q.b, q.p, q.n00, q.m1n);
+        job(P[3], q.a, q.p, q.n10, q.m1n);
+        const int64_t L = q.p + std::max(q.m1n + 1, q.nn) + 1;
+        comb.push_back(CombineJob{P[0], P[1], lp, lp, q.a, L, q.sa, 1, 0, q.ra, q.nn, q.p, q.n_ab});
+        comb.push_back(CombineJob{P[2], P[3], lp, lp, q.b, L, q.sb, 1, 0, q.rb, q.nn, q.p, q.n_ab});
+        // hgcd_matrix_mul!(M, M1): row (r0, r1) times M1's columns (hgcd_mul2!)
+        const int64_t lq = q.mn + q.m1n;
+        uint64_t *Q[8];
+        for (int k = 0; k < 8; k++) Q[k] = t + k * lq;
+        t += 8 * lq;
+        uint64_t *row[2][2] = {{q.m00, q.m01}, {q.m10, q.m11}};
+        const uint64_t *col[2][2] = {{q.n00, q.n10}, {q.n01, q.n11}};
+        for (int rr = 0; rr < 2; rr++)
+            for (int cc = 0; cc < 2; cc++) {
+                uint64_t *A = Q[4 * rr + 2 * cc], *B = Q[4 * rr + 2 * cc + 1];
+                job(A, row[rr][0], q.mn, col[cc][0], q.m1n);
+                job(B, row[rr][1], q.mn, col[cc][1], q.m1n);
+                comb.push_back(CombineJob{A, B, lq, lq, row[rr][cc], lq + 1, nullptr, 0, 0, nullptr, 0, 0, q.n_m});
+            }
+    }
+    RCCHK(batch_create(prods.data(), (int64_t)prods.size(), &H->products));
+    H->ncomb = (int64_t)comb.size();
+    H->lens = true;
     HIPCHK(hipMalloc((void **)&H->dcomb, sizeof(CombineJob) * comb.size()));
     HIPCHK(hipMemcpy(H->dcomb, comb.data(), sizeof(CombineJob) * comb.size(), hipMemcpyHostToDevice));
     *out = H.release();
@@ -2525,7 +2589,20 @@ int32_t nbh_hgcd_run(nbh_hgcd h, void *stream) {
         hipStream_t s = stream ? (hipStream_t)stream : g.stream;
         RCCHK(batch_run(h->products, s));
         ProfScope ps(5, s);
-        HIPCHK(launch_combine(h->dcomb, h->ncomb, s));
+        HIPCHK(launch_combine(h->dcomb, h->ncomb, s, h->lens));
+        return NBH_OK;
+    NBH_GUARD_END
+}
+
+int32_t nbh_hgcd_level_create(const nbh_hgcd_level_job *jobs, int64_t count, nbh_hgcd *out) {
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        if (!out) return fail(NBH_ERR_ARG, "null handle");
+        nbh_hgcd_s *H = nullptr;
+        RCCHK(hgcd_level_create(jobs, count, &H));
+        *out = H;
+        g.live++;
         return NBH_OK;
     NBH_GUARD_END
 }

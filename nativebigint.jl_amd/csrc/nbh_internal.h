@@ -144,6 +144,9 @@ struct CombineJob {
     int32_t sub, pad;
     const uint64_t *C;  // when set: A is first replaced by A + C * beta^cshift, written to r
     int64_t lc, cshift;
+    // when set: max(*len, 1 + index of r's highest nonzero limb) is stored here
+    // (normlen; the combine's init launch first sets every *len to 1)
+    int64_t *len;
 };
 
 // Barrett correction of one reduction (barrett_reduce!, src/barrett.jl:

@@ -1388,10 +1388,13 @@ __device__ __forceinline__ uint32_t block_carry(bool gen, bool prop, uint32_t ci
 // between threads from the ballot prefix and between chunks in a register.
 // A thread reads limbs k0..k0+3 of X and Y before it writes those of r, so r
 // may be X or Y.
-__device__ __forceinline__ void combine_pass(const uint64_t *X, int64_t lx, const uint64_t *Y, int64_t ly,
-                                             int64_t ysh, uint64_t *r, int64_t lr, bool sub, uint32_t *wsum) {
+// Returns 1 + the highest index k < lr with r[k] != 0 among the calling
+// thread's limbs (0 if none).
+__device__ __forceinline__ int64_t combine_pass(const uint64_t *X, int64_t lx, const uint64_t *Y, int64_t ly,
+                                                int64_t ysh, uint64_t *r, int64_t lr, bool sub, uint32_t *wsum) {
     const int tid = threadIdx.x;
     uint32_t carry = 0;
+    int64_t top = 0;
     for (int64_t c0 = 0; c0 < lr; c0 += UNPACK_LPC) {
         const int64_t k0 = c0 + UNPACK_LPT * tid;
         uint64_t d[UNPACK_LPT];
@@ -1433,10 +1436,14 @@ __device__ __forceinline__ void combine_pass(const uint64_t *X, int64_t lx, cons
                 v = d[w] - cv;
                 cv = d[w] < cv;
             }
-            if (k < lr) r[k] = v;
+            if (k < lr) {
+                r[k] = v;
+                if (v) top = k + 1;
+            }
         }
         carry = cout;
     }
+    return top;
 }
 
 __global__ void __launch_bounds__(UNPACK_THREADS) k_combine(const CombineJob *__restrict__ jobs) {
@@ -1482,7 +1489,25 @@ __global__ void __launch_bounds__(UNPACK_THREADS) k_combine(const CombineJob *__
     }
     const uint64_t *X = swap ? B : A, *Y = swap ? A : B;
     const int64_t lx = swap ? lb : la, ly = swap ? la : lb;
-    combine_pass(X, lx, Y, ly, 0, r, lr, sub, wsum);
+    const int64_t mytop = combine_pass(X, lx, Y, ly, 0, r, lr, sub, wsum);
+    int64_t *len = unip(J->len);
+    if (len) {
+        // normlen of r: block maximum, then one atomic on the shared length
+        if (tid == 0) top = 0;
+        __syncthreads();
+        if (mytop) atomicMax(&top, (unsigned long long)mytop);
+        __syncthreads();
+        if (tid == 0 && top) atomicMax((unsigned long long *)len, top);
+    }
+}
+
+// Sets every combine job's length output to 1 (normlen's floor) before the
+// combine launch takes maxima into it.
+__global__ void __launch_bounds__(256) k_combine_init(const CombineJob *__restrict__ jobs, int64_t count) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    int64_t *len = jobs[i].len;
+    if (len) *len = 1;
 }
 
 // One workgroup per reduction.  Thread t owns limbs c0 + 4t .. c0 + 4t + 3 of
@@ -1834,8 +1859,13 @@ hipError_t launch_fill_u64(uint64_t *p, int64_t n, uint64_t v, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_combine(const CombineJob *jobs, int64_t count, hipStream_t s) {
+hipError_t launch_combine(const CombineJob *jobs, int64_t count, hipStream_t s, bool lens) {
     if (count <= 0) return hipSuccess;
+    if (lens) {
+        hipLaunchKernelGGL(k_combine_init, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, jobs, count);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(k_combine, dim3((unsigned)count), dim3(UNPACK_THREADS), 0, s, jobs);
     return hipGetLastError();
 }

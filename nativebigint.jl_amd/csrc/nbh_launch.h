@@ -27,8 +27,9 @@ hipError_t launch_carry_fix(const DevJob *jobs, const Tile *tiles, const TileGen
                             hipStream_t s);
 hipError_t launch_store_job(const DevJob &j, DevJob *dst, hipStream_t s);
 hipError_t launch_expand_tiles(const TileRange *ranges, int64_t nranges, Tile *tiles, hipStream_t s);
-// One workgroup per cofactor-step output (sum, or |difference| and sign).
-hipError_t launch_combine(const CombineJob *jobs, int64_t count, hipStream_t s);
+// One workgroup per cofactor-step output (sum, or |difference| and sign);
+// lens: some jobs carry a length output (initialised by a launch before).
+hipError_t launch_combine(const CombineJob *jobs, int64_t count, hipStream_t s, bool lens = false);
 // One workgroup per product below the NTT thresholds (schoolbook).
 hipError_t launch_basecase(const BaseJob *jobs, int64_t count, hipStream_t s);
 hipError_t launch_basecase_one(const BaseJob &j, hipStream_t s);

@@ -49,6 +49,17 @@ class nbh_hgcd_job(ctypes.Structure):
                 ("ra", ctypes.c_void_p), ("rb", ctypes.c_void_p), ("nn", ctypes.c_int64)]
 
 
+class nbh_hgcd_level_job(ctypes.Structure):
+    _fields_ = [("m00", ctypes.c_void_p), ("m01", ctypes.c_void_p), ("m10", ctypes.c_void_p),
+                ("m11", ctypes.c_void_p), ("mn", ctypes.c_int64),
+                ("n00", ctypes.c_void_p), ("n01", ctypes.c_void_p), ("n10", ctypes.c_void_p),
+                ("n11", ctypes.c_void_p), ("m1n", ctypes.c_int64),
+                ("a", ctypes.c_void_p), ("b", ctypes.c_void_p), ("p", ctypes.c_int64),
+                ("ra", ctypes.c_void_p), ("rb", ctypes.c_void_p), ("nn", ctypes.c_int64),
+                ("sa", ctypes.c_void_p), ("sb", ctypes.c_void_p),
+                ("n_ab", ctypes.c_void_p), ("n_m", ctypes.c_void_p)]
+
+
 class nbh_spec_job(ctypes.Structure):
     _fields_ = [("r", ctypes.c_void_p), ("a", ctypes.c_void_p), ("m", ctypes.c_int64),
                 ("bits_a", ctypes.c_int64), ("s", ctypes.c_void_p)]
@@ -100,6 +111,7 @@ SIGNATURES = {
     "nbh_hgcd_create": (_i32, [_P(nbh_hgcd_job), _i64, _i32, _P(_vp)]),
     "nbh_hgcd_run": (_i32, [_vp, _vp]),
     "nbh_hgcd_destroy": (None, [_vp]),
+    "nbh_hgcd_level_create": (_i32, [_P(nbh_hgcd_level_job), _i64, _P(_vp)]),
     "nbh_spectrum_create": (_i32, [_vp, _i64, _i64, _i64, _vp, _P(_vp)]),
     "nbh_spectrum_destroy": (None, [_vp]),
     "nbh_spectrum_info": (_i32, [_vp, _P(_i64), _P(_i32), _P(_i64), _P(_i64)]),

@@ -12,7 +12,7 @@ import numpy as np
 import torch
 
 from ._lib import (NBH_HGCD_ADJUST, NBH_HGCD_ADJUST_FULL, NBH_HGCD_FORWARD, check, init, lib, nbh_hgcd_job,
-                   nbh_job, nbh_spec_job)
+                   nbh_hgcd_level_job, nbh_job, nbh_spec_job)
 
 
 def _stream():
@@ -201,6 +201,60 @@ class HgcdMatrixMul:
 
     def close(self) -> None:
         self.step.close()
+
+
+class HgcdLevel:
+    """One level of hgcd! (src/gcd.jl:546-554) for many instances, every
+    buffer resident and updated in place: hgcd_matrix_adjust!(M1, a, b, p,
+    ra, rb, nn) (src/gcd.jl:398-437) then hgcd_matrix_mul!(M, M1)
+    (src/gcd.jl:372-391), run as one planned batch of 12 products per
+    instance and one combine launch.
+
+    instances: dicts of device tensors M = m00..m11 (capacity >= mn + m1n + 1
+    limbs, the first mn used), M1 = n00..n11 (m1n limbs), a, b (capacity >=
+    p + max(m1n + 1, nn) + 1, the low p limbs used), ra, rb (nn limbs), and
+    ints mn, p.  After run(): a, b and the M entries hold the new values
+    (zero-padded), signs[i] the adjust's signs (-1 where the reference's
+    assertion would fail) and lens[i] = (normlen of a/b, new M.n)."""
+
+    def __init__(self, instances):
+        init()
+        arr = (nbh_hgcd_level_job * len(instances))()
+        count = len(instances)
+        self.signs = torch.zeros((count, 2), dtype=torch.int32, device="cuda")
+        self.lens = torch.zeros((count, 2), dtype=torch.int64, device="cuda")
+        self._keep = instances
+        for i, q in enumerate(instances):
+            mn, p = int(q["mn"]), int(q["p"])
+            m1n, nn = q["n00"].numel(), q["ra"].numel()
+            L = p + max(m1n + 1, nn) + 1
+            assert q["a"].numel() >= L and q["b"].numel() >= L and q["rb"].numel() == nn
+            for k in ("m00", "m01", "m10", "m11"):
+                assert q[k].numel() >= mn + m1n + 1
+            for k in ("m00", "m01", "m10", "m11", "n00", "n01", "n10", "n11", "a", "b", "ra", "rb"):
+                setattr(arr[i], k, q[k].data_ptr())
+            arr[i].mn, arr[i].m1n, arr[i].p, arr[i].nn = mn, m1n, p, nn
+            sp, lp = self.signs.data_ptr() + 8 * i, self.lens.data_ptr() + 16 * i
+            arr[i].sa, arr[i].sb, arr[i].n_ab, arr[i].n_m = sp, sp + 4, lp, lp + 8
+        h = ctypes.c_void_p()
+        torch.cuda.synchronize()
+        check(lib().nbh_hgcd_level_create(arr, count, ctypes.byref(h)))
+        self._h = h
+
+    def run(self, stream=None) -> None:
+        s = ctypes.c_void_p(stream.cuda_stream) if stream is not None else _stream()
+        check(lib().nbh_hgcd_run(self._h, s))
+
+    def close(self) -> None:
+        if self._h:
+            lib().nbh_hgcd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Spectrum:

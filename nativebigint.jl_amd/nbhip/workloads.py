@@ -14,6 +14,8 @@ Workloads (sizes in 64-bit limbs):
            src/mul.jl:388)
   hgcd     1024 instances of a 2x2 matrix of n/2-limb cofactors times a
            2-vector of n-limb operands, n log-uniform in [600, 32768]
+  hgcdlevel 1024 instances of one hgcd! level's second half (the adjust by
+           M1 and M <- M * M1), n0 log-uniform in [600, 32768]
   spectra  a fixed k-limb modulus m and (k+1)-limb reciprocal mu, and per
            Barrett reduction a (k+1)-limb q1 and q3 (barrett_reduce!,
            src/barrett.jl:44-81, multiplies q1*mu and q3*m)
@@ -32,7 +34,7 @@ _M2 = 0x94D049BB133111EB
 MASK64 = (1 << 64) - 1
 
 WORKLOAD_IDS = {"single": 1, "large": 2, "batch": 3, "hgcd": 4, "serve": 5, "spectra": 6, "powermod": 7, "small": 8,
-                "test": 9, "shortprod": 10}
+                "test": 9, "shortprod": 10, "hgcdlevel": 11}
 
 
 def _mix(z: int) -> int:
@@ -136,6 +138,37 @@ def hgcd_instance(k: int, n: int, workload: str = "hgcd") -> dict:
     mn = max(1, n // 2)
     names = ("m00", "m01", "m10", "m11", "u", "v")
     return {name: operand(workload, k, i, mn if i < 4 else n) for i, name in enumerate(names)}
+
+
+def hgcd_level_shape(n0: int) -> dict:
+    """Buffer sizes of the second half of one hgcd! level on n0-limb inputs
+    (src/gcd.jl:525-554): after the first recursion and the stray steps the
+    values have n = 3 n0 / 4 limbs, s = n0 / 2 + 1, the second recursion runs
+    on the top n - p limbs with p = 2 s - n + 1 and returns M1 (about (n - p)/2
+    limbs) and the reduced tops ra, rb (nn = (n - p)/2 + 1 limbs); M, from the
+    first recursion on the top n0 / 2 limbs, has about n0 / 4."""
+    s = n0 // 2 + 1
+    n = 3 * n0 // 4
+    p = 2 * s - n + 1
+    m1n = max(1, (n - p) // 2)
+    nn = (n - p) // 2 + 1
+    mn = max(1, n0 // 4)
+    return {"n": n, "p": p, "mn": mn, "m1n": m1n, "nn": nn, "L": p + max(m1n + 1, nn) + 1,
+            "Lm": mn + m1n + 1}
+
+
+def hgcd_level_instance(k: int, n0: int, workload: str = "hgcdlevel") -> dict:
+    """Instance k of the level workload: M (mn-limb entries), M1 (m1n-limb
+    entries), a, b (n limbs), ra, rb (nn limbs), uniform limbs with nonzero
+    top limbs.  Entries are host arrays at their own lengths; the caller pads
+    them to the capacities in hgcd_level_shape."""
+    sh = hgcd_level_shape(n0)
+    out = {}
+    names = [("m00", "mn"), ("m01", "mn"), ("m10", "mn"), ("m11", "mn"), ("n00", "m1n"), ("n01", "m1n"),
+             ("n10", "m1n"), ("n11", "m1n"), ("a", "n"), ("b", "n"), ("ra", "nn"), ("rb", "nn")]
+    for i, (name, size) in enumerate(names):
+        out[name] = operand(workload, k, i, sh[size])
+    return out
 
 
 def barrett_operands(count: int = 1024, k: int = 4096, workload: str = "spectra"):
