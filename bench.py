@@ -1,13 +1,18 @@
 """Benchmark: bit-exact limb-products/s of the MI355X NTT product engine.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W]
-                [--workload batch|large|hgcd|spectra|powermod|serve|small|single|shortprod|hostbatch]
+                [--workload batch|large|hgcd|hgcdlevel|spectra|powermod|serve|small|single|shortprod|hostbatch]
 
 workload "batch" (default): 4096 independent products, larger size
 log-uniform in [152, 65536] limbs, 25% squarings, size ratio uniform in
 [1, 4].  A step is the whole job list in one planned batch (a fixed number of
 launches covering every job).  The same list is also run one product per
-launch for the batched/one-per-launch ratio.
+launch for the batched/one-per-launch ratio; a short serving leg (Poisson
+arrivals at 0.5x and 0.8x of capacity through nbh_server_*) gives p50/p99
+latency; the CPU baseline (the oracle, with libgmp beside it) runs on the
+host threads for about 15 s.
+workload "hgcdlevel": 1024 hgcd! levels on resident buffers (the adjust by M1
+and M <- M * M1), 12 products per instance.
 workload "large": 2^20 x 2^20, 2^20 x 2^18 and a 2^22-limb square, one
 product per launch; a step is the three products.
 workload "hgcd": 1024 batched 2x2 cofactor steps (u', v') = (m11 u - m01 v,
@@ -91,11 +96,77 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-one-per-launch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="batch: skip the short serving leg (p50/p99)")
+    ap.add_argument("--serve-requests", type=int, default=20000,
+                    help="batch: requests per offered load in the short serving leg")
     ap.add_argument("--tuning", default="",
                     help="grp_lds,mid_max,cmin,cmax schedule knobs (default: the library's)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run a plain device-to-device copy of this many MiB (HBM counter calibration)")
     return ap.parse_args()
+
+
+ORACLE_FLAGS = "gcc -O3 -march=x86-64-v3 -ffp-contract=off, scalar"
+
+
+def host_info() -> dict:
+    """CPU model, logical CPUs, physical cores and this process's affinity."""
+    model, phys = "unknown", set()
+    cur = {}
+    try:
+        for line in open("/proc/cpuinfo"):
+            if ":" not in line:
+                if cur:
+                    phys.add((cur.get("physical id"), cur.get("core id")))
+                cur = {}
+                continue
+            k, v = (x.strip() for x in line.split(":", 1))
+            cur[k] = v
+            if k == "model name":
+                model = v
+        if cur:
+            phys.add((cur.get("physical id"), cur.get("core id")))
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count(), "physical_cores": len(phys) or None,
+            "affinity_cpus": len(os.sched_getaffinity(0))}
+
+
+def cpu_threads() -> int:
+    """Host threads for the CPU legs: every CPU this process may run on, at
+    most 16 (the host share a one-GPU box gives a process)."""
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def cpu_throughput(ops, fn, threads: int, seconds: float) -> dict:
+    """Limb-products/s of fn over the job list on `threads` threads, one
+    product per thread at a time, cycling the list until `seconds` have
+    passed (whole products only).  ctypes calls release the GIL."""
+    lock = threading.Lock()
+    state = {"next": 0, "limbs": 0, "products": 0}
+    t_end = time.perf_counter() + seconds
+
+    def worker():
+        while True:
+            with lock:
+                if time.perf_counter() >= t_end:
+                    return
+                i = state["next"] % len(ops)
+                state["next"] += 1
+            a, b = ops[i]
+            r = fn(a, b)
+            with lock:
+                state["limbs"] += r.size
+                state["products"] += 1
+    ts = [threading.Thread(target=worker) for _ in range(threads)]
+    c0 = time.perf_counter()
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    dt = time.perf_counter() - c0
+    return {"value": state["limbs"] / dt, "seconds": dt, "products": state["products"],
+            "passes": state["products"] / len(ops)}
 
 
 def kernel_bytes(plan_groups, N, m, n, sqr, nconv):
@@ -409,42 +480,40 @@ def main():
                "batched_speedup": opl_ms / ms}
         log(f"one-per-launch: {opl_ms:.3f} ms/step, batched speedup {opl_ms / ms:.2f}x")
 
-    # ---- CPU baseline: the oracle on a bounded sample ---------------------------------
+    # ---- CPU baseline: the oracle (and libgmp beside it) on a bounded sample ----
     cpu = None
     # the oracle restates the NTT engine; below its thresholds the reference
     # runs Karatsuba / basecase, which the oracle does not restate
     if not args.no_cpu and wname != "small":
         from oracle import oracle as O
-        ncores = min(16, os.cpu_count() or 1)
-        # the whole job list, three times over: about 13 s of CPU work for the
-        # batch list (4 s per pass at ~1.7e7 limb-products/s per core)
-        passes = 3
-        sample = list(range(len(jobs)))
-        lock = threading.Lock()
-        work = sample * passes
-        done_limbs = [0]
+        from tests import gmp_ref
+        threads = cpu_threads()
+        port = cpu_throughput(ops, lambda a, b: O.sqr(a) if b is None else O.mul(a, b), threads, 10.0)
+        hi = host_info()
+        cpu = {"value": port["value"], "unit": "limb-products/s", "cores": threads, "kind": "port",
+               "sample": f"{wname} job list cycled for {port['seconds']:.1f} s ({port['products']} products, "
+                         f"{port['passes']:.2f} passes of all {len(jobs)}), one product per thread; C restatement "
+                         f"of src/fpntt.jl (oracle/, {ORACLE_FLAGS}), not the Julia original",
+               "host": hi}
+        if gmp_ref.available():
+            gm = cpu_throughput(ops, lambda a, b: gmp_ref.sqr(a) if b is None else gmp_ref.mul(a, b), threads, 5.0)
+            cpu["libgmp"] = {"value": gm["value"], "unit": "limb-products/s", "threads": threads,
+                             "sample": f"__gmpn_mul/__gmpn_sqr over the same list for {gm['seconds']:.1f} s "
+                                       f"({gm['products']} products)"}
+        log(f"cpu baseline {cpu['value']:.4g} limb-products/s on {threads} threads"
+            + (f", libgmp {cpu['libgmp']['value']:.4g}" if "libgmp" in cpu else "") + f" ({hi['model']})")
 
-        def worker():
-            while True:
-                with lock:
-                    if not work:
-                        return
-                    i = work.pop()
-                a, b = ops[i]
-                r = O.sqr(a) if b is None else O.mul(a, b)
-                with lock:
-                    done_limbs[0] += r.size
-        threads = [threading.Thread(target=worker) for _ in range(ncores)]
-        c0 = time.perf_counter()
-        for t in threads:
-            t.start()
-        for t in threads:
-            t.join()
-        cdt = time.perf_counter() - c0
-        cpu = {"value": done_limbs[0] / cdt, "unit": "limb-products/s", "cores": ncores, "kind": "port",
-               "sample": f"all {len(sample)} {wname} products, {passes} passes, C restatement of src/fpntt.jl "
-                         f"(oracle/, gcc -O3 -march=x86-64-v3), one product per thread, {cdt:.1f}s wall"}
-        log(f"cpu baseline {cpu['value']:.4g} limb-products/s on {ncores} threads")
+    # ---- serving: p50/p99 latency under Poisson arrivals at two offered loads ----
+    latency = None
+    if wname == "batch" and not args.no_serve:
+        cap, native, _ = serve_leg(args, D, engine, wl, stream, args.serve_requests, 400, python_loop=False)
+        latency = {"server": "nbh_server_* (continuous batching, 8 GiB HBM budget, 4 streams)",
+                   "capacity_limb_products_per_s": cap,
+                   "requests": "Poisson arrivals, limbs log-uniform [152, 262144], 30% squares, 20% unbalanced "
+                               "(ratio <= 8), operands resident in HBM",
+                   **{load: {k: v[k] for k in ("p50_ms", "p99_ms", "mean_ms", "max_ms", "offered_limb_products_per_s",
+                                               "achieved_limb_products_per_s", "requests", "micro_batches")}
+                      for load, v in native.items()}}
 
     out = {
         "metric": "bit-exact limb-products/s",
@@ -472,6 +541,7 @@ def main():
         "fp64_valu": valu,
         "kernels": kernels,
         "one_per_launch": opl,
+        "latency": latency,
         "cpu_baseline": cpu,
         "device": info,
     }
@@ -1010,14 +1080,16 @@ def bench_hostbatch(args, D, engine, wl, info, stream):
     print(json.dumps(out))
 
 
-def bench_serve(args, D, engine, wl, info, stream):
-    """Poisson arrivals into the continuous-batching server at two offered
-    loads (fractions of the capacity measured on the same request mix)."""
+def serve_leg(args, D, engine, wl, stream, count, pool, python_loop):
+    """Poisson arrivals at 0.5x and 0.8x of the capacity measured on the same
+    request mix (one planned batch of the first requests), through the
+    library's queue (nbh_server_*) and, with python_loop, the Python loop over
+    the arena planner on the same arrivals.  A sample of every load's results
+    is checked against libgmp.  Returns (capacity, native, python)."""
     import torch
     from nbhip import serve
     from tests import gmp_ref
-    count = args.requests
-    pool = min(1000, count)            # distinct operand pairs; requests draw from them
+    pool = min(pool, count)            # distinct operand pairs; requests draw from them
     jobs = wl.serve_jobs(pool)
     ops = wl.batch_operands(jobs, workload="serve")
     dev, bits = [], []
@@ -1054,19 +1126,6 @@ def bench_serve(args, D, engine, wl, info, stream):
     budget = int(args.budget_gib * (1 << 30))
     keep = set(range(0, count, max(1, count // 64)))
 
-    def drive(srv, reqs, arrivals):
-        """Submit each request at its arrival time, polling in between."""
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        i = 0
-        while i < count or not srv.idle():
-            now = time.perf_counter() - t0
-            while i < count and arrivals[i] <= now:
-                srv.submit(reqs[i], t=t0 + arrivals[i])
-                i += 1
-            srv.poll()
-        return t0, max(r.t_done for r in srv.done)
-
     def summary(srv, reqs, pick, load, t0, t_end, host_s, label):
         lat = serve.latency_stats(srv.done)
         thr = sum(r.limbs for r in srv.done) / (t_end - t0)
@@ -1088,14 +1147,15 @@ def bench_serve(args, D, engine, wl, info, stream):
 
     results, native = {}, {}
     for load, pick, arrivals in streams:
-        # the Python loop over the arena planner (results in the arena, sample copied out)
-        srv = serve.Server(budget_bytes=budget, nstreams=args.serve_streams, max_batch=args.serve_max_batch)
-        reqs = [serve.Request(dev[p][0], dev[p][1], *bits[p], keep=i in keep) for i, p in enumerate(pick)]
-        srv.prepare(reqs)                 # start-up: plans of every request shape
-        t0, t_end = drive(srv, reqs, arrivals)
-        results[str(load)] = summary(srv, reqs, pick, load, t0, t_end, srv.host_admit_s, "python loop")
-        del srv, reqs
-        torch.cuda.empty_cache()
+        if python_loop:
+            # the Python loop over the arena planner (results in the arena, sample copied out)
+            srv = serve.Server(budget_bytes=budget, nstreams=args.serve_streams, max_batch=args.serve_max_batch)
+            reqs = [serve.Request(dev[p][0], dev[p][1], *bits[p], keep=i in keep) for i, p in enumerate(pick)]
+            srv.prepare(reqs)                 # start-up: plans of every request shape
+            t0, t_end = serve.drive(srv, reqs, arrivals)
+            results[str(load)] = summary(srv, reqs, pick, load, t0, t_end, srv.host_admit_s, "python loop")
+            del srv, reqs
+            torch.cuda.empty_cache()
         # the library's own serving queue: results straight into per-request buffers
         res = limbs[pick].astype(np.int64)
         offs = np.concatenate([[0], np.cumsum(res)[:-1]])
@@ -1115,11 +1175,18 @@ def bench_serve(args, D, engine, wl, info, stream):
         srv.done.clear()
         srv.host_s = 0.0
         del warm
-        t0, t_end = drive(srv, reqs, arrivals)
+        t0, t_end = serve.drive(srv, reqs, arrivals)
         native[str(load)] = summary(srv, reqs, pick, load, t0, t_end, srv.host_s, "native queue")
         srv.close()
         del srv, reqs, out
         torch.cuda.empty_cache()
+    return cap, native, results
+
+
+def bench_serve(args, D, engine, wl, info, stream):
+    """Poisson arrivals into the continuous-batching server at two offered
+    loads (fractions of the capacity measured on the same request mix)."""
+    cap, native, results = serve_leg(args, D, engine, wl, stream, args.requests, 1000, python_loop=True)
     out = {"metric": "bit-exact limb-products/s", "value": native["0.8"]["achieved_limb_products_per_s"],
            "unit": "limb-products/s", "n_gpus": 1, "steps": 1, "warmup": 0,
            "ms_per_step": native["0.8"]["duration_s"] * 1e3, "higher_is_better": True, "scaling": "weak",
