@@ -66,6 +66,24 @@ int32_t nbh_mul_dev(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b
                     int64_t bits_b, void *stream);
 int32_t nbh_sqr_dev(uint64_t *r, const uint64_t *a, int64_t n, int64_t bits_a, void *stream);
 
+/* nbh_mul_dev without any allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc): the same product, limbs
+ * and arguments as nbh_mul_dev, computed in a static device workspace of
+ * 64 MiB that the library holds on each GPU it runs on until the process
+ * ends, with only plans already cached.  It calls no hipMalloc / hipFree and
+ * creates nothing (profiling events included).  Returns NBH_ERR_ARG with a
+ * clear message when the product needs more than 64 MiB of workspace
+ * (4 spectra of N doubles plus 4 bytes per 1020 result limbs: transform
+ * lengths up to about 2^21, e.g. 2^20 x 2^18 limbs) or when no plan of its
+ * transform length is cached yet -- one nbh_mul_dev of the same length
+ * builds it.  Every call on a GPU shares that GPU's workspace: the library
+ * runs its single-product calls one after another (each waits for the end of
+ * the previous one on whatever stream), so only work that overlaps outside
+ * that order (e.g. a captured replay beside a call) races on it. */
+int32_t nbh_mul_dev_noalloc(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int64_t n,
+                            int64_t bits_a, int64_t bits_b, void *stream);
+
 /* Low short products: mullo! (src/mul.jl:305-354) r[0..k) = a[0..la) *
  * b[0..lb) mod beta^k, and sqrlo! (:360-378) r[0..k) = a[0..la)^2 mod beta^k,
  * la, lb, k >= 1.  Exactly k limbs of r are written (the reference's two

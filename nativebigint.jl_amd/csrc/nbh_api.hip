@@ -35,6 +35,17 @@ using namespace nbh;
 
 // A fixed operand's finished forward spectra (both primes) at one length and
 // chunk width.
+// Workspace of nbh_mul_dev_noalloc, for callers that cannot allocate device
+// memory: a static device array, so it exists on every GPU the library runs
+// on until the process ends (each GPU holds its own copy).  Host code never
+// uses its name as an address: nbh_init looks up each device's address once
+// (Lib::scratch_dev) and every launch gets that.  Its contents persist from
+// call to call; nothing in it needs clearing (the pack writes every spectrum
+// point and every unpack chunk writes its carry summary before it is read).
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+constexpr int SCRATCH_MAX_DEVICES = 64;
+
 struct nbh_spectrum_s {
     const uint64_t *b = nullptr;
     int64_t n = 0, bits_b = 0, bits_a_max = 0, N = 0, ncb = 0;
@@ -282,6 +293,8 @@ struct Lib {
     int64_t live = 0;
     // bumped whenever a schedule knob changes: arena sizes depend on it
     uint64_t tuning_gen = 0;
+    // device address of `scratch` per device ordinal (looked up at nbh_init)
+    void *scratch_dev[SCRATCH_MAX_DEVICES] = {};
 };
 Lib g;
 
@@ -301,8 +314,8 @@ struct ProfScope {
     int kind;
     hipStream_t s;
     hipEvent_t a = nullptr;
-    ProfScope(int k, hipStream_t st) : kind(k), s(st) {
-        if (g.prof && (a = prof_event())) (void)hipEventRecord(a, s);
+    ProfScope(int k, hipStream_t st, bool on = true) : kind(k), s(st) {
+        if (on && g.prof && (a = prof_event())) (void)hipEventRecord(a, s);
     }
     ~ProfScope() {
         if (!a) return;
@@ -583,7 +596,22 @@ int get_plan(int64_t N, int k, PlanEntry **out, bool single = false) {
     return NBH_OK;
 }
 
+// The cached plan of (N, k) under the current schedule, or null: never builds
+// one (building allocates the plan record and the twiddle tables).
+PlanEntry *find_plan(int64_t N, int k, bool single) {
+    const Tuning T = tuning_for(N, single);
+    auto it = g.plans.find(std::make_tuple(N, k, T.grp_lds, T.mid_max, T.cmin, T.cmax));
+    return it == g.plans.end() ? nullptr : it->second.get();
+}
+
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// A caller-independent workspace for products that must not allocate: the
+// library's static device array (its device address and size).
+struct Scratch {
+    void *p;
+    size_t bytes;
+};
 
 // Host-pointer products at most this large (operands + result) go through
 // the pinned bounce buffer: a host memcpy plus a DMA from pinned memory beats
@@ -696,9 +724,12 @@ int read_bits(const uint64_t *a, int64_t n, int64_t *bits, hipStream_t s = nullp
 // One product on stream s (null: g.stream): pack+forward groups, mid,
 // inverse groups, unpack.  keep > 0: only r[0..keep) is written, the product
 // mod beta^keep.  The caller orders it with lib_begin / lib_end.
+// sc: use that workspace and only cached plans; nothing is allocated (no
+// device memory, no profiling events), and a length without a cached plan or
+// a workspace need beyond sc->bytes is an error.
 int run_single(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int64_t n, int64_t bits_a,
                int64_t bits_b, bool sqr, const nbh_spectrum_s *spec = nullptr, int64_t keep = 0,
-               hipStream_t s = nullptr) {
+               hipStream_t s = nullptr, const Scratch *sc = nullptr) {
     if (!s) s = g.stream;
     if (spec) { b = spec->b; n = spec->n; bits_b = spec->bits_b; }
     if (!r || !a || m < 1 || (!sqr && (!b || n < 1))) return fail(NBH_ERR_ARG, "null buffer or empty operand");
@@ -723,14 +754,29 @@ int run_single(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int
         return NBH_OK;
     }
     PlanEntry *P[2];
-    RCCHK(get_plan(N, 0, &P[0], true));
-    RCCHK(get_plan(N, 1, &P[1], true));
     // operands transformed here: a, and b unless it is a's square or cached
     const int nops = sqr || spec ? 1 : 2;
     const size_t xbytes = align256(sizeof(double) * (size_t)N);
     const size_t need = align256(sizeof(uint32_t) * (size_t)J.nchunks) + xbytes * 2 * nops;
-    RCCHK(ensure_buf(&g.ws, &g.ws_size, need));
-    char *w = (char *)g.ws;
+    char *w = nullptr;
+    if (sc) {
+        if (need > sc->bytes)
+            return fail(NBH_ERR_ARG, "product needs " + std::to_string(need) + " bytes of workspace (transform length " +
+                                         std::to_string(N) + "); the no-allocation entry point has " +
+                                         std::to_string(sc->bytes));
+        P[0] = find_plan(N, 0, true);
+        P[1] = find_plan(N, 1, true);
+        if (!P[0] || !P[1])
+            return fail(NBH_ERR_ARG, "no cached plan for transform length " + std::to_string(N) +
+                                         ": run one allocating product of this length (nbh_mul_dev) first");
+        w = (char *)sc->p;
+    } else {
+        RCCHK(get_plan(N, 0, &P[0], true));
+        RCCHK(get_plan(N, 1, &P[1], true));
+        RCCHK(ensure_buf(&g.ws, &g.ws_size, need));
+        w = (char *)g.ws;
+    }
+    const bool prof = sc == nullptr;
     J.chunk_flags = (uint32_t *)w;
     w += align256(sizeof(uint32_t) * (size_t)J.nchunks);
     for (int k = 0; k < 2; k++) {
@@ -750,30 +796,30 @@ int run_single(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int
     const int K = H.ngroups;
     for (int gi = 0; gi + 1 < K; gi++) {
         TileGen tg{gi, nops, H.g[gi].tiles, gi == 0 ? TILE_PACK : 0, 0};
-        ProfScope ps(0, s);
+        ProfScope ps(0, s, prof);
         HIPCHK(launch_group(slot, nullptr, tg, 2 * nops * H.g[gi].tiles, 1, P[0]->grp_lds_bytes[gi],
                             P[0]->grp_threads[gi], s));
     }
     {
         TileGen tg{K - 1, 1, H.g[K - 1].tiles, K == 1 ? TILE_PACK : 0, 0};
-        ProfScope ps(1, s);
+        ProfScope ps(1, s, prof);
         // a cached b still takes an LDS array, but only a is transformed
         HIPCHK(launch_mid(slot, nullptr, tg, 2 * H.g[K - 1].tiles, P[0]->mid_lds_bytes[sqr ? 1 : 0],
                           P[0]->mid_threads[sqr || spec ? 1 : 0], s));
     }
     for (int gi = K - 2; gi >= 0; gi--) {
         TileGen tg{gi, 1, H.g[gi].tiles, 0, 0};
-        ProfScope ps(2, s);
+        ProfScope ps(2, s, prof);
         HIPCHK(launch_group(slot, nullptr, tg, 2 * H.g[gi].tiles, 0, P[0]->grp_lds_bytes[gi], P[0]->grp_threads[gi],
                             s));
     }
     TileGen ug{0, 1, J.nchunks, 0, 0};
     {
-        ProfScope ps(3, s);
+        ProfScope ps(3, s, prof);
         HIPCHK(launch_unpack(slot, nullptr, ug, J.nchunks, J.bch, s));
     }
     if (J.nchunks > 1) {  // one chunk has no carry to pass on
-        ProfScope ps(4, s);
+        ProfScope ps(4, s, prof);
         HIPCHK(launch_carry_fix(slot, nullptr, ug, J.nchunks, s));
     }
     return NBH_OK;
@@ -1457,6 +1503,13 @@ int32_t nbh_init(int32_t device) {
         HIPCHK(hipGetDeviceCount(&n));
         if (device < 0 || device >= n) return fail(NBH_ERR_ARG, "no such HIP device");
         HIPCHK(hipSetDevice(device));
+        if (device < SCRATCH_MAX_DEVICES && !g.scratch_dev[device]) {
+            // with `device` current: its copy of the static workspace
+            void *ptr = nullptr;
+            HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+            if (!ptr) return fail(NBH_ERR_HIP, "no device address for the no-allocation workspace");
+            g.scratch_dev[device] = ptr;
+        }
         hipDeviceProp_t prop;
         HIPCHK(hipGetDeviceProperties(&prop, device));
         g.device = device;
@@ -1565,6 +1618,24 @@ int32_t nbh_mul_dev(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b
         std::lock_guard<std::recursive_mutex> lk(g.mu);
         RCCHK(ensure_init());
         return run_on(stream, [&](hipStream_t s) { return run_single(r, a, m, b, n, bits_a, bits_b, false, nullptr, 0, s); });
+    NBH_GUARD_END
+}
+
+int32_t nbh_mul_dev_noalloc(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int64_t n,
+                            int64_t bits_a, int64_t bits_b, void *stream) {
+    NBH_GUARD_BEGIN
+        std::lock_guard<std::recursive_mutex> lk(g.mu);
+        RCCHK(ensure_init());
+        void *p = g.device >= 0 && g.device < SCRATCH_MAX_DEVICES ? g.scratch_dev[g.device] : nullptr;
+        if (!p) return fail(NBH_ERR_HIP, "the no-allocation workspace has no address on this device");
+        const Scratch sc{p, NOALLOC_SCRATCH_BYTES};
+        const hipStream_t s = stream ? (hipStream_t)stream : g.stream;
+        // ordered after the previous single-product call, like nbh_mul_dev
+        // (no std::function here: its capture could take host memory)
+        RCCHK(lib_begin(s));
+        const int rc = run_single(r, a, m, b, n, bits_a, bits_b, false, nullptr, 0, s, &sc);
+        const int rc2 = lib_end(s);
+        return rc != NBH_OK ? rc : rc2;
     NBH_GUARD_END
 }
 

@@ -86,6 +86,7 @@ SIGNATURES = {
     "nbh_sqr": (_i32, [_vp, _vp, _i64]),
     "nbh_mul_dev": (_i32, [_vp, _vp, _i64, _vp, _i64, _i64, _i64, _vp]),
     "nbh_sqr_dev": (_i32, [_vp, _vp, _i64, _i64, _vp]),
+    "nbh_mul_dev_noalloc": (_i32, [_vp, _vp, _i64, _vp, _i64, _i64, _i64, _vp]),
     "nbh_mullo": (_i32, [_vp, _vp, _i64, _vp, _i64, _i64]),
     "nbh_sqrlo": (_i32, [_vp, _vp, _i64, _i64]),
     "nbh_mullo_dev": (_i32, [_vp, _vp, _i64, _vp, _i64, _i64, _i64, _i64, _vp]),
