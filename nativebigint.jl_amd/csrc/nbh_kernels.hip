@@ -895,32 +895,125 @@ __device__ __forceinline__ Tile fetch_tile(const Tile *__restrict__ tiles, const
 }
 
 // The LDS tile (rows of 2^logC columns, row r at gbase + r*Lb) back to HBM,
-// SU points per thread per round.
-constexpr int SU = 4;
+// SU points per thread per round.  When the block's thread count is a
+// multiple of the row width (always, for the schedules built here), point
+// e0 + k*T of thread e0 sits k*step elements past the thread's first point
+// and k*(T + T/16) doubles past its first padded LDS slot: one index per
+// thread, then pointer steps by uniform amounts.
+constexpr int SU = 8;
 
 __device__ __forceinline__ void store_tile(double *__restrict__ x, const double *__restrict__ lds, int total,
                                            const TileGeo &g, int64_t gbase) {
     const int T = blockDim.x;
     const int cmask = (1 << g.logC) - 1;
-    for (int r0 = 0; r0 < total; r0 += SU * T) {
-        const int e0 = r0 + (int)threadIdx.x;
-        if (r0 + SU * T <= total) {
+    const int e0 = threadIdx.x;
+    if ((T & cmask) == 0) {
+        gdouble *__restrict__ p = (gdouble *)x + (gbase + (int64_t)(e0 >> g.logC) * g.Lb + (e0 & cmask));
+        const int64_t step = (int64_t)(T >> g.logC) * g.Lb;
+        const double *__restrict__ l = lds + pidx(e0);
+        const int lstep = T + (T >> 4);
+        int r0 = 0;
+        for (; r0 + SU * T <= total; r0 += SU * T) {
             double v[SU];
 #pragma unroll
-            for (int k = 0; k < SU; k++) v[k] = lds[pidx(e0 + k * T)];
+            for (int k = 0; k < SU; k++) v[k] = l[k * lstep];
 #pragma unroll
-            for (int k = 0; k < SU; k++) {
-                const int e = e0 + k * T;
-                gst(x, gbase + (int64_t)(e >> g.logC) * g.Lb + (e & cmask), v[k]);
-            }
-        } else {
-#pragma unroll
-            for (int k = 0; k < SU; k++) {
-                const int e = e0 + k * T;
-                if (e < total) gst(x, gbase + (int64_t)(e >> g.logC) * g.Lb + (e & cmask), lds[pidx(e)]);
-            }
+            for (int k = 0; k < SU; k++) p[k * step] = v[k];
+            p += SU * step;
+            l += SU * lstep;
         }
+        for (int k = 0; r0 + k * T < total; k++)
+            if (r0 + k * T + e0 < total) p[k * step] = l[k * lstep];
+        return;
     }
+    for (int r0 = 0; r0 < total; r0 += T) {
+        const int e = r0 + e0;
+        if (e < total) gst(x, gbase + (int64_t)(e >> g.logC) * g.Lb + (e & cmask), lds[pidx(e)]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Edge passes.  When a tile's first two stages are radix 4 and the block has
+// exactly one thread per 16 points (T = S*C/16), thread e0's round loads
+// e0 + k*T (k = 0..15) are rows r + k*S/16 of column c (r = e0 >> logC,
+// c = e0 & (C-1)): exactly the sub-problem (a, b), k = a + 4b, of the pair's
+// pass44 with sl = e0 (legs S/16 and S/4 rows apart).  So the first forward
+// pair runs on the loaded registers and writes LDS once, and the inverse of
+// that pair -- the last inverse pass -- stores from registers to the same
+// addresses the loads read.  One LDS round trip and one barrier fewer on each
+// side; the butterflies are the pass44 ones.
+
+__device__ __forceinline__ bool edge_ok(const DevPlan *__restrict__ P, int s0, int s1, int total, int min_stages) {
+    return total == 16 * (int)blockDim.x && s1 - s0 >= min_stages && uni(cst(P)->st[s0].m) == 4 &&
+           uni(cst(P)->st[s0 + 1].m) == 4;
+}
+
+// Loads the thread's 16 points from src (rounds of NU), runs stages s0, s0+1
+// forward on them and writes them to the LDS tile.
+template <int K, int NU>
+__device__ __forceinline__ void edge_load44(double *__restrict__ lds, const PointSrc &src, bool pack,
+                                            const TileGeo &g, int64_t gbase, const DevPlan *__restrict__ P, int s0) {
+    static_assert(16 % NU == 0, "rounds of NU points cover the 16");
+    const int T = blockDim.x;
+    const int total = 16 * T;
+    double y[4][4];
+#pragma unroll
+    for (int r = 0; r < 16 / NU; r++) {
+        double v[NU];
+        if (pack) load_round<true, NU>(v, src, (int)threadIdx.x + r * NU * T, total, true, g, gbase);
+        else load_round<false, NU>(v, src, (int)threadIdx.x + r * NU * T, total, true, g, gbase);
+#pragma unroll
+        for (int k = 0; k < NU; k++) y[(r * NU + k) & 3][(r * NU + k) >> 2] = v[k];
+    }
+    const double *tw = unip(cst(P)->tw);
+    const StageU sa = stage_u(&P->st[s0], tw), sb = stage_u(&P->st[s0 + 1], tw);
+    const Sub44 u = sub44((int)threadIdx.x, sa, sb, g);
+    double w0[4][3], w1[3];
+    tw44<false>(w0, w1, u, sa, sb);
+    regs44<K, true>(y, w0, w1, sa, sb);
+    if ((((sa.q | sb.q) << g.logC) & 15) == 0) {
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) lds[ix44<true>(u, a, b)] = y[a][b];
+    } else {
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) lds[ix44<false>(u, a, b)] = y[a][b];
+    }
+}
+
+// Reads the thread's pass44 sub-problem of stages s0, s0+1 from the LDS tile,
+// runs the pair inverse and stores the 16 points to x.
+template <int K>
+__device__ __forceinline__ void edge_store44(double *__restrict__ x, const double *__restrict__ lds,
+                                             const TileGeo &g, int64_t gbase, const DevPlan *__restrict__ P, int s0) {
+    const int T = blockDim.x;
+    const int e0 = threadIdx.x;
+    const double *tw = unip(cst(P)->tw);
+    const StageU sa = stage_u(&P->st[s0], tw), sb = stage_u(&P->st[s0 + 1], tw);
+    const Sub44 u = sub44(e0, sa, sb, g);
+    double y[4][4];
+    if ((((sa.q | sb.q) << g.logC) & 15) == 0) {
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) y[a][b] = lds[ix44<true>(u, a, b)];
+    } else {
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) y[a][b] = lds[ix44<false>(u, a, b)];
+    }
+    double w0[4][3], w1[3];
+    tw44<false>(w0, w1, u, sa, sb);
+    regs44<K, false>(y, w0, w1, sa, sb);
+    const int cmask = (1 << g.logC) - 1;
+    gdouble *__restrict__ p = (gdouble *)x + (gbase + (int64_t)(e0 >> g.logC) * g.Lb + (e0 & cmask));
+    const int64_t step = (int64_t)(T >> g.logC) * g.Lb;
+#pragma unroll
+    for (int k = 0; k < 16; k++) p[k * step] = y[k & 3][k >> 2];
 }
 
 // ---------------------------------------------------------------------------
@@ -953,11 +1046,30 @@ __device__ __forceinline__ void group_body(const DevJob *__restrict__ Jp, const 
     src.n = uni64(t.op ? J->n : J->m);
     src.nch = uni64(t.op ? J->ncb : J->nca);
     src.b = uni(J->bch);
-    load_tile<NU>(lds, src, (t.flags & TILE_PACK) != 0, total, g, gbase);
-    __syncthreads();
-    if (fwd) run_stages<K, true>(lds, nullptr, 1, P, s0, s1, g);
-    else run_stages<K, false>(lds, nullptr, 1, P, s0, s1, g);
-    store_tile(x, lds, total, g, gbase);
+    const bool pack = (t.flags & TILE_PACK) != 0;
+    const bool edge = edge_ok(P, s0, s1, total, 2);
+    if (fwd) {
+        if (edge) {
+            edge_load44<K, (NU < 8 ? NU : 8)>(lds, src, pack, g, gbase, P, s0);
+            __syncthreads();
+            run_stages<K, true>(lds, nullptr, 1, P, s0 + 2, s1, g);
+        } else {
+            load_tile<NU>(lds, src, pack, total, g, gbase);
+            __syncthreads();
+            run_stages<K, true>(lds, nullptr, 1, P, s0, s1, g);
+        }
+        store_tile(x, lds, total, g, gbase);
+    } else {
+        load_tile<NU>(lds, src, pack, total, g, gbase);
+        __syncthreads();
+        if (edge) {
+            run_stages<K, false>(lds, nullptr, 1, P, s0 + 2, s1, g);
+            edge_store44<K>(x, lds, g, gbase, P, s0);
+        } else {
+            run_stages<K, false>(lds, nullptr, 1, P, s0, s1, g);
+            store_tile(x, lds, total, g, gbase);
+        }
+    }
 }
 
 // Forward and inverse group passes are separate kernels (one code path each,
@@ -1039,24 +1151,37 @@ __device__ __forceinline__ void mid_body(const DevJob *__restrict__ Jp, const Ti
     sb.nch = uni64(J->ncb);
     sa.b = sb.b = uni(J->bch);
     double hb[HOLD];
+    // the first radix-4 pair straight from the loads, the last inverse one
+    // straight to HBM (when the pair is not also the last forward pass)
+    const bool edge = edge_ok(P, s0, s1, S, 3);
+    const int sf = edge ? s0 + 2 : s0;
     if (!sq && !bspec) {
         // b first: its forward passes, then its last pass held in registers
-        load_tile<LU>(la, sb, pack, S, g, gbase);
+        if (edge) {
+            edge_load44<K, 8>(la, sb, pack, g, gbase, P, s0);
+        } else {
+            load_tile<LU>(la, sb, pack, S, g, gbase);
+        }
         __syncthreads();
-        const int sl = fwd_until_last<K>(la, P, s0, s1, g);
+        const int sl = fwd_until_last<K>(la, P, sf, s1, g);
         last_pass<K, true, SRC_HELD>(la, hb, nullptr, 0, P, sl, s1, g);
         __syncthreads();
     }
-    load_tile<LU>(la, sa, pack, S, g, gbase);
+    if (edge) {
+        edge_load44<K, 8>(la, sa, pack, g, gbase, P, s0);
+    } else {
+        load_tile<LU>(la, sa, pack, S, g, gbase);
+    }
     __syncthreads();
-    const int sl = fwd_until_last<K>(la, P, s0, s1, g);
+    const int sl = fwd_until_last<K>(la, P, sf, s1, g);
     if (sq) last_pass<K, false, SRC_SELF>(la, hb, nullptr, 0, P, sl, s1, g);
     else if (bspec) last_pass<K, false, SRC_GLOB>(la, hb, sb.x, gbase, P, sl, s1, g);
     else last_pass<K, false, SRC_HELD>(la, hb, nullptr, 0, P, sl, s1, g);
     __syncthreads();
-    run_stages<K, false>(la, nullptr, 1, P, s0, sl, g);
+    run_stages<K, false>(la, nullptr, 1, P, sf, sl, g);
     double *__restrict__ xo = unip(J->xa[K]);
-    store_tile(xo, la, S, g, gbase);
+    if (edge) edge_store44<K>(xo, la, g, gbase, P, s0);
+    else store_tile(xo, la, S, g, gbase);
 }
 
 __global__ void __launch_bounds__(MID_MAX_THREADS) k_mid(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles,
