@@ -22,6 +22,9 @@ end
 
 function __init__()
     check(ccall((:nbh_init, libnbh), Int32, (Int32,), 0))
+    # nbh_shutdown refuses (and changes nothing) while handles are live; at
+    # exit that only means the process frees them
+    atexit(() -> ccall((:nbh_shutdown, libnbh), Int32, ()))
 end
 
 function mul_fpntt2!(r::Memory{UInt64}, ro::Int, a::Memory{UInt64}, ao::Int, m::Int,
