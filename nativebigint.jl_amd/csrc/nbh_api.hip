@@ -448,6 +448,25 @@ int get_stages(int64_t N, int k, StageList **out) {
 int64_t pow2_part(int64_t x) { return x & -x; }
 int64_t floor_pow2(int64_t x) { return x < 1 ? 0 : (int64_t)1 << (63 - __builtin_clzll((uint64_t)x)); }
 
+// The column-kernel shape of a strided group (radices in forward order as
+// M0 | M1 << 4 | M2 << 8), or 0 when the column kernels do not take it: one
+// to three stages of radix 3, 4, 5 or 7 with at most 36 rows (the shapes
+// col_switch lists).
+int col_code(const StageList &L, const GroupDesc &G) {
+    if (G.S > 36 || G.s1 - G.s0 > 3) return 0;
+    int code = 0;
+    for (int si = G.s0; si < G.s1; si++) {
+        const int m = L.st[si].m;
+        if (m != 3 && m != 4 && m != 5 && m != 7) return 0;
+        code |= m << (4 * (si - G.s0));
+    }
+    switch (code) {
+    case 0x3: case 0x4: case 0x5: case 0x7: case 0x33: case 0x43: case 0x35: case 0x44:
+    case 0x45: case 0x37: case 0x47: case 0x57: case 0x433: return code;
+    default: return 0;
+    }
+}
+
 // Cut the stage list into HBM passes: the last group is every stage of span
 // <= mid_max (contiguous blocks); the earlier stages are packed greedily into
 // strided groups whose S x C tile fits grp_lds doubles with C >= cmin columns.
@@ -501,6 +520,17 @@ int make_schedule(const StageList &L, const Tuning &T, DevPlan &P, PlanEntry &E)
         }
         E.grp_lds_bytes[gi] = sizeof(double) * (size_t)lds_doubles(Gd.S * Gd.C);
         E.grp_threads[gi] = tile_threads((int64_t)Gd.S * Gd.C);
+        // a strided group of at most 36 rows in a supported shape runs as a
+        // column group: one thread per column, up to COL_THREADS columns a tile
+        const int code = gi + 1 < ng ? col_code(L, Gd) : 0;
+        if (code) {
+            Gd.col = code;
+            Gd.C = (int32_t)std::min<int64_t>(pow2_part(Gd.Lb), COL_THREADS);
+            Gd.logC = __builtin_ctz((unsigned)Gd.C);
+            Gd.tiles = (L.N / Gd.La) * (Gd.Lb / Gd.C);
+            E.grp_lds_bytes[gi] = Gd.S <= 16 ? COL_SMALL : COL_BIG;
+            E.grp_threads[gi] = Gd.C;
+        }
     }
     // the mid kernel holds one array per tile (products keep b's last pass in registers)
     E.mid_lds_bytes[0] = sizeof(double) * (size_t)lds_doubles(P.g[ng - 1].S);

@@ -40,7 +40,7 @@ struct GroupDesc {
     int32_t S;          // rows: La / Lb
     int32_t C;          // columns per tile (power of two dividing Lb)
     int32_t logC;
-    int32_t pad;
+    int32_t col;        // column group: radices M0 | M1 << 4 | M2 << 8 (0: LDS tiles)
     int64_t La, Lb;
     int64_t tiles;      // tiles per transform: (N / La) * (Lb / C)
 };

@@ -18,6 +18,7 @@
 
 #include "nbh_field.h"
 #include "nbh_internal.h"
+#include "nbh_launch.h"
 
 namespace nbh {
 
@@ -1117,6 +1118,150 @@ k_group_inv_w6(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles, 
 }
 
 // ---------------------------------------------------------------------------
+// Column groups: a strided group of one to three stages with S <= 36 rows
+// (radices 7, 5, 3, 4 and their products up to 36).  Its tile is C adjacent
+// columns of the S rows; every column is an independent S-point problem of
+// the group's stages (the legs of each stage are rows of the same column), so
+// thread c loads its column's S points (coalesced across the wave: adjacent
+// columns are adjacent in memory), runs every stage of the group on them in
+// registers and stores them back.  No LDS and no barrier: the waves only
+// stream, and as many stay resident as their registers allow.  The
+// butterflies are those of pass1 / pass44, so any schedule gives the same
+// spectra.
+
+// One stage of radix M on the column's S points: legs STRIDE rows apart in
+// blocks of STRIDE*M rows; the butterfly at row jr of a block has twiddle
+// index jr*Lb + q (q = the column), as in pass1.
+template <int K, bool FWD, int M, int STRIDE, int S>
+__device__ __forceinline__ void col_stage(double (&y)[S], const StageU &st, int64_t Lb, int64_t q) {
+    static_assert(S % (STRIDE * M) == 0, "stage blocks tile the column");
+#pragma unroll
+    for (int b = 0; b < S / (STRIDE * M); b++)
+#pragma unroll
+        for (int jr = 0; jr < STRIDE; jr++) {
+            const int64_t j = jr * Lb + q;
+            double w[M - 1], v[M];
+#pragma unroll
+            for (int t = 1; t < M; t++) w[t - 1] = gld(st.tw, (t - 1) * st.Q + j);
+#pragma unroll
+            for (int t = 0; t < M; t++) v[t] = y[b * STRIDE * M + jr + t * STRIDE];
+            stage_regs<K, FWD, M>(v, w, st);
+#pragma unroll
+            for (int t = 0; t < M; t++) y[b * STRIDE * M + jr + t * STRIDE] = v[t];
+        }
+}
+
+template <int K, bool FWD, int M0, int M1, int M2>
+__device__ void col_body(const DevJob *__restrict__ Jp, const Tile &t) {
+    constexpr int S = M0 * (M1 ? M1 : 1) * (M2 ? M2 : 1);
+    const auto *J = cst(Jp);
+    const DevPlan *__restrict__ P = unip(J->plan[K]);
+    const auto *G = &cst(P)->g[t.group];
+    const int64_t La = uni64(G->La), Lb = uni64(G->Lb);
+    const int logC = uni(G->logC);
+    const int s0 = uni(G->s0);
+    const int64_t tpr = Lb >> logC;
+    const int64_t blk = (int64_t)t.tile / tpr;
+    const int64_t jc = (int64_t)t.tile - blk * tpr;
+    const int64_t q = (jc << logC) + threadIdx.x;
+    const int64_t base = blk * La + q;
+    double *__restrict__ x = unip(t.op ? J->xb[K] : J->xa[K]);
+    double y[S];
+    if ((t.flags & TILE_PACK) != 0) {
+        const uint64_t *limbs = unip(t.op ? J->b : J->a);
+        const int64_t n = uni64(t.op ? J->n : J->m);
+        const int64_t nch = uni64(t.op ? J->ncb : J->nca);
+        const int b = uni(J->bch);
+        if (n <= BUF_MAX_LIMBS) {
+            const __amdgpu_buffer_rsrc_t r = limb_rsrc(limbs, n);
+            const uint32_t lmask = b >= 32 ? ~0u : (1u << b) - 1u;
+            const uint32_t hmask = b >= 32 ? (1u << (b - 32)) - 1u : 0u;
+#pragma unroll
+            for (int k = 0; k < S; k++) {
+                const int64_t i = base + k * Lb;
+                y[k] = pack_buf(r, lmask, hmask, i < nch, i * b);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < S; k++) y[k] = pack_chunk(limbs, n, b, nch, base + k * Lb);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < S; k++) y[k] = gld(x, base + k * Lb);
+    }
+    const double *tw = unip(cst(P)->tw);
+    if (FWD) {
+        col_stage<K, true, M0, S / M0>(y, stage_u(&P->st[s0], tw), Lb, q);
+        if constexpr (M1 != 0) col_stage<K, true, M1, S / (M0 * M1)>(y, stage_u(&P->st[s0 + 1], tw), Lb, q);
+        if constexpr (M2 != 0) col_stage<K, true, M2, 1>(y, stage_u(&P->st[s0 + 2], tw), Lb, q);
+    } else {
+        if constexpr (M2 != 0) col_stage<K, false, M2, 1>(y, stage_u(&P->st[s0 + 2], tw), Lb, q);
+        if constexpr (M1 != 0) col_stage<K, false, M1, S / (M0 * M1)>(y, stage_u(&P->st[s0 + 1], tw), Lb, q);
+        col_stage<K, false, M0, S / M0>(y, stage_u(&P->st[s0], tw), Lb, q);
+    }
+#pragma unroll
+    for (int k = 0; k < S; k++) gst(x, base + k * Lb, y[k]);
+}
+
+// The group shapes a column kernel takes (GroupDesc::col = M0 | M1 << 4 | M2 << 8).
+template <int K, bool FWD, bool BIG>
+__device__ __forceinline__ void col_switch(const DevJob *J, const Tile &t, int code) {
+    if (!BIG) {
+        switch (code) {
+        case 0x3: col_body<K, FWD, 3, 0, 0>(J, t); break;
+        case 0x4: col_body<K, FWD, 4, 0, 0>(J, t); break;
+        case 0x5: col_body<K, FWD, 5, 0, 0>(J, t); break;
+        case 0x7: col_body<K, FWD, 7, 0, 0>(J, t); break;
+        case 0x33: col_body<K, FWD, 3, 3, 0>(J, t); break;
+        case 0x43: col_body<K, FWD, 3, 4, 0>(J, t); break;
+        case 0x35: col_body<K, FWD, 5, 3, 0>(J, t); break;
+        case 0x44: col_body<K, FWD, 4, 4, 0>(J, t); break;
+        default: break;
+        }
+    } else {
+        switch (code) {
+        case 0x45: col_body<K, FWD, 5, 4, 0>(J, t); break;
+        case 0x37: col_body<K, FWD, 7, 3, 0>(J, t); break;
+        case 0x47: col_body<K, FWD, 7, 4, 0>(J, t); break;
+        case 0x57: col_body<K, FWD, 7, 5, 0>(J, t); break;
+        case 0x433: col_body<K, FWD, 3, 3, 4>(J, t); break;
+        default: break;
+        }
+    }
+}
+
+template <bool FWD, bool BIG>
+__device__ __forceinline__ void col_kernel(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles,
+                                           const TileGen &gen) {
+    const Tile t = fetch_tile(tiles, gen);
+    const DevJob *J = jobs + t.job;
+    if (uni(cst(J)->zero)) return;
+    const DevPlan *P = unip(cst(J)->plan[t.prime ? 1 : 0]);
+    const int code = uni(cst(P)->g[t.group].col);
+    if (t.prime == 0) col_switch<0, FWD, BIG>(J, t, code);
+    else col_switch<1, FWD, BIG>(J, t, code);
+}
+
+// S <= 16: at most 32 VGPRs of points, six waves per SIMD
+__global__ void __launch_bounds__(COL_THREADS) __attribute__((amdgpu_waves_per_eu(6)))
+k_col_fwd_s(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles, TileGen gen) {
+    col_kernel<true, false>(jobs, tiles, gen);
+}
+__global__ void __launch_bounds__(COL_THREADS) __attribute__((amdgpu_waves_per_eu(6)))
+k_col_inv_s(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles, TileGen gen) {
+    col_kernel<false, false>(jobs, tiles, gen);
+}
+// 16 < S <= 36: up to 72 VGPRs of points, three waves per SIMD (four spill)
+__global__ void __launch_bounds__(COL_THREADS) __attribute__((amdgpu_waves_per_eu(3)))
+k_col_fwd_b(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles, TileGen gen) {
+    col_kernel<true, true>(jobs, tiles, gen);
+}
+__global__ void __launch_bounds__(COL_THREADS) __attribute__((amdgpu_waves_per_eu(3)))
+k_col_inv_b(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles, TileGen gen) {
+    col_kernel<false, true>(jobs, tiles, gen);
+}
+
+// ---------------------------------------------------------------------------
 // Mid pass: last forward group on both operands, pointwise product
 // (src/fpntt.jl:770-785), first inverse group; for a single-group transform
 // the pack is fused in too, so the whole transform pair is one kernel.  A
@@ -1914,6 +2059,18 @@ static inline unsigned grid_for(int64_t n, int threads, int64_t cap = 1 << 16) {
 hipError_t launch_group(const DevJob *jobs, const Tile *tiles, const TileGen &gen, int64_t ntiles, int fwd,
                         size_t lds_bytes, int threads, hipStream_t s) {
     if (ntiles <= 0) return hipSuccess;
+    if (lds_bytes == COL_SMALL || lds_bytes == COL_BIG) {
+        const bool big = lds_bytes == COL_BIG;
+        const dim3 grid((unsigned)ntiles), block(threads);
+        if (fwd) {
+            if (big) hipLaunchKernelGGL(k_col_fwd_b, grid, block, 0, s, jobs, tiles, gen);
+            else hipLaunchKernelGGL(k_col_fwd_s, grid, block, 0, s, jobs, tiles, gen);
+        } else {
+            if (big) hipLaunchKernelGGL(k_col_inv_b, grid, block, 0, s, jobs, tiles, gen);
+            else hipLaunchKernelGGL(k_col_inv_s, grid, block, 0, s, jobs, tiles, gen);
+        }
+        return hipGetLastError();
+    }
     // waves per CU that LDS admits: above 16 (four per SIMD) the six-wave build pays
     const int64_t lds_waves = (int64_t)(LDS_PER_CU / std::max<size_t>(lds_bytes, 1)) * (threads / 64);
     const bool w6 = lds_waves > 16;

@@ -14,6 +14,14 @@ inline int tile_threads(int64_t points) {
     return (int)(t < 64 ? 64 : t > 1024 ? 1024 : t);
 }
 
+// Column groups (GroupDesc::col != 0): each thread carries one column of the
+// group's S rows through all its stages in registers, no LDS.  Their
+// lds_bytes entry is one of these markers (no LDS tile is this small), which
+// picks the column kernel sized for S <= 16 or for 16 < S <= 36.
+constexpr size_t COL_SMALL = 1;
+constexpr size_t COL_BIG = 2;
+constexpr int COL_THREADS = 256;  // columns per tile at most
+
 // Tile-driven launches: `tiles` is a device tile list of ntiles entries, or
 // NULL to generate ntiles tiles of job 0 from `gen`.
 hipError_t launch_group(const DevJob *jobs, const Tile *tiles, const TileGen &gen, int64_t ntiles, int fwd,
