@@ -428,10 +428,21 @@ def main():
     bytes_per_launch = bytes_by_kind[dom] / launches_per_step
     avg_launch_s = dom_ms * 1e-3 / dom_launches
     achieved = bytes_per_launch / avg_launch_s / 1e9
-    kernels = {k: {"ms_per_step": prof[k][0] / psteps, "launches_per_step": prof[k][1] / psteps,
-                   "algorithmic_GBps": (bytes_by_kind[k] / (prof[k][0] * 1e-3 / psteps) / 1e9)
-                   if prof[k][0] > 0 else None}
-               for k in D.PROF_KINDS}
+    def kind_line(k):
+        ms_k = prof[k][0] / psteps
+        line = {"ms_per_step": ms_k, "launches_per_step": prof[k][1] / psteps,
+                "algorithmic_GBps": None, "hbm_frac": None, "fp64_frac": None}
+        if ms_k > 0:
+            gbps = bytes_by_kind[k] / (ms_k * 1e-3) / 1e9
+            line["algorithmic_GBps"] = gbps
+            line["hbm_frac"] = round(gbps / HBM_PEAK_GBS, 4)
+            if fp64_kind.get(k):
+                line["fp64_frac"] = round(fp64_kind[k] / (ms_k * 1e-3) / FP64_INSTR_PEAK, 4)
+        return line
+
+    # per kind: algorithmic bytes and model FP64 instructions of one step over
+    # that kind's summed launch time (launches issued in order)
+    kernels = {k: kind_line(k) for k in D.PROF_KINDS}
     step_bytes = sum(bytes_by_kind.values())
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_{wname}.json")

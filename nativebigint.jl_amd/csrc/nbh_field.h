@@ -6,6 +6,7 @@
 // every operation is an IEEE double +, -, * or fma in round-to-nearest and the
 // sequence of rounding-sensitive operations is the reference's:
 //   fp_round(v)   = (v + 1.5*2^52) - 1.5*2^52      exact nearest integer, |v| <= 2^51
+//                   (device code: one v_rndne_f64, the same integer -- see fp_round)
 //   fp_reduce(x)  = fma(round(x * fl(1/p)), -p, x)
 //   fp_mulmod3    = h = x*y; l = fma(x, y, -h); reduce(h) + l
 //   fp_mulmod4    = h = x*w; l = fma(x, w, -h); q = round(x*wpinv); fma(q, -p, h) + l
@@ -50,7 +51,20 @@ template <> struct Prime<1> {
     static constexpr uint64_t pu = P2_U;
 };
 
-NBH_HD double fp_round(double v) { return (v + FP_MAGIC) - FP_MAGIC; }
+// Nearest integer, ties to even (src/fpntt.jl:81-83).  For |v| <= 2^51 the
+// magic-constant sum rounds v to the nearest integer with ties to even (1.5*2^52
+// is even and its ulp is 1), which is exactly what the rndne instruction
+// returns; the device uses the one instruction instead of two adds.  The two
+// can differ only in the sign of a zero result (rint(-0.3) is -0.0, the sum
+// gives +0.0): the value of every later result is the same, so spectra compare
+// equal to the reference's as numbers and every product limb is bit-identical.
+NBH_HD double fp_round(double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_rint(v);
+#else
+    return (v + FP_MAGIC) - FP_MAGIC;
+#endif
+}
 
 template <int K> NBH_HD double fp_reduce(double x) {
     double q = fp_round(x * Prime<K>::pinv);
