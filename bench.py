@@ -106,9 +106,6 @@ def parse():
     return ap.parse_args()
 
 
-ORACLE_FLAGS = "gcc -O3 -march=x86-64-v3 -ffp-contract=off, scalar"
-
-
 def host_info() -> dict:
     """CPU model, logical CPUs, physical cores and this process's affinity."""
     model, phys = "unknown", set()
@@ -499,12 +496,21 @@ def main():
         from oracle import oracle as O
         from tests import gmp_ref
         threads = cpu_threads()
-        port = cpu_throughput(ops, lambda a, b: O.sqr(a) if b is None else O.mul(a, b), threads, 10.0)
+        fn = lambda a, b: O.sqr(a) if b is None else O.mul(a, b)  # noqa: E731
+        # each build this host can run (AVX2; AVX-512 where present) on a short
+        # sample, then the faster one on the timed sample
+        trials = {}
+        for isa in O.isas():
+            O.use(isa)
+            trials[isa] = cpu_throughput(ops, fn, threads, 2.0)["value"]
+        O.use(max(trials, key=trials.get))
+        port = cpu_throughput(ops, fn, threads, 10.0)
         hi = host_info()
         cpu = {"value": port["value"], "unit": "limb-products/s", "cores": threads, "kind": "port",
                "sample": f"{wname} job list cycled for {port['seconds']:.1f} s ({port['products']} products, "
                          f"{port['passes']:.2f} passes of all {len(jobs)}), one product per thread; C restatement "
-                         f"of src/fpntt.jl (oracle/, {ORACLE_FLAGS}), not the Julia original",
+                         f"of src/fpntt.jl (oracle/, {O.variant()}), not the Julia original",
+               "builds_2s_sample": {k: round(v, 1) for k, v in trials.items()},
                "host": hi}
         if gmp_ref.available():
             gm = cpu_throughput(ops, lambda a, b: gmp_ref.sqr(a) if b is None else gmp_ref.mul(a, b), threads, 5.0)

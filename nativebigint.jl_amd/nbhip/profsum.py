@@ -148,7 +148,7 @@ def counters_report(pass1: str, pass2: str, trace: str, commands: list[str], ker
     out += ["", "Per-dispatch counter values:", ""]
     for path in (pass1, pass2):
         txt = open(path).read().split("per dispatch (sum / dispatches):")[1]
-        listed = {k.rsplit("_w6", 1)[0] for k in kernels}     # both builds of the group kernels
+        listed = {k.rsplit("_w6", 1)[0] for k in kernels}     # both builds of the tile group kernels
         out += [x for x in txt.strip().split("\n")
                 if x.startswith("|") and ("kernel" in x or "---" in x or any(k in x for k in listed))]
         out.append("")
@@ -168,11 +168,13 @@ def main():
     ap.add_argument("--traffic-write-db", default="",
                     help="traffic mode: db is the FETCH_SIZE run, this the WRITE_SIZE run; prints JSON")
     ap.add_argument("--copy-bytes", type=int, default=256 << 20)
-    ap.add_argument("--kernels", default="k_mid,k_group_fwd,k_group_inv,k_group_fwd_w6,k_group_inv_w6,k_unpack,"
-                                          "k_carry_fix,k_combine,k_barrett_fix")
+    ap.add_argument("--kernels", default="k_mid,k_group_fwd,k_group_inv,k_group_fwd_w6,k_group_inv_w6,k_col_fwd_s,"
+                                          "k_col_fwd_b,k_col_inv_s,k_col_inv_b,k_unpack,k_carry_fix,k_combine,"
+                                          "k_barrett_fix,k_basecase")
     a = ap.parse_args()
     if a.counters:
-        ks = ["nbh::k_mid", "nbh::k_group_fwd_w6", "nbh::k_group_inv_w6", "nbh::k_unpack"]
+        ks = ["nbh::k_mid", "nbh::k_group_fwd_w6", "nbh::k_group_inv_w6", "nbh::k_col_fwd_s", "nbh::k_col_fwd_b",
+              "nbh::k_col_inv_s", "nbh::k_col_inv_b", "nbh::k_unpack"]
         print(counters_report(*a.counters, [c for c in a.counter_commands.split(";;") if c], ks))
         return
     if not a.db:

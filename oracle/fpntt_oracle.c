@@ -304,20 +304,30 @@ void or_plan_free(plan_t *P) {
         a = y0_; b = y1_; c = y2_;                   \
     } while (0)
 
+/* The stage loops below run over the butterfly index j with one pointer per
+ * leg: the legs are disjoint runs of Q points, so each loop vectorizes over j
+ * (8 doubles a vector in the AVX-512 build, as the reference's VF8 kernels do)
+ * with the same per-lane IEEE operations as the scalar loop. */
+
 /* fp_radix3! (src/fpntt.jl:315-354) */
 static void radix3(double *x, int64_t o, const stage_t *s, const ctx_t *F, int fwd) {
-    int64_t Q = s->Q;
-    double w3 = s->rot[0];
-    const double *t1 = s->tw[0], *t2 = s->tw[1];
+    const int64_t Q = s->Q;
+    const double w3 = s->rot[0];
+    const ctx_t Fl = *F;
+    const double *restrict t1 = s->tw[0], *restrict t2 = s->tw[1];
+    double *restrict x0 = x + o, *restrict x1 = x0 + Q, *restrict x2 = x1 + Q;
+#define F (&Fl)
+#pragma GCC ivdep
     for (int64_t j = 0; j < Q; j++) {
-        double y0 = x[o + j], y1 = x[o + j + Q], y2 = x[o + j + 2 * Q];
+        double y0 = x0[j], y1 = x1[j], y2 = x2[j];
         if (fwd) DFT3(y0, y1, y2, w3);
         y0 = fp_reduce(y0, F);
         y1 = fp_mulmod3(y1, t1[j], F);
         y2 = fp_mulmod3(y2, t2[j], F);
         if (!fwd) DFT3(y0, y1, y2, w3);
-        x[o + j] = y0; x[o + j + Q] = y1; x[o + j + 2 * Q] = y2;
+        x0[j] = y0; x1[j] = y1; x2[j] = y2;
     }
+#undef F
 }
 
 /* fp_dft5 (src/fpntt.jl:371-382) */
@@ -337,19 +347,30 @@ static inline void dft5(double *y, const double *k, const ctx_t *F) {
 
 /* fp_radix5! (src/fpntt.jl:384-442) */
 static void radix5(double *x, int64_t o, const stage_t *s, const ctx_t *F, int fwd) {
-    int64_t Q = s->Q;
+    const int64_t Q = s->Q;
+    const ctx_t Fl = *F;
+    double rot[8];
+    memcpy(rot, s->rot, sizeof rot);
+    const double *restrict t1 = s->tw[0], *restrict t2 = s->tw[1], *restrict t3 = s->tw[2], *restrict t4 = s->tw[3];
+    double *restrict x0 = x + o, *restrict x1 = x0 + Q, *restrict x2 = x1 + Q, *restrict x3 = x2 + Q,
+           *restrict x4 = x3 + Q;
+#define F (&Fl)
+#pragma GCC ivdep
     for (int64_t j = 0; j < Q; j++) {
-        double y[5];
-        for (int t = 0; t < 5; t++) y[t] = x[o + j + t * Q];
-        if (fwd) dft5(y, s->rot, F);
+        double y[5] = {x0[j], x1[j], x2[j], x3[j], x4[j]};
+        if (fwd) dft5(y, rot, F);
         y[0] = fp_reduce(y[0], F);
-        for (int t = 1; t < 5; t++) y[t] = fp_mulmod3(y[t], s->tw[t - 1][j], F);
+        y[1] = fp_mulmod3(y[1], t1[j], F);
+        y[2] = fp_mulmod3(y[2], t2[j], F);
+        y[3] = fp_mulmod3(y[3], t3[j], F);
+        y[4] = fp_mulmod3(y[4], t4[j], F);
         if (!fwd) {
-            dft5(y, s->rot, F);
+            dft5(y, rot, F);
             y[0] = fp_reduce(y[0], F);
         }
-        for (int t = 0; t < 5; t++) x[o + j + t * Q] = y[t];
+        x0[j] = y[0]; x1[j] = y[1]; x2[j] = y[2]; x3[j] = y[3]; x4[j] = y[4];
     }
+#undef F
 }
 
 /* fp_dft7 (src/fpntt.jl:459-492) */
@@ -388,19 +409,33 @@ static inline void dft7(double *y, const double *rot, const ctx_t *F, int red) {
 
 /* fp_radix7! (src/fpntt.jl:494-537) */
 static void radix7(double *x, int64_t o, const stage_t *s, const ctx_t *F, int fwd) {
-    int64_t Q = s->Q;
+    const int64_t Q = s->Q;
+    const ctx_t Fl = *F;
+    double rot[8];
+    memcpy(rot, s->rot, sizeof rot);
+    const double *restrict t1 = s->tw[0], *restrict t2 = s->tw[1], *restrict t3 = s->tw[2], *restrict t4 = s->tw[3],
+                 *restrict t5 = s->tw[4], *restrict t6 = s->tw[5];
+    double *restrict x0 = x + o, *restrict x1 = x0 + Q, *restrict x2 = x1 + Q, *restrict x3 = x2 + Q,
+           *restrict x4 = x3 + Q, *restrict x5 = x4 + Q, *restrict x6 = x5 + Q;
+#define F (&Fl)
+#pragma GCC ivdep
     for (int64_t j = 0; j < Q; j++) {
-        double y[7];
-        for (int t = 0; t < 7; t++) y[t] = x[o + j + t * Q];
-        if (fwd) dft7(y, s->rot, F, 0);
+        double y[7] = {x0[j], x1[j], x2[j], x3[j], x4[j], x5[j], x6[j]};
+        if (fwd) dft7(y, rot, F, 0);
         y[0] = fp_reduce(y[0], F);
-        for (int t = 1; t < 7; t++) y[t] = fp_mulmod3(y[t], s->tw[t - 1][j], F);
+        y[1] = fp_mulmod3(y[1], t1[j], F);
+        y[2] = fp_mulmod3(y[2], t2[j], F);
+        y[3] = fp_mulmod3(y[3], t3[j], F);
+        y[4] = fp_mulmod3(y[4], t4[j], F);
+        y[5] = fp_mulmod3(y[5], t5[j], F);
+        y[6] = fp_mulmod3(y[6], t6[j], F);
         if (!fwd) {
-            dft7(y, s->rot, F, 1);
+            dft7(y, rot, F, 1);
             y[0] = fp_reduce(y[0], F);
         }
-        for (int t = 0; t < 7; t++) x[o + j + t * Q] = y[t];
+        x0[j] = y[0]; x1[j] = y[1]; x2[j] = y[2]; x3[j] = y[3]; x4[j] = y[4]; x5[j] = y[5]; x6[j] = y[6];
     }
+#undef F
 }
 
 /* fp_dft4 (src/fpntt.jl:543-549) */
@@ -415,22 +450,26 @@ static void radix7(double *x, int64_t o, const stage_t *s, const ctx_t *F, int f
 
 /* fp_radix4! (src/fpntt.jl:645-701), all three Q regimes */
 static void radix4(double *x, int64_t o, const stage_t *s, const ctx_t *F, int fwd) {
-    int64_t Q = s->Q;
-    double wi = s->rot[0];
-    const double *t1 = s->tw[0], *t2 = s->tw[1], *t3 = s->tw[2];
+    const int64_t Q = s->Q;
+    const double wi = s->rot[0];
+    const ctx_t Fl = *F;
+    const double *restrict t1 = s->tw[0], *restrict t2 = s->tw[1], *restrict t3 = s->tw[2];
+#define F (&Fl)
     for (int64_t b0 = o; b0 < o + s->span; b0 += 4 * Q) {
+        double *restrict x0 = x + b0, *restrict x1 = x0 + Q, *restrict x2 = x1 + Q, *restrict x3 = x2 + Q;
+#pragma GCC ivdep
         for (int64_t j = 0; j < Q; j++) {
-            int64_t i0 = b0 + j;
-            double y0 = x[i0], y1 = x[i0 + Q], y2 = x[i0 + 2 * Q], y3 = x[i0 + 3 * Q];
+            double y0 = x0[j], y1 = x1[j], y2 = x2[j], y3 = x3[j];
             if (fwd) DFT4(y0, y1, y2, y3, wi);
             y0 = fp_reduce(y0, F);
             y1 = fp_mulmod3(y1, t1[j], F);
             y2 = fp_mulmod3(y2, t2[j], F);
             y3 = fp_mulmod3(y3, t3[j], F);
             if (!fwd) DFT4(y0, y1, y2, y3, wi);
-            x[i0] = y0; x[i0 + Q] = y1; x[i0 + 2 * Q] = y2; x[i0 + 3 * Q] = y3;
+            x0[j] = y0; x1[j] = y1; x2[j] = y2; x3[j] = y3;
         }
     }
+#undef F
 }
 
 /* fp_radix2! (src/fpntt.jl:630-638) */

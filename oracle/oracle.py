@@ -17,16 +17,60 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "libfpntt_oracle.so")
+_LIB_V3 = os.path.join(_HERE, "libfpntt_oracle.so")       # AVX2 + FMA (x86-64-v3)
+_LIB_V4 = os.path.join(_HERE, "libfpntt_oracle_v4.so")    # AVX-512 (x86-64-v4), 8 doubles a vector
 
 _lib = None
+_variant = None
+
+
+def _has_avx512() -> bool:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    fl = set(line.split(":", 1)[1].split())
+                    return {"avx512f", "avx512dq", "avx512bw", "avx512vl"} <= fl
+    except OSError:
+        pass
+    return False
+
+
+_forced = None
+
+
+def _lib_path() -> str:
+    """The AVX-512 build where the host has AVX-512 (NBH_ORACLE_ISA=v3, or
+    use("v3"), selects the AVX2 build); both give the same results."""
+    isa = _forced or os.environ.get("NBH_ORACLE_ISA", "")
+    if isa != "v3" and _has_avx512() and os.path.exists(_LIB_V4):
+        return _LIB_V4
+    return _LIB_V3
+
+
+def isas() -> list:
+    """The builds this host can run: "v3", and "v4" where it has AVX-512."""
+    return ["v3"] + (["v4"] if _has_avx512() and os.path.exists(_LIB_V4) else [])
+
+
+def use(isa: str) -> None:
+    """Load the "v3" (AVX2) or "v4" (AVX-512) build for later calls."""
+    global _lib, _forced
+    _forced = isa
+    _lib = None
+
+
+def variant() -> str:
+    """Which build lib() loaded, with its compiler flags."""
+    lib()
+    return _variant
 
 
 def build(force: bool = False) -> str:
-    """Compile the oracle library in place (make -C oracle)."""
-    if force or not os.path.exists(_LIB_PATH):
+    """Compile the oracle libraries in place (make -C oracle)."""
+    if force or not (os.path.exists(_LIB_V3) and os.path.exists(_LIB_V4)):
         subprocess.run(["make", "-s", "-B" if force else "all", "-C", _HERE], check=True)
-    return _LIB_PATH
+    return _LIB_V3
 
 
 _d = ctypes.c_double
@@ -37,11 +81,15 @@ _vp = ctypes.c_void_p
 
 
 def lib():
-    global _lib
+    global _lib, _variant
     if _lib is None:
-        if not os.path.exists(_LIB_PATH):
+        if not os.path.exists(_LIB_V3):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
+        path = _lib_path()
+        L = ctypes.CDLL(path)
+        _variant = ("gcc -O3 -march=x86-64-v4 -mprefer-vector-width=512 -ffp-contract=off (AVX-512: the stage "
+                    "loops 8 doubles a vector)" if path == _LIB_V4 else
+                    "gcc -O3 -march=x86-64-v3 -ffp-contract=off (AVX2: the stage loops 4 doubles a vector)")
         sig = {
             "or_prime": (_u64, [_int]),
             "or_gen": (_u64, [_int]),

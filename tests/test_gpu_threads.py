@@ -89,3 +89,67 @@ def test_single_products_across_streams():
     torch.cuda.synchronize()
     for dr, a, b in cases:
         assert np.array_equal(D.to_host(dr), gmp_ref.mul(a, b))
+
+
+def test_barrett_batches_created_and_destroyed_beside_products():
+    """Barrett contexts and batches are created, run and destroyed on one
+    thread while four others run host-path products: the destroy calls share
+    the library's event pool and plan cache with every other call, so they
+    must take the same lock (each result is still exact)."""
+    if not gmp_ref.available():
+        pytest.skip("libgmp not found")
+    from nbhip import barrett as BR
+    from nbhip import device as D
+    from tests.helpers import ints, wl
+    engine.init()
+    errors = []
+    stop = threading.Event()
+
+    def products(seed):
+        try:
+            rng = np.random.default_rng(seed)
+            while not stop.is_set():
+                m = int(rng.integers(160, 12000))
+                n = int(rng.integers(152, m + 1))
+                a, b = rand_big_style(rng, m, 1), rand_big_style(rng, n, 1)
+                r = np.zeros(m + n, dtype=np.uint64)
+                engine.mul_fpntt2(r, 0, a, 0, m, b, 0, n)
+                if not np.array_equal(r, gmp_ref.mul(a, b)):
+                    errors.append(("product", seed, m, n))
+        except Exception as e:  # surfaced below
+            errors.append(("product", seed, repr(e)))
+
+    def barrett():
+        try:
+            rng = np.random.default_rng(0xBA)
+            for it in range(10):
+                k = int(rng.integers(160, 2000))
+                m = rng.integers(0, 2**64, k, dtype=np.uint64)
+                m[-1] |= np.uint64(1)
+                M = ints(m)
+                ctx = BR.Barrett(D.to_dev(m))
+                vals = [int.from_bytes(rng.bytes(16 * k), "little") for _ in range(4)]
+                ts = [D.to_dev(wl.int_to_limbs(v, 2 * k)) for v in vals]
+                rs = [D.empty_limbs(k) for _ in vals]
+                RB = BR.BarrettBatch(ctx, list(zip(ts, rs)))
+                RB.run()
+                torch.cuda.synchronize()
+                for r, v in zip(rs, vals):
+                    if ints(D.to_host(r)) != v % M:
+                        errors.append(("barrett", it, k))
+                RB.close()
+                ctx.close()
+        except Exception as e:  # surfaced below
+            errors.append(("barrett", repr(e)))
+
+    workers = [threading.Thread(target=products, args=(0x51 + s,)) for s in range(4)]
+    for t in workers:
+        t.start()
+    b = threading.Thread(target=barrett)
+    b.start()
+    b.join(timeout=100)
+    stop.set()
+    for t in workers:
+        t.join(timeout=100)
+    assert not b.is_alive() and not any(t.is_alive() for t in workers)
+    assert not errors, errors
