@@ -182,17 +182,20 @@ def kernel_bytes(plan_groups, N, m, n, sqr, nconv):
     return out
 
 
+# FP64 VALU instructions (each add/mul/fma/rndne counts once; a reduce is 3,
+# a 3- or 4-argument mulmod 6, the round being one rndne on the device) per
+# point per stage: radix-4 35/4, radix-2 8/2, radix-3 28/3, radix-5 82/5,
+# radix-7 154/7; pointwise 6 per point per prime; unpack ~31 per coefficient.
+FP64_STAGE_COST = {4: 8.75, 2: 4.0, 3: 28.0 / 3, 5: 16.4, 7: 22.0}
+FP64_POINTWISE = 6.0
+FP64_UNPACK = 31.0
+
+
 def fp64_flops(N, plan_stage_radices, sqr, nconv):
-    """FP64 VALU instructions per product (each add/mul/fma counts once):
-    per point per stage radix-4 10, radix-2 5, radix-3 10.67, radix-5 18.8,
-    radix-7 25; pointwise 7 per point per prime; unpack ~35 per coefficient."""
-    cost = {4: 10.0, 2: 5.0, 3: 32.0 / 3, 5: 18.8, 7: 25.0}
-    c = sum(cost[r] for r in plan_stage_radices)
+    """FP64 VALU instructions per product, by the costs above."""
+    c = sum(FP64_STAGE_COST[r] for r in plan_stage_radices)
     T = 2 if sqr else 3
-    return 2 * (T * c * N + 7 * N) + 35 * nconv
-
-
-FP64_STAGE_COST = {4: 10.0, 2: 5.0, 3: 32.0 / 3, 5: 18.8, 7: 25.0}
+    return 2 * (T * c * N + FP64_POINTWISE * N) + FP64_UNPACK * nconv
 
 
 def fp64_by_kind(plan_groups, plan_stage_radices, N, sqr, nconv):
@@ -211,11 +214,11 @@ def fp64_by_kind(plan_groups, plan_stage_radices, N, sqr, nconv):
             prod *= r
             i += 1
         costs.append(c)
-    out = {"group_fwd": 0.0, "mid": 0.0, "group_inv": 0.0, "unpack": 35.0 * nconv}
+    out = {"group_fwd": 0.0, "mid": 0.0, "group_inv": 0.0, "unpack": FP64_UNPACK * nconv}
     for c in costs[:-1]:
         out["group_fwd"] += 2 * nops * c * N
         out["group_inv"] += 2 * c * N
-    out["mid"] += 2 * ((nops + 1) * costs[-1] * N + 7 * N)
+    out["mid"] += 2 * ((nops + 1) * costs[-1] * N + FP64_POINTWISE * N)
     return out
 
 
