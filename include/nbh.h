@@ -357,9 +357,9 @@ int32_t nbh_get_tuning_for_single(int64_t N, int32_t *knobs4);
  * synchronises, returns the summed milliseconds and launch counts per kernel
  * kind (0 forward strided group + pack, 1 mid, 2 inverse strided group,
  * 3 unpack, 4 carry fix, 5 cofactor-step combine, 6 schoolbook products below
- * the NTT thresholds) and clears them.  Instrumentation only: no reference
- * counterpart. */
-#define NBH_PROF_KINDS 7
+ * the NTT thresholds, 7 single products run whole in one workgroup) and
+ * clears them.  Instrumentation only: no reference counterpart. */
+#define NBH_PROF_KINDS 8
 int32_t nbh_profile_enable(int32_t on);
 /* A batch step's launches (one per tile shape) are independent; by default
  * they fork onto 4 library streams so small shapes run beside big ones.

@@ -735,6 +735,55 @@ void keep_limbs(DevJob &J, int64_t keep) {
     J.nchunks = (keep + UNPACK_CHUNK - 1) / UNPACK_CHUNK;
 }
 
+// Twiddle doubles of a length-N plan (build_twiddles' table): (m-1) Q per
+// stage of radix 3, 5, 7 or 4, none for radix 2.  Depends on N only.
+int64_t twiddle_count(int64_t N) {
+    int64_t span = N, m = N >> __builtin_ctzll((uint64_t)N), total = 0;
+    for (int mf : {7, 5, 3, 3}) {
+        if (m % mf) continue;
+        total += (int64_t)(mf - 1) * (span / mf);
+        span /= mf;
+        m /= mf;
+    }
+    for (int64_t L = span; L >= 4; L >>= 2) total += 3 * (L / 4);
+    return total;
+}
+
+// Geometry of a sized product in the one-workgroup kernel (k_small): half the
+// block per prime, each half one thread per 4 points of a prime's two arrays
+// (at least a wave, at most 512 threads); every stage its own pass when the
+// half has a thread per 4 points of one array, else radix-4 pairs (then one
+// thread per 16 points of both arrays suffices).  LDS for a's and b's arrays
+// of both primes and both twiddle tables.  False when it does not fit one
+// block: more threads than the kernel's bound, result segments than threads
+// (the limb sums run in one round), staged Garner words than an array holds,
+// loads than one round of four per thread, or LDS than a block has.
+struct SmallGeo {
+    int threads, ld, twl, nseg, cnt, single;
+    size_t lds;
+};
+bool small_geo(const DevJob &J, int64_t N, SmallGeo *out) {
+    if (J.zero || J.bspec || N < 16 || N > 16384) return false;
+    const int64_t half = std::min<int64_t>(SMALL_MAX_THREADS / 2, std::max<int64_t>(64, (2 * (N / 4) + 63) / 64 * 64));
+    const int single = 4 * half >= N ? 1 : 0;
+    const int64_t nt = 2 * half;
+    const int64_t nseg = (J.rn + UNPACK_LPT - 1) / UNPACK_LPT;
+    // the segment sums read the coefficients that start below limb 4 nseg
+    const int64_t cnt = (64 * UNPACK_LPT * nseg + J.bch - 1) / J.bch + 1;
+    const int64_t ld = lds_doubles((int)N);
+    const int64_t twl = twiddle_count(N);
+    const size_t lds = sizeof(double) * (size_t)(4 * ld + 2 * twl);
+    const size_t lds_static = 2 * SMALL_MAX_STAGES * sizeof(StageDesc);
+    int nst = 0;
+    for (int64_t L = N; L > 1; nst++) L /= (L % 7 == 0 ? 7 : L % 5 == 0 ? 5 : L % 3 == 0 ? 3 : L % 4 == 0 ? 4 : 2);
+    const int64_t sw = 2 * (int64_t)(sizeof(StageDesc) / sizeof(double)) * nst;
+    if (nt > SMALL_MAX_THREADS || nseg > nt || cnt > ld || 2 * nseg > ld || lds + lds_static > g.lds_block ||
+        nst > SMALL_MAX_STAGES || std::max(N, twl) > 4 * nt || sw > nt)
+        return false;
+    *out = SmallGeo{(int)nt, (int)ld, (int)twl, (int)nseg, (int)cnt, single, lds};
+    return true;
+}
+
 // The products mul!/sqr! keep out of the NTT engine (Karatsuba and
 // mul_basecase!, src/mul.jl:24-26,131,145,388): smaller operand below 64
 // limbs or m + n below 304; squares below 160 limbs.
@@ -788,6 +837,26 @@ int run_single(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int
     const int nops = sqr || spec ? 1 : 2;
     const size_t xbytes = align256(sizeof(double) * (size_t)N);
     const size_t need = align256(sizeof(uint32_t) * (size_t)J.nchunks) + xbytes * 2 * nops;
+    const bool prof = sc == nullptr;
+    SmallGeo sg;
+    if (small_geo(J, N, &sg)) {
+        // the whole product in one workgroup: no workspace, one launch
+        if (sc) {
+            P[0] = find_plan(N, 0, true);
+            P[1] = find_plan(N, 1, true);
+            if (!P[0] || !P[1])
+                return fail(NBH_ERR_ARG, "no cached plan for transform length " + std::to_string(N) +
+                                             ": run one allocating product of this length (nbh_mul_dev) first");
+        } else {
+            RCCHK(get_plan(N, 0, &P[0], true));
+            RCCHK(get_plan(N, 1, &P[1], true));
+        }
+        J.plan[0] = P[0]->dev;
+        J.plan[1] = P[1]->dev;
+        ProfScope ps(7, s, prof);
+        HIPCHK(launch_small(J, sg.threads, sg.ld, sg.twl, sg.nseg, sg.cnt, sg.single, sg.lds, s));
+        return NBH_OK;
+    }
     char *w = nullptr;
     if (sc) {
         if (need > sc->bytes)
@@ -806,7 +875,6 @@ int run_single(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int
         RCCHK(ensure_buf(&g.ws, &g.ws_size, need));
         w = (char *)g.ws;
     }
-    const bool prof = sc == nullptr;
     J.chunk_flags = (uint32_t *)w;
     w += align256(sizeof(uint32_t) * (size_t)J.nchunks);
     for (int k = 0; k < 2; k++) {
@@ -926,6 +994,22 @@ int run_on(void *stream, const std::function<int(hipStream_t)> &f) {
     return rc != NBH_OK ? rc : rc2;
 }
 
+// Whether a host-pointer product (m, n already clamped to keep) runs whole in
+// the one-workgroup kernel.
+bool host_small(int64_t m, int64_t n, int64_t bits_a, int64_t bits_b, bool sqr, int64_t keep) {
+    if (keep && schoolbook_route(m, n, sqr)) return false;
+    DevJob J;
+    std::memset(&J, 0, sizeof J);
+    J.m = m;
+    J.n = n;
+    J.sqr = sqr ? 1 : 0;
+    int64_t N = 0;
+    if (size_job(J, bits_a, bits_b, &N) != NBH_OK) return false;
+    keep_limbs(J, keep);
+    SmallGeo sg;
+    return small_geo(J, N, &sg);
+}
+
 // Host buffers in and out; keep > 0 is the low short product r[0..keep).
 int host_product(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int64_t n, bool sqr,
                  int64_t keep = 0) {
@@ -947,7 +1031,9 @@ int host_product(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, i
     uint64_t *db = (uint64_t *)((char *)g.io + sa);
     uint64_t *dr = (uint64_t *)((char *)g.io + sa + sb);
     const bool pinned = sa + sb + sr <= PIN_MAX_BYTES;
-    if (pinned && sa + sb + sr >= ZERO_COPY_MIN_BYTES) {
+    // the one-workgroup kernel reads each limb once, in one round of loads:
+    // over the bus that costs less than two copy commands at any size it takes
+    if (pinned && (sa + sb + sr >= ZERO_COPY_MIN_BYTES || host_small(m, n, bits_a, bits_b, sqr, keep))) {
         RCCHK(ensure_pinned(sa + sb + sr));
         if (g.pin_dev) {
             // the kernels read the operands from and write r to the pinned

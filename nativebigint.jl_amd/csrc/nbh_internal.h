@@ -98,6 +98,11 @@ constexpr size_t LDS_PER_CU = 160 * 1024;
 // bytes per point: 139 KB).
 constexpr int MID_MAX_THREADS = 1024;
 
+// Block-size bound of the one-workgroup product kernel (k_small), and the
+// most stages of a plan it takes (it copies both plans' descriptors to LDS).
+constexpr int SMALL_MAX_THREADS = 1024;
+constexpr int SMALL_MAX_STAGES = 10;
+
 // Doubles of LDS that an n-point tile occupies with one pad double per 16
 // (the kernels index LDS as i + i/16).
 NBH_HDI constexpr int lds_doubles(int n) { return n + (n >> 4) + 1; }

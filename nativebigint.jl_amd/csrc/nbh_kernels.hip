@@ -1653,6 +1653,296 @@ __device__ __forceinline__ uint32_t block_carry(bool gen, bool prop, uint32_t ci
     return (uint32_t)((cv >> lane) & 1);
 }
 
+// ---------------------------------------------------------------------------
+// A whole product in one workgroup, for single products of short transforms
+// (the launcher admits N when the arrays below fit one block's LDS): the chunk
+// pack (src/fpntt.jl:744-768), both primes' forward transforms of both
+// operands (:703-733), the pointwise products (:770-785), the inverse
+// transforms, the Garner CRT and the limb sums with their carry chain
+// (:806-925) all run on LDS, so only operand limbs are read and result limbs
+// written, by one launch.  Both primes' twiddle tables are copied to LDS in
+// the same round of loads as the limbs, so no pass waits on a twiddle load.
+// The lower half of the block works on p1, the upper half on p2 (each half a
+// whole number of waves, so the split is wave-uniform); the radices of a
+// length depend on N only, so both halves run the same pass sequence and
+// every barrier is reached by every wave.  The butterflies are those of pass44
+// and pass1 (the same stages, the same reduce/mulmod order).
+//
+// LDS (doubles, each array padded as pidx): a's points for p1 and p2, b's for
+// p1 and p2 (ld each), then the two twiddle tables (twl each).  After the
+// inverse transforms, b's arrays hold the staged Garner words and a's the
+// spills between threads.
+
+// One stage's constants for a whole-transform pass (rows = N, one column):
+// the row stride q is the stage's Q, and tw points at its LDS twiddle rows.
+// sd is the kernel's LDS copy of the stage descriptor: a scalar load from the
+// plan would be an L2 round trip at every pass of so short a kernel.
+__device__ __forceinline__ StageU small_stage(const StageDesc *sd, const double *twl) {
+    StageU u;
+    u.m = uni(sd->m);
+    u.Q = uni64(sd->Q);
+    u.q = (int)u.Q;
+    u.qmagic = (uint64_t)(0xFFFFFFFFu / (uint32_t)u.q) + 1;  // ceil(2^32 / q)
+    u.tw = twl + uni64(sd->tw_off);
+#pragma unroll
+    for (int i = 0; i < 8; i++) u.rot[i] = unid(sd->rot[i]);
+    return u;
+}
+
+// Radix-4 pair (stages s0, s1) over narr arrays, by the lt-th of H threads.
+template <int K, bool FWD, bool LIN>
+__device__ void small_pass44(double *__restrict__ x0, double *__restrict__ x1, int narr, const StageU &s0,
+                             const StageU &s1, int N, int lt, int H) {
+    TileGeo g;
+    g.S = N;
+    g.logC = 0;
+    g.Lb = 1;
+    g.jcol0 = 0;
+    const int nsub = N >> 4;
+    for (int sg = lt; sg < nsub * narr; sg += H) {
+        double *__restrict__ x = sg < nsub ? x0 : x1;
+        const Sub44 u = sub44(sg < nsub ? sg : sg - nsub, s0, s1, g);
+        double w0[4][3], w1[3];
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int t = 0; t < 3; t++) w0[a][t] = s0.tw[t * s0.Q + u.jt + a * s1.Q];
+#pragma unroll
+        for (int t = 0; t < 3; t++) w1[t] = s1.tw[t * s1.Q + u.jt];
+        double y[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) y[a][b] = x[ix44<LIN>(u, a, b)];
+        regs44<K, FWD>(y, w0, w1, s0, s1);
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) x[ix44<LIN>(u, a, b)] = y[a][b];
+    }
+}
+
+// One radix-M stage over narr arrays, by the lt-th of H threads.
+template <int K, bool FWD, int M>
+__device__ void small_pass1(double *__restrict__ x0, double *__restrict__ x1, int narr, const StageU &s, int N,
+                            int lt, int H) {
+    TileGeo g;
+    g.S = N;
+    g.logC = 0;
+    g.Lb = 1;
+    g.jcol0 = 0;
+    const int nb = N / M;
+    for (int bg = lt; bg < nb * narr; bg += H) {
+        double *__restrict__ x = bg < nb ? x0 : x1;
+        const Sub1 u = sub1<M>(bg < nb ? bg : bg - nb, s, g);
+        double w[M > 1 ? M - 1 : 1] = {};
+        if constexpr (M > 2) {
+#pragma unroll
+            for (int t = 1; t < M; t++) w[t - 1] = s.tw[(t - 1) * s.Q + u.j];
+        }
+        double y[M];
+#pragma unroll
+        for (int t = 0; t < M; t++) y[t] = x[ix1<false>(u, t)];
+        stage_regs<K, FWD, M>(y, w, s);
+#pragma unroll
+        for (int t = 0; t < M; t++) x[ix1<false>(u, t)] = y[t];
+    }
+}
+
+// Stages [s, s + (pair ? 2 : 1)) of prime K's plan over its narr arrays.
+template <int K, bool FWD>
+__device__ __forceinline__ void small_pass(double *x0, double *x1, int narr, const StageDesc *st, int s, bool pair,
+                                           const double *twl, int N, int lt, int H) {
+    const StageU a = small_stage(&st[s], twl);
+    if (pair) {
+        const StageU b = small_stage(&st[s + 1], twl);
+        if (((a.q | b.q) & 15) == 0) small_pass44<K, FWD, true>(x0, x1, narr, a, b, N, lt, H);
+        else small_pass44<K, FWD, false>(x0, x1, narr, a, b, N, lt, H);
+        return;
+    }
+    switch (a.m) {
+    case 4: small_pass1<K, FWD, 4>(x0, x1, narr, a, N, lt, H); break;
+    case 2: small_pass1<K, FWD, 2>(x0, x1, narr, a, N, lt, H); break;
+    case 3: small_pass1<K, FWD, 3>(x0, x1, narr, a, N, lt, H); break;
+    case 5: small_pass1<K, FWD, 5>(x0, x1, narr, a, N, lt, H); break;
+    default: small_pass1<K, FWD, 7>(x0, x1, narr, a, N, lt, H); break;
+    }
+}
+
+// Loads per thread per round of the small kernel's first copies: chunks of
+// a and b, both twiddle tables (the launcher gives the block at least a
+// quarter as many threads as points, and the tables hold at most about
+// 1.1 N doubles, so one round covers them).
+constexpr int SMALL_LU = 4;
+// Doubles of one stage descriptor (the kernel copies both plans' to LDS).
+constexpr int STAGE_WORDS = (int)(sizeof(StageDesc) / sizeof(double));
+static_assert(sizeof(StageDesc) % sizeof(double) == 0, "stage descriptors copy as doubles");
+
+// single: run every stage as its own pass (one radix-4 butterfly per thread
+// and array when the block has a thread per 4 points of a prime's arrays:
+// four times the waves of the 16-point register pairs, which a block this
+// small needs to hide its FP64 and LDS latencies); otherwise radix-4 pairs.
+__global__ void __launch_bounds__(SMALL_MAX_THREADS) k_small(const DevJob J, int ld, int twl, int nseg, int cnt,
+                                                             int single) {
+    extern __shared__ double lds[];
+    __shared__ StageDesc sst[2][SMALL_MAX_STAGES];
+    const int tid = threadIdx.x, NT = blockDim.x, H = NT >> 1;
+    const int k = uni(tid >= H ? 1 : 0);  // prime of this thread's half (H is a multiple of 64)
+    const int lt = tid - k * H;
+    const DevPlan *__restrict__ P0 = unip(J.plan[0]);
+    const DevPlan *__restrict__ P1 = unip(J.plan[1]);
+    const int N = (int)uni64(cst(P0)->N);
+    const int nst = uni(cst(P0)->nst);
+    const bool sq = J.sqr != 0;
+    const int narr = sq ? 1 : 2;
+    double *const xa0 = lds, *const xa1 = lds + ld, *const xb0 = lds + 2 * ld, *const xb1 = lds + 3 * ld;
+    double *const tw0 = lds + 4 * ld, *const tw1 = tw0 + twl;
+    double *const xa = k ? xa1 : xa0, *const xb = k ? xb1 : xb0;
+    const double *const twk = k ? tw1 : tw0;
+    const StageDesc *const stk = sst[k];
+
+    // chunks of a (and b) for both primes, both twiddle tables and both
+    // plans' stage descriptors: every load of a round is issued before any
+    // result is used, so the copies cost one memory latency
+    {
+        const int b = J.bch;
+        const uint32_t lmask = b >= 32 ? ~0u : (1u << b) - 1u;
+        const uint32_t hmask = b >= 32 ? (1u << (b - 32)) - 1u : 0u;
+        const __amdgpu_buffer_rsrc_t ra = limb_rsrc(J.a, J.m), rb = limb_rsrc(J.b, J.n);
+        const double *__restrict__ g0 = unip(cst(P0)->tw);
+        const double *__restrict__ g1 = unip(cst(P1)->tw);
+        const int sw = STAGE_WORDS * nst;
+        const int top = max(max(N, twl), 2 * sw);
+        for (int r0 = 0; r0 < top; r0 += SMALL_LU * NT) {
+            double va[SMALL_LU], vb[SMALL_LU], v0[SMALL_LU], v1[SMALL_LU];
+            const int i0 = r0 + tid;
+            const int pk = i0 >= sw ? 1 : 0, w = i0 - pk * sw;
+            const double *sd = (const double *)(pk ? &P1->st[0] : &P0->st[0]);
+            const double vs = gld(sd, i0 < 2 * sw ? w : 0);
+#pragma unroll
+            for (int u = 0; u < SMALL_LU; u++) {
+                const int i = r0 + u * NT + tid;
+                va[u] = pack_buf(ra, lmask, hmask, i < J.nca, (int64_t)i * b);
+                vb[u] = sq ? 0.0 : pack_buf(rb, lmask, hmask, i < J.ncb, (int64_t)i * b);
+                const int it = min(i, twl - 1);
+                v0[u] = gld(g0, it);
+                v1[u] = gld(g1, it);
+            }
+            if (i0 < 2 * sw) ((double *)&sst[pk][0])[w] = vs;
+#pragma unroll
+            for (int u = 0; u < SMALL_LU; u++) {
+                const int i = r0 + u * NT + tid;
+                if (i < N) {
+                    const int p = pidx(i);
+                    xa0[p] = va[u];
+                    xa1[p] = va[u];
+                    if (!sq) {
+                        xb0[p] = vb[u];
+                        xb1[p] = vb[u];
+                    }
+                }
+                if (i < twl) {
+                    tw0[i] = v0[u];
+                    tw1[i] = v1[u];
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    // forward transforms of a and b (radix-4 pairs from the front)
+    for (int s = 0; s < nst;) {
+        const bool pair = !single && uni(sst[0][s].m) == 4 && s + 1 < nst && uni(sst[0][s + 1].m) == 4;
+        if (k) small_pass<1, true>(xa, xb, narr, stk, s, pair, twk, N, lt, H);
+        else small_pass<0, true>(xa, xb, narr, stk, s, pair, twk, N, lt, H);
+        __syncthreads();
+        s += pair ? 2 : 1;
+    }
+    // pointwise product (a square multiplies a by itself)
+    {
+        const double *__restrict__ yb = sq ? xa : xb;
+        for (int i = lt; i < N; i += H) {
+            const int p = pidx(i);
+            xa[p] = k ? fp_mulmod3<1>(xa[p], yb[p]) : fp_mulmod3<0>(xa[p], yb[p]);
+        }
+    }
+    __syncthreads();
+    // inverse transforms of the products (radix-4 pairs from the back)
+    for (int s = nst - 1; s >= 0;) {
+        const bool pair = !single && uni(sst[0][s].m) == 4 && s - 1 >= 0 && uni(sst[0][s - 1].m) == 4;
+        const int s0 = pair ? s - 1 : s;
+        if (k) small_pass<1, false>(xa, nullptr, 1, stk, s0, pair, twk, N, lt, H);
+        else small_pass<0, false>(xa, nullptr, 1, stk, s0, pair, twk, N, lt, H);
+        __syncthreads();
+        s -= pair ? 2 : 1;
+    }
+
+    // Garner words of coefficients [0, cnt) (zero past nconv) into b's arrays
+    UnpackU U;
+    U.x1 = U.x2 = nullptr;
+    U.N = N;
+    U.nconv = J.nconv;
+    U.rn = J.rn;
+    U.n1 = unid(cst(P0)->ninv);
+    U.n1p = unid(cst(P0)->ninvp);
+    U.n2 = unid(cst(P1)->ninv);
+    U.n2p = unid(cst(P1)->ninvp);
+    U.bmagic = J.bmagic;
+    U.b = J.bch;
+    uint64_t *const clo = (uint64_t *)xb0, *const chi = (uint64_t *)xb1;
+    for (int i = tid; i < cnt; i += NT) {
+        uint64_t lo = 0, hi = 0;
+        if (i < U.nconv) {
+            const int p = pidx(i == 0 ? 0 : N - i);
+            garner(U, xa0[p], xa1[p], lo, hi);
+        }
+        clo[i] = lo;
+        chi[i] = hi;
+    }
+    __syncthreads();
+    // thread t sums the coefficients starting in limbs [4t, 4t + 4) (one round:
+    // the launcher makes the block at least nseg threads)
+    uint64_t *const spill0 = (uint64_t *)xa0, *const spill1 = spill0 + nseg;
+    uint32_t *const wsum = (uint32_t *)xa1;
+    const int64_t k0 = (int64_t)UNPACK_LPT * tid;
+    const bool mine = tid < nseg;
+    const Seg sg = mine ? segment_sum(U, k0, 0, clo, chi) : Seg{0, 0, 0, 0, 0, 0};
+    if (mine) {
+        spill0[tid] = sg.s0;
+        spill1[tid] = sg.s1;
+    }
+    __syncthreads();
+    const bool below = tid > 0 && tid <= nseg;  // threads with a segment under them
+    const uint64_t p0 = below ? spill0[tid - 1] : 0, p1 = below ? spill1[tid - 1] : 0;
+    uint64_t q0 = sg.o0 + p0;
+    uint64_t c = q0 < p0;
+    uint64_t q1 = sg.o1 + p1;
+    uint64_t c1 = q1 < p1;
+    q1 += c;
+    c1 += (q1 < c);
+    uint64_t q2 = sg.o2 + c1;
+    c = q2 < c1;
+    uint64_t q3 = sg.o3 + c;
+    c = q3 < c;
+    uint32_t cout = 0;
+    const uint32_t my = block_carry(c != 0, (q0 & q1 & q2 & q3) == ~0ull, 0, wsum, cout);
+    q0 += my;
+    c = q0 < my;
+    q1 += c;
+    c = q1 < c;
+    q2 += c;
+    c = q2 < c;
+    q3 += c;
+    if (mine) {
+        uint64_t *__restrict__ r = J.r;
+        const int64_t rn = J.rn;
+        if (k0 + 0 < rn) r[k0 + 0] = q0;
+        if (k0 + 1 < rn) r[k0 + 1] = q1;
+        if (k0 + 2 < rn) r[k0 + 2] = q2;
+        if (k0 + 3 < rn) r[k0 + 3] = q3;
+    }
+}
+
 // r[0..lr) = X + Y*beta^ysh (add) or X - Y*beta^ysh (sub, X >= Y*beta^ysh):
 // one block walks the limbs in chunks of UNPACK_LPC, the carry (borrow)
 // between threads from the ballot prefix and between chunks in a register.
@@ -2092,6 +2382,12 @@ hipError_t launch_mid(const DevJob *jobs, const Tile *tiles, const TileGen &gen,
     return hipGetLastError();
 }
 
+hipError_t launch_small(const DevJob &J, int threads, int ld, int twl, int nseg, int cnt, int single,
+                        size_t lds_bytes, hipStream_t s) {
+    hipLaunchKernelGGL(k_small, dim3(1), dim3(threads), lds_bytes, s, J, ld, twl, nseg, cnt, single);
+    return hipGetLastError();
+}
+
 hipError_t launch_unpack(const DevJob *jobs, const Tile *tiles, const TileGen &gen, int64_t ntiles, int bmin,
                          hipStream_t s) {
     if (ntiles <= 0) return hipSuccess;
@@ -2195,6 +2491,10 @@ hipError_t set_lds_limit(size_t bytes) {
     e = hipFuncSetAttribute((const void *)k_group_fwd_w6, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (e != hipSuccess) return e;
     e = hipFuncSetAttribute((const void *)k_group_inv_w6, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
+    // k_small also holds both plans' stage descriptors in static LDS
+    e = hipFuncSetAttribute((const void *)k_small, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(bytes - 2 * SMALL_MAX_STAGES * sizeof(StageDesc)));
     if (e != hipSuccess) return e;
     return hipFuncSetAttribute((const void *)k_mid, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }

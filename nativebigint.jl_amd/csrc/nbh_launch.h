@@ -28,6 +28,12 @@ hipError_t launch_group(const DevJob *jobs, const Tile *tiles, const TileGen &ge
                         size_t lds_bytes, int threads, hipStream_t s);
 hipError_t launch_mid(const DevJob *jobs, const Tile *tiles, const TileGen &gen, int64_t ntiles, size_t lds_bytes,
                       int threads, hipStream_t s);
+// One single product in one workgroup (k_small): `threads` threads, a's and
+// b's arrays of ld doubles per prime, twiddle tables of twl doubles per prime,
+// nseg 4-limb result segments and cnt staged Garner words; single: every
+// stage its own pass (else radix-4 pairs).
+hipError_t launch_small(const DevJob &J, int threads, int ld, int twl, int nseg, int cnt, int single,
+                        size_t lds_bytes, hipStream_t s);
 // bmin: the smallest chunk width among the launch's jobs (sizes the staging LDS)
 hipError_t launch_unpack(const DevJob *jobs, const Tile *tiles, const TileGen &gen, int64_t ntiles, int bmin,
                          hipStream_t s);

@@ -447,7 +447,7 @@ def device_info():
     return {"cus": cus.value, "lds_per_block": lds.value, "hbm_bytes": hbm.value, "arch": name.value.decode()}
 
 
-PROF_KINDS = ("group_fwd", "mid", "group_inv", "unpack", "carry_fix", "combine", "basecase")
+PROF_KINDS = ("group_fwd", "mid", "group_inv", "unpack", "carry_fix", "combine", "basecase", "small")
 
 
 def set_streams(nside: int) -> None:
