@@ -601,9 +601,12 @@ __device__ int fwd_until_last(double *x, const DevPlan *__restrict__ P, int s0, 
     int s = s0;
     for (;;) {
         const StageU a = stage_u(&P->st[s], tw);
-        const bool pair = a.m == 4 && s + 1 < s1 && uni(cst(P)->st[s + 1].m) == 4;
+        const int m1 = s + 1 < s1 ? uni(cst(P)->st[s + 1].m) : 0;
+        const bool pair = a.m == 4 && m1 == 4;
         const int e = s + (pair ? 2 : 1);
         if (e >= s1) return s;
+        // a radix-4 stage followed by the final radix 2 is the last pass too
+        if (a.m == 4 && m1 == 2 && s + 2 == s1) return s;
         if (pair) {
             const StageU b = stage_u(&P->st[s + 1], tw);
             if ((((a.q | b.q) << g.logC) & 15) == 0) pass44f<K, true, 1>(x, nullptr, a, b, g);
@@ -693,6 +696,83 @@ __device__ void last1(double *__restrict__ x, double (&hb)[HOLD], const double *
     }
 }
 
+// Last pass = a radix-4 stage s0 then the transform's final radix-2 stage
+// s1 (lengths with an odd power of two): 8-point sub-problems, rows
+// row0 + a q1 + b q0 (a in 0..1, b in 0..3), two per thread.  s0's
+// butterflies run over b (twiddle index jt + a Q1), s1's over a (no
+// twiddles); the inverse runs them in the opposite order.  Exactly the
+// butterflies of pass1<4> and pass1<2>, with one LDS round trip instead of
+// two.
+template <int K, bool LIN, bool HOLDB, int SRC>
+__device__ void last42(double *__restrict__ x, double (&hb)[HOLD], const double *__restrict__ gb, int64_t gbase,
+                       const StageU &s0, const StageU &s1, const TileGeo &g) {
+    const int nsub = (g.S >> 3) << g.logC;
+    const int q1 = s1.q, q0 = s0.q;
+    const int logq1 = __builtin_ctz((unsigned)q1);
+    const int cmask = (1 << g.logC) - 1;
+    const int sa = q1 << g.logC, sb = q0 << g.logC;
+    const int pa = sa + (sa >> 4), pb = sb + (sb >> 4);
+    const double wnone[1] = {};
+#pragma unroll
+    for (int it = 0; it < 2; it++) {  // blockDim >= S/16: at most two sub-problems per thread
+        const int sl = (int)threadIdx.x + it * (int)blockDim.x;
+        if (sl >= nsub) break;
+        const int c = sl & cmask;
+        const int r = sl >> g.logC;
+        const int jj = r & (q1 - 1);
+        const int row0 = ((r >> logq1) << 3) * q1 + jj;
+        const int base = (row0 << g.logC) + c;
+        const int p0 = pidx(base);
+        auto ix = [&](int a, int b) { return LIN ? p0 + a * pa + b * pb : pidx(base + a * sa + b * sb); };
+        const int64_t jt = (int64_t)jj * g.Lb + g.jcol0 + c;
+        double y[2][4], w0[2][3];
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) y[a][b] = x[ix(a, b)];
+#pragma unroll
+        for (int a = 0; a < 2; a++) load_tw<4>(w0[a], s0, jt + a * s1.Q);
+#pragma unroll
+        for (int a = 0; a < 2; a++) stage_regs<K, true, 4>(y[a], w0[a], s0);
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            double t[2] = {y[0][b], y[1][b]};
+            stage_regs<K, true, 2>(t, wnone, s1);
+            y[0][b] = t[0];
+            y[1][b] = t[1];
+        }
+        if (HOLDB) {
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < 4; b++) hb[8 * it + 4 * a + b] = y[a][b];
+            continue;
+        }
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const double v = SRC == SRC_SELF ? y[a][b]
+                                 : SRC == SRC_HELD ? hb[8 * it + 4 * a + b]
+                                                   : gld(gb, gbase + base + a * sa + b * sb);
+                y[a][b] = fp_mulmod3<K>(y[a][b], v);
+            }
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            double t[2] = {y[0][b], y[1][b]};
+            stage_regs<K, false, 2>(t, wnone, s1);
+            y[0][b] = t[0];
+            y[1][b] = t[1];
+        }
+#pragma unroll
+        for (int a = 0; a < 2; a++) stage_regs<K, false, 4>(y[a], w0[a], s0);
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) x[ix(a, b)] = y[a][b];
+    }
+}
+
 template <int K, bool HOLDB, int SRC>
 __device__ __forceinline__ void last_pass(double *x, double (&hb)[HOLD], const double *gb, int64_t gbase,
                                           const DevPlan *__restrict__ P, int s, int s1, const TileGeo &g) {
@@ -700,8 +780,14 @@ __device__ __forceinline__ void last_pass(double *x, double (&hb)[HOLD], const d
     const StageU a = stage_u(&P->st[s], tw);
     if (s1 - s == 2) {
         const StageU b = stage_u(&P->st[s + 1], tw);
-        if ((((a.q | b.q) << g.logC) & 15) == 0) last44<K, true, HOLDB, SRC>(x, hb, gb, gbase, a, b, g);
-        else last44<K, false, HOLDB, SRC>(x, hb, gb, gbase, a, b, g);
+        const bool lin = (((a.q | b.q) << g.logC) & 15) == 0;
+        if (b.m == 2) {
+            if (lin) last42<K, true, HOLDB, SRC>(x, hb, gb, gbase, a, b, g);
+            else last42<K, false, HOLDB, SRC>(x, hb, gb, gbase, a, b, g);
+        } else {
+            if (lin) last44<K, true, HOLDB, SRC>(x, hb, gb, gbase, a, b, g);
+            else last44<K, false, HOLDB, SRC>(x, hb, gb, gbase, a, b, g);
+        }
         return;
     }
     const bool lin = ((a.q << g.logC) & 15) == 0;
