@@ -228,8 +228,8 @@ struct StageList {
     int prime = 0;
     std::vector<StageDesc> st;
     double ninv = 0, ninvp = 0;
-    double *tw = nullptr;
-    int64_t ntw = 0;
+    double *twc = nullptr;  // contiguous copy of the stages' rows (N <= SMALL_TWC_MAX_N)
+    int64_t ntw = 0;        // doubles of all the stages' rows
 };
 
 struct PlanEntry {
@@ -262,6 +262,8 @@ struct Lib {
     // transform over the chip, so they favour smaller tiles than batches do
     std::vector<std::pair<int64_t, Tuning>> table_single;
     std::map<std::pair<int64_t, int>, std::unique_ptr<StageList>> stages;
+    // shared twiddle tables by (prime, radix, span)
+    std::map<std::tuple<int, int, int64_t>, double *> twtab;
     std::map<std::tuple<int64_t, int, int, int, int, int>, std::unique_ptr<PlanEntry>> plans;
     void *ws = nullptr;  // single-product workspace
     size_t ws_size = 0;
@@ -336,22 +338,24 @@ int ensure_init() {
     return fail(NBH_ERR_NOT_INIT, "nbh_init has not been called");
 }
 
-// Device twiddle tables for every stage: w_N^e via exact lo/hi power tables.
-int build_twiddles(StageList &L, uint64_t w) {
-    const int k = L.prime;
-    const uint64_t p = prime_u(k);
-    int64_t total = 0;
-    for (auto &s : L.st) {
-        if (s.m == 2) continue;
-        s.tw_off = total;
-        total += (int64_t)(s.m - 1) * s.Q;
+// The shared twiddle table of a radix-m stage of span L for prime k: rows
+// t = 1..m-1 of minbal(w_L^(t j)), j < L/m, with w_L = g^((p-1)/L) (the
+// reference's w_N^(N/L) for any N the stage occurs in, src/fpntt.jl:173-225).
+// Built on the device from exact lo/hi power tables of w_L.
+int get_twtab(int k, int m, int64_t L, const double **out) {
+    const auto key = std::make_tuple(k, m, L);
+    auto it = g.twtab.find(key);
+    if (it != g.twtab.end()) {
+        *out = it->second;
+        return NBH_OK;
     }
-    L.ntw = total;
-    if (total == 0) return NBH_OK;
-    const int logN = top_set_bit((uint64_t)(L.N - 1));
-    const int h = (logN + 1) / 2;
+    const uint64_t p = prime_u(k);
+    const uint64_t w = powmod(generator(k), (p - 1) / (uint64_t)L, p);
+    const int64_t Q = L / m;
+    const int logL = top_set_bit((uint64_t)(L - 1));
+    const int h = (logL + 1) / 2;
     const int64_t nlo = 1ll << h;
-    const int64_t nhi = (L.N + nlo - 1) / nlo;
+    const int64_t nhi = (L + nlo - 1) / nlo;
     std::vector<double> tab((size_t)(nlo + nhi));
     const double half = prime_d(k) / 2;
     auto minbal = [&](uint64_t v) { return (double)v > half ? (double)v - prime_d(k) : (double)v; };
@@ -366,16 +370,38 @@ int build_twiddles(StageList &L, uint64_t w) {
         tab[(size_t)(nlo + i)] = minbal(x);
         x = (uint64_t)((u128)x * step % p);
     }
-    double *dtab = nullptr;
-    HIPCHK(hipMalloc((void **)&L.tw, sizeof(double) * (size_t)total));
+    double *t = nullptr, *dtab = nullptr;
+    HIPCHK(hipMalloc((void **)&t, sizeof(double) * (size_t)((m - 1) * Q)));
+    g.twtab[key] = t;
     HIPCHK(hipMalloc((void **)&dtab, sizeof(double) * tab.size()));
     HIPCHK(hipMemcpyAsync(dtab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice, g.stream));
-    for (auto &s : L.st) {
-        if (s.m == 2) continue;
-        HIPCHK(launch_twiddle_stage(k, L.tw + s.tw_off, s.m, s.Q, L.N / s.L, dtab, dtab + nlo, h, g.stream));
-    }
+    HIPCHK(launch_twiddle_stage(k, t, m, Q, 1, dtab, dtab + nlo, h, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
     HIPCHK(hipFree(dtab));
+    *out = t;
+    return NBH_OK;
+}
+
+// Every stage's shared table, and (short lengths) the contiguous copy of the
+// rows the one-workgroup kernel loads in one round.
+int build_twiddles(StageList &L) {
+    int64_t total = 0;
+    for (auto &s : L.st) {
+        s.tw = nullptr;
+        s.tw_off = 0;
+        if (s.m == 2) continue;
+        RCCHK(get_twtab(L.prime, s.m, s.L, &s.tw));
+        s.tw_off = total;
+        total += (int64_t)(s.m - 1) * s.Q;
+    }
+    L.ntw = total;
+    if (total == 0 || L.N > SMALL_TWC_MAX_N) return NBH_OK;
+    HIPCHK(hipMalloc((void **)&L.twc, sizeof(double) * (size_t)total));
+    for (auto &s : L.st)
+        if (s.m != 2)
+            HIPCHK(hipMemcpyAsync(L.twc + s.tw_off, s.tw, sizeof(double) * (size_t)((s.m - 1) * s.Q),
+                                  hipMemcpyDeviceToDevice, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
     return NBH_OK;
 }
 
@@ -439,7 +465,7 @@ int get_stages(int64_t N, int k, StageList **out) {
     if ((int)L->st.size() > NBH_MAX_STAGES) return fail(NBH_ERR_LENGTH, "too many stages");
     L->ninv = fp_reduce_k((double)invmod((uint64_t)N % p, p), k);
     L->ninvp = L->ninv / prime_d(k);
-    RCCHK(build_twiddles(*L, w));
+    RCCHK(build_twiddles(*L));
     *out = L.get();
     g.stages[key] = std::move(L);
     return NBH_OK;
@@ -617,7 +643,7 @@ int get_plan(int64_t N, int k, PlanEntry **out, bool single = false) {
     P.ninv = L->ninv;
     P.ninvp = L->ninvp;
     for (int i = 0; i < P.nst; i++) P.st[i] = L->st[i];
-    P.tw = L->tw;
+    P.twc = L->twc;
     RCCHK(make_schedule(*L, T, P, *E));
     HIPCHK(hipMalloc((void **)&E->dev, sizeof(DevPlan)));
     HIPCHK(hipMemcpy(E->dev, &P, sizeof(DevPlan), hipMemcpyHostToDevice));
@@ -763,7 +789,7 @@ struct SmallGeo {
     size_t lds;
 };
 bool small_geo(const DevJob &J, int64_t N, SmallGeo *out) {
-    if (J.zero || J.bspec || N < 16 || N > 16384) return false;
+    if (J.zero || J.bspec || N < 16 || N > SMALL_TWC_MAX_N) return false;
     const int64_t half = std::min<int64_t>(SMALL_MAX_THREADS / 2, std::max<int64_t>(64, (2 * (N / 4) + 63) / 64 * 64));
     const int single = 4 * half >= N ? 1 : 0;
     const int64_t nt = 2 * half;
@@ -1662,9 +1688,11 @@ int32_t nbh_shutdown(void) {
     // single products may have run on callers' streams: wait for all of them
     (void)hipDeviceSynchronize();
     for (auto &e : g.plans) (void)hipFree(e.second->dev);
-    for (auto &e : g.stages) (void)hipFree(e.second->tw);
+    for (auto &e : g.stages) (void)hipFree(e.second->twc);
+    for (auto &e : g.twtab) (void)hipFree(e.second);
     g.plans.clear();
     g.stages.clear();
+    g.twtab.clear();
     if (g.ws) (void)hipFree(g.ws);
     if (g.io) (void)hipFree(g.io);
     if (g.pin) (void)hipHostFree(g.pin);
@@ -2942,8 +2970,7 @@ int32_t nbh_plan_twiddle_row(int64_t N, int32_t prime, int32_t i, int32_t t, dou
         if (i < 0 || i >= P->host.nst) return fail(NBH_ERR_ARG, "no such stage");
         const StageDesc &s = P->host.st[i];
         if (t < 1 || t >= s.m || s.m == 2) return fail(NBH_ERR_ARG, "no such twiddle row");
-        HIPCHK(hipMemcpy(out, P->host.tw + s.tw_off + (int64_t)(t - 1) * s.Q, sizeof(double) * (size_t)s.Q,
-                         hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(out, s.tw + (int64_t)(t - 1) * s.Q, sizeof(double) * (size_t)s.Q, hipMemcpyDeviceToHost));
         return NBH_OK;
     NBH_GUARD_END
 }

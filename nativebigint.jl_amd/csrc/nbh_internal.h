@@ -30,7 +30,14 @@ struct StageDesc {
     int32_t q;          // Q / Lb of the group this stage belongs to (row stride)
     int64_t L;          // span (block length) of the stage
     int64_t Q;          // L / m
-    int64_t tw_off;     // twiddle row t (1..m-1) is tw[tw_off + (t-1)*Q + j]
+    // twiddle row t (1..m-1) is tw[(t-1)*Q + j] = minbal(w_L^(t j)).  The table
+    // depends on (prime, m, L) only, so every plan with such a stage shares
+    // one copy (the kernels of a batch of many lengths touch one table per
+    // span, not one per length).  Null for radix 2.
+    const double *tw;
+    // offset of the same rows in the plan's contiguous copy DevPlan::twc
+    // (lengths the one-workgroup kernel takes): twc[tw_off + (t-1)*Q + j]
+    int64_t tw_off;
     uint64_t q_magic;   // ceil(2^32 / q): floor(r / q) = (r * q_magic) >> 32
     double rot[8];      // combine constants (radix-4: rot[0] = wi = r^(N2/4))
 };
@@ -51,7 +58,10 @@ struct DevPlan {
     double ninv, ninvp;           // 1/N mod p (minimal balanced) and fl(ninv/p)
     StageDesc st[NBH_MAX_STAGES];
     GroupDesc g[NBH_MAX_GROUPS];
-    const double *tw;             // device twiddle tables
+    // every stage's twiddle rows back to back (stage order, radix 2 none),
+    // for the one-workgroup kernel's single copy to LDS; null for lengths
+    // above SMALL_TWC_MAX_N
+    const double *twc;
 };
 
 // One product r = a * b (or a^2 when sqr) resident on the device.
@@ -102,6 +112,8 @@ constexpr int MID_MAX_THREADS = 1024;
 // most stages of a plan it takes (it copies both plans' descriptors to LDS).
 constexpr int SMALL_MAX_THREADS = 1024;
 constexpr int SMALL_MAX_STAGES = 10;
+// Plans of at most this length keep a contiguous copy of their twiddle rows.
+constexpr int64_t SMALL_TWC_MAX_N = 4096;
 
 // Doubles of LDS that an n-point tile occupies with one pad double per 16
 // (the kernels index LDS as i + i/16).

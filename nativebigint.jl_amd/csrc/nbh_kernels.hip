@@ -151,14 +151,14 @@ struct StageU {
     double rot[8];
 };
 
-__device__ __forceinline__ StageU stage_u(const StageDesc *sdp, const double *twbase) {
+__device__ __forceinline__ StageU stage_u(const StageDesc *sdp) {
     const auto *sd = cst(sdp);
     StageU s;
     s.m = uni(sd->m);
     s.q = uni(sd->q);
     s.qmagic = (uint64_t)uni64((int64_t)sd->q_magic);
     s.Q = uni64(sd->Q);
-    s.tw = twbase + uni64(sd->tw_off);
+    s.tw = (const double *)uni64((int64_t)sd->tw);
 #pragma unroll
     for (int i = 0; i < 8; i++) s.rot[i] = unid(sd->rot[i]);
     return s;
@@ -333,13 +333,12 @@ __device__ __forceinline__ void pass_44(double *x0, double *x1, int narr, const 
 template <int K, bool FWD>
 __device__ void run_stages(double *x0, double *x1, int narr, const DevPlan *__restrict__ P, int s0, int s1,
                            const TileGeo &g) {
-    const double *tw = unip(cst(P)->tw);
     if (FWD) {
         int s = s0;
         while (s < s1) {
-            const StageU a = stage_u(&P->st[s], tw);
+            const StageU a = stage_u(&P->st[s]);
             if (a.m == 4 && s + 1 < s1 && uni(cst(P)->st[s + 1].m) == 4) {
-                const StageU b = stage_u(&P->st[s + 1], tw);
+                const StageU b = stage_u(&P->st[s + 1]);
                 pass_44<K, true>(x0, x1, narr, a, b, g);
                 s += 2;
             } else {
@@ -351,9 +350,9 @@ __device__ void run_stages(double *x0, double *x1, int narr, const DevPlan *__re
     } else {
         int s = s1 - 1;
         while (s >= s0) {
-            const StageU b = stage_u(&P->st[s], tw);
+            const StageU b = stage_u(&P->st[s]);
             if (b.m == 4 && s - 1 >= s0 && uni(cst(P)->st[s - 1].m) == 4) {
-                const StageU a = stage_u(&P->st[s - 1], tw);
+                const StageU a = stage_u(&P->st[s - 1]);
                 pass_44<K, false>(x0, x1, narr, a, b, g);
                 s -= 2;
             } else {
@@ -597,10 +596,9 @@ constexpr int HOLD = 16;
 // one array; returns the first stage of the last pass.
 template <int K>
 __device__ int fwd_until_last(double *x, const DevPlan *__restrict__ P, int s0, int s1, const TileGeo &g) {
-    const double *tw = unip(cst(P)->tw);
     int s = s0;
     for (;;) {
-        const StageU a = stage_u(&P->st[s], tw);
+        const StageU a = stage_u(&P->st[s]);
         const int m1 = s + 1 < s1 ? uni(cst(P)->st[s + 1].m) : 0;
         const bool pair = a.m == 4 && m1 == 4;
         const int e = s + (pair ? 2 : 1);
@@ -608,7 +606,7 @@ __device__ int fwd_until_last(double *x, const DevPlan *__restrict__ P, int s0, 
         // a radix-4 stage followed by the final radix 2 is the last pass too
         if (a.m == 4 && m1 == 2 && s + 2 == s1) return s;
         if (pair) {
-            const StageU b = stage_u(&P->st[s + 1], tw);
+            const StageU b = stage_u(&P->st[s + 1]);
             if ((((a.q | b.q) << g.logC) & 15) == 0) pass44f<K, true, 1>(x, nullptr, a, b, g);
             else pass44f<K, false, 1>(x, nullptr, a, b, g);
         } else {
@@ -776,10 +774,9 @@ __device__ void last42(double *__restrict__ x, double (&hb)[HOLD], const double 
 template <int K, bool HOLDB, int SRC>
 __device__ __forceinline__ void last_pass(double *x, double (&hb)[HOLD], const double *gb, int64_t gbase,
                                           const DevPlan *__restrict__ P, int s, int s1, const TileGeo &g) {
-    const double *tw = unip(cst(P)->tw);
-    const StageU a = stage_u(&P->st[s], tw);
+    const StageU a = stage_u(&P->st[s]);
     if (s1 - s == 2) {
-        const StageU b = stage_u(&P->st[s + 1], tw);
+        const StageU b = stage_u(&P->st[s + 1]);
         const bool lin = (((a.q | b.q) << g.logC) & 15) == 0;
         if (b.m == 2) {
             if (lin) last42<K, true, HOLDB, SRC>(x, hb, gb, gbase, a, b, g);
@@ -1052,8 +1049,7 @@ __device__ __forceinline__ void edge_load44(double *__restrict__ lds, const Poin
 #pragma unroll
         for (int k = 0; k < NU; k++) y[(r * NU + k) & 3][(r * NU + k) >> 2] = v[k];
     }
-    const double *tw = unip(cst(P)->tw);
-    const StageU sa = stage_u(&P->st[s0], tw), sb = stage_u(&P->st[s0 + 1], tw);
+    const StageU sa = stage_u(&P->st[s0]), sb = stage_u(&P->st[s0 + 1]);
     const Sub44 u = sub44((int)threadIdx.x, sa, sb, g);
     double w0[4][3], w1[3];
     tw44<false>(w0, w1, u, sa, sb);
@@ -1078,8 +1074,7 @@ __device__ __forceinline__ void edge_store44(double *__restrict__ x, const doubl
                                              const TileGeo &g, int64_t gbase, const DevPlan *__restrict__ P, int s0) {
     const int T = blockDim.x;
     const int e0 = threadIdx.x;
-    const double *tw = unip(cst(P)->tw);
-    const StageU sa = stage_u(&P->st[s0], tw), sb = stage_u(&P->st[s0 + 1], tw);
+    const StageU sa = stage_u(&P->st[s0]), sb = stage_u(&P->st[s0 + 1]);
     const Sub44 u = sub44(e0, sa, sb, g);
     double y[4][4];
     if ((((sa.q | sb.q) << g.logC) & 15) == 0) {
@@ -1275,15 +1270,14 @@ __device__ void col_body(const DevJob *__restrict__ Jp, const Tile &t) {
 #pragma unroll
         for (int k = 0; k < S; k++) y[k] = gld(x, base + k * Lb);
     }
-    const double *tw = unip(cst(P)->tw);
     if (FWD) {
-        col_stage<K, true, M0, S / M0>(y, stage_u(&P->st[s0], tw), Lb, q);
-        if constexpr (M1 != 0) col_stage<K, true, M1, S / (M0 * M1)>(y, stage_u(&P->st[s0 + 1], tw), Lb, q);
-        if constexpr (M2 != 0) col_stage<K, true, M2, 1>(y, stage_u(&P->st[s0 + 2], tw), Lb, q);
+        col_stage<K, true, M0, S / M0>(y, stage_u(&P->st[s0]), Lb, q);
+        if constexpr (M1 != 0) col_stage<K, true, M1, S / (M0 * M1)>(y, stage_u(&P->st[s0 + 1]), Lb, q);
+        if constexpr (M2 != 0) col_stage<K, true, M2, 1>(y, stage_u(&P->st[s0 + 2]), Lb, q);
     } else {
-        if constexpr (M2 != 0) col_stage<K, false, M2, 1>(y, stage_u(&P->st[s0 + 2], tw), Lb, q);
-        if constexpr (M1 != 0) col_stage<K, false, M1, S / (M0 * M1)>(y, stage_u(&P->st[s0 + 1], tw), Lb, q);
-        col_stage<K, false, M0, S / M0>(y, stage_u(&P->st[s0], tw), Lb, q);
+        if constexpr (M2 != 0) col_stage<K, false, M2, 1>(y, stage_u(&P->st[s0 + 2]), Lb, q);
+        if constexpr (M1 != 0) col_stage<K, false, M1, S / (M0 * M1)>(y, stage_u(&P->st[s0 + 1]), Lb, q);
+        col_stage<K, false, M0, S / M0>(y, stage_u(&P->st[s0]), Lb, q);
     }
 #pragma unroll
     for (int k = 0; k < S; k++) gst(x, base + k * Lb, y[k]);
@@ -1895,8 +1889,8 @@ __global__ void __launch_bounds__(SMALL_MAX_THREADS) k_small(const DevJob J, int
         const uint32_t lmask = b >= 32 ? ~0u : (1u << b) - 1u;
         const uint32_t hmask = b >= 32 ? (1u << (b - 32)) - 1u : 0u;
         const __amdgpu_buffer_rsrc_t ra = limb_rsrc(J.a, J.m), rb = limb_rsrc(J.b, J.n);
-        const double *__restrict__ g0 = unip(cst(P0)->tw);
-        const double *__restrict__ g1 = unip(cst(P1)->tw);
+        const double *__restrict__ g0 = unip(cst(P0)->twc);
+        const double *__restrict__ g1 = unip(cst(P1)->twc);
         const int sw = STAGE_WORDS * nst;
         const int top = max(max(N, twl), 2 * sw);
         for (int r0 = 0; r0 < top; r0 += SMALL_LU * NT) {
