@@ -89,6 +89,24 @@ template <int K> NBH_HD double fp_minbal(double x) {
     return x > half ? x - Prime<K>::p : (x < -half ? x + Prime<K>::p : x);
 }
 
+// A prime as a runtime (wave-uniform) value: the kernels take it as an
+// argument instead of a template parameter, so both primes run the same
+// machine code (half the instruction-cache footprint).  The operations and
+// their order are those above, bit for bit.
+struct Fld {
+    double p, pinv;
+};
+NBH_HD Fld fld(int k) { return k ? Fld{P2_D, P2_INV} : Fld{P1_D, P1_INV}; }
+NBH_HD double fp_reduce(const Fld &f, double x) {
+    double q = fp_round(x * f.pinv);
+    return __builtin_fma(q, -f.p, x);
+}
+NBH_HD double fp_mulmod3(const Fld &f, double x, double y) {
+    double h = x * y;
+    double l = __builtin_fma(x, y, -h);
+    return fp_reduce(f, h) + l;
+}
+
 // Runtime-prime forms for host plan building.
 NBH_HD double fp_reduce_k(double x, int k) { return k ? fp_reduce<1>(x) : fp_reduce<0>(x); }
 NBH_HD double fp_mulmod3_k(double x, double y, int k) { return k ? fp_mulmod3<1>(x, y) : fp_mulmod3<0>(x, y); }

@@ -25,37 +25,34 @@ namespace nbh {
 // ---------------------------------------------------------------------------
 // DFT combines (src/fpntt.jl:310-313, 371-382, 459-492, 543-549)
 
-template <int K>
-__device__ __forceinline__ void dft3(double &a, double &b, double &c, double w3) {
-    double u = fp_mulmod3<K>(b - c, w3);
+__device__ __forceinline__ void dft3(const Fld &f, double &a, double &b, double &c, double w3) {
+    double u = fp_mulmod3(f, b - c, w3);
     double y0 = a + (b + c);
     double y1 = (a - c) + u;
     double y2 = (a - b) - u;
     a = y0; b = y1; c = y2;
 }
 
-template <int K>
-__device__ __forceinline__ void dft4(double &a, double &b, double &c, double &d, double wi) {
+__device__ __forceinline__ void dft4(const Fld &f, double &a, double &b, double &c, double &d, double wi) {
     double apc = a + c, amc = a - c;
     double bpd = b + d;
-    double ibmd = fp_mulmod3<K>(b - d, wi);
+    double ibmd = fp_mulmod3(f, b - d, wi);
     a = apc + bpd;
     b = amc + ibmd;
     c = apc - bpd;
     d = amc - ibmd;
 }
 
-template <int K>
-__device__ __forceinline__ void dft5(double *y, const double *k) {
+__device__ __forceinline__ void dft5(const Fld &f, double *y, const double *k) {
     double x0 = y[0];
     double t1 = y[1] + y[4], t3 = y[1] - y[4];
     double t2 = y[2] + y[3], t4 = y[2] - y[3];
     double S = t1 + t2, D = t1 - t2;
     double y0 = x0 + S;
-    double r = fp_reduce<K>(x0 + fp_mulmod3<K>(S, k[0]));
-    double q = fp_mulmod3<K>(D, k[1]);
-    double g1 = fp_mulmod3<K>(t3, k[2]) + fp_mulmod3<K>(t4, k[3]);
-    double g2 = fp_mulmod3<K>(t3, k[3]) - fp_mulmod3<K>(t4, k[2]);
+    double r = fp_reduce(f, x0 + fp_mulmod3(f, S, k[0]));
+    double q = fp_mulmod3(f, D, k[1]);
+    double g1 = fp_mulmod3(f, t3, k[2]) + fp_mulmod3(f, t4, k[3]);
+    double g2 = fp_mulmod3(f, t3, k[3]) - fp_mulmod3(f, t4, k[2]);
     double u = r + q, v = r - q;
     y[0] = y0;
     y[1] = u + g1;
@@ -64,35 +61,35 @@ __device__ __forceinline__ void dft5(double *y, const double *k) {
     y[4] = u - g1;
 }
 
-template <int K, bool RED>
-__device__ __forceinline__ void dft7(double *y, const double *rot) {
+template <bool RED>
+__device__ __forceinline__ void dft7(const Fld &f, double *y, const double *rot) {
     double x0 = y[0];
     double s1 = y[1] + y[6], d1 = y[1] - y[6];
     double s2 = y[2] + y[5], d2 = y[2] - y[5];
     double s3 = y[3] + y[4], d3 = y[3] - y[4];
     if (RED) {
-        s1 = fp_reduce<K>(s1); s2 = fp_reduce<K>(s2); s3 = fp_reduce<K>(s3);
-        d1 = fp_reduce<K>(d1); d2 = fp_reduce<K>(d2); d3 = fp_reduce<K>(d3);
+        s1 = fp_reduce(f, s1); s2 = fp_reduce(f, s2); s3 = fp_reduce(f, s3);
+        d1 = fp_reduce(f, d1); d2 = fp_reduce(f, d2); d3 = fp_reduce(f, d3);
     }
     double S = (s1 + s2) + s3;
     double y0 = x0 + S;
-    double m1 = fp_mulmod3<K>(S, rot[0]);
+    double m1 = fp_mulmod3(f, S, rot[0]);
     double va = s1 - s3, vb = s2 - s3;
-    double w0 = fp_mulmod3<K>(va, rot[1]);
-    double w1 = fp_mulmod3<K>(vb, rot[2]);
-    double w2 = fp_mulmod3<K>(va + vb, rot[3]);
-    double mn = fp_mulmod3<K>((d1 + d2) - d3, rot[4]);
+    double w0 = fp_mulmod3(f, va, rot[1]);
+    double w1 = fp_mulmod3(f, vb, rot[2]);
+    double w2 = fp_mulmod3(f, va + vb, rot[3]);
+    double mn = fp_mulmod3(f, (d1 + d2) - d3, rot[4]);
     double wa = d1 + d3, wb = -(d2 + d3);
-    double q0 = fp_mulmod3<K>(wa, rot[5]);
-    double q1 = fp_mulmod3<K>(wb, rot[6]);
-    double q2 = fp_mulmod3<K>(wa + wb, rot[7]);
-    double u1 = fp_reduce<K>(m1 + (3.0 * w0 - w2));
-    double u3 = fp_reduce<K>((m1 + 2.0 * w2) - 3.0 * (w0 + w1));
-    double u2 = fp_reduce<K>(m1 + (3.0 * w1 - w2));
+    double q0 = fp_mulmod3(f, wa, rot[5]);
+    double q1 = fp_mulmod3(f, wb, rot[6]);
+    double q2 = fp_mulmod3(f, wa + wb, rot[7]);
+    double u1 = fp_reduce(f, m1 + (3.0 * w0 - w2));
+    double u3 = fp_reduce(f, (m1 + 2.0 * w2) - 3.0 * (w0 + w1));
+    double u2 = fp_reduce(f, m1 + (3.0 * w1 - w2));
     double r0 = q0 - q1, r1 = q2 - q0;
-    double v1 = fp_reduce<K>(mn + (2.0 * r0 + r1));
-    double v3 = fp_reduce<K>((r0 + 2.0 * r1) - mn);
-    double v2 = fp_reduce<K>((mn - r0) + r1);
+    double v1 = fp_reduce(f, mn + (2.0 * r0 + r1));
+    double v3 = fp_reduce(f, (r0 + 2.0 * r1) - mn);
+    double v2 = fp_reduce(f, (mn - r0) + r1);
     double a1 = x0 + u1, a2 = x0 + u2, a3 = x0 + u3;
     y[0] = y0;
     y[1] = a1 + v1;
@@ -166,16 +163,16 @@ __device__ __forceinline__ StageU stage_u(const StageDesc *sdp) {
 
 // Radix-4 butterfly with its twiddles (src/fpntt.jl:645-701): forward
 // combines then reduces leg 0 and twiddles legs 1..3; reverse twiddles first.
-template <int K, bool FWD>
-__device__ __forceinline__ void bfly4(double &y0, double &y1, double &y2, double &y3, double wi,
+template <bool FWD>
+__device__ __forceinline__ void bfly4(const Fld &f, double &y0, double &y1, double &y2, double &y3, double wi,
                                       const double *__restrict__ tw, int64_t Q, int64_t j) {
     const double w1 = gld(tw, j), w2 = gld(tw, Q + j), w3 = gld(tw, 2 * Q + j);
-    if (FWD) dft4<K>(y0, y1, y2, y3, wi);
-    y0 = fp_reduce<K>(y0);
-    y1 = fp_mulmod3<K>(y1, w1);
-    y2 = fp_mulmod3<K>(y2, w2);
-    y3 = fp_mulmod3<K>(y3, w3);
-    if (!FWD) dft4<K>(y0, y1, y2, y3, wi);
+    if (FWD) dft4(f, y0, y1, y2, y3, wi);
+    y0 = fp_reduce(f, y0);
+    y1 = fp_mulmod3(f, y1, w1);
+    y2 = fp_mulmod3(f, y2, w2);
+    y3 = fp_mulmod3(f, y3, w3);
+    if (!FWD) dft4(f, y0, y1, y2, y3, wi);
 }
 
 // Tile geometry shared by every pass: S rows x C columns, legs q rows apart,
@@ -195,8 +192,8 @@ struct TileGeo {
 // LIN: both leg strides are multiples of 16 points, so the padded index of
 // leg (a, b) is pidx(base) + a*pidx-stride(q1) + b*pidx-stride(q0), one add
 // per point instead of a pidx per point.
-template <int K, bool FWD, bool LIN>
-__device__ void pass44(double *__restrict__ x0, double *__restrict__ x1, int narr, const StageU &s0,
+template <bool FWD, bool LIN>
+__device__ void pass44(const Fld &f, double *__restrict__ x0, double *__restrict__ x1, int narr, const StageU &s0,
                        const StageU &s1, const TileGeo &g) {
     const int q1 = s1.q;
     const int logq1 = __builtin_ctz((unsigned)q1);
@@ -224,16 +221,16 @@ __device__ void pass44(double *__restrict__ x0, double *__restrict__ x1, int nar
         if (FWD) {
 #pragma unroll
             for (int a = 0; a < 4; a++)
-                bfly4<K, true>(y[a][0], y[a][1], y[a][2], y[a][3], s0.rot[0], s0.tw, s0.Q, jt + a * s1.Q);
+                bfly4<true>(f, y[a][0], y[a][1], y[a][2], y[a][3], s0.rot[0], s0.tw, s0.Q, jt + a * s1.Q);
 #pragma unroll
-            for (int b = 0; b < 4; b++) bfly4<K, true>(y[0][b], y[1][b], y[2][b], y[3][b], s1.rot[0], s1.tw, s1.Q, jt);
+            for (int b = 0; b < 4; b++) bfly4<true>(f, y[0][b], y[1][b], y[2][b], y[3][b], s1.rot[0], s1.tw, s1.Q, jt);
         } else {
 #pragma unroll
             for (int b = 0; b < 4; b++)
-                bfly4<K, false>(y[0][b], y[1][b], y[2][b], y[3][b], s1.rot[0], s1.tw, s1.Q, jt);
+                bfly4<false>(f, y[0][b], y[1][b], y[2][b], y[3][b], s1.rot[0], s1.tw, s1.Q, jt);
 #pragma unroll
             for (int a = 0; a < 4; a++)
-                bfly4<K, false>(y[a][0], y[a][1], y[a][2], y[a][3], s0.rot[0], s0.tw, s0.Q, jt + a * s1.Q);
+                bfly4<false>(f, y[a][0], y[a][1], y[a][2], y[a][3], s0.rot[0], s0.tw, s0.Q, jt + a * s1.Q);
         }
 #pragma unroll
         for (int a = 0; a < 4; a++)
@@ -246,8 +243,8 @@ __device__ void pass44(double *__restrict__ x0, double *__restrict__ x1, int nar
 // 494-535, 630-638, 645-701): forward = combine, reduce leg 0, twiddle legs
 // 1..M-1; reverse = reduce + twiddle, then combine (radix 5/7 re-reduce leg
 // 0); radix 2 is twiddle-free and the same both ways.
-template <int K, bool FWD, int M, bool LIN>
-__device__ void pass1(double *__restrict__ x0, double *__restrict__ x1, int narr, const StageU &s,
+template <bool FWD, int M, bool LIN>
+__device__ void pass1(const Fld &f, double *__restrict__ x0, double *__restrict__ x1, int narr, const StageU &s,
                       const TileGeo &g) {
     const int q = s.q;
     const int C = 1 << g.logC;
@@ -269,34 +266,34 @@ __device__ void pass1(double *__restrict__ x0, double *__restrict__ x1, int narr
         for (int t = 0; t < M; t++) y[t] = x[ix(t)];
         if constexpr (M == 2) {
             const double u = y[0], v = y[1];
-            y[0] = fp_reduce<K>(u + v);
-            y[1] = fp_reduce<K>(u - v);
+            y[0] = fp_reduce(f, u + v);
+            y[1] = fp_reduce(f, u - v);
         } else if constexpr (M == 4) {
-            bfly4<K, FWD>(y[0], y[1], y[2], y[3], s.rot[0], s.tw, s.Q, j);
+            bfly4<FWD>(f, y[0], y[1], y[2], y[3], s.rot[0], s.tw, s.Q, j);
         } else if constexpr (M == 3) {
             const double w1 = gld(s.tw, j), w2 = gld(s.tw, s.Q + j);
-            if (FWD) dft3<K>(y[0], y[1], y[2], s.rot[0]);
-            y[0] = fp_reduce<K>(y[0]);
-            y[1] = fp_mulmod3<K>(y[1], w1);
-            y[2] = fp_mulmod3<K>(y[2], w2);
-            if (!FWD) dft3<K>(y[0], y[1], y[2], s.rot[0]);
+            if (FWD) dft3(f, y[0], y[1], y[2], s.rot[0]);
+            y[0] = fp_reduce(f, y[0]);
+            y[1] = fp_mulmod3(f, y[1], w1);
+            y[2] = fp_mulmod3(f, y[2], w2);
+            if (!FWD) dft3(f, y[0], y[1], y[2], s.rot[0]);
         } else if constexpr (M == 5) {
-            if (FWD) dft5<K>(y, s.rot);
-            y[0] = fp_reduce<K>(y[0]);
+            if (FWD) dft5(f, y, s.rot);
+            y[0] = fp_reduce(f, y[0]);
 #pragma unroll
-            for (int t = 1; t < 5; t++) y[t] = fp_mulmod3<K>(y[t], gld(s.tw, (t - 1) * s.Q + j));
+            for (int t = 1; t < 5; t++) y[t] = fp_mulmod3(f, y[t], gld(s.tw, (t - 1) * s.Q + j));
             if (!FWD) {
-                dft5<K>(y, s.rot);
-                y[0] = fp_reduce<K>(y[0]);
+                dft5(f, y, s.rot);
+                y[0] = fp_reduce(f, y[0]);
             }
         } else {
-            if (FWD) dft7<K, false>(y, s.rot);
-            y[0] = fp_reduce<K>(y[0]);
+            if (FWD) dft7<false>(f, y, s.rot);
+            y[0] = fp_reduce(f, y[0]);
 #pragma unroll
-            for (int t = 1; t < 7; t++) y[t] = fp_mulmod3<K>(y[t], gld(s.tw, (t - 1) * s.Q + j));
+            for (int t = 1; t < 7; t++) y[t] = fp_mulmod3(f, y[t], gld(s.tw, (t - 1) * s.Q + j));
             if (!FWD) {
-                dft7<K, true>(y, s.rot);
-                y[0] = fp_reduce<K>(y[0]);
+                dft7<true>(f, y, s.rot);
+                y[0] = fp_reduce(f, y[0]);
             }
         }
 #pragma unroll
@@ -304,34 +301,34 @@ __device__ void pass1(double *__restrict__ x0, double *__restrict__ x1, int narr
     }
 }
 
-template <int K, bool FWD, bool LIN>
-__device__ __forceinline__ void pass_one_t(double *x0, double *x1, int narr, const StageU &s, const TileGeo &g) {
+template <bool FWD, bool LIN>
+__device__ __forceinline__ void pass_one_t(const Fld &f, double *x0, double *x1, int narr, const StageU &s, const TileGeo &g) {
     switch (s.m) {
-    case 4: pass1<K, FWD, 4, LIN>(x0, x1, narr, s, g); break;
-    case 2: pass1<K, FWD, 2, LIN>(x0, x1, narr, s, g); break;
-    case 3: pass1<K, FWD, 3, LIN>(x0, x1, narr, s, g); break;
-    case 5: pass1<K, FWD, 5, LIN>(x0, x1, narr, s, g); break;
-    default: pass1<K, FWD, 7, LIN>(x0, x1, narr, s, g); break;
+    case 4: pass1<FWD, 4, LIN>(f, x0, x1, narr, s, g); break;
+    case 2: pass1<FWD, 2, LIN>(f, x0, x1, narr, s, g); break;
+    case 3: pass1<FWD, 3, LIN>(f, x0, x1, narr, s, g); break;
+    case 5: pass1<FWD, 5, LIN>(f, x0, x1, narr, s, g); break;
+    default: pass1<FWD, 7, LIN>(f, x0, x1, narr, s, g); break;
     }
 }
 
-template <int K, bool FWD>
-__device__ __forceinline__ void pass_one(double *x0, double *x1, int narr, const StageU &s, const TileGeo &g) {
-    if (((s.q << g.logC) & 15) == 0) pass_one_t<K, FWD, true>(x0, x1, narr, s, g);
-    else pass_one_t<K, FWD, false>(x0, x1, narr, s, g);
+template <bool FWD>
+__device__ __forceinline__ void pass_one(const Fld &f, double *x0, double *x1, int narr, const StageU &s, const TileGeo &g) {
+    if (((s.q << g.logC) & 15) == 0) pass_one_t<FWD, true>(f, x0, x1, narr, s, g);
+    else pass_one_t<FWD, false>(f, x0, x1, narr, s, g);
 }
 
-template <int K, bool FWD>
-__device__ __forceinline__ void pass_44(double *x0, double *x1, int narr, const StageU &a, const StageU &b,
+template <bool FWD>
+__device__ __forceinline__ void pass_44(const Fld &f, double *x0, double *x1, int narr, const StageU &a, const StageU &b,
                                         const TileGeo &g) {
-    if ((((a.q | b.q) << g.logC) & 15) == 0) pass44<K, FWD, true>(x0, x1, narr, a, b, g);
-    else pass44<K, FWD, false>(x0, x1, narr, a, b, g);
+    if ((((a.q | b.q) << g.logC) & 15) == 0) pass44<FWD, true>(f, x0, x1, narr, a, b, g);
+    else pass44<FWD, false>(f, x0, x1, narr, a, b, g);
 }
 
 // Stages [s0, s1) of a plan over the LDS tile(s), radix-4 pairs fused.  Any
 // grouping of consecutive stages into passes applies the same butterflies.
-template <int K, bool FWD>
-__device__ void run_stages(double *x0, double *x1, int narr, const DevPlan *__restrict__ P, int s0, int s1,
+template <bool FWD>
+__device__ void run_stages(const Fld &f, double *x0, double *x1, int narr, const DevPlan *__restrict__ P, int s0, int s1,
                            const TileGeo &g) {
     if (FWD) {
         int s = s0;
@@ -339,10 +336,10 @@ __device__ void run_stages(double *x0, double *x1, int narr, const DevPlan *__re
             const StageU a = stage_u(&P->st[s]);
             if (a.m == 4 && s + 1 < s1 && uni(cst(P)->st[s + 1].m) == 4) {
                 const StageU b = stage_u(&P->st[s + 1]);
-                pass_44<K, true>(x0, x1, narr, a, b, g);
+                pass_44<true>(f, x0, x1, narr, a, b, g);
                 s += 2;
             } else {
-                pass_one<K, true>(x0, x1, narr, a, g);
+                pass_one<true>(f, x0, x1, narr, a, g);
                 s += 1;
             }
             __syncthreads();
@@ -353,10 +350,10 @@ __device__ void run_stages(double *x0, double *x1, int narr, const DevPlan *__re
             const StageU b = stage_u(&P->st[s]);
             if (b.m == 4 && s - 1 >= s0 && uni(cst(P)->st[s - 1].m) == 4) {
                 const StageU a = stage_u(&P->st[s - 1]);
-                pass_44<K, false>(x0, x1, narr, a, b, g);
+                pass_44<false>(f, x0, x1, narr, a, b, g);
                 s -= 2;
             } else {
-                pass_one<K, false>(x0, x1, narr, b, g);
+                pass_one<false>(f, x0, x1, narr, b, g);
                 s -= 1;
             }
             __syncthreads();
@@ -374,45 +371,45 @@ __device__ void run_stages(double *x0, double *x1, int narr, const DevPlan *__re
 // x times the twiddle w^0 = 1, which is every twiddle of a transform's last
 // stage (span = radix): fp_mulmod3(x, 1) = reduce(x) + fma(x, 1, -x) =
 // reduce(x) + (+0.0), bit for bit, without the product and the fma.
-template <int K> __device__ __forceinline__ double fp_mulmod_one(double x) { return fp_reduce<K>(x) + 0.0; }
+__device__ __forceinline__ double fp_mulmod_one(const Fld &f, double x) { return fp_reduce(f, x) + 0.0; }
 
 // One radix-M stage on registers y[0..M) with its twiddles w[0..M-2].
 // UNIT: the transform's last stage, whose twiddles are all 1 (w unread).
-template <int K, bool FWD, int M, bool UNIT = false>
-__device__ __forceinline__ void stage_regs(double *y, const double *w, const StageU &s) {
+template <bool FWD, int M, bool UNIT = false>
+__device__ __forceinline__ void stage_regs(const Fld &f, double *y, const double *w, const StageU &s) {
     if constexpr (M == 2) {
         const double u = y[0], v = y[1];
-        y[0] = fp_reduce<K>(u + v);
-        y[1] = fp_reduce<K>(u - v);
+        y[0] = fp_reduce(f, u + v);
+        y[1] = fp_reduce(f, u - v);
     } else if constexpr (M == 4) {
-        if (FWD) dft4<K>(y[0], y[1], y[2], y[3], s.rot[0]);
-        y[0] = fp_reduce<K>(y[0]);
+        if (FWD) dft4(f, y[0], y[1], y[2], y[3], s.rot[0]);
+        y[0] = fp_reduce(f, y[0]);
 #pragma unroll
-        for (int t = 1; t < 4; t++) y[t] = UNIT ? fp_mulmod_one<K>(y[t]) : fp_mulmod3<K>(y[t], w[t - 1]);
-        if (!FWD) dft4<K>(y[0], y[1], y[2], y[3], s.rot[0]);
+        for (int t = 1; t < 4; t++) y[t] = UNIT ? fp_mulmod_one(f, y[t]) : fp_mulmod3(f, y[t], w[t - 1]);
+        if (!FWD) dft4(f, y[0], y[1], y[2], y[3], s.rot[0]);
     } else if constexpr (M == 3) {
-        if (FWD) dft3<K>(y[0], y[1], y[2], s.rot[0]);
-        y[0] = fp_reduce<K>(y[0]);
-        y[1] = fp_mulmod3<K>(y[1], w[0]);
-        y[2] = fp_mulmod3<K>(y[2], w[1]);
-        if (!FWD) dft3<K>(y[0], y[1], y[2], s.rot[0]);
+        if (FWD) dft3(f, y[0], y[1], y[2], s.rot[0]);
+        y[0] = fp_reduce(f, y[0]);
+        y[1] = fp_mulmod3(f, y[1], w[0]);
+        y[2] = fp_mulmod3(f, y[2], w[1]);
+        if (!FWD) dft3(f, y[0], y[1], y[2], s.rot[0]);
     } else if constexpr (M == 5) {
-        if (FWD) dft5<K>(y, s.rot);
-        y[0] = fp_reduce<K>(y[0]);
+        if (FWD) dft5(f, y, s.rot);
+        y[0] = fp_reduce(f, y[0]);
 #pragma unroll
-        for (int t = 1; t < 5; t++) y[t] = fp_mulmod3<K>(y[t], w[t - 1]);
+        for (int t = 1; t < 5; t++) y[t] = fp_mulmod3(f, y[t], w[t - 1]);
         if (!FWD) {
-            dft5<K>(y, s.rot);
-            y[0] = fp_reduce<K>(y[0]);
+            dft5(f, y, s.rot);
+            y[0] = fp_reduce(f, y[0]);
         }
     } else {
-        if (FWD) dft7<K, false>(y, s.rot);
-        y[0] = fp_reduce<K>(y[0]);
+        if (FWD) dft7<false>(f, y, s.rot);
+        y[0] = fp_reduce(f, y[0]);
 #pragma unroll
-        for (int t = 1; t < 7; t++) y[t] = fp_mulmod3<K>(y[t], w[t - 1]);
+        for (int t = 1; t < 7; t++) y[t] = fp_mulmod3(f, y[t], w[t - 1]);
         if (!FWD) {
-            dft7<K, true>(y, s.rot);
-            y[0] = fp_reduce<K>(y[0]);
+            dft7<true>(f, y, s.rot);
+            y[0] = fp_reduce(f, y[0]);
         }
     }
 }
@@ -453,16 +450,16 @@ __device__ __forceinline__ int ix44(const Sub44 &u, int a, int b) {
 }
 
 // Forward stages s0 (over b) and s1 (over a) of one sub-problem on registers.
-template <int K, bool FWD, bool UNIT1 = false>
-__device__ __forceinline__ void regs44(double (&y)[4][4], const double (&w0)[4][3], const double (&w1)[3],
+template <bool FWD, bool UNIT1 = false>
+__device__ __forceinline__ void regs44(const Fld &f, double (&y)[4][4], const double (&w0)[4][3], const double (&w1)[3],
                                        const StageU &s0, const StageU &s1) {
     if (FWD) {
 #pragma unroll
-        for (int a = 0; a < 4; a++) stage_regs<K, true, 4>(y[a], w0[a], s0);
+        for (int a = 0; a < 4; a++) stage_regs<true, 4>(f, y[a], w0[a], s0);
 #pragma unroll
         for (int b = 0; b < 4; b++) {
             double t[4] = {y[0][b], y[1][b], y[2][b], y[3][b]};
-            stage_regs<K, true, 4, UNIT1>(t, w1, s1);
+            stage_regs<true, 4, UNIT1>(f, t, w1, s1);
 #pragma unroll
             for (int a = 0; a < 4; a++) y[a][b] = t[a];
         }
@@ -470,12 +467,12 @@ __device__ __forceinline__ void regs44(double (&y)[4][4], const double (&w0)[4][
 #pragma unroll
         for (int b = 0; b < 4; b++) {
             double t[4] = {y[0][b], y[1][b], y[2][b], y[3][b]};
-            stage_regs<K, false, 4, UNIT1>(t, w1, s1);
+            stage_regs<false, 4, UNIT1>(f, t, w1, s1);
 #pragma unroll
             for (int a = 0; a < 4; a++) y[a][b] = t[a];
         }
 #pragma unroll
-        for (int a = 0; a < 4; a++) stage_regs<K, false, 4>(y[a], w0[a], s0);
+        for (int a = 0; a < 4; a++) stage_regs<false, 4>(f, y[a], w0[a], s0);
     }
 }
 
@@ -492,8 +489,8 @@ __device__ __forceinline__ void tw44(double (&w0)[4][3], double (&w1)[3], const 
 }
 
 // Forward radix-4 pair on NA arrays (a, and b when NA == 2).
-template <int K, bool LIN, int NA>
-__device__ void pass44f(double *__restrict__ xa, double *__restrict__ xb, const StageU &s0, const StageU &s1,
+template <bool LIN, int NA>
+__device__ void pass44f(const Fld &f, double *__restrict__ xa, double *__restrict__ xb, const StageU &s0, const StageU &s1,
                         const TileGeo &g) {
     const int nsub = (g.S >> 4) << g.logC;
     for (int sl = threadIdx.x; sl < nsub; sl += blockDim.x) {
@@ -508,7 +505,7 @@ __device__ void pass44f(double *__restrict__ xa, double *__restrict__ xb, const 
             for (int a = 0; a < 4; a++)
 #pragma unroll
                 for (int b = 0; b < 4; b++) y[a][b] = x[ix44<LIN>(u, a, b)];
-            regs44<K, true>(y, w0, w1, s0, s1);
+            regs44<true>(f, y, w0, w1, s0, s1);
 #pragma unroll
             for (int a = 0; a < 4; a++)
 #pragma unroll
@@ -545,8 +542,8 @@ __device__ __forceinline__ int ix1(const Sub1 &u, int t) {
 }
 
 // Forward single stage on NA arrays.
-template <int K, int M, bool LIN, int NA>
-__device__ void pass1f(double *__restrict__ xa, double *__restrict__ xb, const StageU &s, const TileGeo &g) {
+template <int M, bool LIN, int NA>
+__device__ void pass1f(const Fld &f, double *__restrict__ xa, double *__restrict__ xb, const StageU &s, const TileGeo &g) {
     const int nb = (g.S / M) << g.logC;
     for (int beta = threadIdx.x; beta < nb; beta += blockDim.x) {
         const Sub1 u = sub1<M>(beta, s, g);
@@ -558,22 +555,22 @@ __device__ void pass1f(double *__restrict__ xa, double *__restrict__ xb, const S
             double y[M];
 #pragma unroll
             for (int t = 0; t < M; t++) y[t] = x[ix1<LIN>(u, t)];
-            stage_regs<K, true, M>(y, w, s);
+            stage_regs<true, M>(f, y, w, s);
 #pragma unroll
             for (int t = 0; t < M; t++) x[ix1<LIN>(u, t)] = y[t];
         }
     }
 }
 
-template <int K, int NA>
-__device__ __forceinline__ void pass1f_any(double *xa, double *xb, const StageU &s, const TileGeo &g) {
+template <int NA>
+__device__ __forceinline__ void pass1f_any(const Fld &f, double *xa, double *xb, const StageU &s, const TileGeo &g) {
     const bool lin = ((s.q << g.logC) & 15) == 0;
     switch (s.m) {
-    case 4: lin ? pass1f<K, 4, true, NA>(xa, xb, s, g) : pass1f<K, 4, false, NA>(xa, xb, s, g); break;
-    case 2: lin ? pass1f<K, 2, true, NA>(xa, xb, s, g) : pass1f<K, 2, false, NA>(xa, xb, s, g); break;
-    case 3: lin ? pass1f<K, 3, true, NA>(xa, xb, s, g) : pass1f<K, 3, false, NA>(xa, xb, s, g); break;
-    case 5: lin ? pass1f<K, 5, true, NA>(xa, xb, s, g) : pass1f<K, 5, false, NA>(xa, xb, s, g); break;
-    default: lin ? pass1f<K, 7, true, NA>(xa, xb, s, g) : pass1f<K, 7, false, NA>(xa, xb, s, g); break;
+    case 4: lin ? pass1f<4, true, NA>(f, xa, xb, s, g) : pass1f<4, false, NA>(f, xa, xb, s, g); break;
+    case 2: lin ? pass1f<2, true, NA>(f, xa, xb, s, g) : pass1f<2, false, NA>(f, xa, xb, s, g); break;
+    case 3: lin ? pass1f<3, true, NA>(f, xa, xb, s, g) : pass1f<3, false, NA>(f, xa, xb, s, g); break;
+    case 5: lin ? pass1f<5, true, NA>(f, xa, xb, s, g) : pass1f<5, false, NA>(f, xa, xb, s, g); break;
+    default: lin ? pass1f<7, true, NA>(f, xa, xb, s, g) : pass1f<7, false, NA>(f, xa, xb, s, g); break;
     }
 }
 
@@ -594,8 +591,7 @@ constexpr int HOLD = 16;
 
 // Forward passes from s0 up to (not including) the last pass of [s0, s1), on
 // one array; returns the first stage of the last pass.
-template <int K>
-__device__ int fwd_until_last(double *x, const DevPlan *__restrict__ P, int s0, int s1, const TileGeo &g) {
+__device__ int fwd_until_last(const Fld &f, double *x, const DevPlan *__restrict__ P, int s0, int s1, const TileGeo &g) {
     int s = s0;
     for (;;) {
         const StageU a = stage_u(&P->st[s]);
@@ -607,10 +603,10 @@ __device__ int fwd_until_last(double *x, const DevPlan *__restrict__ P, int s0, 
         if (a.m == 4 && m1 == 2 && s + 2 == s1) return s;
         if (pair) {
             const StageU b = stage_u(&P->st[s + 1]);
-            if ((((a.q | b.q) << g.logC) & 15) == 0) pass44f<K, true, 1>(x, nullptr, a, b, g);
-            else pass44f<K, false, 1>(x, nullptr, a, b, g);
+            if ((((a.q | b.q) << g.logC) & 15) == 0) pass44f<true, 1>(f, x, nullptr, a, b, g);
+            else pass44f<false, 1>(f, x, nullptr, a, b, g);
         } else {
-            pass1f_any<K, 1>(x, nullptr, a, g);
+            pass1f_any<1>(f, x, nullptr, a, g);
         }
         __syncthreads();
         s = e;
@@ -619,8 +615,8 @@ __device__ int fwd_until_last(double *x, const DevPlan *__restrict__ P, int s0, 
 
 // Last pass = a radix-4 pair.  HOLDB: compute x's last forward pass into hb
 // (no store).  Otherwise: the fused pass on x with b from SRC.
-template <int K, bool LIN, bool HOLDB, int SRC>
-__device__ void last44(double *__restrict__ x, double (&hb)[HOLD], const double *__restrict__ gb, int64_t gbase,
+template <bool LIN, bool HOLDB, int SRC>
+__device__ void last44(const Fld &f, double *__restrict__ x, double (&hb)[HOLD], const double *__restrict__ gb, int64_t gbase,
                        const StageU &s0, const StageU &s1, const TileGeo &g) {
     const int nsub = (g.S >> 4) << g.logC;
     const int sl = threadIdx.x;  // one sub-problem per thread: blockDim >= S/16
@@ -634,7 +630,7 @@ __device__ void last44(double *__restrict__ x, double (&hb)[HOLD], const double 
     // s1 is the transform's last stage: its twiddles are all 1
     double w0[4][3], w1[3];
     tw44<LIN, true>(w0, w1, u, s0, s1);
-    regs44<K, true, true>(y, w0, w1, s0, s1);
+    regs44<true, true>(f, y, w0, w1, s0, s1);
     if (HOLDB) {
 #pragma unroll
         for (int a = 0; a < 4; a++)
@@ -649,9 +645,9 @@ __device__ void last44(double *__restrict__ x, double (&hb)[HOLD], const double 
             const double v = SRC == SRC_SELF ? y[a][b]
                              : SRC == SRC_HELD ? hb[4 * a + b]
                                                : gld(gb, gbase + u.base + a * u.sa + b * u.sb);
-            y[a][b] = fp_mulmod3<K>(y[a][b], v);
+            y[a][b] = fp_mulmod3(f, y[a][b], v);
         }
-    regs44<K, false, true>(y, w0, w1, s0, s1);
+    regs44<false, true>(f, y, w0, w1, s0, s1);
 #pragma unroll
     for (int a = 0; a < 4; a++)
 #pragma unroll
@@ -659,8 +655,8 @@ __device__ void last44(double *__restrict__ x, double (&hb)[HOLD], const double 
 }
 
 // Last pass = one radix-M stage: the thread's butterflies beta = t + it*T.
-template <int K, int M, bool LIN, bool HOLDB, int SRC>
-__device__ void last1(double *__restrict__ x, double (&hb)[HOLD], const double *__restrict__ gb, int64_t gbase,
+template <int M, bool LIN, bool HOLDB, int SRC>
+__device__ void last1(const Fld &f, double *__restrict__ x, double (&hb)[HOLD], const double *__restrict__ gb, int64_t gbase,
                       const StageU &s, const TileGeo &g) {
     static_assert(16 % M == 0, "the last pass is radix 4 or 2");
     constexpr int MAXIT = 16 / M;  // blockDim >= S/16, so nb/blockDim <= 16/M
@@ -675,7 +671,7 @@ __device__ void last1(double *__restrict__ x, double (&hb)[HOLD], const double *
         double y[M];
 #pragma unroll
         for (int t = 0; t < M; t++) y[t] = x[ix1<LIN>(u, t)];
-        stage_regs<K, true, M, true>(y, w, s);
+        stage_regs<true, M, true>(f, y, w, s);
         if (HOLDB) {
 #pragma unroll
             for (int t = 0; t < M; t++) hb[it * M + t] = y[t];
@@ -686,9 +682,9 @@ __device__ void last1(double *__restrict__ x, double (&hb)[HOLD], const double *
             const double v = SRC == SRC_SELF ? y[t]
                              : SRC == SRC_HELD ? hb[it * M + t]
                                                : gld(gb, gbase + u.base + t * u.strideC);
-            y[t] = fp_mulmod3<K>(y[t], v);
+            y[t] = fp_mulmod3(f, y[t], v);
         }
-        stage_regs<K, false, M, true>(y, w, s);
+        stage_regs<false, M, true>(f, y, w, s);
 #pragma unroll
         for (int t = 0; t < M; t++) x[ix1<LIN>(u, t)] = y[t];
     }
@@ -701,8 +697,8 @@ __device__ void last1(double *__restrict__ x, double (&hb)[HOLD], const double *
 // twiddles); the inverse runs them in the opposite order.  Exactly the
 // butterflies of pass1<4> and pass1<2>, with one LDS round trip instead of
 // two.
-template <int K, bool LIN, bool HOLDB, int SRC>
-__device__ void last42(double *__restrict__ x, double (&hb)[HOLD], const double *__restrict__ gb, int64_t gbase,
+template <bool LIN, bool HOLDB, int SRC>
+__device__ void last42(const Fld &f, double *__restrict__ x, double (&hb)[HOLD], const double *__restrict__ gb, int64_t gbase,
                        const StageU &s0, const StageU &s1, const TileGeo &g) {
     const int nsub = (g.S >> 3) << g.logC;
     const int q1 = s1.q, q0 = s0.q;
@@ -731,11 +727,11 @@ __device__ void last42(double *__restrict__ x, double (&hb)[HOLD], const double 
 #pragma unroll
         for (int a = 0; a < 2; a++) load_tw<4>(w0[a], s0, jt + a * s1.Q);
 #pragma unroll
-        for (int a = 0; a < 2; a++) stage_regs<K, true, 4>(y[a], w0[a], s0);
+        for (int a = 0; a < 2; a++) stage_regs<true, 4>(f, y[a], w0[a], s0);
 #pragma unroll
         for (int b = 0; b < 4; b++) {
             double t[2] = {y[0][b], y[1][b]};
-            stage_regs<K, true, 2>(t, wnone, s1);
+            stage_regs<true, 2>(f, t, wnone, s1);
             y[0][b] = t[0];
             y[1][b] = t[1];
         }
@@ -753,17 +749,17 @@ __device__ void last42(double *__restrict__ x, double (&hb)[HOLD], const double 
                 const double v = SRC == SRC_SELF ? y[a][b]
                                  : SRC == SRC_HELD ? hb[8 * it + 4 * a + b]
                                                    : gld(gb, gbase + base + a * sa + b * sb);
-                y[a][b] = fp_mulmod3<K>(y[a][b], v);
+                y[a][b] = fp_mulmod3(f, y[a][b], v);
             }
 #pragma unroll
         for (int b = 0; b < 4; b++) {
             double t[2] = {y[0][b], y[1][b]};
-            stage_regs<K, false, 2>(t, wnone, s1);
+            stage_regs<false, 2>(f, t, wnone, s1);
             y[0][b] = t[0];
             y[1][b] = t[1];
         }
 #pragma unroll
-        for (int a = 0; a < 2; a++) stage_regs<K, false, 4>(y[a], w0[a], s0);
+        for (int a = 0; a < 2; a++) stage_regs<false, 4>(f, y[a], w0[a], s0);
 #pragma unroll
         for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -771,27 +767,27 @@ __device__ void last42(double *__restrict__ x, double (&hb)[HOLD], const double 
     }
 }
 
-template <int K, bool HOLDB, int SRC>
-__device__ __forceinline__ void last_pass(double *x, double (&hb)[HOLD], const double *gb, int64_t gbase,
+template <bool HOLDB, int SRC>
+__device__ __forceinline__ void last_pass(const Fld &f, double *x, double (&hb)[HOLD], const double *gb, int64_t gbase,
                                           const DevPlan *__restrict__ P, int s, int s1, const TileGeo &g) {
     const StageU a = stage_u(&P->st[s]);
     if (s1 - s == 2) {
         const StageU b = stage_u(&P->st[s + 1]);
         const bool lin = (((a.q | b.q) << g.logC) & 15) == 0;
         if (b.m == 2) {
-            if (lin) last42<K, true, HOLDB, SRC>(x, hb, gb, gbase, a, b, g);
-            else last42<K, false, HOLDB, SRC>(x, hb, gb, gbase, a, b, g);
+            if (lin) last42<true, HOLDB, SRC>(f, x, hb, gb, gbase, a, b, g);
+            else last42<false, HOLDB, SRC>(f, x, hb, gb, gbase, a, b, g);
         } else {
-            if (lin) last44<K, true, HOLDB, SRC>(x, hb, gb, gbase, a, b, g);
-            else last44<K, false, HOLDB, SRC>(x, hb, gb, gbase, a, b, g);
+            if (lin) last44<true, HOLDB, SRC>(f, x, hb, gb, gbase, a, b, g);
+            else last44<false, HOLDB, SRC>(f, x, hb, gb, gbase, a, b, g);
         }
         return;
     }
     const bool lin = ((a.q << g.logC) & 15) == 0;
     if (a.m == 2) {
-        lin ? last1<K, 2, true, HOLDB, SRC>(x, hb, gb, gbase, a, g) : last1<K, 2, false, HOLDB, SRC>(x, hb, gb, gbase, a, g);
+        lin ? last1<2, true, HOLDB, SRC>(f, x, hb, gb, gbase, a, g) : last1<2, false, HOLDB, SRC>(f, x, hb, gb, gbase, a, g);
     } else {
-        lin ? last1<K, 4, true, HOLDB, SRC>(x, hb, gb, gbase, a, g) : last1<K, 4, false, HOLDB, SRC>(x, hb, gb, gbase, a, g);
+        lin ? last1<4, true, HOLDB, SRC>(f, x, hb, gb, gbase, a, g) : last1<4, false, HOLDB, SRC>(f, x, hb, gb, gbase, a, g);
     }
 }
 
@@ -1034,8 +1030,8 @@ __device__ __forceinline__ bool edge_ok(const DevPlan *__restrict__ P, int s0, i
 
 // Loads the thread's 16 points from src (rounds of NU), runs stages s0, s0+1
 // forward on them and writes them to the LDS tile.
-template <int K, int NU>
-__device__ __forceinline__ void edge_load44(double *__restrict__ lds, const PointSrc &src, bool pack,
+template <int NU>
+__device__ __forceinline__ void edge_load44(const Fld &f, double *__restrict__ lds, const PointSrc &src, bool pack,
                                             const TileGeo &g, int64_t gbase, const DevPlan *__restrict__ P, int s0) {
     static_assert(16 % NU == 0, "rounds of NU points cover the 16");
     const int T = blockDim.x;
@@ -1053,7 +1049,7 @@ __device__ __forceinline__ void edge_load44(double *__restrict__ lds, const Poin
     const Sub44 u = sub44((int)threadIdx.x, sa, sb, g);
     double w0[4][3], w1[3];
     tw44<false>(w0, w1, u, sa, sb);
-    regs44<K, true>(y, w0, w1, sa, sb);
+    regs44<true>(f, y, w0, w1, sa, sb);
     if ((((sa.q | sb.q) << g.logC) & 15) == 0) {
 #pragma unroll
         for (int a = 0; a < 4; a++)
@@ -1069,8 +1065,7 @@ __device__ __forceinline__ void edge_load44(double *__restrict__ lds, const Poin
 
 // Reads the thread's pass44 sub-problem of stages s0, s0+1 from the LDS tile,
 // runs the pair inverse and stores the 16 points to x.
-template <int K>
-__device__ __forceinline__ void edge_store44(double *__restrict__ x, const double *__restrict__ lds,
+__device__ __forceinline__ void edge_store44(const Fld &f, double *__restrict__ x, const double *__restrict__ lds,
                                              const TileGeo &g, int64_t gbase, const DevPlan *__restrict__ P, int s0) {
     const int T = blockDim.x;
     const int e0 = threadIdx.x;
@@ -1090,7 +1085,7 @@ __device__ __forceinline__ void edge_store44(double *__restrict__ x, const doubl
     }
     double w0[4][3], w1[3];
     tw44<false>(w0, w1, u, sa, sb);
-    regs44<K, false>(y, w0, w1, sa, sb);
+    regs44<false>(f, y, w0, w1, sa, sb);
     const int cmask = (1 << g.logC) - 1;
     gdouble *__restrict__ p = (gdouble *)x + (gbase + (int64_t)(e0 >> g.logC) * g.Lb + (e0 & cmask));
     const int64_t step = (int64_t)(T >> g.logC) * g.Lb;
@@ -1101,10 +1096,10 @@ __device__ __forceinline__ void edge_store44(double *__restrict__ x, const doubl
 // ---------------------------------------------------------------------------
 // Group pass: one tile of one group of one transform, forward or inverse.
 
-template <int K, int NU>
-__device__ __forceinline__ void group_body(const DevJob *__restrict__ Jp, const Tile &t, bool fwd, double *lds) {
+template <int NU>
+__device__ __forceinline__ void group_body(const Fld &f, const DevJob *__restrict__ Jp, const Tile &t, bool fwd, double *lds) {
     const auto *J = cst(Jp);
-    const DevPlan *__restrict__ P = unip(J->plan[K]);
+    const DevPlan *__restrict__ P = unip(J->plan[t.prime]);
     const auto *G = &cst(P)->g[t.group];
     TileGeo g;
     g.S = uni(G->S);
@@ -1112,7 +1107,7 @@ __device__ __forceinline__ void group_body(const DevJob *__restrict__ Jp, const 
     g.Lb = uni64(G->Lb);
     const int64_t La = uni64(G->La);
     const int s0 = uni(G->s0), s1 = uni(G->s1);
-    double *__restrict__ x = unip(t.op ? J->xb[K] : J->xa[K]);
+    double *__restrict__ x = unip(t.op ? J->xb[t.prime] : J->xa[t.prime]);
     const int64_t tpr = g.Lb >> g.logC;
     const int64_t blk = (int64_t)t.tile / tpr;
     const int64_t jc = (int64_t)t.tile - blk * tpr;
@@ -1132,23 +1127,23 @@ __device__ __forceinline__ void group_body(const DevJob *__restrict__ Jp, const 
     const bool edge = edge_ok(P, s0, s1, total, 2);
     if (fwd) {
         if (edge) {
-            edge_load44<K, (NU < 8 ? NU : 8)>(lds, src, pack, g, gbase, P, s0);
+            edge_load44<(NU < 8 ? NU : 8)>(f, lds, src, pack, g, gbase, P, s0);
             __syncthreads();
-            run_stages<K, true>(lds, nullptr, 1, P, s0 + 2, s1, g);
+            run_stages<true>(f, lds, nullptr, 1, P, s0 + 2, s1, g);
         } else {
             load_tile<NU>(lds, src, pack, total, g, gbase);
             __syncthreads();
-            run_stages<K, true>(lds, nullptr, 1, P, s0, s1, g);
+            run_stages<true>(f, lds, nullptr, 1, P, s0, s1, g);
         }
         store_tile(x, lds, total, g, gbase);
     } else {
         load_tile<NU>(lds, src, pack, total, g, gbase);
         __syncthreads();
         if (edge) {
-            run_stages<K, false>(lds, nullptr, 1, P, s0 + 2, s1, g);
-            edge_store44<K>(x, lds, g, gbase, P, s0);
+            run_stages<false>(f, lds, nullptr, 1, P, s0 + 2, s1, g);
+            edge_store44(f, x, lds, g, gbase, P, s0);
         } else {
-            run_stages<K, false>(lds, nullptr, 1, P, s0, s1, g);
+            run_stages<false>(f, lds, nullptr, 1, P, s0, s1, g);
             store_tile(x, lds, total, g, gbase);
         }
     }
@@ -1163,8 +1158,7 @@ __device__ __forceinline__ void group_kernel(const DevJob *__restrict__ jobs, co
     const Tile t = fetch_tile(tiles, gen);
     const DevJob *J = jobs + t.job;
     if (uni(cst(J)->zero)) return;
-    if (t.prime == 0) group_body<0, NU>(J, t, FWD, lds);
-    else group_body<1, NU>(J, t, FWD, lds);
+    group_body<NU>(fld(t.prime), J, t, FWD, lds);
 }
 
 __global__ void __launch_bounds__(1024) k_group_fwd(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles,
@@ -1213,8 +1207,8 @@ k_group_inv_w6(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles, 
 // One stage of radix M on the column's S points: legs STRIDE rows apart in
 // blocks of STRIDE*M rows; the butterfly at row jr of a block has twiddle
 // index jr*Lb + q (q = the column), as in pass1.
-template <int K, bool FWD, int M, int STRIDE, int S>
-__device__ __forceinline__ void col_stage(double (&y)[S], const StageU &st, int64_t Lb, int64_t q) {
+template <bool FWD, int M, int STRIDE, int S>
+__device__ __forceinline__ void col_stage(const Fld &f, double (&y)[S], const StageU &st, int64_t Lb, int64_t q) {
     static_assert(S % (STRIDE * M) == 0, "stage blocks tile the column");
 #pragma unroll
     for (int b = 0; b < S / (STRIDE * M); b++)
@@ -1226,17 +1220,17 @@ __device__ __forceinline__ void col_stage(double (&y)[S], const StageU &st, int6
             for (int t = 1; t < M; t++) w[t - 1] = gld(st.tw, (t - 1) * st.Q + j);
 #pragma unroll
             for (int t = 0; t < M; t++) v[t] = y[b * STRIDE * M + jr + t * STRIDE];
-            stage_regs<K, FWD, M>(v, w, st);
+            stage_regs<FWD, M>(f, v, w, st);
 #pragma unroll
             for (int t = 0; t < M; t++) y[b * STRIDE * M + jr + t * STRIDE] = v[t];
         }
 }
 
-template <int K, bool FWD, int M0, int M1, int M2>
-__device__ void col_body(const DevJob *__restrict__ Jp, const Tile &t) {
+template <bool FWD, int M0, int M1, int M2>
+__device__ void col_body(const Fld &f, const DevJob *__restrict__ Jp, const Tile &t) {
     constexpr int S = M0 * (M1 ? M1 : 1) * (M2 ? M2 : 1);
     const auto *J = cst(Jp);
-    const DevPlan *__restrict__ P = unip(J->plan[K]);
+    const DevPlan *__restrict__ P = unip(J->plan[t.prime]);
     const auto *G = &cst(P)->g[t.group];
     const int64_t La = uni64(G->La), Lb = uni64(G->Lb);
     const int logC = uni(G->logC);
@@ -1246,7 +1240,7 @@ __device__ void col_body(const DevJob *__restrict__ Jp, const Tile &t) {
     const int64_t jc = (int64_t)t.tile - blk * tpr;
     const int64_t q = (jc << logC) + threadIdx.x;
     const int64_t base = blk * La + q;
-    double *__restrict__ x = unip(t.op ? J->xb[K] : J->xa[K]);
+    double *__restrict__ x = unip(t.op ? J->xb[t.prime] : J->xa[t.prime]);
     double y[S];
     if ((t.flags & TILE_PACK) != 0) {
         const uint64_t *limbs = unip(t.op ? J->b : J->a);
@@ -1271,40 +1265,40 @@ __device__ void col_body(const DevJob *__restrict__ Jp, const Tile &t) {
         for (int k = 0; k < S; k++) y[k] = gld(x, base + k * Lb);
     }
     if (FWD) {
-        col_stage<K, true, M0, S / M0>(y, stage_u(&P->st[s0]), Lb, q);
-        if constexpr (M1 != 0) col_stage<K, true, M1, S / (M0 * M1)>(y, stage_u(&P->st[s0 + 1]), Lb, q);
-        if constexpr (M2 != 0) col_stage<K, true, M2, 1>(y, stage_u(&P->st[s0 + 2]), Lb, q);
+        col_stage<true, M0, S / M0>(f, y, stage_u(&P->st[s0]), Lb, q);
+        if constexpr (M1 != 0) col_stage<true, M1, S / (M0 * M1)>(f, y, stage_u(&P->st[s0 + 1]), Lb, q);
+        if constexpr (M2 != 0) col_stage<true, M2, 1>(f, y, stage_u(&P->st[s0 + 2]), Lb, q);
     } else {
-        if constexpr (M2 != 0) col_stage<K, false, M2, 1>(y, stage_u(&P->st[s0 + 2]), Lb, q);
-        if constexpr (M1 != 0) col_stage<K, false, M1, S / (M0 * M1)>(y, stage_u(&P->st[s0 + 1]), Lb, q);
-        col_stage<K, false, M0, S / M0>(y, stage_u(&P->st[s0]), Lb, q);
+        if constexpr (M2 != 0) col_stage<false, M2, 1>(f, y, stage_u(&P->st[s0 + 2]), Lb, q);
+        if constexpr (M1 != 0) col_stage<false, M1, S / (M0 * M1)>(f, y, stage_u(&P->st[s0 + 1]), Lb, q);
+        col_stage<false, M0, S / M0>(f, y, stage_u(&P->st[s0]), Lb, q);
     }
 #pragma unroll
     for (int k = 0; k < S; k++) gst(x, base + k * Lb, y[k]);
 }
 
 // The group shapes a column kernel takes (GroupDesc::col = M0 | M1 << 4 | M2 << 8).
-template <int K, bool FWD, bool BIG>
-__device__ __forceinline__ void col_switch(const DevJob *J, const Tile &t, int code) {
+template <bool FWD, bool BIG>
+__device__ __forceinline__ void col_switch(const Fld &f, const DevJob *J, const Tile &t, int code) {
     if (!BIG) {
         switch (code) {
-        case 0x3: col_body<K, FWD, 3, 0, 0>(J, t); break;
-        case 0x4: col_body<K, FWD, 4, 0, 0>(J, t); break;
-        case 0x5: col_body<K, FWD, 5, 0, 0>(J, t); break;
-        case 0x7: col_body<K, FWD, 7, 0, 0>(J, t); break;
-        case 0x33: col_body<K, FWD, 3, 3, 0>(J, t); break;
-        case 0x43: col_body<K, FWD, 3, 4, 0>(J, t); break;
-        case 0x35: col_body<K, FWD, 5, 3, 0>(J, t); break;
-        case 0x44: col_body<K, FWD, 4, 4, 0>(J, t); break;
+        case 0x3: col_body<FWD, 3, 0, 0>(f, J, t); break;
+        case 0x4: col_body<FWD, 4, 0, 0>(f, J, t); break;
+        case 0x5: col_body<FWD, 5, 0, 0>(f, J, t); break;
+        case 0x7: col_body<FWD, 7, 0, 0>(f, J, t); break;
+        case 0x33: col_body<FWD, 3, 3, 0>(f, J, t); break;
+        case 0x43: col_body<FWD, 3, 4, 0>(f, J, t); break;
+        case 0x35: col_body<FWD, 5, 3, 0>(f, J, t); break;
+        case 0x44: col_body<FWD, 4, 4, 0>(f, J, t); break;
         default: break;
         }
     } else {
         switch (code) {
-        case 0x45: col_body<K, FWD, 5, 4, 0>(J, t); break;
-        case 0x37: col_body<K, FWD, 7, 3, 0>(J, t); break;
-        case 0x47: col_body<K, FWD, 7, 4, 0>(J, t); break;
-        case 0x57: col_body<K, FWD, 7, 5, 0>(J, t); break;
-        case 0x433: col_body<K, FWD, 3, 3, 4>(J, t); break;
+        case 0x45: col_body<FWD, 5, 4, 0>(f, J, t); break;
+        case 0x37: col_body<FWD, 7, 3, 0>(f, J, t); break;
+        case 0x47: col_body<FWD, 7, 4, 0>(f, J, t); break;
+        case 0x57: col_body<FWD, 7, 5, 0>(f, J, t); break;
+        case 0x433: col_body<FWD, 3, 3, 4>(f, J, t); break;
         default: break;
         }
     }
@@ -1318,8 +1312,7 @@ __device__ __forceinline__ void col_kernel(const DevJob *__restrict__ jobs, cons
     if (uni(cst(J)->zero)) return;
     const DevPlan *P = unip(cst(J)->plan[t.prime ? 1 : 0]);
     const int code = uni(cst(P)->g[t.group].col);
-    if (t.prime == 0) col_switch<0, FWD, BIG>(J, t, code);
-    else col_switch<1, FWD, BIG>(J, t, code);
+    col_switch<FWD, BIG>(fld(t.prime), J, t, code);
 }
 
 // S <= 16: at most 32 VGPRs of points, six waves per SIMD
@@ -1348,10 +1341,9 @@ k_col_inv_b(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles, Til
 // job whose b is a cached spectrum (bspec) reads b's finished forward
 // spectrum and transforms a only.
 
-template <int K>
-__device__ __forceinline__ void mid_body(const DevJob *__restrict__ Jp, const Tile &t, double *lds) {
+__device__ __forceinline__ void mid_body(const Fld &f, const DevJob *__restrict__ Jp, const Tile &t, double *lds) {
     const auto *J = cst(Jp);
-    const DevPlan *__restrict__ P = unip(J->plan[K]);
+    const DevPlan *__restrict__ P = unip(J->plan[t.prime]);
     const auto *G = &cst(P)->g[uni(cst(P)->ngroups) - 1];
     TileGeo g;
     g.S = uni(G->S);
@@ -1366,8 +1358,8 @@ __device__ __forceinline__ void mid_body(const DevJob *__restrict__ Jp, const Ti
     double *la = lds;  // the one LDS array
     const bool pack = (t.flags & TILE_PACK) != 0;
     PointSrc sa, sb;
-    sa.x = unip(J->xa[K]);
-    sb.x = unip(J->xb[K]);
+    sa.x = unip(J->xa[t.prime]);
+    sb.x = unip(J->xb[t.prime]);
     sa.limbs = unip(J->a);
     sb.limbs = unip(J->b);
     sa.n = uni64(J->m);
@@ -1383,29 +1375,29 @@ __device__ __forceinline__ void mid_body(const DevJob *__restrict__ Jp, const Ti
     if (!sq && !bspec) {
         // b first: its forward passes, then its last pass held in registers
         if (edge) {
-            edge_load44<K, 8>(la, sb, pack, g, gbase, P, s0);
+            edge_load44<8>(f, la, sb, pack, g, gbase, P, s0);
         } else {
             load_tile<LU>(la, sb, pack, S, g, gbase);
         }
         __syncthreads();
-        const int sl = fwd_until_last<K>(la, P, sf, s1, g);
-        last_pass<K, true, SRC_HELD>(la, hb, nullptr, 0, P, sl, s1, g);
+        const int sl = fwd_until_last(f, la, P, sf, s1, g);
+        last_pass<true, SRC_HELD>(f, la, hb, nullptr, 0, P, sl, s1, g);
         __syncthreads();
     }
     if (edge) {
-        edge_load44<K, 8>(la, sa, pack, g, gbase, P, s0);
+        edge_load44<8>(f, la, sa, pack, g, gbase, P, s0);
     } else {
         load_tile<LU>(la, sa, pack, S, g, gbase);
     }
     __syncthreads();
-    const int sl = fwd_until_last<K>(la, P, sf, s1, g);
-    if (sq) last_pass<K, false, SRC_SELF>(la, hb, nullptr, 0, P, sl, s1, g);
-    else if (bspec) last_pass<K, false, SRC_GLOB>(la, hb, sb.x, gbase, P, sl, s1, g);
-    else last_pass<K, false, SRC_HELD>(la, hb, nullptr, 0, P, sl, s1, g);
+    const int sl = fwd_until_last(f, la, P, sf, s1, g);
+    if (sq) last_pass<false, SRC_SELF>(f, la, hb, nullptr, 0, P, sl, s1, g);
+    else if (bspec) last_pass<false, SRC_GLOB>(f, la, hb, sb.x, gbase, P, sl, s1, g);
+    else last_pass<false, SRC_HELD>(f, la, hb, nullptr, 0, P, sl, s1, g);
     __syncthreads();
-    run_stages<K, false>(la, nullptr, 1, P, sf, sl, g);
-    double *__restrict__ xo = unip(J->xa[K]);
-    if (edge) edge_store44<K>(xo, la, g, gbase, P, s0);
+    run_stages<false>(f, la, nullptr, 1, P, sf, sl, g);
+    double *__restrict__ xo = unip(J->xa[t.prime]);
+    if (edge) edge_store44(f, xo, la, g, gbase, P, s0);
     else store_tile(xo, la, S, g, gbase);
 }
 
@@ -1415,8 +1407,7 @@ __global__ void __launch_bounds__(MID_MAX_THREADS) k_mid(const DevJob *__restric
     const Tile t = fetch_tile(tiles, gen);
     const DevJob *J = jobs + t.job;
     if (uni(cst(J)->zero)) return;
-    if (t.prime == 0) mid_body<0>(J, t, lds);
-    else mid_body<1>(J, t, lds);
+    mid_body(fld(t.prime), J, t, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -1770,8 +1761,8 @@ __device__ __forceinline__ StageU small_stage(const StageDesc *sd, const double 
 }
 
 // Radix-4 pair (stages s0, s1) over narr arrays, by the lt-th of H threads.
-template <int K, bool FWD, bool LIN>
-__device__ void small_pass44(double *__restrict__ x0, double *__restrict__ x1, int narr, const StageU &s0,
+template <bool FWD, bool LIN>
+__device__ void small_pass44(const Fld &f, double *__restrict__ x0, double *__restrict__ x1, int narr, const StageU &s0,
                              const StageU &s1, int N, int lt, int H) {
     TileGeo g;
     g.S = N;
@@ -1794,7 +1785,7 @@ __device__ void small_pass44(double *__restrict__ x0, double *__restrict__ x1, i
         for (int a = 0; a < 4; a++)
 #pragma unroll
             for (int b = 0; b < 4; b++) y[a][b] = x[ix44<LIN>(u, a, b)];
-        regs44<K, FWD>(y, w0, w1, s0, s1);
+        regs44<FWD>(f, y, w0, w1, s0, s1);
 #pragma unroll
         for (int a = 0; a < 4; a++)
 #pragma unroll
@@ -1803,8 +1794,8 @@ __device__ void small_pass44(double *__restrict__ x0, double *__restrict__ x1, i
 }
 
 // One radix-M stage over narr arrays, by the lt-th of H threads.
-template <int K, bool FWD, int M>
-__device__ void small_pass1(double *__restrict__ x0, double *__restrict__ x1, int narr, const StageU &s, int N,
+template <bool FWD, int M>
+__device__ void small_pass1(const Fld &f, double *__restrict__ x0, double *__restrict__ x1, int narr, const StageU &s, int N,
                             int lt, int H) {
     TileGeo g;
     g.S = N;
@@ -1823,29 +1814,29 @@ __device__ void small_pass1(double *__restrict__ x0, double *__restrict__ x1, in
         double y[M];
 #pragma unroll
         for (int t = 0; t < M; t++) y[t] = x[ix1<false>(u, t)];
-        stage_regs<K, FWD, M>(y, w, s);
+        stage_regs<FWD, M>(f, y, w, s);
 #pragma unroll
         for (int t = 0; t < M; t++) x[ix1<false>(u, t)] = y[t];
     }
 }
 
 // Stages [s, s + (pair ? 2 : 1)) of prime K's plan over its narr arrays.
-template <int K, bool FWD>
-__device__ __forceinline__ void small_pass(double *x0, double *x1, int narr, const StageDesc *st, int s, bool pair,
+template <bool FWD>
+__device__ __forceinline__ void small_pass(const Fld &f, double *x0, double *x1, int narr, const StageDesc *st, int s, bool pair,
                                            const double *twl, int N, int lt, int H) {
     const StageU a = small_stage(&st[s], twl);
     if (pair) {
         const StageU b = small_stage(&st[s + 1], twl);
-        if (((a.q | b.q) & 15) == 0) small_pass44<K, FWD, true>(x0, x1, narr, a, b, N, lt, H);
-        else small_pass44<K, FWD, false>(x0, x1, narr, a, b, N, lt, H);
+        if (((a.q | b.q) & 15) == 0) small_pass44<FWD, true>(f, x0, x1, narr, a, b, N, lt, H);
+        else small_pass44<FWD, false>(f, x0, x1, narr, a, b, N, lt, H);
         return;
     }
     switch (a.m) {
-    case 4: small_pass1<K, FWD, 4>(x0, x1, narr, a, N, lt, H); break;
-    case 2: small_pass1<K, FWD, 2>(x0, x1, narr, a, N, lt, H); break;
-    case 3: small_pass1<K, FWD, 3>(x0, x1, narr, a, N, lt, H); break;
-    case 5: small_pass1<K, FWD, 5>(x0, x1, narr, a, N, lt, H); break;
-    default: small_pass1<K, FWD, 7>(x0, x1, narr, a, N, lt, H); break;
+    case 4: small_pass1<FWD, 4>(f, x0, x1, narr, a, N, lt, H); break;
+    case 2: small_pass1<FWD, 2>(f, x0, x1, narr, a, N, lt, H); break;
+    case 3: small_pass1<FWD, 3>(f, x0, x1, narr, a, N, lt, H); break;
+    case 5: small_pass1<FWD, 5>(f, x0, x1, narr, a, N, lt, H); break;
+    default: small_pass1<FWD, 7>(f, x0, x1, narr, a, N, lt, H); break;
     }
 }
 
@@ -1868,6 +1859,7 @@ __global__ void __launch_bounds__(SMALL_MAX_THREADS) k_small(const DevJob J, int
     __shared__ StageDesc sst[2][SMALL_MAX_STAGES];
     const int tid = threadIdx.x, NT = blockDim.x, H = NT >> 1;
     const int k = uni(tid >= H ? 1 : 0);  // prime of this thread's half (H is a multiple of 64)
+    const Fld fk = fld(k);
     const int lt = tid - k * H;
     const DevPlan *__restrict__ P0 = unip(J.plan[0]);
     const DevPlan *__restrict__ P1 = unip(J.plan[1]);
@@ -1933,8 +1925,7 @@ __global__ void __launch_bounds__(SMALL_MAX_THREADS) k_small(const DevJob J, int
     // forward transforms of a and b (radix-4 pairs from the front)
     for (int s = 0; s < nst;) {
         const bool pair = !single && uni(sst[0][s].m) == 4 && s + 1 < nst && uni(sst[0][s + 1].m) == 4;
-        if (k) small_pass<1, true>(xa, xb, narr, stk, s, pair, twk, N, lt, H);
-        else small_pass<0, true>(xa, xb, narr, stk, s, pair, twk, N, lt, H);
+        small_pass<true>(fk, xa, xb, narr, stk, s, pair, twk, N, lt, H);
         __syncthreads();
         s += pair ? 2 : 1;
     }
@@ -1943,7 +1934,7 @@ __global__ void __launch_bounds__(SMALL_MAX_THREADS) k_small(const DevJob J, int
         const double *__restrict__ yb = sq ? xa : xb;
         for (int i = lt; i < N; i += H) {
             const int p = pidx(i);
-            xa[p] = k ? fp_mulmod3<1>(xa[p], yb[p]) : fp_mulmod3<0>(xa[p], yb[p]);
+            xa[p] = fp_mulmod3(fk, xa[p], yb[p]);
         }
     }
     __syncthreads();
@@ -1951,8 +1942,7 @@ __global__ void __launch_bounds__(SMALL_MAX_THREADS) k_small(const DevJob J, int
     for (int s = nst - 1; s >= 0;) {
         const bool pair = !single && uni(sst[0][s].m) == 4 && s - 1 >= 0 && uni(sst[0][s - 1].m) == 4;
         const int s0 = pair ? s - 1 : s;
-        if (k) small_pass<1, false>(xa, nullptr, 1, stk, s0, pair, twk, N, lt, H);
-        else small_pass<0, false>(xa, nullptr, 1, stk, s0, pair, twk, N, lt, H);
+        small_pass<false>(fk, xa, nullptr, 1, stk, s0, pair, twk, N, lt, H);
         __syncthreads();
         s -= pair ? 2 : 1;
     }
