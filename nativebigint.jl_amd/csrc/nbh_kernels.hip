@@ -767,28 +767,21 @@ __device__ void last42(const Fld &f, double *__restrict__ x, double (&hb)[HOLD],
     }
 }
 
+// The mid tile's last pass: its last one or two stages, which are the
+// transform's last, so their strides are below 16 and the plain padded index
+// serves (no linear-index variant).
 template <bool HOLDB, int SRC>
 __device__ __forceinline__ void last_pass(const Fld &f, double *x, double (&hb)[HOLD], const double *gb, int64_t gbase,
                                           const DevPlan *__restrict__ P, int s, int s1, const TileGeo &g) {
     const StageU a = stage_u(&P->st[s]);
     if (s1 - s == 2) {
         const StageU b = stage_u(&P->st[s + 1]);
-        const bool lin = (((a.q | b.q) << g.logC) & 15) == 0;
-        if (b.m == 2) {
-            if (lin) last42<true, HOLDB, SRC>(f, x, hb, gb, gbase, a, b, g);
-            else last42<false, HOLDB, SRC>(f, x, hb, gb, gbase, a, b, g);
-        } else {
-            if (lin) last44<true, HOLDB, SRC>(f, x, hb, gb, gbase, a, b, g);
-            else last44<false, HOLDB, SRC>(f, x, hb, gb, gbase, a, b, g);
-        }
+        if (b.m == 2) last42<false, HOLDB, SRC>(f, x, hb, gb, gbase, a, b, g);
+        else last44<false, HOLDB, SRC>(f, x, hb, gb, gbase, a, b, g);
         return;
     }
-    const bool lin = ((a.q << g.logC) & 15) == 0;
-    if (a.m == 2) {
-        lin ? last1<2, true, HOLDB, SRC>(f, x, hb, gb, gbase, a, g) : last1<2, false, HOLDB, SRC>(f, x, hb, gb, gbase, a, g);
-    } else {
-        lin ? last1<4, true, HOLDB, SRC>(f, x, hb, gb, gbase, a, g) : last1<4, false, HOLDB, SRC>(f, x, hb, gb, gbase, a, g);
-    }
+    if (a.m == 2) last1<2, false, HOLDB, SRC>(f, x, hb, gb, gbase, a, g);
+    else last1<4, false, HOLDB, SRC>(f, x, hb, gb, gbase, a, g);
 }
 
 // Points each thread moves per round in the tile load loops.
