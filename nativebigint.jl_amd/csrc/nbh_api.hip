@@ -222,7 +222,8 @@ struct Tuning {
     int cmax = 64;
 };
 
-// Stage list and device twiddle tables of one (N, prime); schedule-free.
+// Stage list of one (N, prime), its stages pointing at the shared twiddle
+// tables (plus the contiguous copy for short lengths); schedule-free.
 struct StageList {
     int64_t N = 0, N2 = 0;
     int prime = 0;
