@@ -199,7 +199,12 @@ struct TileGeo {
 // fp_radix4! sweeps, one LDS round trip instead of two.
 // LIN: both leg strides are multiples of 16 points, so the padded index of
 // leg (a, b) is pidx(base) + a*pidx-stride(q1) + b*pidx-stride(q0), one add
-// per point instead of a pidx per point.
+// per point instead of a pidx per point.  Otherwise the padded offsets still
+// add: pidx(base + a sa + b sb) = pidx(base) + oa[a] + ob[b] with
+// oa[a] = pidx(base + a sa) - pidx(base) and ob[b] likewise, because q1 is a
+// power of two, sb = 4 sa and base mod 16 < sa whenever sa < 16, so the a and
+// b offsets never cross a 16-point boundary together: six pidx per
+// sub-problem instead of sixteen.
 template <bool FWD, bool LIN>
 __device__ void pass44(const Fld &f, double *__restrict__ x0, double *__restrict__ x1, int narr, const StageU &s0,
                        const StageU &s1, const TileGeo &g) {
@@ -219,7 +224,13 @@ __device__ void pass44(const Fld &f, double *__restrict__ x0, double *__restrict
         const int p0 = pidx(base);
         const int sa = q1 << g.logC, sb = q0 << g.logC;
         const int pa = sa + (sa >> 4), pb = sb + (sb >> 4);
-        auto ix = [&](int a, int b) { return LIN ? p0 + a * pa + b * pb : pidx(base + a * sa + b * sb); };
+        int oa[4], ob[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            oa[k] = pidx(base + k * sa) - p0;
+            ob[k] = pidx(base + k * sb) - p0;
+        }
+        auto ix = [&](int a, int b) { return LIN ? p0 + a * pa + b * pb : p0 + oa[a] + ob[b]; };
         double y[4][4];
 #pragma unroll
         for (int a = 0; a < 4; a++)
@@ -431,6 +442,7 @@ __device__ __forceinline__ void load_tw(double *w, const StageU &s, int64_t j) {
 // Index geometry of a pass44 sub-problem (see pass44).
 struct Sub44 {
     int p0, pa, pb, base, sa, sb;
+    int oa[4], ob[4];  // padded leg offsets (non-LIN form, see pass44)
     int64_t jt;
 };
 
@@ -448,13 +460,18 @@ __device__ __forceinline__ Sub44 sub44(int sl, const StageU &s0, const StageU &s
     u.sb = q0 << g.logC;
     u.pa = u.sa + (u.sa >> 4);
     u.pb = u.sb + (u.sb >> 4);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        u.oa[k] = pidx(u.base + k * u.sa) - u.p0;
+        u.ob[k] = pidx(u.base + k * u.sb) - u.p0;
+    }
     u.jt = (int64_t)jj * g.Lb + g.jcol0 + c;
     return u;
 }
 
 template <bool LIN>
 __device__ __forceinline__ int ix44(const Sub44 &u, int a, int b) {
-    return LIN ? u.p0 + a * u.pa + b * u.pb : pidx(u.base + a * u.sa + b * u.sb);
+    return LIN ? u.p0 + a * u.pa + b * u.pb : u.p0 + u.oa[a] + u.ob[b];
 }
 
 // Forward stages s0 (over b) and s1 (over a) of one sub-problem on registers.
@@ -621,6 +638,17 @@ __device__ int fwd_until_last(const Fld &f, double *x, const DevPlan *__restrict
     }
 }
 
+// The mid tile's last pass holds the transform's last stages on a
+// contiguous tile (one column): the last stage's legs are adjacent (q = 1)
+// and the stage before it has q = 4 (radix 4) or 2 (radix 2 last), so a
+// sub-problem's points are one run starting at a multiple of its size, which
+// never crosses a pad: constant offsets from pidx(base), which the compiler
+// folds into the LDS instructions' offset fields.  LIN = false is that form.
+template <bool LIN>
+__device__ __forceinline__ int ix_last44(const Sub44 &u, int a, int b) {
+    return LIN ? ix44<true>(u, a, b) : u.p0 + a + 4 * b;
+}
+
 // Last pass = a radix-4 pair.  HOLDB: compute x's last forward pass into hb
 // (no store).  Otherwise: the fused pass on x with b from SRC.
 template <bool LIN, bool HOLDB, int SRC>
@@ -634,7 +662,7 @@ __device__ void last44(const Fld &f, double *__restrict__ x, double (&hb)[HOLD],
 #pragma unroll
     for (int a = 0; a < 4; a++)
 #pragma unroll
-        for (int b = 0; b < 4; b++) y[a][b] = x[ix44<LIN>(u, a, b)];
+        for (int b = 0; b < 4; b++) y[a][b] = x[ix_last44<LIN>(u, a, b)];
     // s1 is the transform's last stage: its twiddles are all 1
     double w0[4][3], w1[3];
     tw44<LIN, true>(w0, w1, u, s0, s1);
@@ -659,7 +687,7 @@ __device__ void last44(const Fld &f, double *__restrict__ x, double (&hb)[HOLD],
 #pragma unroll
     for (int a = 0; a < 4; a++)
 #pragma unroll
-        for (int b = 0; b < 4; b++) x[ix44<LIN>(u, a, b)] = y[a][b];
+        for (int b = 0; b < 4; b++) x[ix_last44<LIN>(u, a, b)] = y[a][b];
 }
 
 // Last pass = one radix-M stage: the thread's butterflies beta = t + it*T.
@@ -677,8 +705,11 @@ __device__ void last1(const Fld &f, double *__restrict__ x, double (&hb)[HOLD], 
         // s is the transform's last stage: its twiddles are all 1 (w unread)
         const double w[M > 1 ? M - 1 : 1] = {};
         double y[M];
+        // not LIN: the mid tile's last stage, q = 1, M-point runs at
+        // multiples of M (see ix_last44)
+        auto ix = [&](int t) { return LIN ? ix1<true>(u, t) : u.p0 + t; };
 #pragma unroll
-        for (int t = 0; t < M; t++) y[t] = x[ix1<LIN>(u, t)];
+        for (int t = 0; t < M; t++) y[t] = x[ix(t)];
         stage_regs<true, M, true>(f, y, w, s);
         if (HOLDB) {
 #pragma unroll
@@ -694,7 +725,7 @@ __device__ void last1(const Fld &f, double *__restrict__ x, double (&hb)[HOLD], 
         }
         stage_regs<false, M, true>(f, y, w, s);
 #pragma unroll
-        for (int t = 0; t < M; t++) x[ix1<LIN>(u, t)] = y[t];
+        for (int t = 0; t < M; t++) x[ix(t)] = y[t];
     }
 }
 
@@ -725,7 +756,9 @@ __device__ void last42(const Fld &f, double *__restrict__ x, double (&hb)[HOLD],
         const int row0 = ((r >> logq1) << 3) * q1 + jj;
         const int base = (row0 << g.logC) + c;
         const int p0 = pidx(base);
-        auto ix = [&](int a, int b) { return LIN ? p0 + a * pa + b * pb : pidx(base + a * sa + b * sb); };
+        // not LIN: the mid tile's last pass, legs q1 = 1 and q0 = 2 apart in
+        // 8-point runs at multiples of 8 (see ix_last44)
+        auto ix = [&](int a, int b) { return LIN ? p0 + a * pa + b * pb : p0 + a + 2 * b; };
         const int64_t jt = (int64_t)jj * g.Lb + g.jcol0 + c;
         double y[2][4], w0[2][3];
 #pragma unroll
@@ -776,8 +809,8 @@ __device__ void last42(const Fld &f, double *__restrict__ x, double (&hb)[HOLD],
 }
 
 // The mid tile's last pass: its last one or two stages, which are the
-// transform's last, so their strides are below 16 and the plain padded index
-// serves (no linear-index variant).
+// transform's last, so their strides are below 16: the LIN = false forms,
+// which index unit-stride runs (ix_last44).
 template <bool HOLDB, int SRC>
 __device__ __forceinline__ void last_pass(const Fld &f, double *x, double (&hb)[HOLD], const double *gb, int64_t gbase,
                                           const DevPlan *__restrict__ P, int s, int s1, const TileGeo &g) {
