@@ -140,6 +140,22 @@ __device__ __forceinline__ double gldn(const double *p, int64_t i) {
 __device__ __forceinline__ void gstn(double *p, int64_t i, double v) { __builtin_nontemporal_store(v, &((gdouble *)p)[i]); }
 __device__ __forceinline__ uint64_t gldu(const uint64_t *p, int64_t i) { return ((gcu64 *)p)[i]; }
 
+// Twiddle at column j of the row that starts rowoff doubles into a stage
+// table.  TW32: the row start is wave-uniform (scalar registers) and j * 8
+// fits 32 bits, so the load takes the scalar-base form with one shared 32-bit
+// lane offset, and no 64-bit address add per load.  The mid tile's passes
+// read columns j < 3 * (tile size), always in range.
+template <bool TW32>
+__device__ __forceinline__ double twld(const double *tw, int64_t rowoff, int64_t j) {
+    if (TW32) {
+        // raw buffer over the table: row start as the scalar offset, column as the lane offset
+        const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc((void *)tw, (short)0, -1, 0x00020000);
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (uint32_t)j << 3, (uint32_t)rowoff << 3, 0);
+        return __builtin_bit_cast(double, v);
+    }
+    return gld(tw, rowoff + j);
+}
+
 // Descriptor records (plans, jobs, tiles) are read-only while any kernel runs;
 // reading them through the constant address space lets the compiler use
 // scalar loads (scalar cache) instead of vector loads after every barrier.
@@ -171,10 +187,11 @@ __device__ __forceinline__ StageU stage_u(const StageDesc *sdp) {
 
 // Radix-4 butterfly with its twiddles (src/fpntt.jl:645-701): forward
 // combines then reduces leg 0 and twiddles legs 1..3; reverse twiddles first.
-template <bool FWD>
+template <bool FWD, bool TW32 = false>
 __device__ __forceinline__ void bfly4(const Fld &f, double &y0, double &y1, double &y2, double &y3, double wi,
-                                      const double *__restrict__ tw, int64_t Q, int64_t j) {
-    const double w1 = gld(tw, j), w2 = gld(tw, Q + j), w3 = gld(tw, 2 * Q + j);
+                                      const double *__restrict__ tw, int64_t Q, int64_t j, int64_t rowoff = 0) {
+    const double w1 = twld<TW32>(tw, rowoff, j), w2 = twld<TW32>(tw, Q + rowoff, j),
+                 w3 = twld<TW32>(tw, 2 * Q + rowoff, j);
     if (FWD) dft4(f, y0, y1, y2, y3, wi);
     y0 = fp_reduce(f, y0);
     y1 = fp_mulmod3(f, y1, w1);
@@ -205,7 +222,7 @@ struct TileGeo {
 // power of two, sb = 4 sa and base mod 16 < sa whenever sa < 16, so the a and
 // b offsets never cross a 16-point boundary together: six pidx per
 // sub-problem instead of sixteen.
-template <bool FWD, bool LIN>
+template <bool FWD, bool LIN, bool TW32 = false>
 __device__ void pass44(const Fld &f, double *__restrict__ x0, double *__restrict__ x1, int narr, const StageU &s0,
                        const StageU &s1, const TileGeo &g) {
     const int q1 = s1.q;
@@ -240,16 +257,16 @@ __device__ void pass44(const Fld &f, double *__restrict__ x0, double *__restrict
         if (FWD) {
 #pragma unroll
             for (int a = 0; a < 4; a++)
-                bfly4<true>(f, y[a][0], y[a][1], y[a][2], y[a][3], s0.rot[0], s0.tw, s0.Q, jt + a * s1.Q);
+                bfly4<true, TW32>(f, y[a][0], y[a][1], y[a][2], y[a][3], s0.rot[0], s0.tw, s0.Q, jt, a * s1.Q);
 #pragma unroll
-            for (int b = 0; b < 4; b++) bfly4<true>(f, y[0][b], y[1][b], y[2][b], y[3][b], s1.rot[0], s1.tw, s1.Q, jt);
+            for (int b = 0; b < 4; b++) bfly4<true, TW32>(f, y[0][b], y[1][b], y[2][b], y[3][b], s1.rot[0], s1.tw, s1.Q, jt);
         } else {
 #pragma unroll
             for (int b = 0; b < 4; b++)
-                bfly4<false>(f, y[0][b], y[1][b], y[2][b], y[3][b], s1.rot[0], s1.tw, s1.Q, jt);
+                bfly4<false, TW32>(f, y[0][b], y[1][b], y[2][b], y[3][b], s1.rot[0], s1.tw, s1.Q, jt);
 #pragma unroll
             for (int a = 0; a < 4; a++)
-                bfly4<false>(f, y[a][0], y[a][1], y[a][2], y[a][3], s0.rot[0], s0.tw, s0.Q, jt + a * s1.Q);
+                bfly4<false, TW32>(f, y[a][0], y[a][1], y[a][2], y[a][3], s0.rot[0], s0.tw, s0.Q, jt, a * s1.Q);
         }
 #pragma unroll
         for (int a = 0; a < 4; a++)
@@ -262,7 +279,7 @@ __device__ void pass44(const Fld &f, double *__restrict__ x0, double *__restrict
 // 494-535, 630-638, 645-701): forward = combine, reduce leg 0, twiddle legs
 // 1..M-1; reverse = reduce + twiddle, then combine (radix 5/7 re-reduce leg
 // 0); radix 2 is twiddle-free and the same both ways.
-template <bool FWD, int M, bool LIN>
+template <bool FWD, int M, bool LIN, bool TW32 = false>
 __device__ void pass1(const Fld &f, double *__restrict__ x0, double *__restrict__ x1, int narr, const StageU &s,
                       const TileGeo &g) {
     const int q = s.q;
@@ -288,9 +305,9 @@ __device__ void pass1(const Fld &f, double *__restrict__ x0, double *__restrict_
             y[0] = fp_reduce(f, u + v);
             y[1] = fp_reduce(f, u - v);
         } else if constexpr (M == 4) {
-            bfly4<FWD>(f, y[0], y[1], y[2], y[3], s.rot[0], s.tw, s.Q, j);
+            bfly4<FWD, TW32>(f, y[0], y[1], y[2], y[3], s.rot[0], s.tw, s.Q, j);
         } else if constexpr (M == 3) {
-            const double w1 = gld(s.tw, j), w2 = gld(s.tw, s.Q + j);
+            const double w1 = twld<TW32>(s.tw, 0, j), w2 = twld<TW32>(s.tw, s.Q, j);
             if (FWD) dft3(f, y[0], y[1], y[2], s.rot[0]);
             y[0] = fp_reduce(f, y[0]);
             y[1] = fp_mulmod3(f, y[1], w1);
@@ -300,7 +317,7 @@ __device__ void pass1(const Fld &f, double *__restrict__ x0, double *__restrict_
             if (FWD) dft5(f, y, s.rot);
             y[0] = fp_reduce(f, y[0]);
 #pragma unroll
-            for (int t = 1; t < 5; t++) y[t] = fp_mulmod3(f, y[t], gld(s.tw, (t - 1) * s.Q + j));
+            for (int t = 1; t < 5; t++) y[t] = fp_mulmod3(f, y[t], twld<TW32>(s.tw, (t - 1) * s.Q, j));
             if (!FWD) {
                 dft5(f, y, s.rot);
                 y[0] = fp_reduce(f, y[0]);
@@ -309,7 +326,7 @@ __device__ void pass1(const Fld &f, double *__restrict__ x0, double *__restrict_
             if (FWD) dft7<false>(f, y, s.rot);
             y[0] = fp_reduce(f, y[0]);
 #pragma unroll
-            for (int t = 1; t < 7; t++) y[t] = fp_mulmod3(f, y[t], gld(s.tw, (t - 1) * s.Q + j));
+            for (int t = 1; t < 7; t++) y[t] = fp_mulmod3(f, y[t], twld<TW32>(s.tw, (t - 1) * s.Q, j));
             if (!FWD) {
                 dft7<true>(f, y, s.rot);
                 y[0] = fp_reduce(f, y[0]);
@@ -320,33 +337,34 @@ __device__ void pass1(const Fld &f, double *__restrict__ x0, double *__restrict_
     }
 }
 
-template <bool FWD, bool LIN>
+template <bool FWD, bool LIN, bool TW32>
 __device__ __forceinline__ void pass_one_t(const Fld &f, double *x0, double *x1, int narr, const StageU &s, const TileGeo &g) {
     switch (s.m) {
-    case 4: pass1<FWD, 4, LIN>(f, x0, x1, narr, s, g); break;
-    case 2: pass1<FWD, 2, LIN>(f, x0, x1, narr, s, g); break;
-    case 3: pass1<FWD, 3, LIN>(f, x0, x1, narr, s, g); break;
-    case 5: pass1<FWD, 5, LIN>(f, x0, x1, narr, s, g); break;
-    default: pass1<FWD, 7, LIN>(f, x0, x1, narr, s, g); break;
+    case 4: pass1<FWD, 4, LIN, TW32>(f, x0, x1, narr, s, g); break;
+    case 2: pass1<FWD, 2, LIN, TW32>(f, x0, x1, narr, s, g); break;
+    case 3: pass1<FWD, 3, LIN, TW32>(f, x0, x1, narr, s, g); break;
+    case 5: pass1<FWD, 5, LIN, TW32>(f, x0, x1, narr, s, g); break;
+    default: pass1<FWD, 7, LIN, TW32>(f, x0, x1, narr, s, g); break;
     }
 }
 
-template <bool FWD>
+template <bool FWD, bool TW32>
 __device__ __forceinline__ void pass_one(const Fld &f, double *x0, double *x1, int narr, const StageU &s, const TileGeo &g) {
-    if (((s.q << g.logC) & 15) == 0) pass_one_t<FWD, true>(f, x0, x1, narr, s, g);
-    else pass_one_t<FWD, false>(f, x0, x1, narr, s, g);
+    if (((s.q << g.logC) & 15) == 0) pass_one_t<FWD, true, TW32>(f, x0, x1, narr, s, g);
+    else pass_one_t<FWD, false, TW32>(f, x0, x1, narr, s, g);
 }
 
-template <bool FWD>
+template <bool FWD, bool TW32>
 __device__ __forceinline__ void pass_44(const Fld &f, double *x0, double *x1, int narr, const StageU &a, const StageU &b,
                                         const TileGeo &g) {
-    if ((((a.q | b.q) << g.logC) & 15) == 0) pass44<FWD, true>(f, x0, x1, narr, a, b, g);
-    else pass44<FWD, false>(f, x0, x1, narr, a, b, g);
+    if ((((a.q | b.q) << g.logC) & 15) == 0) pass44<FWD, true, TW32>(f, x0, x1, narr, a, b, g);
+    else pass44<FWD, false, TW32>(f, x0, x1, narr, a, b, g);
 }
 
 // Stages [s0, s1) of a plan over the LDS tile(s), radix-4 pairs fused.  Any
 // grouping of consecutive stages into passes applies the same butterflies.
-template <bool FWD>
+// TW32: twiddle columns below 2^29 (the mid tile's; see twld).
+template <bool FWD, bool TW32 = false>
 __device__ void run_stages(const Fld &f, double *x0, double *x1, int narr, const DevPlan *__restrict__ P, int s0, int s1,
                            const TileGeo &g) {
     if (FWD) {
@@ -355,10 +373,10 @@ __device__ void run_stages(const Fld &f, double *x0, double *x1, int narr, const
             const StageU a = stage_u(&P->st[s]);
             if (a.m == 4 && s + 1 < s1 && uni(cst(P)->st[s + 1].m) == 4) {
                 const StageU b = stage_u(&P->st[s + 1]);
-                pass_44<true>(f, x0, x1, narr, a, b, g);
+                pass_44<true, TW32>(f, x0, x1, narr, a, b, g);
                 s += 2;
             } else {
-                pass_one<true>(f, x0, x1, narr, a, g);
+                pass_one<true, TW32>(f, x0, x1, narr, a, g);
                 s += 1;
             }
             __syncthreads();
@@ -369,10 +387,10 @@ __device__ void run_stages(const Fld &f, double *x0, double *x1, int narr, const
             const StageU b = stage_u(&P->st[s]);
             if (b.m == 4 && s - 1 >= s0 && uni(cst(P)->st[s - 1].m) == 4) {
                 const StageU a = stage_u(&P->st[s - 1]);
-                pass_44<false>(f, x0, x1, narr, a, b, g);
+                pass_44<false, TW32>(f, x0, x1, narr, a, b, g);
                 s -= 2;
             } else {
-                pass_one<false>(f, x0, x1, narr, b, g);
+                pass_one<false, TW32>(f, x0, x1, narr, b, g);
                 s -= 1;
             }
             __syncthreads();
@@ -433,10 +451,10 @@ __device__ __forceinline__ void stage_regs(const Fld &f, double *y, const double
     }
 }
 
-template <int M>
-__device__ __forceinline__ void load_tw(double *w, const StageU &s, int64_t j) {
+template <int M, bool TW32 = false>
+__device__ __forceinline__ void load_tw(double *w, const StageU &s, int64_t j, int64_t rowoff = 0) {
 #pragma unroll
-    for (int t = 1; t < M; t++) w[t - 1] = gld(s.tw, (t - 1) * s.Q + j);
+    for (int t = 1; t < M; t++) w[t - 1] = twld<TW32>(s.tw, (t - 1) * s.Q + rowoff, j);
 }
 
 // Index geometry of a pass44 sub-problem (see pass44).
@@ -501,15 +519,15 @@ __device__ __forceinline__ void regs44(const Fld &f, double (&y)[4][4], const do
     }
 }
 
-template <bool LIN, bool UNIT1 = false>
+template <bool LIN, bool UNIT1 = false, bool TW32 = false>
 __device__ __forceinline__ void tw44(double (&w0)[4][3], double (&w1)[3], const Sub44 &u, const StageU &s0,
                                      const StageU &s1) {
 #pragma unroll
-    for (int a = 0; a < 4; a++) load_tw<4>(w0[a], s0, u.jt + a * s1.Q);
+    for (int a = 0; a < 4; a++) load_tw<4, TW32>(w0[a], s0, u.jt, a * s1.Q);
     if (UNIT1) {
         w1[0] = w1[1] = w1[2] = 1.0;  // unread
     } else {
-        load_tw<4>(w1, s1, u.jt);
+        load_tw<4, TW32>(w1, s1, u.jt);
     }
 }
 
@@ -521,7 +539,7 @@ __device__ void pass44f(const Fld &f, double *__restrict__ xa, double *__restric
     for (int sl = threadIdx.x; sl < nsub; sl += blockDim.x) {
         const Sub44 u = sub44(sl, s0, s1, g);
         double w0[4][3], w1[3];
-        tw44<LIN>(w0, w1, u, s0, s1);
+        tw44<LIN, false, true>(w0, w1, u, s0, s1);  // the mid tile's passes only
 #pragma unroll
         for (int n = 0; n < NA; n++) {
             double *__restrict__ x = n ? xb : xa;
@@ -573,7 +591,7 @@ __device__ void pass1f(const Fld &f, double *__restrict__ xa, double *__restrict
     for (int beta = threadIdx.x; beta < nb; beta += blockDim.x) {
         const Sub1 u = sub1<M>(beta, s, g);
         double w[M > 1 ? M - 1 : 1];
-        load_tw<M>(w, s, u.j);
+        load_tw<M, true>(w, s, u.j);  // the mid tile's passes only
 #pragma unroll
         for (int n = 0; n < NA; n++) {
             double *__restrict__ x = n ? xb : xa;
@@ -665,7 +683,7 @@ __device__ void last44(const Fld &f, double *__restrict__ x, double (&hb)[HOLD],
         for (int b = 0; b < 4; b++) y[a][b] = x[ix_last44<LIN>(u, a, b)];
     // s1 is the transform's last stage: its twiddles are all 1
     double w0[4][3], w1[3];
-    tw44<LIN, true>(w0, w1, u, s0, s1);
+    tw44<LIN, true, true>(w0, w1, u, s0, s1);
     regs44<true, true>(f, y, w0, w1, s0, s1);
     if (HOLDB) {
 #pragma unroll
@@ -766,7 +784,7 @@ __device__ void last42(const Fld &f, double *__restrict__ x, double (&hb)[HOLD],
 #pragma unroll
             for (int b = 0; b < 4; b++) y[a][b] = x[ix(a, b)];
 #pragma unroll
-        for (int a = 0; a < 2; a++) load_tw<4>(w0[a], s0, jt + a * s1.Q);
+        for (int a = 0; a < 2; a++) load_tw<4, true>(w0[a], s0, jt, a * s1.Q);
 #pragma unroll
         for (int a = 0; a < 2; a++) stage_regs<true, 4>(f, y[a], w0[a], s0);
 #pragma unroll
@@ -1064,7 +1082,7 @@ __device__ __forceinline__ bool edge_ok(const DevPlan *__restrict__ P, int s0, i
 
 // Loads the thread's 16 points from src (rounds of NU), runs stages s0, s0+1
 // forward on them and writes them to the LDS tile.
-template <int NU>
+template <int NU, bool TW32 = false>
 __device__ __forceinline__ void edge_load44(const Fld &f, double *__restrict__ lds, const PointSrc &src, bool pack,
                                             const TileGeo &g, int64_t gbase, const DevPlan *__restrict__ P, int s0) {
     static_assert(16 % NU == 0, "rounds of NU points cover the 16");
@@ -1082,7 +1100,7 @@ __device__ __forceinline__ void edge_load44(const Fld &f, double *__restrict__ l
     const StageU sa = stage_u(&P->st[s0]), sb = stage_u(&P->st[s0 + 1]);
     const Sub44 u = sub44((int)threadIdx.x, sa, sb, g);
     double w0[4][3], w1[3];
-    tw44<false>(w0, w1, u, sa, sb);
+    tw44<false, false, TW32>(w0, w1, u, sa, sb);
     regs44<true>(f, y, w0, w1, sa, sb);
     if ((((sa.q | sb.q) << g.logC) & 15) == 0) {
 #pragma unroll
@@ -1099,6 +1117,7 @@ __device__ __forceinline__ void edge_load44(const Fld &f, double *__restrict__ l
 
 // Reads the thread's pass44 sub-problem of stages s0, s0+1 from the LDS tile,
 // runs the pair inverse and stores the 16 points to x.
+template <bool TW32 = false>
 __device__ __forceinline__ void edge_store44(const Fld &f, double *__restrict__ x, const double *__restrict__ lds,
                                              const TileGeo &g, int64_t gbase, const DevPlan *__restrict__ P, int s0) {
     const int T = blockDim.x;
@@ -1118,7 +1137,7 @@ __device__ __forceinline__ void edge_store44(const Fld &f, double *__restrict__ 
             for (int b = 0; b < 4; b++) y[a][b] = lds[ix44<false>(u, a, b)];
     }
     double w0[4][3], w1[3];
-    tw44<false>(w0, w1, u, sa, sb);
+    tw44<false, false, TW32>(w0, w1, u, sa, sb);
     regs44<false>(f, y, w0, w1, sa, sb);
     const int cmask = (1 << g.logC) - 1;
     gdouble *__restrict__ p = (gdouble *)x + (gbase + (int64_t)(e0 >> g.logC) * g.Lb + (e0 & cmask));
@@ -1409,7 +1428,7 @@ __device__ __forceinline__ void mid_body(const Fld &f, const DevJob *__restrict_
     if (!sq && !bspec) {
         // b first: its forward passes, then its last pass held in registers
         if (edge) {
-            edge_load44<8>(f, la, sb, pack, g, gbase, P, s0);
+            edge_load44<8, true>(f, la, sb, pack, g, gbase, P, s0);
         } else {
             load_tile<LU>(la, sb, pack, S, g, gbase);
         }
@@ -1419,7 +1438,7 @@ __device__ __forceinline__ void mid_body(const Fld &f, const DevJob *__restrict_
         __syncthreads();
     }
     if (edge) {
-        edge_load44<8>(f, la, sa, pack, g, gbase, P, s0);
+        edge_load44<8, true>(f, la, sa, pack, g, gbase, P, s0);
     } else {
         load_tile<LU>(la, sa, pack, S, g, gbase);
     }
@@ -1429,9 +1448,9 @@ __device__ __forceinline__ void mid_body(const Fld &f, const DevJob *__restrict_
     else if (bspec) last_pass<false, SRC_GLOB>(f, la, hb, sb.x, gbase, P, sl, s1, g);
     else last_pass<false, SRC_HELD>(f, la, hb, nullptr, 0, P, sl, s1, g);
     __syncthreads();
-    run_stages<false>(f, la, nullptr, 1, P, sf, sl, g);
+    run_stages<false, true>(f, la, nullptr, 1, P, sf, sl, g);
     double *__restrict__ xo = unip(J->xa[t.prime]);
-    if (edge) edge_store44(f, xo, la, g, gbase, P, s0);
+    if (edge) edge_store44<true>(f, xo, la, g, gbase, P, s0);
     else store_tile(xo, la, S, g, gbase);
 }
 
