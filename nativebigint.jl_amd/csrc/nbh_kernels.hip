@@ -222,32 +222,36 @@ struct TileGeo {
 // power of two, sb = 4 sa and base mod 16 < sa whenever sa < 16, so the a and
 // b offsets never cross a 16-point boundary together: six pidx per
 // sub-problem instead of sixteen.
-template <bool FWD, bool LIN, bool TW32 = false>
+// CQ1 > 0: a one-column tile (logC = 0) whose pair has q1 = CQ1, so every
+// offset is a constant, pidx(a q1) + pidx(4 b q1) (the decomposition above
+// with base mod 16 < q1), folded into the LDS instructions.
+template <bool FWD, bool LIN, bool TW32 = false, int CQ1 = 0>
 __device__ void pass44(const Fld &f, double *__restrict__ x0, double *__restrict__ x1, int narr, const StageU &s0,
                        const StageU &s1, const TileGeo &g) {
-    const int q1 = s1.q;
+    const int q1 = CQ1 ? CQ1 : s1.q;
     const int logq1 = __builtin_ctz((unsigned)q1);
-    const int q0 = s0.q;
-    const int C = 1 << g.logC;
-    const int nsub = (g.S >> 4) << g.logC;
+    const int q0 = CQ1 ? 4 * CQ1 : s0.q;
+    const int logC = CQ1 ? 0 : g.logC;
+    const int C = 1 << logC;
+    const int nsub = (g.S >> 4) << logC;
     for (int sg = threadIdx.x; sg < nsub * narr; sg += blockDim.x) {
         double *__restrict__ x = sg < nsub ? x0 : x1;
         const int sl = sg < nsub ? sg : sg - nsub;
         const int c = sl & (C - 1);
-        const int r = sl >> g.logC;
+        const int r = sl >> logC;
         const int jj = r & (q1 - 1);
         const int row0 = ((r >> logq1) << 4) * q1 + jj;
-        const int base = (row0 << g.logC) + c;
+        const int base = (row0 << logC) + c;
         const int p0 = pidx(base);
-        const int sa = q1 << g.logC, sb = q0 << g.logC;
+        const int sa = q1 << logC, sb = q0 << logC;
         const int pa = sa + (sa >> 4), pb = sb + (sb >> 4);
         int oa[4], ob[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            oa[k] = pidx(base + k * sa) - p0;
-            ob[k] = pidx(base + k * sb) - p0;
+            oa[k] = CQ1 ? pidx(k * CQ1) : pidx(base + k * sa) - p0;
+            ob[k] = CQ1 ? pidx(4 * k * CQ1) : pidx(base + k * sb) - p0;
         }
-        auto ix = [&](int a, int b) { return LIN ? p0 + a * pa + b * pb : p0 + oa[a] + ob[b]; };
+        auto ix = [&](int a, int b) { return LIN && !CQ1 ? p0 + a * pa + b * pb : p0 + oa[a] + ob[b]; };
         double y[4][4];
 #pragma unroll
         for (int a = 0; a < 4; a++)
@@ -357,6 +361,15 @@ __device__ __forceinline__ void pass_one(const Fld &f, double *x0, double *x1, i
 template <bool FWD, bool TW32>
 __device__ __forceinline__ void pass_44(const Fld &f, double *x0, double *x1, int narr, const StageU &a, const StageU &b,
                                         const TileGeo &g) {
+    if (TW32) {  // the mid tile (one column): the common pair strides take constant offsets
+        switch (g.logC == 0 ? b.q : 0) {
+        case 4: pass44<FWD, false, TW32, 4>(f, x0, x1, narr, a, b, g); return;
+        case 8: pass44<FWD, false, TW32, 8>(f, x0, x1, narr, a, b, g); return;
+        case 16: pass44<FWD, true, TW32, 16>(f, x0, x1, narr, a, b, g); return;
+        case 64: pass44<FWD, true, TW32, 64>(f, x0, x1, narr, a, b, g); return;
+        default: break;
+        }
+    }
     if ((((a.q | b.q) << g.logC) & 15) == 0) pass44<FWD, true, TW32>(f, x0, x1, narr, a, b, g);
     else pass44<FWD, false, TW32>(f, x0, x1, narr, a, b, g);
 }
@@ -532,14 +545,22 @@ __device__ __forceinline__ void tw44(double (&w0)[4][3], double (&w1)[3], const 
 }
 
 // Forward radix-4 pair on NA arrays (a, and b when NA == 2).
-template <bool LIN, int NA>
+// CQ1 > 0: constant offsets for q1 = CQ1 on the one-column mid tile (see pass44).
+template <bool LIN, int NA, int CQ1 = 0>
 __device__ void pass44f(const Fld &f, double *__restrict__ xa, double *__restrict__ xb, const StageU &s0, const StageU &s1,
                         const TileGeo &g) {
     const int nsub = (g.S >> 4) << g.logC;
+    StageU t1 = s1;
+    TileGeo gc = g;
+    if (CQ1) {
+        t1.q = CQ1;
+        gc.logC = 0;
+    }
     for (int sl = threadIdx.x; sl < nsub; sl += blockDim.x) {
-        const Sub44 u = sub44(sl, s0, s1, g);
+        const Sub44 u = sub44(sl, s0, t1, gc);
         double w0[4][3], w1[3];
         tw44<LIN, false, true>(w0, w1, u, s0, s1);  // the mid tile's passes only
+        auto ix = [&](int a, int b) { return CQ1 ? u.p0 + pidx(a * CQ1) + pidx(4 * b * CQ1) : ix44<LIN>(u, a, b); };
 #pragma unroll
         for (int n = 0; n < NA; n++) {
             double *__restrict__ x = n ? xb : xa;
@@ -547,12 +568,12 @@ __device__ void pass44f(const Fld &f, double *__restrict__ xa, double *__restric
 #pragma unroll
             for (int a = 0; a < 4; a++)
 #pragma unroll
-                for (int b = 0; b < 4; b++) y[a][b] = x[ix44<LIN>(u, a, b)];
+                for (int b = 0; b < 4; b++) y[a][b] = x[ix(a, b)];
             regs44<true>(f, y, w0, w1, s0, s1);
 #pragma unroll
             for (int a = 0; a < 4; a++)
 #pragma unroll
-                for (int b = 0; b < 4; b++) x[ix44<LIN>(u, a, b)] = y[a][b];
+                for (int b = 0; b < 4; b++) x[ix(a, b)] = y[a][b];
         }
     }
 }
@@ -646,8 +667,16 @@ __device__ int fwd_until_last(const Fld &f, double *x, const DevPlan *__restrict
         if (a.m == 4 && m1 == 2 && s + 2 == s1) return s;
         if (pair) {
             const StageU b = stage_u(&P->st[s + 1]);
-            if ((((a.q | b.q) << g.logC) & 15) == 0) pass44f<true, 1>(f, x, nullptr, a, b, g);
-            else pass44f<false, 1>(f, x, nullptr, a, b, g);
+            // the mid tile is one column: the common pair strides take constant offsets
+            switch (g.logC == 0 ? b.q : 0) {
+            case 4: pass44f<false, 1, 4>(f, x, nullptr, a, b, g); break;
+            case 8: pass44f<false, 1, 8>(f, x, nullptr, a, b, g); break;
+            case 16: pass44f<true, 1, 16>(f, x, nullptr, a, b, g); break;
+            case 64: pass44f<true, 1, 64>(f, x, nullptr, a, b, g); break;
+            default:
+                if ((((a.q | b.q) << g.logC) & 15) == 0) pass44f<true, 1>(f, x, nullptr, a, b, g);
+                else pass44f<false, 1>(f, x, nullptr, a, b, g);
+            }
         } else {
             pass1f_any<1>(f, x, nullptr, a, g);
         }
@@ -910,11 +939,34 @@ struct PointSrc {
 // (e & (C-1)).  In a full round (every e < total) with T a multiple of C the
 // element and bit positions advance by a uniform step; otherwise a lane past
 // `total` re-reads point 0.
-template <bool PACK, int NU>
+// Spectra points of a tile through a raw buffer based at the tile's first
+// point: the per-lane offset is the lane's first point, the per-load offset a
+// wave-uniform multiple of the round step, both 32-bit byte offsets (the mid
+// tile spans at most 3 * 2^27 bytes), so no 64-bit address add per load.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t tile_rsrc(const double *x) {
+    return __builtin_amdgcn_make_buffer_rsrc((void *)x, (short)0, -1, 0x00020000);
+}
+__device__ __forceinline__ double bufldn(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 2));  // nt
+}
+__device__ __forceinline__ void bufstn(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, double v) {
+    using V = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, v), r, voff, soff, 2);  // nt
+}
+
+// OFF32: the mid tile (Lb = 1, one column), whose spectra loads take the
+// buffer form above.
+template <bool PACK, int NU, bool OFF32 = false>
 __device__ __forceinline__ void load_round(double (&v)[NU], const PointSrc &src, int e0, int total, bool full,
                                            const TileGeo &g, int64_t gbase) {
     const int T = blockDim.x;
     const int cmask = (1 << g.logC) - 1;
+    if (!PACK && OFF32 && full) {
+        const __amdgpu_buffer_rsrc_t r = tile_rsrc(src.x + gbase);
+#pragma unroll
+        for (int k = 0; k < NU; k++) v[k] = bufldn(r, (uint32_t)e0 << 3, (uint32_t)(k * T) << 3);
+        return;
+    }
     if (full && (T & cmask) == 0) {
         const int64_t i0 = gbase + (int64_t)(e0 >> g.logC) * g.Lb + (e0 & cmask);
         const int64_t step = (int64_t)(T >> g.logC) * g.Lb;
@@ -1093,7 +1145,7 @@ __device__ __forceinline__ void edge_load44(const Fld &f, double *__restrict__ l
     for (int r = 0; r < 16 / NU; r++) {
         double v[NU];
         if (pack) load_round<true, NU>(v, src, (int)threadIdx.x + r * NU * T, total, true, g, gbase);
-        else load_round<false, NU>(v, src, (int)threadIdx.x + r * NU * T, total, true, g, gbase);
+        else load_round<false, NU, TW32>(v, src, (int)threadIdx.x + r * NU * T, total, true, g, gbase);
 #pragma unroll
         for (int k = 0; k < NU; k++) y[(r * NU + k) & 3][(r * NU + k) >> 2] = v[k];
     }
@@ -1139,6 +1191,12 @@ __device__ __forceinline__ void edge_store44(const Fld &f, double *__restrict__ 
     double w0[4][3], w1[3];
     tw44<false, false, TW32>(w0, w1, u, sa, sb);
     regs44<false>(f, y, w0, w1, sa, sb);
+    if (TW32) {  // the mid tile: points e0 + k*T of the tile (see load_round)
+        const __amdgpu_buffer_rsrc_t r = tile_rsrc(x + gbase);
+#pragma unroll
+        for (int k = 0; k < 16; k++) bufstn(r, (uint32_t)e0 << 3, (uint32_t)(k * T) << 3, y[k & 3][k >> 2]);
+        return;
+    }
     const int cmask = (1 << g.logC) - 1;
     gdouble *__restrict__ p = (gdouble *)x + (gbase + (int64_t)(e0 >> g.logC) * g.Lb + (e0 & cmask));
     const int64_t step = (int64_t)(T >> g.logC) * g.Lb;
