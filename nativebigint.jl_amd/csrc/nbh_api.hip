@@ -249,7 +249,7 @@ struct Lib {
     hipStream_t stream = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     // end of the last call that used the shared single-product resources
-    // (workspace, job slots, I/O buffers), on whatever stream it ran
+    // (workspace, I/O buffers), on whatever stream it ran
     hipEvent_t last = nullptr;
     bool last_valid = false;
     bool last_on_lib = false;  // `last` was recorded on g.stream
@@ -281,8 +281,6 @@ struct Lib {
         size_t pin_size = 0, dev_size = 0;
         hipEvent_t ev = nullptr;
     } hs[2];
-    DevJob *slots = nullptr;
-    int slot_next = 0;
     hipStream_t side[8] = {};  // batch phases fork onto these (default priority)
     int nside = 4;             // how many of them a step uses (0: all launches in order)
     // per-kernel event timing
@@ -331,7 +329,6 @@ struct ProfScope {
         g.prof_open.emplace_back(kind, a, b);
     }
 };
-constexpr int NSLOTS = 256;
 constexpr int NSIDE = 4;
 
 int ensure_init() {
@@ -915,37 +912,36 @@ int run_single(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int
             w += xbytes;
         }
     }
-    DevJob *slot = g.slots + (g.slot_next++ % NSLOTS);
-    HIPCHK(launch_store_job(J, slot, s));
+    // every launch carries the job record in its TileGen (no job array)
     const DevPlan &H = P[0]->host;
     const int K = H.ngroups;
     for (int gi = 0; gi + 1 < K; gi++) {
-        TileGen tg{gi, nops, H.g[gi].tiles, gi == 0 ? TILE_PACK : 0, 0};
+        TileGen tg{gi, nops, H.g[gi].tiles, gi == 0 ? TILE_PACK : 0, 0, J};
         ProfScope ps(0, s, prof);
-        HIPCHK(launch_group(slot, nullptr, tg, 2 * nops * H.g[gi].tiles, 1, P[0]->grp_lds_bytes[gi],
+        HIPCHK(launch_group(nullptr, nullptr, tg, 2 * nops * H.g[gi].tiles, 1, P[0]->grp_lds_bytes[gi],
                             P[0]->grp_threads[gi], s));
     }
     {
-        TileGen tg{K - 1, 1, H.g[K - 1].tiles, K == 1 ? TILE_PACK : 0, 0};
+        TileGen tg{K - 1, 1, H.g[K - 1].tiles, K == 1 ? TILE_PACK : 0, 0, J};
         ProfScope ps(1, s, prof);
         // a cached b still takes an LDS array, but only a is transformed
-        HIPCHK(launch_mid(slot, nullptr, tg, 2 * H.g[K - 1].tiles, P[0]->mid_lds_bytes[sqr ? 1 : 0],
+        HIPCHK(launch_mid(nullptr, nullptr, tg, 2 * H.g[K - 1].tiles, P[0]->mid_lds_bytes[sqr ? 1 : 0],
                           P[0]->mid_threads[sqr || spec ? 1 : 0], s));
     }
     for (int gi = K - 2; gi >= 0; gi--) {
-        TileGen tg{gi, 1, H.g[gi].tiles, 0, 0};
+        TileGen tg{gi, 1, H.g[gi].tiles, 0, 0, J};
         ProfScope ps(2, s, prof);
-        HIPCHK(launch_group(slot, nullptr, tg, 2 * H.g[gi].tiles, 0, P[0]->grp_lds_bytes[gi], P[0]->grp_threads[gi],
-                            s));
+        HIPCHK(launch_group(nullptr, nullptr, tg, 2 * H.g[gi].tiles, 0, P[0]->grp_lds_bytes[gi],
+                            P[0]->grp_threads[gi], s));
     }
-    TileGen ug{0, 1, J.nchunks, 0, 0};
+    TileGen ug{0, 1, J.nchunks, 0, 0, J};
     {
         ProfScope ps(3, s, prof);
-        HIPCHK(launch_unpack(slot, nullptr, ug, J.nchunks, J.bch, s));
+        HIPCHK(launch_unpack(nullptr, nullptr, ug, J.nchunks, J.bch, s));
     }
     if (J.nchunks > 1) {  // one chunk has no carry to pass on
         ProfScope ps(4, s, prof);
-        HIPCHK(launch_carry_fix(slot, nullptr, ug, J.nchunks, s));
+        HIPCHK(launch_carry_fix(nullptr, nullptr, ug, J.nchunks, s));
     }
     return NBH_OK;
 }
@@ -975,8 +971,7 @@ int run_lo(uint64_t *r, const uint64_t *a, int64_t la, const uint64_t *b, int64_
     return run_single(r, a, la, sqr ? nullptr : b, lb, bits_a, bits_b, sqr, nullptr, k, s);
 }
 
-// The single-product resources (workspace, job slots, I/O and pinned
-// buffers) serve one call at a time, in submission order, whatever stream a
+// The single-product resources (workspace, I/O and pinned buffers) serve one call at a time, in submission order, whatever stream a
 // call runs on: each call's stream first waits for the event recorded at the
 // end of the previous call (lib_begin) and records it again at its own end
 // (lib_end).  Single products on a caller's stream run on that stream, so
@@ -1670,7 +1665,6 @@ int32_t nbh_init(int32_t device) {
         HIPCHK(hipEventCreateWithFlags(&g.ev_out, hipEventDisableTiming));
         HIPCHK(hipEventCreateWithFlags(&g.last, hipEventDisableTiming));
         g.last_valid = g.last_on_lib = false;
-        HIPCHK(hipMalloc((void **)&g.slots, sizeof(DevJob) * NSLOTS));
         for (int i = 0; i < NSIDE; i++) HIPCHK(hipStreamCreateWithFlags(&g.side[i], hipStreamNonBlocking));
         HIPCHK(set_lds_limit(g.lds_block));
         load_tuning_file();
@@ -1705,11 +1699,9 @@ int32_t nbh_shutdown(void) {
         if (h.ev) (void)hipEventDestroy(h.ev);
         h = Lib::HostSlot{};
     }
-    if (g.slots) (void)hipFree(g.slots);
     g.ws = g.io = nullptr;
     g.pin = g.pin_dev = nullptr;
     g.ws_size = g.io_size = g.pin_size = 0;
-    g.slots = nullptr;
     for (int i = 0; i < NSIDE; i++) (void)hipStreamDestroy(g.side[i]);
     (void)hipEventDestroy(g.ev_in);
     (void)hipEventDestroy(g.ev_out);
@@ -2600,13 +2592,11 @@ int32_t nbh_spectrum_create(const uint64_t *b, int64_t n, int64_t bits_b, int64_
             J.plan[k] = P[k]->dev;
             J.xa[k] = S->x[k];
         }
-        DevJob *slot = g.slots + (g.slot_next++ % NSLOTS);
-        HIPCHK(launch_store_job(J, slot, g.stream));
         const DevPlan &H = P[0]->host;
         for (int gi = 0; gi < H.ngroups; gi++) {
-            TileGen tg{gi, 1, H.g[gi].tiles, gi == 0 ? TILE_PACK : 0, 0};
-            HIPCHK(launch_group(slot, nullptr, tg, 2 * H.g[gi].tiles, 1, P[0]->grp_lds_bytes[gi], P[0]->grp_threads[gi],
-                                g.stream));
+            TileGen tg{gi, 1, H.g[gi].tiles, gi == 0 ? TILE_PACK : 0, 0, J};
+            HIPCHK(launch_group(nullptr, nullptr, tg, 2 * H.g[gi].tiles, 1, P[0]->grp_lds_bytes[gi],
+                                P[0]->grp_threads[gi], g.stream));
         }
         *out = S.release();
         g.live++;
@@ -2846,12 +2836,11 @@ int32_t nbh_ntt_fwd_dev(double *x, int64_t N, int32_t prime, void *stream) {
         std::memset(&J, 0, sizeof J);
         J.plan[prime] = P->dev;
         J.xa[prime] = x;
-        DevJob *slot = g.slots + (g.slot_next++ % NSLOTS);
-        HIPCHK(launch_store_job(J, slot, g.stream));
         const DevPlan &H = P->host;
         for (int gi = 0; gi < H.ngroups; gi++) {
-            TileGen tg{gi, 1, H.g[gi].tiles, 0, prime};
-            HIPCHK(launch_group(slot, nullptr, tg, H.g[gi].tiles, 1, P->grp_lds_bytes[gi], P->grp_threads[gi], g.stream));
+            TileGen tg{gi, 1, H.g[gi].tiles, 0, prime, J};
+            HIPCHK(launch_group(nullptr, nullptr, tg, H.g[gi].tiles, 1, P->grp_lds_bytes[gi], P->grp_threads[gi],
+                                g.stream));
         }
         return stream_leave(stream);
     NBH_GUARD_END
@@ -2869,12 +2858,11 @@ int32_t nbh_ntt_rev_dev(double *x, int64_t N, int32_t prime, void *stream) {
         std::memset(&J, 0, sizeof J);
         J.plan[prime] = P->dev;
         J.xa[prime] = x;
-        DevJob *slot = g.slots + (g.slot_next++ % NSLOTS);
-        HIPCHK(launch_store_job(J, slot, g.stream));
         const DevPlan &H = P->host;
         for (int gi = H.ngroups - 1; gi >= 0; gi--) {
-            TileGen tg{gi, 1, H.g[gi].tiles, 0, prime};
-            HIPCHK(launch_group(slot, nullptr, tg, H.g[gi].tiles, 0, P->grp_lds_bytes[gi], P->grp_threads[gi], g.stream));
+            TileGen tg{gi, 1, H.g[gi].tiles, 0, prime, J};
+            HIPCHK(launch_group(nullptr, nullptr, tg, H.g[gi].tiles, 0, P->grp_lds_bytes[gi], P->grp_threads[gi],
+                                g.stream));
         }
         return stream_leave(stream);
     NBH_GUARD_END
@@ -2928,11 +2916,9 @@ int32_t nbh_unpack_dev(uint64_t *r, int64_t rn, const double *x1, const double *
         J.xa[1] = const_cast<double *>(x2);
         RCCHK(ensure_buf(&g.ws, &g.ws_size, align256(sizeof(uint32_t) * (size_t)J.nchunks)));
         J.chunk_flags = (uint32_t *)g.ws;
-        DevJob *slot = g.slots + (g.slot_next++ % NSLOTS);
-        HIPCHK(launch_store_job(J, slot, g.stream));
-        TileGen ug{0, 1, J.nchunks, 0, 0};
-        HIPCHK(launch_unpack(slot, nullptr, ug, J.nchunks, b, g.stream));
-        HIPCHK(launch_carry_fix(slot, nullptr, ug, J.nchunks, g.stream));
+        TileGen ug{0, 1, J.nchunks, 0, 0, J};
+        HIPCHK(launch_unpack(nullptr, nullptr, ug, J.nchunks, b, g.stream));
+        HIPCHK(launch_carry_fix(nullptr, nullptr, ug, J.nchunks, g.stream));
         return stream_leave(stream);
     NBH_GUARD_END
 }

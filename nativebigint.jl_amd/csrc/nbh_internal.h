@@ -92,12 +92,16 @@ enum TileFlags : uint8_t { TILE_PACK = 1 };
 
 // Tiles generated from blockIdx when a launch covers a single job: block b is
 // (prime, op, tile) = (prime0 + b / (nops*per), (b / per) % nops, b % per).
+// A launch over one job also carries that job's record here (its kernels are
+// passed no job array), so no device copy of the record has to be stored by
+// a launch of its own first.
 struct TileGen {
     int32_t group;
     int32_t nops;
     int64_t per;
     int32_t flags;
     int32_t prime0;
+    DevJob job;
 };
 
 // LDS of one gfx950 CU (bytes).

@@ -1051,6 +1051,12 @@ __device__ __forceinline__ void load_tile(double *__restrict__ lds, const PointS
 }
 
 // The tile a block works on: from the launch's tile list, or generated.
+// The job of a tile: from the launch's job array, or the record the launch
+// carries in its TileGen when it covers one job (no job array).
+__device__ __forceinline__ const DevJob *job_of(const DevJob *jobs, const Tile &t, const TileGen &gen) {
+    return jobs ? jobs + t.job : &gen.job;
+}
+
 __device__ __forceinline__ Tile fetch_tile(const Tile *__restrict__ tiles, const TileGen &g) {
     Tile t;
     if (tiles) {
@@ -1267,7 +1273,7 @@ template <bool FWD, int NU>
 __device__ __forceinline__ void group_kernel(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles,
                                              const TileGen &gen, double *lds) {
     const Tile t = fetch_tile(tiles, gen);
-    const DevJob *J = jobs + t.job;
+    const DevJob *J = job_of(jobs, t, gen);
     if (uni(cst(J)->zero)) return;
     group_body<NU>(fld(t.prime), J, t, FWD, lds);
 }
@@ -1419,7 +1425,7 @@ template <bool FWD, bool BIG>
 __device__ __forceinline__ void col_kernel(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles,
                                            const TileGen &gen) {
     const Tile t = fetch_tile(tiles, gen);
-    const DevJob *J = jobs + t.job;
+    const DevJob *J = job_of(jobs, t, gen);
     if (uni(cst(J)->zero)) return;
     const DevPlan *P = unip(cst(J)->plan[t.prime ? 1 : 0]);
     const int code = uni(cst(P)->g[t.group].col);
@@ -1516,7 +1522,7 @@ __global__ void __launch_bounds__(MID_MAX_THREADS) k_mid(const DevJob *__restric
                                               TileGen gen) {
     extern __shared__ double lds[];
     const Tile t = fetch_tile(tiles, gen);
-    const DevJob *J = jobs + t.job;
+    const DevJob *J = job_of(jobs, t, gen);
     if (uni(cst(J)->zero)) return;
     mid_body(fld(t.prime), J, t, lds);
 }
@@ -1635,7 +1641,7 @@ k_unpack(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles, TileGe
     __shared__ uint64_t spill0[UNPACK_THREADS], spill1[UNPACK_THREADS];
     __shared__ uint32_t wsum[UNPACK_THREADS / 64];
     const Tile t = fetch_tile(tiles, gen);
-    const DevJob *__restrict__ Jp = jobs + t.job;
+    const DevJob *__restrict__ Jp = job_of(jobs, t, gen);
     const auto *J = cst(Jp);
     const int tid = threadIdx.x;
     const int64_t L0 = (int64_t)t.tile * UNPACK_CHUNK;
@@ -1752,7 +1758,7 @@ __global__ void __launch_bounds__(UNPACK_THREADS) k_carry_fix(const DevJob *__re
                                                               const Tile *__restrict__ tiles, TileGen gen) {
     __shared__ int first;
     const Tile t = fetch_tile(tiles, gen);
-    const auto *J = cst(jobs + t.job);
+    const auto *J = cst(job_of(jobs, t, gen));
     if (uni(J->zero) || t.tile == 0) return;
     const uint32_t *__restrict__ flags = unip(J->chunk_flags);
     int cin = 0;
@@ -2497,11 +2503,6 @@ __global__ void k_expand_tiles(const TileRange *__restrict__ ranges, Tile *__res
     }
 }
 
-// Stream-ordered upload of a single job descriptor (no host staging buffer).
-__global__ void k_store_job(DevJob j, DevJob *__restrict__ dst) {
-    if (threadIdx.x == 0) *dst = j;
-}
-
 // Top limbs of every operand of a batch (magnitude_bits needs them,
 // src/kernels/addsub.jl:150).
 __global__ void k_top_limbs(const DevJob *__restrict__ jobs, int64_t count, uint64_t *__restrict__ out) {
@@ -2649,11 +2650,6 @@ hipError_t launch_barrett_fix(const BarrettFixJob *jobs, int64_t count, hipStrea
 hipError_t launch_expand_tiles(const TileRange *ranges, int64_t nranges, Tile *tiles, hipStream_t s) {
     if (nranges <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_expand_tiles, dim3((unsigned)nranges), dim3(256), 0, s, ranges, tiles);
-    return hipGetLastError();
-}
-
-hipError_t launch_store_job(const DevJob &j, DevJob *dst, hipStream_t s) {
-    hipLaunchKernelGGL(k_store_job, dim3(1), dim3(64), 0, s, j, dst);
     return hipGetLastError();
 }
 

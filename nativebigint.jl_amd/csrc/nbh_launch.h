@@ -39,7 +39,6 @@ hipError_t launch_unpack(const DevJob *jobs, const Tile *tiles, const TileGen &g
                          hipStream_t s);
 hipError_t launch_carry_fix(const DevJob *jobs, const Tile *tiles, const TileGen &gen, int64_t ntiles,
                             hipStream_t s);
-hipError_t launch_store_job(const DevJob &j, DevJob *dst, hipStream_t s);
 hipError_t launch_expand_tiles(const TileRange *ranges, int64_t nranges, Tile *tiles, hipStream_t s);
 // One workgroup per cofactor-step output (sum, or |difference| and sign);
 // lens: some jobs carry a length output (initialised by a launch before).
