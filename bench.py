@@ -84,7 +84,7 @@ def parse():
     ap.add_argument("--bases", type=int, default=64, help="powermod: bases sharing the exponent and modulus")
     ap.add_argument("--exp-bits", type=int, default=64, help="powermod: exponent bits")
     ap.add_argument("--modulus-limbs", type=int, default=4096, help="spectra: limbs of the fixed modulus")
-    ap.add_argument("--streams", type=int, default=4,
+    ap.add_argument("--streams", type=int, default=8,
                     help="side streams a batch step forks its launches onto (0: in order)")
     ap.add_argument("--requests", type=int, default=20000, help="serve: requests per offered load")
     ap.add_argument("--budget-gib", type=float, default=8.0, help="serve: HBM budget")

@@ -17,7 +17,7 @@ from nbhip import engine  # noqa: E402
 def _init():
     engine.init()
     yield
-    D.set_streams(4)
+    D.set_streams(8)
 
 
 def _batch(count, seed):
@@ -30,7 +30,7 @@ def _batch(count, seed):
     return tens
 
 
-@pytest.mark.parametrize("nside", [4, 0])
+@pytest.mark.parametrize("nside", [8, 4, 0])
 def test_graph_replays_equal_eager(nside):
     D.set_streams(nside)
     tens = _batch(80, 0)
@@ -71,7 +71,7 @@ def test_graph_replays_equal_eager(nside):
 def test_stream_settings_are_checked():
     from nbhip._lib import NbhError, check, lib
     with pytest.raises(NbhError):
-        D.set_streams(5)
+        D.set_streams(9)
     with pytest.raises(NbhError):
         D.set_streams(-1)
     tens = _batch(4, 0)
@@ -82,4 +82,4 @@ def test_stream_settings_are_checked():
     B.run()
     torch.cuda.synchronize()
     B.close()
-    D.set_streams(4)
+    D.set_streams(8)
