@@ -1139,8 +1139,10 @@ __device__ __forceinline__ bool edge_ok(const DevPlan *__restrict__ P, int s0, i
 }
 
 // Loads the thread's 16 points from src (rounds of NU), runs stages s0, s0+1
-// forward on them and writes them to the LDS tile.
-template <int NU, bool TW32 = false>
+// forward on them and writes them to the LDS tile.  TW32: twiddles through
+// 32-bit offsets (see twld); MIDIO: the one-column mid tile's spectra through
+// a tile-based buffer (see load_round).
+template <int NU, bool TW32 = false, bool MIDIO = false>
 __device__ __forceinline__ void edge_load44(const Fld &f, double *__restrict__ lds, const PointSrc &src, bool pack,
                                             const TileGeo &g, int64_t gbase, const DevPlan *__restrict__ P, int s0) {
     static_assert(16 % NU == 0, "rounds of NU points cover the 16");
@@ -1151,7 +1153,7 @@ __device__ __forceinline__ void edge_load44(const Fld &f, double *__restrict__ l
     for (int r = 0; r < 16 / NU; r++) {
         double v[NU];
         if (pack) load_round<true, NU>(v, src, (int)threadIdx.x + r * NU * T, total, true, g, gbase);
-        else load_round<false, NU, TW32>(v, src, (int)threadIdx.x + r * NU * T, total, true, g, gbase);
+        else load_round<false, NU, MIDIO>(v, src, (int)threadIdx.x + r * NU * T, total, true, g, gbase);
 #pragma unroll
         for (int k = 0; k < NU; k++) y[(r * NU + k) & 3][(r * NU + k) >> 2] = v[k];
     }
@@ -1175,7 +1177,7 @@ __device__ __forceinline__ void edge_load44(const Fld &f, double *__restrict__ l
 
 // Reads the thread's pass44 sub-problem of stages s0, s0+1 from the LDS tile,
 // runs the pair inverse and stores the 16 points to x.
-template <bool TW32 = false>
+template <bool TW32 = false, bool MIDIO = false>
 __device__ __forceinline__ void edge_store44(const Fld &f, double *__restrict__ x, const double *__restrict__ lds,
                                              const TileGeo &g, int64_t gbase, const DevPlan *__restrict__ P, int s0) {
     const int T = blockDim.x;
@@ -1197,7 +1199,7 @@ __device__ __forceinline__ void edge_store44(const Fld &f, double *__restrict__ 
     double w0[4][3], w1[3];
     tw44<false, false, TW32>(w0, w1, u, sa, sb);
     regs44<false>(f, y, w0, w1, sa, sb);
-    if (TW32) {  // the mid tile: points e0 + k*T of the tile (see load_round)
+    if (MIDIO) {  // the mid tile: points e0 + k*T of the tile (see load_round)
         const __amdgpu_buffer_rsrc_t r = tile_rsrc(x + gbase);
 #pragma unroll
         for (int k = 0; k < 16; k++) bufstn(r, (uint32_t)e0 << 3, (uint32_t)(k * T) << 3, y[k & 3][k >> 2]);
@@ -1492,7 +1494,7 @@ __device__ __forceinline__ void mid_body(const Fld &f, const DevJob *__restrict_
     if (!sq && !bspec) {
         // b first: its forward passes, then its last pass held in registers
         if (edge) {
-            edge_load44<8, true>(f, la, sb, pack, g, gbase, P, s0);
+            edge_load44<8, true, true>(f, la, sb, pack, g, gbase, P, s0);
         } else {
             load_tile<LU>(la, sb, pack, S, g, gbase);
         }
@@ -1502,7 +1504,7 @@ __device__ __forceinline__ void mid_body(const Fld &f, const DevJob *__restrict_
         __syncthreads();
     }
     if (edge) {
-        edge_load44<8, true>(f, la, sa, pack, g, gbase, P, s0);
+        edge_load44<8, true, true>(f, la, sa, pack, g, gbase, P, s0);
     } else {
         load_tile<LU>(la, sa, pack, S, g, gbase);
     }
@@ -1514,7 +1516,7 @@ __device__ __forceinline__ void mid_body(const Fld &f, const DevJob *__restrict_
     __syncthreads();
     run_stages<false, true>(f, la, nullptr, 1, P, sf, sl, g);
     double *__restrict__ xo = unip(J->xa[t.prime]);
-    if (edge) edge_store44<true>(f, xo, la, g, gbase, P, s0);
+    if (edge) edge_store44<true, true>(f, xo, la, g, gbase, P, s0);
     else store_tile(xo, la, S, g, gbase);
 }
 
