@@ -919,6 +919,8 @@ int run_single(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int
     // every launch carries the job record in its TileGen (no job array)
     const DevPlan &H = P[0]->host;
     const int K = H.ngroups;
+    // few enough chunks: the unpack resolves the carries itself (no k_carry_fix)
+    const bool lookback = J.nchunks > 1 && J.nchunks <= UNPACK_LB_MAX_CHUNKS;
     for (int gi = 0; gi + 1 < K; gi++) {
         TileGen tg{gi, nops, H.g[gi].tiles, gi == 0 ? TILE_PACK : 0, 0, J};
         ProfScope ps(0, s, prof);
@@ -926,7 +928,7 @@ int run_single(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int
                             P[0]->grp_threads[gi], s));
     }
     {
-        TileGen tg{K - 1, 1, H.g[K - 1].tiles, K == 1 ? TILE_PACK : 0, 0, J};
+        TileGen tg{K - 1, 1, H.g[K - 1].tiles, (K == 1 ? TILE_PACK : 0) | (lookback ? TILE_CLEAR : 0), 0, J};
         ProfScope ps(1, s, prof);
         // a cached b still takes an LDS array, but only a is transformed
         HIPCHK(launch_mid(nullptr, nullptr, tg, 2 * H.g[K - 1].tiles, P[0]->mid_lds_bytes[sqr ? 1 : 0],
@@ -941,9 +943,9 @@ int run_single(uint64_t *r, const uint64_t *a, int64_t m, const uint64_t *b, int
     TileGen ug{0, 1, J.nchunks, 0, 0, J};
     {
         ProfScope ps(3, s, prof);
-        HIPCHK(launch_unpack(nullptr, nullptr, ug, J.nchunks, J.bch, s));
+        HIPCHK(launch_unpack(nullptr, nullptr, ug, J.nchunks, J.bch, s, lookback));
     }
-    if (J.nchunks > 1) {  // one chunk has no carry to pass on
+    if (J.nchunks > 1 && !lookback) {  // one chunk has no carry to pass on
         ProfScope ps(4, s, prof);
         HIPCHK(launch_carry_fix(nullptr, nullptr, ug, J.nchunks, s));
     }

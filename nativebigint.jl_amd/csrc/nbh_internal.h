@@ -88,7 +88,10 @@ struct Tile {
     uint32_t pad;
 };
 
-enum TileFlags : uint8_t { TILE_PACK = 1 };
+// TILE_PACK: the tile's points are packed from limbs.  TILE_CLEAR (a
+// single-job mid launch): tile 0 of prime 0 clears the job's chunk flags for
+// an unpack that resolves its own carries.
+enum TileFlags : uint8_t { TILE_PACK = 1, TILE_CLEAR = 2 };
 
 // Tiles generated from blockIdx when a launch covers a single job: block b is
 // (prime, op, tile) = (prime0 + b / (nops*per), (b / per) % nops, b % per).
@@ -142,6 +145,10 @@ constexpr int GU = 8;  // coefficients per thread per staging round
 
 constexpr uint32_t CHUNK_G = 1;  // carry out with carry-in 0
 constexpr uint32_t CHUNK_P = 2;  // every limb all-ones: carry passes through
+// Single-job unpack grids up to this many chunks resolve their carries in the
+// unpack itself (k_unpack<true>): far fewer blocks than one launch holds
+// resident at once (at least four unpack blocks per CU fit, 256 CUs).
+constexpr int64_t UNPACK_LB_MAX_CHUNKS = 256;
 
 }  // namespace nbh
 

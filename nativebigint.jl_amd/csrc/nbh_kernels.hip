@@ -1526,6 +1526,12 @@ __global__ void __launch_bounds__(MID_MAX_THREADS) k_mid(const DevJob *__restric
     const Tile t = fetch_tile(tiles, gen);
     const DevJob *J = job_of(jobs, t, gen);
     if (uni(cst(J)->zero)) return;
+    if ((t.flags & TILE_CLEAR) && t.prime == 0 && t.tile == 0) {
+        // the unpack after this launch resolves its carries itself (k_unpack<true>)
+        uint32_t *__restrict__ fl = unip(cst(J)->chunk_flags);
+        const int64_t nch = uni64(cst(J)->nchunks);
+        for (int64_t i = threadIdx.x; i < nch; i += blockDim.x) fl[i] = 0;
+    }
     mid_body(fld(t.prime), J, t, lds);
 }
 
@@ -1637,11 +1643,27 @@ __device__ __forceinline__ Seg segment_sum(const UnpackU &U, int64_t k0, int64_t
 // so every thread sums exactly one segment.
 // Held to five waves per SIMD (at most 96 VGPRs): with the staging sized to
 // the chunk width, five blocks' LDS fit a CU for b >= 40.
+//
+// LB (one job whose chunks all fit the GPU at once, UNPACK_LB_MAX_CHUNKS):
+// the carries between chunks are resolved here instead of by k_carry_fix.
+// Each chunk publishes its summary (carry out with carry-in 0, all-ones) with
+// a ready bit as soon as its own sums are done, then walks back over the
+// summaries of the chunks below it -- waiting for each to be published -- to
+// find its carry-in, and adds it before storing.  A chunk publishes before it
+// waits, so the lowest unfinished chunk never waits on anything; the grid is
+// far below the blocks one launch can hold resident at once, so every chunk
+// waited on is running or gets a slot as other work leaves.  The flags start
+// cleared (the mid kernel clears them, TILE_CLEAR) and each wait is bounded.
+constexpr uint32_t CHUNK_READY = 4;
+constexpr int LB_SPIN_MAX = 1 << 26;
+
+template <bool LB>
 __global__ void __launch_bounds__(UNPACK_THREADS) __attribute__((amdgpu_waves_per_eu(5)))
 k_unpack(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles, TileGen gen) {
     extern __shared__ uint64_t ustage[];
     __shared__ uint64_t spill0[UNPACK_THREADS], spill1[UNPACK_THREADS];
     __shared__ uint32_t wsum[UNPACK_THREADS / 64];
+    __shared__ int lb_cin, lb_first;
     const Tile t = fetch_tile(tiles, gen);
     const DevJob *__restrict__ Jp = job_of(jobs, t, gen);
     const auto *J = cst(Jp);
@@ -1738,18 +1760,63 @@ k_unpack(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles, TileGe
     q2 += c;
     c = q2 < c;
     q3 += c;
-    if (own) {
-        if (k0 + 0 < rn) r[k0 + 0] = q0;
-        if (k0 + 1 < rn) r[k0 + 1] = q1;
-        if (k0 + 2 < rn) r[k0 + 2] = q2;
-        if (k0 + 3 < rn) r[k0 + 3] = q3;
-    }
     if (tid == UNPACK_THREADS - 1) {
         // carry out of the chunk with carry-in 0, and whether all limbs propagate
         const uint32_t cg = (G || (Pp && my)) ? CHUNK_G : 0u;
         uint32_t allp = 1;
         for (int w = 0; w < UNPACK_THREADS / 64; w++) allp &= (wsum[w] >> 1);
-        flags[t.tile] = cg | (allp ? CHUNK_P : 0u);
+        if (LB)
+            __hip_atomic_store(&flags[t.tile], cg | (allp ? CHUNK_P : 0u) | CHUNK_READY, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        else
+            flags[t.tile] = cg | (allp ? CHUNK_P : 0u);
+    }
+    if (LB) {
+        // carry into the chunk = G(ch-1) | (P(ch-1) & carry into ch-1), as k_carry_fix
+        if (tid == 0) {
+            int cin = 0;
+            for (int64_t ch = (int64_t)t.tile - 1; ch >= 0; ch--) {
+                uint32_t f = 0;
+                for (int spin = 0; spin < LB_SPIN_MAX; spin++) {
+                    f = __hip_atomic_load(&flags[ch], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (f & CHUNK_READY) break;
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                if (f & CHUNK_G) {
+                    cin = 1;
+                    break;
+                }
+                if (!(f & CHUNK_P)) break;
+            }
+            lb_cin = cin;
+            lb_first = UNPACK_THREADS;
+        }
+        __syncthreads();
+        if (lb_cin) {
+            // add it into the leading run of all-ones limbs (thread 0 owns none)
+            const bool ones = !own || ((k0 + 0 < rn && q0 == ~0ull) && (k0 + 1 < rn && q1 == ~0ull) &&
+                                       (k0 + 2 < rn && q2 == ~0ull) && (k0 + 3 < rn && q3 == ~0ull));
+            if (!ones) atomicMin(&lb_first, tid);
+            __syncthreads();
+            const int first = lb_first;
+            if (own && tid < first) {
+                q0 = q1 = q2 = q3 = 0;
+            } else if (own && tid == first) {
+                q0 += 1;
+                c = q0 == 0;
+                q1 += c;
+                c = c && q1 == 0;
+                q2 += c;
+                c = c && q2 == 0;
+                q3 += c;
+            }
+        }
+    }
+    if (own) {
+        if (k0 + 0 < rn) r[k0 + 0] = q0;
+        if (k0 + 1 < rn) r[k0 + 1] = q1;
+        if (k0 + 2 < rn) r[k0 + 2] = q2;
+        if (k0 + 3 < rn) r[k0 + 3] = q3;
     }
 }
 
@@ -2573,10 +2640,12 @@ hipError_t launch_small(const DevJob &J, int threads, int ld, int twl, int nseg,
 }
 
 hipError_t launch_unpack(const DevJob *jobs, const Tile *tiles, const TileGen &gen, int64_t ntiles, int bmin,
-                         hipStream_t s) {
+                         hipStream_t s, bool lookback) {
     if (ntiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_unpack, dim3((unsigned)ntiles), dim3(UNPACK_THREADS), unpack_lds_bytes(bmin < UNPACK_MIN_B ? UNPACK_MIN_B : bmin), s,
-                       jobs, tiles, gen);
+    if (lookback && (tiles || ntiles > UNPACK_LB_MAX_CHUNKS)) return hipErrorInvalidValue;
+    const size_t lds = unpack_lds_bytes(bmin < UNPACK_MIN_B ? UNPACK_MIN_B : bmin);
+    if (lookback) hipLaunchKernelGGL(k_unpack<true>, dim3((unsigned)ntiles), dim3(UNPACK_THREADS), lds, s, jobs, tiles, gen);
+    else hipLaunchKernelGGL(k_unpack<false>, dim3((unsigned)ntiles), dim3(UNPACK_THREADS), lds, s, jobs, tiles, gen);
     return hipGetLastError();
 }
 

@@ -35,8 +35,10 @@ hipError_t launch_mid(const DevJob *jobs, const Tile *tiles, const TileGen &gen,
 hipError_t launch_small(const DevJob &J, int threads, int ld, int twl, int nseg, int cnt, int single,
                         size_t lds_bytes, hipStream_t s);
 // bmin: the smallest chunk width among the launch's jobs (sizes the staging LDS)
+// lookback: one job (no tile list) of at most UNPACK_LB_MAX_CHUNKS chunks
+// whose flags its mid launch cleared (TILE_CLEAR); no k_carry_fix after it.
 hipError_t launch_unpack(const DevJob *jobs, const Tile *tiles, const TileGen &gen, int64_t ntiles, int bmin,
-                         hipStream_t s);
+                         hipStream_t s, bool lookback = false);
 hipError_t launch_carry_fix(const DevJob *jobs, const Tile *tiles, const TileGen &gen, int64_t ntiles,
                             hipStream_t s);
 hipError_t launch_expand_tiles(const TileRange *ranges, int64_t nranges, Tile *tiles, hipStream_t s);
