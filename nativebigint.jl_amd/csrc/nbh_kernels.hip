@@ -141,10 +141,11 @@ __device__ __forceinline__ void gstn(double *p, int64_t i, double v) { __builtin
 __device__ __forceinline__ uint64_t gldu(const uint64_t *p, int64_t i) { return ((gcu64 *)p)[i]; }
 
 // Twiddle at column j of the row that starts rowoff doubles into a stage
-// table.  TW32: the row start is wave-uniform (scalar registers) and j * 8
-// fits 32 bits, so the load takes the scalar-base form with one shared 32-bit
-// lane offset, and no 64-bit address add per load.  The mid tile's passes
-// read columns j < 3 * (tile size), always in range.
+// table.  TW32: the row start and j are both below 2^29, so the load goes
+// through a raw buffer over the table with the row start as the wave-uniform
+// (scalar) offset and j as the 32-bit lane offset: no 64-bit address add per
+// load.  The mid tile's passes read rows and columns below 4 * (tile size),
+// always in range.
 template <bool TW32>
 __device__ __forceinline__ double twld(const double *tw, int64_t rowoff, int64_t j) {
     if (TW32) {
@@ -941,8 +942,8 @@ struct PointSrc {
 // `total` re-reads point 0.
 // Spectra points of a tile through a raw buffer based at the tile's first
 // point: the per-lane offset is the lane's first point, the per-load offset a
-// wave-uniform multiple of the round step, both 32-bit byte offsets (the mid
-// tile spans at most 3 * 2^27 bytes), so no 64-bit address add per load.
+// wave-uniform multiple of the round step, both 32-bit byte offsets (a mid
+// tile is at most 16384 points), so no 64-bit address add per load.
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t tile_rsrc(const double *x) {
     return __builtin_amdgcn_make_buffer_rsrc((void *)x, (short)0, -1, 0x00020000);
 }
