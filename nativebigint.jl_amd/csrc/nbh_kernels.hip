@@ -364,9 +364,11 @@ __device__ __forceinline__ void pass_44(const Fld &f, double *x0, double *x1, in
                                         const TileGeo &g) {
     if (TW32) {  // the mid tile (one column): the common pair strides take constant offsets
         switch (g.logC == 0 ? b.q : 0) {
+        case 2: pass44<FWD, false, TW32, 2>(f, x0, x1, narr, a, b, g); return;
         case 4: pass44<FWD, false, TW32, 4>(f, x0, x1, narr, a, b, g); return;
         case 8: pass44<FWD, false, TW32, 8>(f, x0, x1, narr, a, b, g); return;
         case 16: pass44<FWD, true, TW32, 16>(f, x0, x1, narr, a, b, g); return;
+        case 32: pass44<FWD, true, TW32, 32>(f, x0, x1, narr, a, b, g); return;
         case 64: pass44<FWD, true, TW32, 64>(f, x0, x1, narr, a, b, g); return;
         default: break;
         }
@@ -670,9 +672,11 @@ __device__ int fwd_until_last(const Fld &f, double *x, const DevPlan *__restrict
             const StageU b = stage_u(&P->st[s + 1]);
             // the mid tile is one column: the common pair strides take constant offsets
             switch (g.logC == 0 ? b.q : 0) {
+            case 2: pass44f<false, 1, 2>(f, x, nullptr, a, b, g); break;
             case 4: pass44f<false, 1, 4>(f, x, nullptr, a, b, g); break;
             case 8: pass44f<false, 1, 8>(f, x, nullptr, a, b, g); break;
             case 16: pass44f<true, 1, 16>(f, x, nullptr, a, b, g); break;
+            case 32: pass44f<true, 1, 32>(f, x, nullptr, a, b, g); break;
             case 64: pass44f<true, 1, 64>(f, x, nullptr, a, b, g); break;
             default:
                 if ((((a.q | b.q) << g.logC) & 15) == 0) pass44f<true, 1>(f, x, nullptr, a, b, g);
