@@ -134,10 +134,17 @@ def counters_report(pass1: str, pass2: str, trace: str, commands: list[str], ker
            "waves", f"per CU = SQ_WAVE_CYCLES (quad-cycles) x 4 / launch cycles / {cus}.", "",
            "| kernel | avg us | VALU instr | FP64 share | VALU issue use | FP64 issue use | SALU instr | LDS instr | "
            "waves/CU |", "|---|---|---|---|---|---|---|---|---|"]
+    def pick(table, k):
+        """The entry of kernel k, or of its one template instance (k<...>)."""
+        if k in table:
+            return table[k]
+        inst = [v for name, v in table.items() if name.startswith(k + "<")]
+        return inst[0] if len(inst) == 1 else None
+
     for k in kernels:
-        if k not in t1 or k not in t2 or k not in avg:
+        a, b, us = pick(t1, k), pick(t2, k), pick(avg, k)
+        if a is None or b is None or us is None:
             continue
-        a, b, us = t1[k], t2[k], avg[k]
         cyc = us * 1e-6 * ghz * 1e9
         cap = cus * 4 * cyc / 4
         valu = a["SQ_INSTS_VALU"]
