@@ -129,7 +129,12 @@ NBH_HDI constexpr int lds_doubles(int n) { return n + (n >> 4) + 1; }
 // Unpack geometry: each thread owns UNPACK_LPT consecutive output limbs.
 constexpr int UNPACK_THREADS = 256;
 constexpr int UNPACK_LPT = 4;
-constexpr int UNPACK_LPC = UNPACK_THREADS * UNPACK_LPT;  // limbs per block-wide pass (combine, basecase, Barrett)
+constexpr int UNPACK_LPC = UNPACK_THREADS * UNPACK_LPT;  // limbs per block-wide pass (basecase, Barrett)
+// Threads of a cofactor-step combine block: it walks its output in steps of
+// COMBINE_THREADS * UNPACK_LPT limbs, one carry exchange per step (512:
+// HGCD steps 3.27 -> 3.24 ms and whole levels 2.97 -> 2.96 ms against 256;
+// 1024 was slower on levels).
+constexpr int COMBINE_THREADS = 512;
 // Limbs per unpack chunk: thread 0 of an unpack block takes the 4 limbs just
 // below the chunk (the previous chunk's top segment, for its spill only) and
 // threads 1..255 the chunk's own limbs.

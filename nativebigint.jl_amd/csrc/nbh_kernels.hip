@@ -1879,7 +1879,7 @@ __global__ void __launch_bounds__(UNPACK_THREADS) k_carry_fix(const DevJob *__re
 // ---------------------------------------------------------------------------
 // 2x2 cofactor step outputs (the adds and subtractions of hgcd_mul2! and
 // hgcd_matrix_adjust!, src/gcd.jl:360-437): one workgroup per output walks
-// its limbs in chunks of UNPACK_LPC, each thread owning UNPACK_LPT limbs, with
+// its limbs in chunks of COMBINE_THREADS * UNPACK_LPT, each thread owning UNPACK_LPT limbs, with
 // the carry (borrow) between threads from the same generate/propagate ballot
 // prefix as the unpack and between chunks in a register.
 
@@ -2205,7 +2205,7 @@ __global__ void __launch_bounds__(SMALL_MAX_THREADS) k_small(const DevJob J, int
 }
 
 // r[0..lr) = X + Y*beta^ysh (add) or X - Y*beta^ysh (sub, X >= Y*beta^ysh):
-// one block walks the limbs in chunks of UNPACK_LPC, the carry (borrow)
+// one block walks the limbs in chunks of blockDim * UNPACK_LPT, the carry (borrow)
 // between threads from the ballot prefix and between chunks in a register.
 // A thread reads limbs k0..k0+3 of X and Y before it writes those of r, so r
 // may be X or Y.
@@ -2214,9 +2214,10 @@ __global__ void __launch_bounds__(SMALL_MAX_THREADS) k_small(const DevJob J, int
 __device__ __forceinline__ int64_t combine_pass(const uint64_t *X, int64_t lx, const uint64_t *Y, int64_t ly,
                                                 int64_t ysh, uint64_t *r, int64_t lr, bool sub, uint32_t *wsum) {
     const int tid = threadIdx.x;
+    const int lpc = (int)blockDim.x * UNPACK_LPT;  // limbs per block-wide step
     uint32_t carry = 0;
     int64_t top = 0;
-    for (int64_t c0 = 0; c0 < lr; c0 += UNPACK_LPC) {
+    for (int64_t c0 = 0; c0 < lr; c0 += lpc) {
         const int64_t k0 = c0 + UNPACK_LPT * tid;
         uint64_t d[UNPACK_LPT];
         bool prop = true;
@@ -2267,8 +2268,8 @@ __device__ __forceinline__ int64_t combine_pass(const uint64_t *X, int64_t lx, c
     return top;
 }
 
-__global__ void __launch_bounds__(UNPACK_THREADS) k_combine(const CombineJob *__restrict__ jobs) {
-    __shared__ uint32_t wsum[UNPACK_THREADS / 64];
+__global__ void __launch_bounds__(COMBINE_THREADS) k_combine(const CombineJob *__restrict__ jobs) {
+    __shared__ uint32_t wsum[COMBINE_THREADS / 64];
     __shared__ unsigned long long top;  // 1 + highest differing limb, 0 if none
     const auto *J = cst(jobs + blockIdx.x);
     const uint64_t *A = unip(J->A), *B = unip(J->B);
@@ -2702,7 +2703,7 @@ hipError_t launch_combine(const CombineJob *jobs, int64_t count, hipStream_t s, 
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_combine, dim3((unsigned)count), dim3(UNPACK_THREADS), 0, s, jobs);
+    hipLaunchKernelGGL(k_combine, dim3((unsigned)count), dim3(COMBINE_THREADS), 0, s, jobs);
     return hipGetLastError();
 }
 
