@@ -2288,17 +2288,24 @@ __global__ void __launch_bounds__(COMBINE_THREADS) k_combine(const CombineJob *_
     }
     bool swap = false;
     if (sub) {
-        // highest limb where A and B differ decides the sign
+        // highest limb where A and B differ decides the sign: scanned from the
+        // top, a block-width of limbs at a time, so it stops at the first
+        // window that differs (almost always the top one) instead of reading
+        // both operands whole
         if (tid == 0) top = 0;
         __syncthreads();
-        unsigned long long hi = 0;
         const int64_t L = la > lb ? la : lb;
-        for (int64_t k = tid; k < L; k += blockDim.x) {
-            const uint64_t a = k < la ? gldu(A, k) : 0, b = k < lb ? gldu(B, k) : 0;
-            if (a != b) hi = (unsigned long long)k + 1;
+        for (int64_t hi0 = L; hi0 > 0; hi0 -= blockDim.x) {
+            const int64_t k = hi0 - 1 - tid;
+            if (k >= 0) {
+                const uint64_t a = k < la ? gldu(A, k) : 0, b = k < lb ? gldu(B, k) : 0;
+                if (a != b) atomicMax(&top, (unsigned long long)k + 1);
+            }
+            __syncthreads();
+            const bool found = top != 0;
+            __syncthreads();  // every thread has read top before the next window may raise it
+            if (found) break;  // the same value in every thread: the loop exits uniformly
         }
-        if (hi) atomicMax(&top, hi);
-        __syncthreads();
         const int64_t t = (int64_t)top - 1;
         int32_t sg = 0;
         if (t >= 0) {
