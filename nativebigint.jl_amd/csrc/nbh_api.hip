@@ -1303,22 +1303,25 @@ int batch_plan(const JobIn *jobs, int64_t count, nbh_batch_s **out, size_t *need
         std::stable_sort(list.begin(), list.end(), [](const TileReq &x, const TileReq &y) {
             return x.threads != y.threads ? x.threads > y.threads : x.lds != y.lds ? x.lds > y.lds : x.t.prime < y.t.prime;
         });
-        // Mid tiles of one block size share a launch when the largest LDS
-        // request keeps as many blocks per CU as the smaller one would: the
-        // kernel takes its tile layout from the plan, not from the launch's
-        // LDS size, and the smaller (mostly whole-transform) tiles then fill
-        // the tail of the big launch instead of queueing launches of their own.
-        auto mid_blocks_per_cu = [](size_t lds, int threads) {
-            const int by_waves = (MID_MAX_THREADS / 64) / ((threads + 63) / 64);
-            return std::min<int64_t>(by_waves, (int64_t)(LDS_PER_CU / std::max<size_t>(lds, 1)));
+        // LDS tiles of one block size share a launch when the largest LDS
+        // request keeps as many blocks per CU as the smaller one would (and,
+        // for strided groups, the same build): the kernels take their tile
+        // layout from the plan, not from the launch's LDS size, and the
+        // smaller tiles then fill the tail of the big launch instead of
+        // queueing launches of their own.  Column groups (marker LDS sizes)
+        // and the unpack stay one launch per class.
+        auto joins = [&](const nbh_batch_s::Phase &ph, size_t lds) {
+            if (lds == ph.lds) return true;
+            if (kind == 1) return lds_blocks_per_cu(false, ph.lds, ph.threads) >= lds_blocks_per_cu(false, lds, ph.threads);
+            if (kind != 0 || ph.lds <= COL_BIG || lds <= COL_BIG) return false;
+            return group_six_waves(ph.lds, ph.threads) == group_six_waves(lds, ph.threads) &&
+                   lds_blocks_per_cu(true, ph.lds, ph.threads) >= lds_blocks_per_cu(true, lds, ph.threads);
         };
         size_t i = 0;
         const int step = B->nsteps++;
         while (i < list.size()) {
             nbh_batch_s::Phase ph{kind, fwd, ntiles, 0, list[i].lds, list[i].threads, step};
-            while (i < list.size() && list[i].threads == ph.threads &&
-                   (list[i].lds == ph.lds ||
-                    (kind == 1 && mid_blocks_per_cu(ph.lds, ph.threads) >= mid_blocks_per_cu(list[i].lds, ph.threads)))) {
+            while (i < list.size() && list[i].threads == ph.threads && joins(ph, list[i].lds)) {
                 ranges.push_back(TileRange{list[i].t, list[i].count, (uint32_t)ntiles});
                 ntiles += list[i].count;
                 ph.count += list[i].count;

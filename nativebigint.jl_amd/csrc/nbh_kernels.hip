@@ -2626,8 +2626,7 @@ hipError_t launch_group(const DevJob *jobs, const Tile *tiles, const TileGen &ge
         return hipGetLastError();
     }
     // waves per CU that LDS admits: above 16 (four per SIMD) the six-wave build pays
-    const int64_t lds_waves = (int64_t)(LDS_PER_CU / std::max<size_t>(lds_bytes, 1)) * (threads / 64);
-    const bool w6 = lds_waves > 16;
+    const bool w6 = group_six_waves(lds_bytes, threads);
     const dim3 grid((unsigned)ntiles), block(threads);
     if (fwd) {
         if (w6) hipLaunchKernelGGL(k_group_fwd_w6, grid, block, lds_bytes, s, jobs, tiles, gen);

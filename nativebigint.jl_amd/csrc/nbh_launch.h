@@ -22,6 +22,21 @@ constexpr size_t COL_SMALL = 1;
 constexpr size_t COL_BIG = 2;
 constexpr int COL_THREADS = 256;  // columns per tile at most
 
+// An LDS group launch takes the six-wave build when LDS admits more than 16
+// waves per CU (four per SIMD, the unrestricted build's VGPR limit).
+inline bool group_six_waves(size_t lds_bytes, int threads) {
+    return (int64_t)(LDS_PER_CU / (lds_bytes > 0 ? lds_bytes : 1)) * (threads / 64) > 16;
+}
+
+// Blocks per CU of an LDS group (mid: false) or mid launch: the lesser of
+// what LDS admits and what the build's waves per CU admit.
+inline int64_t lds_blocks_per_cu(bool group, size_t lds_bytes, int threads) {
+    const int cap = group && group_six_waves(lds_bytes, threads) ? 24 : 16;
+    const int64_t by_lds = (int64_t)(LDS_PER_CU / (lds_bytes > 0 ? lds_bytes : 1));
+    const int64_t by_waves = cap / ((threads + 63) / 64);
+    return by_lds < by_waves ? by_lds : by_waves;
+}
+
 // Tile-driven launches: `tiles` is a device tile list of ntiles entries, or
 // NULL to generate ntiles tiles of job 0 from `gen`.
 hipError_t launch_group(const DevJob *jobs, const Tile *tiles, const TileGen &gen, int64_t ntiles, int fwd,
