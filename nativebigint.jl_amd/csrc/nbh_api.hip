@@ -1310,9 +1310,12 @@ int batch_plan(const JobIn *jobs, int64_t count, nbh_batch_s **out, size_t *need
         // smaller tiles then fill the tail of the big launch instead of
         // queueing launches of their own.  Column groups (marker LDS sizes)
         // and the unpack stay one launch per class.
-        auto joins = [&](const nbh_batch_s::Phase &ph, size_t lds) {
+        auto joins = [&](const nbh_batch_s::Phase &ph, size_t lds, int threads) {
+            if (kind == 1)  // mid tiles also join a launch of larger blocks (their extra threads idle)
+                return lds <= ph.lds && threads <= ph.threads &&
+                       lds_blocks_per_cu(false, ph.lds, ph.threads) >= lds_blocks_per_cu(false, lds, threads);
+            if (threads != ph.threads) return false;
             if (lds == ph.lds) return true;
-            if (kind == 1) return lds_blocks_per_cu(false, ph.lds, ph.threads) >= lds_blocks_per_cu(false, lds, ph.threads);
             if (kind != 0 || ph.lds <= COL_BIG || lds <= COL_BIG) return false;
             return group_six_waves(ph.lds, ph.threads) == group_six_waves(lds, ph.threads) &&
                    lds_blocks_per_cu(true, ph.lds, ph.threads) >= lds_blocks_per_cu(true, lds, ph.threads);
@@ -1321,7 +1324,7 @@ int batch_plan(const JobIn *jobs, int64_t count, nbh_batch_s **out, size_t *need
         const int step = B->nsteps++;
         while (i < list.size()) {
             nbh_batch_s::Phase ph{kind, fwd, ntiles, 0, list[i].lds, list[i].threads, step};
-            while (i < list.size() && list[i].threads == ph.threads && joins(ph, list[i].lds)) {
+            while (i < list.size() && joins(ph, list[i].lds, list[i].threads)) {
                 ranges.push_back(TileRange{list[i].t, list[i].count, (uint32_t)ntiles});
                 ntiles += list[i].count;
                 ph.count += list[i].count;
