@@ -1525,6 +1525,14 @@ struct nbh_hgcd_s {
 
 namespace {
 
+// One combine workgroup walks its whole output, so the longest outputs set the
+// launch's tail: they go first, to be dispatched in the first round of blocks.
+// Jobs are independent (own outputs, sign and length words), so order is free.
+void sort_combine_jobs(std::vector<CombineJob> &comb) {
+    std::stable_sort(comb.begin(), comb.end(),
+                     [](const CombineJob &x, const CombineJob &y) { return x.lr > y.lr; });
+}
+
 void hgcd_free(nbh_hgcd_s *H) {
     if (!H) return;
     batch_free(H->products);
@@ -1578,6 +1586,7 @@ int hgcd_create(const nbh_hgcd_job *jobs, int64_t count, int mode, nbh_hgcd_s **
         }
     }
     RCCHK(batch_create(prods.data(), (int64_t)prods.size(), &H->products));
+    sort_combine_jobs(comb);
     H->ncomb = (int64_t)comb.size();
     HIPCHK(hipMalloc((void **)&H->dcomb, sizeof(CombineJob) * comb.size()));
     HIPCHK(hipMemcpy(H->dcomb, comb.data(), sizeof(CombineJob) * comb.size(), hipMemcpyHostToDevice));
@@ -1640,6 +1649,7 @@ int hgcd_level_create(const nbh_hgcd_level_job *jobs, int64_t count, nbh_hgcd_s 
             }
     }
     RCCHK(batch_create(prods.data(), (int64_t)prods.size(), &H->products));
+    sort_combine_jobs(comb);
     H->ncomb = (int64_t)comb.size();
     H->lens = true;
     HIPCHK(hipMalloc((void **)&H->dcomb, sizeof(CombineJob) * comb.size()));
