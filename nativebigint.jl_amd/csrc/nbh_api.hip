@@ -1385,8 +1385,13 @@ int batch_plan(const JobIn *jobs, int64_t count, nbh_batch_s **out, size_t *need
     }
     push_phase(2, 0, list);
     push_phase(3, 0, list2);
-    // the schoolbook products: one launch, beside the first NTT step
+    // the schoolbook products: one launch, beside the first NTT step; one
+    // workgroup per product, the most limb products first so the longest
+    // blocks are dispatched in the first round (products are independent)
     if (!B->base.empty()) {
+        std::stable_sort(B->base.begin(), B->base.end(), [](const BaseJob &x, const BaseJob &y) {
+            return x.m * std::min(x.n, x.rn) > y.m * std::min(y.n, y.rn);
+        });
         const int step = B->phases.empty() ? B->nsteps++ : B->phases.front().step;
         B->phases.insert(B->phases.begin(), nbh_batch_s::Phase{5, 0, 0, (int64_t)B->base.size(), 0,
                                                               UNPACK_THREADS, step});
