@@ -1311,7 +1311,10 @@ int batch_plan(const JobIn *jobs, int64_t count, nbh_batch_s **out, size_t *need
         // queueing launches of their own.  Column groups (marker LDS sizes)
         // and the unpack stay one launch per class.
         auto joins = [&](const nbh_batch_s::Phase &ph, size_t lds, int threads) {
-            if (kind == 1)  // mid tiles also join a launch of larger blocks (their extra threads idle)
+            // mid tiles also join a launch of larger blocks: their extra threads
+            // idle, and they take the general load/store passes (the edge
+            // passes need exactly one thread per 16 points)
+            if (kind == 1)
                 return lds <= ph.lds && threads <= ph.threads &&
                        lds_blocks_per_cu(false, ph.lds, ph.threads) >= lds_blocks_per_cu(false, lds, threads);
             if (threads != ph.threads) return false;
