@@ -361,8 +361,10 @@ int32_t nbh_get_tuning_for_single(int64_t N, int32_t *knobs4);
  * clears them.  Instrumentation only: no reference counterpart. */
 #define NBH_PROF_KINDS 8
 int32_t nbh_profile_enable(int32_t on);
-/* A batch step's launches (one per tile shape) are independent; by default
- * they fork onto 8 library streams so small shapes run beside big ones.
+/* A batch step's launches (one per block shape; tiles of smaller shapes
+ * join a larger one when that keeps their blocks per CU) are independent; by
+ * default they fork onto 8 library streams so small shapes run beside big
+ * ones.
  * nside = 0 issues them in order on the caller's stream (every launch then
  * has the GPU to itself, which is what a per-launch profile should see). */
 int32_t nbh_set_streams(int32_t nside);

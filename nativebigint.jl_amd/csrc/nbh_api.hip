@@ -1489,7 +1489,7 @@ int launch_phase(nbh_batch_s *B, const nbh_batch_s::Phase &ph, hipStream_t s) {
     return NBH_OK;
 }
 
-// A step's launches (one per tile shape) are independent: they fork onto the
+// A step's launches (one per block shape) are independent: they fork onto the
 // library's side streams after an event on `s` and join back before the next
 // step, so small shape classes run beside the big ones instead of after them.
 int batch_run(nbh_batch_s *B, hipStream_t s) {
