@@ -135,6 +135,9 @@ constexpr int UNPACK_LPC = UNPACK_THREADS * UNPACK_LPT;  // limbs per block-wide
 // HGCD steps 3.27 -> 3.24 ms and whole levels 2.97 -> 2.96 ms against 256;
 // 1024 was slower on levels).
 constexpr int COMBINE_THREADS = 512;
+
+// Chunks per k_carry_fix block (one wave each).
+constexpr int CARRY_FIX_WAVES = 4;
 // Limbs per unpack chunk: thread 0 of an unpack block takes the 4 limbs just
 // below the chunk (the previous chunk's top segment, for its spill only) and
 // threads 1..255 the chunk's own limbs.

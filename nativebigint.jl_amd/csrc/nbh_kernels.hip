@@ -1825,54 +1825,48 @@ k_unpack(const DevJob *__restrict__ jobs, const Tile *__restrict__ tiles, TileGe
     }
 }
 
-// Carry into chunk ch = G(ch-1) | (P(ch-1) & carry into ch-1): walk back over
-// the chunk summaries (all written by the previous launch, so no waiting),
-// then add the carry into the chunk's leading run of all-ones limbs.
-__global__ void __launch_bounds__(UNPACK_THREADS) k_carry_fix(const DevJob *__restrict__ jobs,
-                                                              const Tile *__restrict__ tiles, TileGen gen) {
-    __shared__ int first;
-    const Tile t = fetch_tile(tiles, gen);
-    const auto *J = cst(job_of(jobs, t, gen));
-    if (uni(J->zero) || t.tile == 0) return;
-    const uint32_t *__restrict__ flags = unip(J->chunk_flags);
-    int cin = 0;
-    for (int64_t ch = (int64_t)t.tile - 1; ch >= 0; ch--) {
-        const uint32_t f = uniu(flags[ch]);
-        if (f & CHUNK_G) { cin = 1; break; }
+// Carry into chunk ch = G(ch-1) | (P(ch-1) & carry into ch-1): one wave per
+// chunk (CARRY_FIX_WAVES chunks per block) walks back over the chunk
+// summaries (all written by the previous launch, so no waiting), then adds
+// the carry into the chunk's leading run of all-ones limbs, 64 limbs a round:
+// the run becomes zeros and the first other limb gains one.  Most chunks
+// take no carry, so most waves read one summary word and exit.
+__global__ void __launch_bounds__(64 * CARRY_FIX_WAVES) k_carry_fix(const DevJob *__restrict__ jobs,
+                                                                    const Tile *__restrict__ tiles, TileGen gen,
+                                                                    int64_t ntiles) {
+    const int lane = threadIdx.x & 63;
+    const int64_t idx = (int64_t)blockIdx.x * CARRY_FIX_WAVES + (threadIdx.x >> 6);
+    if (idx >= ntiles) return;  // whole waves
+    uint32_t job = 0, tile;
+    if (tiles) {
+        const Tile tt = tiles[idx];
+        job = tt.job;
+        tile = tt.tile;
+    } else {
+        tile = (uint32_t)(idx % gen.per);
+    }
+    const DevJob *__restrict__ J = jobs ? jobs + job : &gen.job;
+    if (J->zero || tile == 0) return;
+    const uint32_t *__restrict__ flags = J->chunk_flags;
+    bool cin = false;
+    for (int64_t ch = (int64_t)tile - 1; ch >= 0; ch--) {
+        const uint32_t f = flags[ch];
+        if (f & CHUNK_G) { cin = true; break; }
         if (!(f & CHUNK_P)) break;
     }
-    if (!cin) return;
-    const int tid = threadIdx.x;
-    // the unpack's layout: thread t >= 1 owns limbs [L0 + 4(t-1), L0 + 4t), thread 0 none
-    const int64_t k0 = (int64_t)t.tile * UNPACK_CHUNK + UNPACK_LPT * (tid - 1);
-    const bool own = tid > 0;
-    uint64_t *__restrict__ r = unip(J->r);
-    const int64_t rn = uni64(J->rn);
-    uint64_t v[UNPACK_LPT];
-    bool ones = true;
-#pragma unroll
-    for (int w = 0; w < UNPACK_LPT; w++) {
-        v[w] = (own && k0 + w < rn) ? r[k0 + w] : 0;
-        ones = ones && (!own || ((k0 + w < rn) && v[w] == ~0ull));
-    }
-    if (tid == 0) first = UNPACK_THREADS;
-    __syncthreads();
-    if (!ones) atomicMin(&first, tid);
-    __syncthreads();
-    if (!own) {
-        // thread 0 owns no limbs (and is never the first that stops the carry)
-    } else if (tid < first) {
-#pragma unroll
-        for (int w = 0; w < UNPACK_LPT; w++)
-            if (k0 + w < rn) r[k0 + w] = 0;
-    } else if (tid == first) {
-        uint64_t cc = 1;
-#pragma unroll
-        for (int w = 0; w < UNPACK_LPT; w++) {
-            const uint64_t nv = v[w] + cc;
-            cc = nv < cc;
-            if (k0 + w < rn) r[k0 + w] = nv;
-        }
+    if (!cin) return;  // the same for the whole wave
+    uint64_t *__restrict__ r = J->r;
+    const int64_t rn = J->rn;
+    const int64_t lo = (int64_t)tile * UNPACK_CHUNK;
+    for (int base = 0; base < UNPACK_CHUNK; base += 64) {
+        const int64_t k = lo + base + lane;
+        const bool in = base + lane < UNPACK_CHUNK && k < rn;
+        const uint64_t v = in ? r[k] : 0;
+        const uint64_t stop = __ballot(in && v != ~0ull);  // limbs that end the run
+        const int first = stop ? __builtin_ctzll(stop) : 64;
+        if (in && lane < first) r[k] = 0;
+        else if (in && lane == first) r[k] = v + 1;
+        if (stop) break;  // a wave-wide value: the loop exits uniformly
     }
 }
 
@@ -2664,7 +2658,8 @@ hipError_t launch_unpack(const DevJob *jobs, const Tile *tiles, const TileGen &g
 hipError_t launch_carry_fix(const DevJob *jobs, const Tile *tiles, const TileGen &gen, int64_t ntiles,
                             hipStream_t s) {
     if (ntiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_carry_fix, dim3((unsigned)ntiles), dim3(UNPACK_THREADS), 0, s, jobs, tiles, gen);
+    hipLaunchKernelGGL(k_carry_fix, dim3((unsigned)((ntiles + CARRY_FIX_WAVES - 1) / CARRY_FIX_WAVES)),
+                       dim3(64 * CARRY_FIX_WAVES), 0, s, jobs, tiles, gen, ntiles);
     return hipGetLastError();
 }
 
